@@ -1,0 +1,193 @@
+/*
+ * soil_hip.h — C ABI of the MI355X (gfx950) particle substrate.
+ *
+ * Replaces, for one process and one GPU, the per-step substrate systems of
+ * SOIL (SueHeir/soil 0.1.3). Each entry point names the reference function it
+ * stands in for (paths relative to the reference repository root). The Rust
+ * `soil_hip` plugin layer binds these symbols through an `extern "C"` block;
+ * tests and the benchmark bind them through ctypes.
+ *
+ * Conventions
+ *   - Every call returns int: 0 = ok, < 0 = error (SOIL_HIP_E*).
+ *     soil_hip_last_error() returns the message of the last failure on the
+ *     calling thread.
+ *   - Host buffers are caller-owned and AoS-of-3 where the reference uses
+ *     [f64; 3] (pos, vel, force, image). Device memory is owned by the box.
+ *   - A box or batch handle is not thread-safe. Each box and each batch owns
+ *     one HIP stream; calls on different handles may run on different threads.
+ *   - Geometry is f64 end to end; integer outputs (bins, ghost lists, CSR) are
+ *     bit-identical to the reference builder on the same inputs.
+ */
+#ifndef SOIL_HIP_H
+#define SOIL_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* error codes */
+#define SOIL_HIP_OK 0
+#define SOIL_HIP_EINVAL (-1)     /* bad argument */
+#define SOIL_HIP_EDEVICE (-2)    /* HIP runtime / no device */
+#define SOIL_HIP_ECAPACITY (-3)  /* a host buffer is too small (required size in out-param) */
+#define SOIL_HIP_ESTATE (-4)     /* call out of order (e.g. step before setup) */
+#define SOIL_HIP_EUNSUPPORTED (-5) /* case outside the device path (see message) */
+#define SOIL_HIP_EBUDGET (-6)    /* serving: job does not fit the HBM budget */
+
+/* AtomData column flags (soil_derive/src/lib.rs:200-500,
+ * docs/SOIL_ATOMDATA_CONTRACT.md:87-143): FORWARD = overwritten on ghosts by
+ * forward comm, REVERSE = ghost values added onto owners by reverse comm,
+ * ZERO = reset to 0 each step by zero_all_forces. */
+#define SOIL_HIP_COL_FORWARD 1u
+#define SOIL_HIP_COL_REVERSE 2u
+#define SOIL_HIP_COL_ZERO 4u
+
+/* CommState (soil_core/src/lib.rs:87-94) */
+#define SOIL_HIP_FULL_REBUILD 0
+#define SOIL_HIP_COMMUNICATE_ONLY 1
+
+typedef struct soil_hip_ctx soil_hip_ctx;
+typedef struct soil_hip_box soil_hip_box;
+typedef struct soil_hip_batch soil_hip_batch;
+typedef struct soil_hip_server soil_hip_server;
+
+/* Box description: [domain] (soil_core/src/domain.rs:46-78), [neighbor]
+ * (soil_core/src/neighbor.rs:44-80), Atom::dt, and the [deform] xy erate
+ * (soil_deform/src/lib.rs:144-165). Field order is the C layout. */
+typedef struct {
+    double lo[3], hi[3];
+    int32_t periodic[3];
+    int32_t newton;          /* neighbor.newton (half list + reverse comm) */
+    double skin_fraction;    /* neighbor.skin_fraction */
+    double bin_min_size;     /* neighbor.bin_size */
+    uint64_t every;          /* neighbor.every */
+    int32_t check;           /* neighbor.check */
+    int32_t pad0;
+    uint64_t sort_every;     /* neighbor.sort_every */
+    double dt;               /* Atom::dt */
+    double shear_rate;       /* deform xy erate γ̇ (used when has_shear) */
+    int32_t has_shear;
+    int32_t pad1;
+} soil_hip_box_desc;
+
+/* ── context ─────────────────────────────────────────────────────────────── */
+const char* soil_hip_last_error(void);
+/* Opens `device` (ordinal). Replaces the wgpu GpuContext::new of
+ * soil_gpu/src/lib.rs:53-111 for the HIP path. */
+int soil_hip_ctx_create(int device, soil_hip_ctx** out);
+int soil_hip_ctx_destroy(soil_hip_ctx* ctx);
+/* Number of HIP devices visible (0 when none); no context needed. */
+int soil_hip_device_count(int* out);
+
+/* ── box lifecycle ───────────────────────────────────────────────────────── */
+/* Creates a box: Domain (domain.rs:235-296 on one rank), Neighbor::new
+ * (neighbor.rs:240-285), CommTopology (comm.rs:177-182). */
+int soil_hip_box_create(soil_hip_ctx* ctx, const soil_hip_box_desc* desc, soil_hip_box** out);
+int soil_hip_box_destroy(soil_hip_box* box);
+
+/* Registers an AtomData column of `width` f64 per atom with SOIL_HIP_COL_*
+ * flags; mirrors register_atom_data! + #[derive(AtomData)] (atom.rs:106-118,
+ * soil_derive/src/lib.rs:200-356). Must precede soil_hip_upload. */
+int soil_hip_register_column(soil_hip_box* box, int width, uint32_t flags, int* col_id);
+
+/* Uploads n local atoms (Atom columns, atom.rs:406-431; push order = local
+ * index). pos/vel/force/image are AoS-of-3; vel, force, image, tag, type may
+ * be NULL (zeros; tag defaults to the index). inv_mass = 1/mass. */
+int soil_hip_upload(soil_hip_box* box, size_t n, const double* pos, const double* vel, const double* force,
+                    const double* mass, const double* radius, const uint32_t* tag, const uint32_t* type,
+                    const int32_t* image);
+/* Column data for locals [0, n): n*width f64 */
+int soil_hip_upload_column(soil_hip_box* box, int col, const double* data, size_t n);
+
+/* Setup systems: neighbor_setup (neighbor.rs:604-665), comm_setup
+ * (comm.rs:253-283) and setup_deform (soil_deform/src/lib.rs:346-420).
+ * Sizes device buffers for the uploaded atoms. */
+int soil_hip_setup(soil_hip_box* box);
+
+/* Enables the harmonic soft-sphere test consumer F = k(ri+rj-d) n̂ in the
+ * Force phase of soil_hip_step (0 disables; the reference has no built-in
+ * force, physics tiers provide it). */
+int soil_hip_set_consumer(soil_hip_box* box, int enabled, double k);
+
+/* ── per-phase systems (one call = one reference system on this box) ───────
+ * Each enqueues on the box stream; results are read back with the getters. */
+int soil_hip_apply_deform(soil_hip_box* box);          /* soil_deform/src/lib.rs:435-604 (xy LE) */
+int soil_hip_initial_integration(soil_hip_box* box);   /* soil_verlet/src/lib.rs:131-162 */
+/* decide_rebuild (neighbor.rs:796-822, needs_rebuild :762-784,
+ * displacement_exceeded :693-758). Runs only in CommunicateOnly, like its
+ * run_if. *full_rebuild (may be NULL) receives the resulting state == FullRebuild. */
+int soil_hip_decide_rebuild(soil_hip_box* box, int* full_rebuild);
+int soil_hip_remove_ghost_atoms(soil_hip_box* box);    /* atom.rs:625-629 (FullRebuild only) */
+int soil_hip_zero_all_forces(soil_hip_box* box);       /* atom.rs:631-635 */
+int soil_hip_pbc(soil_hip_box* box);                   /* domain.rs:611-729 (FullRebuild only) */
+int soil_hip_sort_atoms_by_bin(soil_hip_box* box);     /* neighbor.rs:834-997 */
+int soil_hip_borders(soil_hip_box* box);               /* comm.rs:682-860 (FullRebuild only) */
+int soil_hip_forward_comm_borders(soil_hip_box* box);  /* comm.rs:582-592 (CommunicateOnly only) */
+int soil_hip_bin_neighbor_list(soil_hip_box* box);     /* neighbor.rs:1013-1319 (FullRebuild only) */
+int soil_hip_harmonic_force(soil_hip_box* box);        /* test consumer (Force phase) */
+int soil_hip_reverse_send_force(soil_hip_box* box);    /* comm.rs:938-1024 (newton only) */
+int soil_hip_final_integration(soil_hip_box* box);     /* soil_verlet/src/lib.rs:175-194 */
+/* n full timesteps in ParticleSimScheduleSet order (schedule.rs:22-52). */
+int soil_hip_step(soil_hip_box* box, int nsteps);
+/* Waits for the box stream; reports any device-side error flag. */
+int soil_hip_sync(soil_hip_box* box);
+/* Sets the xy tilt and boundary velocity directly (a static triclinic box or a
+ * test); marks bounds_changed like apply_deform (domain.rs:126-137 fields). */
+int soil_hip_set_tilt(soil_hip_box* box, double xy, double boundary_vel_x);
+
+/* ── readback (lazy; each waits for the box stream) ──────────────────────── */
+/* counts[0..7] = nlocal, nghost, npairs, ncells, rebuilds, steps, nstencil_fwd, state */
+int soil_hip_get_counts(soil_hip_box* box, uint64_t* counts8);
+/* Neighbor::neighbor_offsets (nlocal+1) and neighbor_indices
+ * (neighbor.rs:91-95). ECAPACITY when cap_idx < npairs (*n_idx = npairs). */
+int soil_hip_get_csr(soil_hip_box* box, uint32_t* offsets, size_t cap_off, uint32_t* indices, size_t cap_idx,
+                     size_t* n_idx);
+/* All atoms, locals then ghosts (cap = nlocal+nghost); any pointer may be NULL. */
+int soil_hip_get_atoms(soil_hip_box* box, size_t cap, double* pos, double* vel, double* force, double* radius,
+                       double* mass, double* inv_mass, uint32_t* tag, int32_t* origin, int32_t* image);
+int soil_hip_get_column(soil_hip_box* box, int col, double* out, size_t cap_atoms);
+/* Last build's bins: bin_start (ncells+1), bin_sorted_atoms, bin_atom_sorted_idx,
+ * bin_atom_cell (nlocal+nghost each); neighbor.rs:149-161. */
+int soil_hip_get_bins(soil_hip_box* box, uint32_t* bin_start, size_t cap_cells1, uint32_t* sorted_atoms,
+                      uint32_t* sorted_idx, uint32_t* atom_cell, size_t cap_atoms);
+/* Swap table (SwapData, comm.rs:114-122): 6 rows of
+ * [recv_start, recv_count, nsend, to_proc, from_proc, off xyz, voff xyz]. */
+int soil_hip_get_swaps(soil_hip_box* box, double* out, size_t cap_rows, size_t* nrows);
+/* Grid of the current step, same layout as the oracle's 21-double grid record:
+ * bin_count[3], bin_size[3], bin_origin[3], ghost_cutoff, max_cutoff,
+ * bin_min_size, tilt[3], boundary_vel[3], cached_min_skin, uniform, uniform_cutoff_sq */
+int soil_hip_get_grid(soil_hip_box* box, double* out21);
+int soil_hip_get_stencil(soil_hip_box* box, int32_t* out, size_t cap, size_t* n);
+
+/* ── batched ensemble (many boxes, one launch per phase, hipGraph step) ─── */
+/* Boxes must be set up and not used individually while in the batch. */
+int soil_hip_batch_create(soil_hip_ctx* ctx, soil_hip_box** boxes, int nboxes, soil_hip_batch** out);
+int soil_hip_batch_destroy(soil_hip_batch* batch);
+/* Autotune launch shapes for this batch (bit-neutral knobs only); may run steps
+ * on a scratch copy, never on the batch state. */
+int soil_hip_batch_autotune(soil_hip_batch* batch);
+/* n timesteps of every box; use_graph = 1 replays one captured step graph. */
+int soil_hip_batch_step(soil_hip_batch* batch, int nsteps, int use_graph);
+int soil_hip_batch_sync(soil_hip_batch* batch);
+
+/* ── kernel timing (HIP events around the CSR build of every rebuild) ──── */
+int soil_hip_set_timing(soil_hip_box* box, int enabled);
+/* out[0] = builds timed, out[1] = total ms of CSR count+fill, out[2] = total
+ * algorithmic bytes of those builds, out[3] = total ms of the fill kernel only */
+int soil_hip_get_timing(soil_hip_box* box, double* out4);
+int soil_hip_batch_set_timing(soil_hip_batch* batch, int enabled);
+int soil_hip_batch_get_timing(soil_hip_batch* batch, double* out4);
+
+/* ── synthetic inputs (host only; SplitMix64 + Box–Muller) ─────────────── */
+/* Writes a box description and atoms for `config` (1..5) / `box_id`.
+ * With pos == NULL only the desc and *n are produced (query size). */
+int soil_hip_gen_box(int config, int box_id, uint64_t seed, soil_hip_box_desc* desc, size_t* n, double* pos,
+                     double* vel, double* mass, double* radius);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SOIL_HIP_H */

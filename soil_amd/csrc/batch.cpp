@@ -1,0 +1,382 @@
+// Batched ensemble: many independent boxes step together, one launch per
+// phase for all of them (the kernels walk a combined tile table), and the
+// whole step is captured once into a hipGraph and replayed.
+//
+// No host decision sits inside a step: every phase kernel reads its box's
+// device state (FullRebuild / CommunicateOnly, counts, error bits) and exits
+// early when the phase does not apply, so one graph serves Lees–Edwards boxes
+// (rebuild every step) and periodic boxes (trigger-driven rebuilds) alike.
+// Per-step geometry comes from a ring of Geom records [K][nbox] that the host
+// fills half a ring ahead of the device.
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+
+#include "box.hpp"
+
+using namespace soil_hip;
+
+#define HIPCHK(call)                                                                                \
+    do {                                                                                            \
+        hipError_t e_ = (call);                                                                     \
+        if (e_ != hipSuccess) return set_error(SOIL_HIP_EDEVICE, "%s: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+namespace soil_hip {
+
+static int batch_tiles(soil_hip_batch* bt) {
+    TileTables tt;
+    for (size_t k = 0; k < bt->boxes.size(); ++k) build_tiles(tt, (uint32_t)k, bt->boxes[k]->caps);
+    uint32_t maxgrid = bt->L.maxgrid ? bt->L.maxgrid : 4096;
+    upload_tiles(tt, bt->L, &bt->arena_tiles);
+    bt->L.boxes = bt->d_ptrs;
+    bt->L.stream = bt->stream;
+    bt->L.maxgrid = maxgrid;
+    return 0;
+}
+
+static void launch_step(const Launch& L) {
+    phase_deform(L);
+    phase_initial(L);
+    phase_decide(L);
+    phase_zero(L);
+    phase_pbc(L);
+    phase_sort(L);
+    phase_borders(L);
+    phase_forward(L);
+    phase_build(L, nullptr, nullptr, nullptr);
+    phase_force(L);
+    phase_reverse(L);
+    phase_final(L);
+    phase_step_end(L);
+}
+
+static void drop_graph(soil_hip_batch* bt) {
+    if (bt->graph_exec) hipGraphExecDestroy(bt->graph_exec);
+    if (bt->graph) hipGraphDestroy(bt->graph);
+    bt->graph_exec = nullptr;
+    bt->graph = nullptr;
+}
+
+static int capture_graph(soil_hip_batch* bt) {
+    drop_graph(bt);
+    HIPCHK(hipStreamBeginCapture(bt->stream, hipStreamCaptureModeThreadLocal));
+    launch_step(bt->L);
+    HIPCHK(hipStreamEndCapture(bt->stream, &bt->graph));
+    HIPCHK(hipGraphInstantiate(&bt->graph_exec, bt->graph, nullptr, nullptr, 0));
+    return 0;
+}
+
+// Reads every box's state; grows boxes whose ghosts or pairs approach capacity.
+// Returns >0 when a box was reallocated (tiles and graph must be rebuilt).
+static int batch_check(soil_hip_batch* bt, int* grown) {
+    *grown = 0;
+    for (auto* b : bt->boxes) {
+        int rc = pull_state(b);
+        if (rc) return rc;
+        const BoxState& st = *b->h_st;
+        if (st.err) return check_err_bits(st.err);
+        const uint64_t ghost_cap = (uint64_t)b->caps.atoms - b->caps.local;
+        if ((uint64_t)st.nghost * 10 > ghost_cap * 8) {
+            Caps nc = b->caps;
+            nc.atoms = (uint32_t)std::min<uint64_t>((uint64_t)b->caps.local + 2 * ghost_cap + 4096, 0x7FFFFFFFull);
+            nc.tiles = (nc.atoms + BORDER_TILE - 1) / BORDER_TILE;
+            rc = alloc_atoms(b, nc, true);
+            if (rc) return rc;
+            *grown = 1;
+        }
+        if ((uint64_t)st.npairs * 10 > (uint64_t)b->caps.pairs * 8) {
+            rc = alloc_pairs(b, (uint32_t)std::min<uint64_t>(2ull * b->caps.pairs + 4096, 0xFFFFFFF0ull), true);
+            if (rc) return rc;
+            *grown = 1;
+        }
+        if (*grown) {
+            rc = push_desc(b);
+            if (rc) return rc;
+        }
+    }
+    return 0;
+}
+
+static int batch_run(soil_hip_batch* bt, int nsteps, bool use_graph) {
+    const uint32_t nb = (uint32_t)bt->boxes.size();
+    const uint32_t half = bt->K / 2;
+    int done = 0;
+    while (done < nsteps) {
+        const uint64_t s0 = bt->steps_done;
+        const uint32_t slot0 = (uint32_t)(s0 % bt->K);
+        const uint32_t h = slot0 / half;
+        const uint32_t run = std::min<uint32_t>((uint32_t)(nsteps - done), half - (slot0 % half));
+        if (bt->half_pending[h]) {
+            HIPCHK(hipEventSynchronize(bt->half_done[h]));
+            bt->half_pending[h] = false;
+        }
+        // host geometry for the run: staging rows [slot][box]
+        for (uint32_t j = 0; j < run; ++j) {
+            Geom* row = bt->h_ring + (size_t)(slot0 + j) * nb;
+            for (uint32_t k = 0; k < nb; ++k) {
+                HostModel& m = bt->boxes[k]->model;
+                m.apply_deform();
+                m.sort_prepare();
+                if (m.fill(row[k])) return set_error(SOIL_HIP_EUNSUPPORTED, "stencil exceeds %d offsets", SOIL_MAX_STENCIL);
+                if (row[k].ncells > bt->boxes[k]->caps.cells)
+                    return set_error(SOIL_HIP_EUNSUPPORTED, "box %u: grid of %u cells exceeds its capacity", k,
+                                     row[k].ncells);
+                bt->boxes[k]->last_geom = row[k];
+            }
+        }
+        HIPCHK(hipMemcpyAsync(bt->d_ring + (size_t)slot0 * nb, bt->h_ring + (size_t)slot0 * nb,
+                              sizeof(Geom) * nb * run, hipMemcpyHostToDevice, bt->stream));
+        for (uint32_t j = 0; j < run; ++j) {
+            if (use_graph) HIPCHK(hipGraphLaunch(bt->graph_exec, bt->stream));
+            else launch_step(bt->L);
+        }
+        HIPCHK(hipEventRecord(bt->half_done[h], bt->stream));
+        bt->half_pending[h] = true;
+        bt->steps_done += run;
+        done += run;
+    }
+    return 0;
+}
+
+}  // namespace soil_hip
+
+extern "C" {
+
+int soil_hip_batch_create(soil_hip_ctx* ctx, soil_hip_box** boxes, int nboxes, soil_hip_batch** out) {
+    if (!ctx || !boxes || nboxes <= 0 || !out) return set_error(SOIL_HIP_EINVAL, "bad arguments");
+    for (int k = 0; k < nboxes; ++k) {
+        if (!boxes[k] || !boxes[k]->setup_done) return set_error(SOIL_HIP_ESTATE, "box %d not set up", k);
+        if (boxes[k]->in_batch) return set_error(SOIL_HIP_ESTATE, "box %d already in a batch", k);
+        for (int j = 0; j < k; ++j)
+            if (boxes[j] == boxes[k]) return set_error(SOIL_HIP_EINVAL, "box %d listed twice", k);
+    }
+    HIPCHK(hipSetDevice(ctx->device));
+    soil_hip_batch* bt = new soil_hip_batch();
+    bt->ctx = ctx;
+    bt->boxes.assign(boxes, boxes + nboxes);
+    HIPCHK(hipStreamCreateWithFlags(&bt->stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&bt->half_done[0], hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&bt->half_done[1], hipEventDisableTiming));
+    HIPCHK(hipEventCreate(&bt->t0));
+    HIPCHK(hipEventCreate(&bt->t1));
+    const uint32_t nb = (uint32_t)nboxes;
+    HIPCHK(hipMalloc(&bt->d_ring, sizeof(Geom) * bt->K * nb));
+    HIPCHK(hipHostMalloc(&bt->h_ring, sizeof(Geom) * bt->K * nb, hipHostMallocDefault));
+    std::vector<BoxDev*> ptrs(nb);
+    for (uint32_t k = 0; k < nb; ++k) {
+        soil_hip_box* b = boxes[k];
+        int rc = pull_desc(b);
+        if (rc) return rc;
+        rc = pull_state(b);
+        if (rc) return rc;
+        bt->saved_rings.push_back(b->h_dev.geom_ring);
+        bt->saved_ring_len.push_back(b->h_dev.ring_len);
+        // repoint the box at the batch ring: slot j of box k is record j*nb + k
+        b->h_dev.geom_ring = bt->d_ring + k;
+        b->h_dev.ring_len = bt->K;
+        b->h_dev.ring_stride = nb;
+        rc = push_desc(b);
+        if (rc) return rc;
+        b->h_st->ring_pos = 0;
+        HIPCHK(hipMemcpy(&b->d_st->ring_pos, &b->h_st->ring_pos, 4, hipMemcpyHostToDevice));
+        b->in_batch = true;
+        ptrs[k] = b->d_dev;
+    }
+    HIPCHK(hipMalloc(&bt->d_ptrs, sizeof(BoxDev*) * nb));
+    HIPCHK(hipMemcpy(bt->d_ptrs, ptrs.data(), sizeof(BoxDev*) * nb, hipMemcpyHostToDevice));
+    bt->L.maxgrid = 4096;
+    batch_tiles(bt);
+    *out = bt;
+    return 0;
+}
+
+int soil_hip_batch_destroy(soil_hip_batch* bt) {
+    if (!bt) return 0;
+    hipStreamSynchronize(bt->stream);
+    drop_graph(bt);
+    for (size_t k = 0; k < bt->boxes.size(); ++k) {
+        soil_hip_box* b = bt->boxes[k];
+        if (pull_desc(b) == 0 && pull_state(b) == 0) {
+            // hand the box back on its own ring, positioned after the batch's steps
+            b->h_dev.geom_ring = bt->saved_rings[k];
+            b->h_dev.ring_len = bt->saved_ring_len[k];
+            b->h_dev.ring_stride = 1;
+            push_desc(b);
+            b->ring_pos = 0;
+            b->h_st->ring_pos = 0;
+            hipMemcpy(&b->d_st->ring_pos, &b->h_st->ring_pos, 4, hipMemcpyHostToDevice);
+            hipMemcpy(b->d_ring, &b->last_geom, sizeof(Geom), hipMemcpyHostToDevice);
+            rebuild_tiles(b);
+        }
+        b->in_batch = false;
+    }
+    if (bt->arena_tiles) hipFree(bt->arena_tiles);
+    if (bt->d_ptrs) hipFree(bt->d_ptrs);
+    if (bt->d_ring) hipFree(bt->d_ring);
+    if (bt->h_ring) hipHostFree(bt->h_ring);
+    for (auto e : bt->half_done)
+        if (e) hipEventDestroy(e);
+    if (bt->t0) hipEventDestroy(bt->t0);
+    if (bt->t1) hipEventDestroy(bt->t1);
+    if (bt->stream) hipStreamDestroy(bt->stream);
+    delete bt;
+    return 0;
+}
+
+int soil_hip_batch_step(soil_hip_batch* bt, int nsteps, int use_graph) {
+    if (!bt || nsteps < 0) return set_error(SOIL_HIP_EINVAL, "bad arguments");
+    HIPCHK(hipSetDevice(bt->ctx->device));
+    if (use_graph && !bt->graph_exec) {
+        int rc = capture_graph(bt);
+        if (rc) return rc;
+    }
+    int rc = batch_run(bt, nsteps, use_graph != 0);
+    if (rc) return rc;
+    return 0;
+}
+
+int soil_hip_batch_sync(soil_hip_batch* bt) {
+    if (!bt) return set_error(SOIL_HIP_EINVAL, "null batch");
+    HIPCHK(hipStreamSynchronize(bt->stream));
+    bt->half_pending[0] = bt->half_pending[1] = false;
+    int grown = 0;
+    int rc = batch_check(bt, &grown);
+    if (rc) return rc;
+    if (grown) {
+        drop_graph(bt);
+        batch_tiles(bt);
+    }
+    return 0;
+}
+
+// Bit-neutral launch-shape tuning: the grid-stride cap of every launch. Each
+// candidate runs a few steps on the live batch after a device-side snapshot of
+// every box (columns, lists, state, descriptor) and the host models; the
+// snapshot is restored after each trial, so the batch state is unchanged.
+int soil_hip_batch_autotune(soil_hip_batch* bt) {
+    if (!bt) return set_error(SOIL_HIP_EINVAL, "null batch");
+    HIPCHK(hipStreamSynchronize(bt->stream));
+    bt->half_pending[0] = bt->half_pending[1] = false;
+    struct Snap {
+        void* atoms = nullptr;
+        void* pairs = nullptr;
+        size_t atoms_bytes = 0, pairs_bytes = 0;
+        BoxDev dev;
+        BoxState st;
+        HostModel model;
+        Geom last;
+    };
+    std::vector<Snap> snaps(bt->boxes.size());
+    // arena sizes: recompute from the carve layout via the allocation sizes
+    for (size_t k = 0; k < bt->boxes.size(); ++k) {
+        soil_hip_box* b = bt->boxes[k];
+        Snap& s = snaps[k];
+        int rc = pull_desc(b);
+        if (rc) return rc;
+        rc = pull_state(b);
+        if (rc) return rc;
+        s.dev = b->h_dev;
+        s.st = *b->h_st;
+        s.model = b->model;
+        s.last = b->last_geom;
+        size_t sz = 0;
+        hipDeviceptr_t base;
+        if (hipMemGetAddressRange(&base, &sz, (hipDeviceptr_t)b->arena_atoms) != hipSuccess) sz = 0;
+        s.atoms_bytes = sz;
+        if (hipMemGetAddressRange(&base, &sz, (hipDeviceptr_t)b->arena_pairs) != hipSuccess) sz = 0;
+        s.pairs_bytes = sz;
+        if (!s.atoms_bytes || !s.pairs_bytes) return set_error(SOIL_HIP_EDEVICE, "autotune: arena size unknown");
+        HIPCHK(hipMalloc(&s.atoms, s.atoms_bytes));
+        HIPCHK(hipMalloc(&s.pairs, s.pairs_bytes));
+        HIPCHK(hipMemcpy(s.atoms, b->arena_atoms, s.atoms_bytes, hipMemcpyDeviceToDevice));
+        HIPCHK(hipMemcpy(s.pairs, b->arena_pairs, s.pairs_bytes, hipMemcpyDeviceToDevice));
+    }
+    const uint64_t steps_before = bt->steps_done;
+    auto restore = [&]() -> int {
+        HIPCHK(hipStreamSynchronize(bt->stream));
+        for (size_t k = 0; k < bt->boxes.size(); ++k) {
+            soil_hip_box* b = bt->boxes[k];
+            Snap& s = snaps[k];
+            HIPCHK(hipMemcpy(b->arena_atoms, s.atoms, s.atoms_bytes, hipMemcpyDeviceToDevice));
+            HIPCHK(hipMemcpy(b->arena_pairs, s.pairs, s.pairs_bytes, hipMemcpyDeviceToDevice));
+            b->h_dev = s.dev;
+            int rc = push_desc(b);
+            if (rc) return rc;
+            *b->h_st = s.st;
+            HIPCHK(hipMemcpy(b->d_st, &s.st, sizeof(BoxState), hipMemcpyHostToDevice));
+            b->model = s.model;
+            b->last_geom = s.last;
+        }
+        bt->steps_done = steps_before;
+        bt->half_pending[0] = bt->half_pending[1] = false;
+        return 0;
+    };
+    const uint32_t cands[5] = {1024, 2048, 4096, 8192, 16384};
+    uint32_t best = bt->L.maxgrid;
+    double best_ms = 1e30;
+    const int trial = 4;
+    for (uint32_t mg : cands) {
+        bt->L.maxgrid = mg;
+        drop_graph(bt);
+        int rc = capture_graph(bt);
+        if (rc) return rc;
+        rc = batch_run(bt, 1, true);  // warm
+        if (rc) return rc;
+        HIPCHK(hipStreamSynchronize(bt->stream));
+        auto t0 = std::chrono::steady_clock::now();
+        rc = batch_run(bt, trial, true);
+        if (rc) return rc;
+        HIPCHK(hipStreamSynchronize(bt->stream));
+        const double ms =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        rc = restore();
+        if (rc) return rc;
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = mg;
+        }
+    }
+    for (auto& s : snaps) {
+        hipFree(s.atoms);
+        hipFree(s.pairs);
+    }
+    bt->L.maxgrid = best;
+    drop_graph(bt);
+    return 0;
+}
+
+int soil_hip_batch_set_timing(soil_hip_batch* bt, int enabled) {
+    if (!bt) return set_error(SOIL_HIP_EINVAL, "null batch");
+    HIPCHK(hipStreamSynchronize(bt->stream));
+    bt->timing = enabled != 0;
+    bt->t_ms = 0.0;
+    uint64_t bytes = 0;
+    for (auto* b : bt->boxes) {
+        int rc = pull_state(b);
+        if (rc) return rc;
+        bytes += b->h_st->csr_bytes;
+    }
+    bt->bytes_at_enable = bytes;
+    return 0;
+}
+
+int soil_hip_batch_get_timing(soil_hip_batch* bt, double* out4) {
+    if (!bt || !out4) return set_error(SOIL_HIP_EINVAL, "null argument");
+    HIPCHK(hipStreamSynchronize(bt->stream));
+    uint64_t bytes = 0, builds = 0;
+    for (auto* b : bt->boxes) {
+        int rc = pull_state(b);
+        if (rc) return rc;
+        bytes += b->h_st->csr_bytes;
+        builds += b->h_st->csr_builds;
+    }
+    out4[0] = (double)builds;
+    out4[1] = bt->t_ms;
+    out4[2] = (double)(bytes - bt->bytes_at_enable);
+    out4[3] = 0.0;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,1009 @@
+// Box lifecycle, device arenas, the plain single-box step and the per-phase
+// C ABI. One box = one HIP stream; all state lives on the device and the host
+// reads it back lazily.
+
+#include "box.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+
+using namespace soil_hip;
+
+namespace soil_hip {
+
+static thread_local std::string g_last_error;
+
+int set_error(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define HIPCHK(call)                                                                                \
+    do {                                                                                            \
+        hipError_t e_ = (call);                                                                     \
+        if (e_ != hipSuccess) return set_error(SOIL_HIP_EDEVICE, "%s: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+static uint32_t ceil_div(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+static void push_tiles(std::vector<Tile>& v, uint32_t box, uint64_t n, uint32_t step) {
+    for (uint64_t s = 0; s < n; s += step) v.push_back(Tile{box, (uint32_t)s});
+}
+
+void build_tiles(TileTables& tt, uint32_t b, const Caps& c) {
+    tt.box.push_back(Tile{b, 0});
+    push_tiles(tt.local, b, c.local, BLK);
+    push_tiles(tt.local1, b, (uint64_t)c.local + 1, BLK);
+    push_tiles(tt.atom, b, c.atoms, BLK);
+    push_tiles(tt.atom1, b, (uint64_t)c.atoms + 1, BLK);
+    push_tiles(tt.ghost, b, (uint64_t)c.atoms - c.local, BLK);
+    push_tiles(tt.cell, b, (uint64_t)c.cells + 1, BLK);
+    push_tiles(tt.pair, b, c.pairs, BLK);
+    push_tiles(tt.border, b, c.atoms, BORDER_TILE);
+    push_tiles(tt.scan_cell, b, (uint64_t)c.cells + 1, SCAN_TILE);
+    push_tiles(tt.scan_local, b, (uint64_t)c.local + 1, SCAN_TILE);
+    push_tiles(tt.scan_atom, b, (uint64_t)c.atoms + 1, SCAN_TILE);
+}
+
+// Tiles of one space are interleaved across boxes in XCD-friendly order: the
+// tables keep box-major order; kernels stride over them with the grid.
+void upload_tiles(const TileTables& tt, Launch& L, void** arena) {
+    const std::vector<Tile>* lists[12] = {&tt.box, &tt.local, &tt.local1, &tt.atom, &tt.atom1, &tt.ghost,
+                                          &tt.cell, &tt.pair, &tt.border, &tt.scan_cell, &tt.scan_local, &tt.scan_atom};
+    TileSet* sets[12] = {&L.t_box, &L.t_local, &L.t_local1, &L.t_atom, &L.t_atom1, &L.t_ghost,
+                         &L.t_cell, &L.t_pair, &L.t_border, &L.t_scan_cell, &L.t_scan_local, &L.t_scan_atom};
+    size_t total = 0;
+    for (auto* l : lists) total += l->size();
+    if (*arena) hipFree(*arena);
+    *arena = nullptr;
+    if (total == 0) return;
+    hipMalloc(arena, total * sizeof(Tile));
+    Tile* base = (Tile*)*arena;
+    size_t off = 0;
+    for (int k = 0; k < 12; ++k) {
+        sets[k]->d = base + off;
+        sets[k]->n = (uint32_t)lists[k]->size();
+        if (!lists[k]->empty()) hipMemcpy(base + off, lists[k]->data(), lists[k]->size() * sizeof(Tile), hipMemcpyHostToDevice);
+        off += lists[k]->size();
+    }
+}
+
+}  // namespace soil_hip
+
+// ── arena layout ───────────────────────────────────────────────────────────
+namespace soil_hip {
+
+struct Carver {
+    char* base;
+    size_t off = 0;
+    bool dry;
+    template <typename T>
+    void take(T*& p, size_t n) {
+        off = (off + 255) & ~(size_t)255;
+        p = dry ? nullptr : (T*)(base + off);
+        off += std::max<size_t>(n, 1) * sizeof(T);
+    }
+};
+
+void carve_atoms(Carver& c, BoxDev& d, const Caps& k, const std::vector<HostCol>& cols) {
+    const size_t A = k.atoms, L1 = (size_t)k.local + 1, C1 = (size_t)k.cells + 1;
+    c.take(d.x, A); c.take(d.y, A); c.take(d.z, A);
+    c.take(d.vx, A); c.take(d.vy, A); c.take(d.vz, A);
+    c.take(d.fx, A); c.take(d.fy, A); c.take(d.fz, A);
+    c.take(d.mass, A); c.take(d.inv_mass, A); c.take(d.rad, A);
+    c.take(d.tag, A); c.take(d.type, A); c.take(d.origin, A);
+    c.take(d.imx, A); c.take(d.imy, A); c.take(d.imz, A);
+    c.take(d.ax, A); c.take(d.ay, A); c.take(d.az, A);
+    c.take(d.avx, A); c.take(d.avy, A); c.take(d.avz, A);
+    c.take(d.amass, A); c.take(d.ainv_mass, A); c.take(d.arad, A);
+    c.take(d.atag, A); c.take(d.atype, A); c.take(d.aorigin, A);
+    c.take(d.aimx, A); c.take(d.aimy, A); c.take(d.aimz, A);
+    c.take(d.gswap, A);
+    c.take(d.lbx, k.local); c.take(d.lby, k.local); c.take(d.lbz, k.local);
+    c.take(d.cell, A); c.take(d.slot, A); c.take(d.tmp, A); c.take(d.sorted, A); c.take(d.sidx, A);
+    c.take(d.spos, 4 * A);
+    c.take(d.bin_start, C1); c.take(d.counts, C1);
+    c.take(d.scan_part, std::max(C1, A + 1) / SCAN_TILE + 2);
+    c.take(d.offsets, L1);
+    c.take(d.tcount, A + 1); c.take(d.tstart, A + 1);
+    c.take(d.btile, 2 * (size_t)k.tiles + 2);
+    for (size_t i = 0; i < cols.size(); ++i) {
+        c.take(d.cols[i].data, A * cols[i].width);
+        c.take(d.cols[i].alt, A * cols[i].width);
+    }
+}
+
+void carve_pairs(Carver& c, BoxDev& d, const Caps& k) {
+    const size_t P = k.pairs;
+    c.take(d.indices, P); c.take(d.tslot, P); c.take(d.ttmp, P); c.take(d.trow, P); c.take(d.row_of, P);
+}
+
+int alloc_arena(void** arena, size_t bytes) {
+    if (*arena) hipFree(*arena);
+    *arena = nullptr;
+    hipError_t e = hipMalloc(arena, bytes);
+    if (e != hipSuccess) return set_error(SOIL_HIP_EDEVICE, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    return 0;
+}
+
+int push_desc(soil_hip_box* b) {
+    HIPCHK(hipMemcpyAsync(b->d_dev, &b->h_dev, sizeof(BoxDev), hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+int pull_desc(soil_hip_box* b) {
+    HIPCHK(hipStreamSynchronize(b->stream));
+    HIPCHK(hipMemcpy(&b->h_dev, b->d_dev, sizeof(BoxDev), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int pull_state(soil_hip_box* b) {
+    HIPCHK(hipMemcpyAsync(b->h_st, b->d_st, sizeof(BoxState), hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+int rebuild_tiles(soil_hip_box* b) {
+    TileTables tt;
+    build_tiles(tt, 0, b->caps);
+    upload_tiles(tt, b->L, &b->arena_tiles);
+    b->L.boxes = b->d_ptrs;
+    b->L.stream = b->stream;
+    return 0;
+}
+
+// Allocates the atoms arena for caps (copying locals from the previous arena
+// when `keep` is set) and the pairs arena.
+int alloc_atoms(soil_hip_box* b, const Caps& nc, bool keep) {
+    BoxDev nd = b->h_dev;
+    Carver dry{nullptr, 0, true};
+    carve_atoms(dry, nd, nc, b->cols);
+    void* arena = nullptr;
+    int rc = alloc_arena(&arena, dry.off + 256);
+    if (rc) return rc;
+    Carver c{(char*)arena, 0, false};
+    carve_atoms(c, nd, nc, b->cols);
+    if (keep) {
+        int r2 = pull_desc(b);
+        if (r2) return r2;
+        r2 = pull_state(b);
+        if (r2) return r2;
+        const BoxDev& od = b->h_dev;
+        const size_t nl = b->nlocal;
+        const size_t n = std::min<size_t>((size_t)b->h_st->nlocal + b->h_st->nghost, nc.atoms);
+        auto cp = [&](void* dst, const void* src, size_t bytes) {
+            if (bytes) hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, b->stream);
+        };
+        cp(nd.x, od.x, n * 8); cp(nd.y, od.y, n * 8); cp(nd.z, od.z, n * 8);
+        cp(nd.vx, od.vx, n * 8); cp(nd.vy, od.vy, n * 8); cp(nd.vz, od.vz, n * 8);
+        cp(nd.fx, od.fx, n * 8); cp(nd.fy, od.fy, n * 8); cp(nd.fz, od.fz, n * 8);
+        cp(nd.mass, od.mass, n * 8); cp(nd.inv_mass, od.inv_mass, n * 8); cp(nd.rad, od.rad, n * 8);
+        cp(nd.tag, od.tag, n * 4); cp(nd.type, od.type, n * 4); cp(nd.origin, od.origin, n * 4);
+        cp(nd.imx, od.imx, n * 4); cp(nd.imy, od.imy, n * 4); cp(nd.imz, od.imz, n * 4);
+        cp(nd.gswap, od.gswap, n);
+        cp(nd.lbx, od.lbx, nl * 8); cp(nd.lby, od.lby, nl * 8); cp(nd.lbz, od.lbz, nl * 8);
+        cp(nd.offsets, od.offsets, (nl + 1) * 4);
+        cp(nd.tstart, od.tstart, (n + 1) * 4);
+        for (size_t i = 0; i < b->cols.size(); ++i)
+            cp(nd.cols[i].data, od.cols[i].data, n * b->cols[i].width * 8);
+        HIPCHK(hipStreamSynchronize(b->stream));
+    }
+    nd.cap_local = nc.local;
+    nd.cap_atoms = nc.atoms;
+    nd.cap_cells = nc.cells;
+    nd.cap_tiles = nc.tiles;
+    if (b->arena_atoms) hipFree(b->arena_atoms);
+    b->arena_atoms = arena;
+    b->h_dev = nd;
+    b->caps.local = nc.local;
+    b->caps.atoms = nc.atoms;
+    b->caps.cells = nc.cells;
+    b->caps.tiles = nc.tiles;
+    return 0;
+}
+
+int alloc_pairs(soil_hip_box* b, uint32_t cap_pairs, bool keep) {
+    Caps nc = b->caps;
+    nc.pairs = cap_pairs;
+    BoxDev nd = b->h_dev;
+    Carver dry{nullptr, 0, true};
+    carve_pairs(dry, nd, nc);
+    void* arena = nullptr;
+    int rc = alloc_arena(&arena, dry.off + 256);
+    if (rc) return rc;
+    Carver c{(char*)arena, 0, false};
+    carve_pairs(c, nd, nc);
+    if (keep && b->arena_pairs) {
+        rc = pull_state(b);
+        if (rc) return rc;
+        const size_t np = std::min<size_t>(b->h_st->npairs, cap_pairs);
+        if (np) {
+            hipMemcpyAsync(nd.indices, b->h_dev.indices, np * 4, hipMemcpyDeviceToDevice, b->stream);
+            hipMemcpyAsync(nd.trow, b->h_dev.trow, np * 4, hipMemcpyDeviceToDevice, b->stream);
+            hipMemcpyAsync(nd.row_of, b->h_dev.row_of, np * 4, hipMemcpyDeviceToDevice, b->stream);
+        }
+        HIPCHK(hipStreamSynchronize(b->stream));
+    }
+    if (b->arena_pairs) hipFree(b->arena_pairs);
+    b->arena_pairs = arena;
+    b->h_dev.indices = nd.indices;
+    b->h_dev.tslot = nd.tslot;
+    b->h_dev.ttmp = nd.ttmp;
+    b->h_dev.trow = nd.trow;
+    b->h_dev.row_of = nd.row_of;
+    b->h_dev.cap_pairs = cap_pairs;
+    b->caps.pairs = cap_pairs;
+    return 0;
+}
+
+int check_box(soil_hip_box* b, bool need_setup) {
+    if (!b) return set_error(SOIL_HIP_EINVAL, "null box");
+    if (need_setup && !b->setup_done) return set_error(SOIL_HIP_ESTATE, "box not set up (call soil_hip_setup)");
+    if (b->in_batch) return set_error(SOIL_HIP_ESTATE, "box belongs to a batch; step the batch");
+    return 0;
+}
+
+int check_err_bits(uint32_t err) {
+    if (err & ERR_ATOM_LEFT_BOX)
+        return set_error(SOIL_HIP_EUNSUPPORTED,
+                         "an atom left the box through a non-periodic face; atom removal (domain.rs:648-661) "
+                         "is not implemented on the device path");
+    if (err & ERR_STENCIL) return set_error(SOIL_HIP_EUNSUPPORTED, "stencil exceeds %d offsets", SOIL_MAX_STENCIL);
+    if (err & (ERR_PAIR_CAPACITY | ERR_GHOST_CAPACITY))
+        return set_error(SOIL_HIP_ECAPACITY, "device capacity exceeded (err=%u)", err);
+    return 0;
+}
+
+// Grows the cell arrays when a tilt set after setup needs a larger grid.
+int ensure_cells(soil_hip_box* b, uint64_t ncells) {
+    if (ncells <= b->caps.cells) return 0;
+    Caps nc = b->caps;
+    nc.cells = (uint32_t)std::max<uint64_t>(ncells, b->model.max_cells());
+    int rc = alloc_atoms(b, nc, true);
+    if (rc) return rc;
+    rc = push_desc(b);
+    if (rc) return rc;
+    return rebuild_tiles(b);
+}
+
+// Uploads the host model's current geometry into ring slot `ring_pos` (synchronous).
+int upload_geom_now(soil_hip_box* b) {
+    Geom& g = b->h_ring[b->ring_pos];
+    if (b->model.fill(g)) return set_error(SOIL_HIP_EUNSUPPORTED, "stencil exceeds %d offsets", SOIL_MAX_STENCIL);
+    int rcc = ensure_cells(b, g.ncells);
+    if (rcc) return rcc;
+    b->last_geom = g;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    HIPCHK(hipMemcpy(b->d_ring + b->ring_pos, &g, sizeof(Geom), hipMemcpyHostToDevice));
+    return 0;
+}
+
+void resolve_timing(soil_hip_box* b) {
+    std::vector<soil_hip_box::Pending> keep;
+    for (auto& p : b->pending) {
+        if (hipEventQuery(p.e2) == hipSuccess) {
+            float ms_all = 0.f, ms_fill = 0.f;
+            hipEventElapsedTime(&ms_all, p.e0, p.e2);
+            hipEventElapsedTime(&ms_fill, p.e1, p.e2);
+            b->t_builds += 1.0;
+            b->t_csr_ms += ms_all;
+            b->t_fill_ms += ms_fill;
+            b->ev_pool.push_back(p.e0);
+            b->ev_pool.push_back(p.e1);
+            b->ev_pool.push_back(p.e2);
+        } else {
+            keep.push_back(p);
+        }
+    }
+    b->pending.swap(keep);
+}
+
+hipEvent_t take_event(soil_hip_box* b) {
+    if (!b->ev_pool.empty()) {
+        hipEvent_t e = b->ev_pool.back();
+        b->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    hipEventCreate(&e);
+    return e;
+}
+
+// Rebuild tail after borders: build + capacity handling. Synchronizes twice
+// (after borders, after the CSR count) so that capacity growth is exact.
+int rebuild_tail(soil_hip_box* b) {
+    for (int attempt = 0; attempt < 4; ++attempt) {
+        phase_borders(b->L);
+        int rc = pull_state(b);
+        if (rc) return rc;
+        if (!(b->h_st->err & ERR_GHOST_CAPACITY)) break;
+        // grow the atoms arena (locals kept) and redo borders
+        Caps nc = b->caps;
+        uint64_t ghost_cap = (uint64_t)(b->caps.atoms - b->caps.local);
+        ghost_cap = ghost_cap * 2 + 4096;
+        nc.atoms = (uint32_t)(b->caps.local + ghost_cap);
+        nc.tiles = ceil_div(nc.atoms, BORDER_TILE);
+        rc = alloc_atoms(b, nc, true);
+        if (rc) return rc;
+        b->h_st->err &= ~ERR_GHOST_CAPACITY;
+        HIPCHK(hipMemcpy(&b->d_st->err, &b->h_st->err, 4, hipMemcpyHostToDevice));
+        rc = push_desc(b);
+        if (rc) return rc;
+        rebuild_tiles(b);
+    }
+    if (b->h_st->err & ERR_GHOST_CAPACITY) return set_error(SOIL_HIP_ECAPACITY, "ghost capacity growth failed");
+    phase_forward(b->L);
+    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+    if (b->timing) {
+        e0 = take_event(b);
+        e1 = take_event(b);
+        e2 = take_event(b);
+    }
+    // build, split at the CSR scan so the pair capacity is known before the fill
+    {
+        Launch L = b->L;
+        // run the build with the fill suppressed? The kernels themselves check capacity:
+        // phase_build runs count, scan (flags capacity), fill (skipped on overflow).
+        phase_build(L, e0, e1, e2);
+    }
+    int rc = pull_state(b);
+    if (rc) return rc;
+    if (b->h_st->err & ERR_PAIR_CAPACITY) {
+        // grow pairs and redo the fill + transpose (state is intact up to the fill)
+        uint32_t np = b->h_st->npairs;
+        rc = alloc_pairs(b, (uint32_t)std::min<uint64_t>((uint64_t)np + np / 2 + 4096, 0xFFFFFFF0u), false);
+        if (rc) return rc;
+        b->h_st->err &= ~ERR_PAIR_CAPACITY;
+        HIPCHK(hipMemcpy(&b->d_st->err, &b->h_st->err, 4, hipMemcpyHostToDevice));
+        rc = push_desc(b);
+        if (rc) return rc;
+        rebuild_tiles(b);
+        // rebuilds was counted once by build_prep; the redo re-runs the whole build,
+        // so compensate the counter first
+        b->h_st->rebuilds -= 1;
+        b->h_st->state = 0;  // build_prep flips to CommunicateOnly again
+        HIPCHK(hipMemcpy(b->d_st, b->h_st, sizeof(BoxState), hipMemcpyHostToDevice));
+        phase_build(b->L, nullptr, nullptr, nullptr);
+        rc = pull_state(b);
+        if (rc) return rc;
+        if (b->h_st->err & ERR_PAIR_CAPACITY) return set_error(SOIL_HIP_ECAPACITY, "pair capacity growth failed");
+        if (e0) {
+            b->ev_pool.push_back(e0);
+            b->ev_pool.push_back(e1);
+            b->ev_pool.push_back(e2);
+            e0 = nullptr;
+        }
+    }
+    if (e0) b->pending.push_back({e0, e1, e2});
+    return 0;
+}
+
+int step_plain(soil_hip_box* b, int nsteps) {
+    for (int s = 0; s < nsteps; ++s) {
+        b->model.apply_deform();
+        b->model.sort_prepare();
+        Geom& g = b->h_ring[b->ring_pos];
+        if (b->model.fill(g)) return set_error(SOIL_HIP_EUNSUPPORTED, "stencil exceeds %d offsets", SOIL_MAX_STENCIL);
+        if (g.ncells > b->caps.cells) {
+            int rcc = ensure_cells(b, g.ncells);
+            if (rcc) return rcc;
+        }
+        b->last_geom = g;
+        HIPCHK(hipMemcpyAsync(b->d_ring + b->ring_pos, &g, sizeof(Geom), hipMemcpyHostToDevice, b->stream));
+        phase_deform(b->L);
+        phase_initial(b->L);
+        phase_decide(b->L);
+        int rc = pull_state(b);
+        if (rc) return rc;
+        if (b->h_st->err) return check_err_bits(b->h_st->err);
+        const bool rebuild = b->h_st->state == 0 || g.periodic_sort || g.bounds_changed;
+        phase_zero(b->L);
+        if (rebuild) {
+            phase_pbc(b->L);
+            phase_sort(b->L);
+            rc = rebuild_tail(b);
+            if (rc) return rc;
+        } else {
+            phase_sort_flags(b->L);
+            phase_forward(b->L);
+        }
+        if (b->consumer) phase_force(b->L);
+        if (b->model.newton) phase_reverse(b->L);
+        phase_final(b->L);
+        phase_step_end(b->L);
+        b->ring_pos = (b->ring_pos + 1) % b->ring_len;
+    }
+    if (b->timing) resolve_timing(b);
+    return 0;
+}
+
+}  // namespace soil_hip
+
+// ─────────────────────────────── C ABI ─────────────────────────────────────
+extern "C" {
+
+const char* soil_hip_last_error(void) { return g_last_error.c_str(); }
+
+int soil_hip_device_count(int* out) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    if (out) *out = n;
+    return 0;
+}
+
+int soil_hip_ctx_create(int device, soil_hip_ctx** out) {
+    if (!out) return set_error(SOIL_HIP_EINVAL, "null out");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return set_error(SOIL_HIP_EDEVICE, "no HIP device");
+    if (device < 0 || device >= n) return set_error(SOIL_HIP_EINVAL, "device %d out of range (%d)", device, n);
+    HIPCHK(hipSetDevice(device));
+    soil_hip_ctx* c = new soil_hip_ctx();
+    c->device = device;
+    *out = c;
+    return 0;
+}
+
+int soil_hip_ctx_destroy(soil_hip_ctx* ctx) {
+    delete ctx;
+    return 0;
+}
+
+int soil_hip_box_create(soil_hip_ctx* ctx, const soil_hip_box_desc* desc, soil_hip_box** out) {
+    if (!ctx || !desc || !out) return set_error(SOIL_HIP_EINVAL, "null argument");
+    for (int k = 0; k < 3; ++k)
+        if (!(desc->hi[k] > desc->lo[k])) return set_error(SOIL_HIP_EINVAL, "box hi must exceed lo on every axis");
+    if (!(desc->skin_fraction >= 1.0)) return set_error(SOIL_HIP_EINVAL, "skin_fraction must be >= 1");
+    if (!(desc->dt > 0.0)) return set_error(SOIL_HIP_EINVAL, "dt must be > 0");
+    if (desc->has_shear && (!desc->periodic[0] || !desc->periodic[1]))
+        return set_error(SOIL_HIP_EINVAL, "[deform] xy shear requires periodic x and y boundaries");
+    HIPCHK(hipSetDevice(ctx->device));
+    soil_hip_box* b = new soil_hip_box();
+    b->ctx = ctx;
+    b->desc = *desc;
+    b->model.init(*desc);
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete b;
+        return set_error(SOIL_HIP_EDEVICE, "hipStreamCreate failed");
+    }
+    *out = b;
+    return 0;
+}
+
+int soil_hip_box_destroy(soil_hip_box* b) {
+    if (!b) return 0;
+    if (b->stream) hipStreamSynchronize(b->stream);
+    for (auto& p : b->pending) {
+        b->ev_pool.push_back(p.e0);
+        b->ev_pool.push_back(p.e1);
+        b->ev_pool.push_back(p.e2);
+    }
+    for (auto e : b->ev_pool) hipEventDestroy(e);
+    if (b->arena_atoms) hipFree(b->arena_atoms);
+    if (b->arena_pairs) hipFree(b->arena_pairs);
+    if (b->arena_tiles) hipFree(b->arena_tiles);
+    if (b->d_dev) hipFree(b->d_dev);
+    if (b->d_ptrs) hipFree(b->d_ptrs);
+    if (b->d_st) hipFree(b->d_st);
+    if (b->d_ring) hipFree(b->d_ring);
+    if (b->h_st) hipHostFree(b->h_st);
+    if (b->h_ring) hipHostFree(b->h_ring);
+    if (b->stream) hipStreamDestroy(b->stream);
+    delete b;
+    return 0;
+}
+
+int soil_hip_register_column(soil_hip_box* b, int width, uint32_t flags, int* col_id) {
+    if (!b) return set_error(SOIL_HIP_EINVAL, "null box");
+    if (b->setup_done) return set_error(SOIL_HIP_ESTATE, "register columns before setup");
+    if (width < 1 || width > 64) return set_error(SOIL_HIP_EINVAL, "column width must be 1..64");
+    if (flags & ~7u) return set_error(SOIL_HIP_EINVAL, "unknown column flags");
+    if (b->cols.size() >= SOIL_MAX_COLS) return set_error(SOIL_HIP_EINVAL, "at most %d columns", SOIL_MAX_COLS);
+    HostCol c;
+    c.width = width;
+    c.flags = flags;
+    c.data.assign(b->nlocal * (size_t)width, 0.0);
+    b->cols.push_back(c);
+    if (col_id) *col_id = (int)b->cols.size() - 1;
+    return 0;
+}
+
+int soil_hip_upload(soil_hip_box* b, size_t n, const double* pos, const double* vel, const double* force,
+                    const double* mass, const double* radius, const uint32_t* tag, const uint32_t* type,
+                    const int32_t* image) {
+    if (!b || !pos || !mass || !radius) return set_error(SOIL_HIP_EINVAL, "null argument");
+    if (b->setup_done) return set_error(SOIL_HIP_ESTATE, "upload before setup");
+    if (b->nlocal + n > 0x7FFFFFFFull) return set_error(SOIL_HIP_EINVAL, "too many atoms");
+    for (size_t i = 0; i < n; ++i) {
+        if (!(mass[i] > 0.0) || !(radius[i] > 0.0)) return set_error(SOIL_HIP_EINVAL, "mass and radius must be > 0");
+        b->hx.push_back(pos[3 * i]);
+        b->hy.push_back(pos[3 * i + 1]);
+        b->hz.push_back(pos[3 * i + 2]);
+        b->hvx.push_back(vel ? vel[3 * i] : 0.0);
+        b->hvy.push_back(vel ? vel[3 * i + 1] : 0.0);
+        b->hvz.push_back(vel ? vel[3 * i + 2] : 0.0);
+        b->hfx.push_back(force ? force[3 * i] : 0.0);
+        b->hfy.push_back(force ? force[3 * i + 1] : 0.0);
+        b->hfz.push_back(force ? force[3 * i + 2] : 0.0);
+        b->hmass.push_back(mass[i]);
+        b->hrad.push_back(radius[i]);
+        b->htag.push_back(tag ? tag[i] : (uint32_t)(b->nlocal + i));
+        b->htype.push_back(type ? type[i] : 0u);
+        b->himx.push_back(image ? image[3 * i] : 0);
+        b->himy.push_back(image ? image[3 * i + 1] : 0);
+        b->himz.push_back(image ? image[3 * i + 2] : 0);
+    }
+    b->nlocal += n;
+    for (auto& c : b->cols) c.data.resize(b->nlocal * (size_t)c.width, 0.0);
+    return 0;
+}
+
+int soil_hip_upload_column(soil_hip_box* b, int col, const double* data, size_t n) {
+    if (!b || !data || col < 0 || col >= (int)b->cols.size()) return set_error(SOIL_HIP_EINVAL, "bad column");
+    if (b->setup_done) return set_error(SOIL_HIP_ESTATE, "upload columns before setup");
+    if (n > b->nlocal) return set_error(SOIL_HIP_EINVAL, "more rows than atoms");
+    std::memcpy(b->cols[col].data.data(), data, n * b->cols[col].width * sizeof(double));
+    return 0;
+}
+
+int soil_hip_setup(soil_hip_box* b) {
+    if (!b) return set_error(SOIL_HIP_EINVAL, "null box");
+    if (b->setup_done) return set_error(SOIL_HIP_ESTATE, "setup already done");
+    HIPCHK(hipSetDevice(b->ctx->device));
+    const size_t n = b->nlocal;
+    b->model.setup(b->hrad.data(), n);
+    if (b->model.max_stencil() > SOIL_MAX_STENCIL)
+        return set_error(SOIL_HIP_EUNSUPPORTED, "stencil of %zu offsets exceeds %d", b->model.max_stencil(),
+                         SOIL_MAX_STENCIL);
+    Caps c;
+    c.local = (uint32_t)n;
+    const uint64_t ghost = b->model.ghost_estimate(n);
+    c.atoms = (uint32_t)std::min<uint64_t>(n + ghost + ghost / 2 + 1024, 0x7FFFFFFFull);
+    c.cells = (uint32_t)b->model.max_cells();
+    c.tiles = ceil_div(c.atoms, BORDER_TILE);
+    b->h_dev = BoxDev{};
+    b->h_dev.ncols = (int32_t)b->cols.size();
+    for (size_t i = 0; i < b->cols.size(); ++i) {
+        b->h_dev.cols[i].width = b->cols[i].width;
+        b->h_dev.cols[i].flags = b->cols[i].flags;
+    }
+    HIPCHK(hipMalloc(&b->d_dev, sizeof(BoxDev)));
+    HIPCHK(hipMalloc(&b->d_ptrs, sizeof(BoxDev*)));
+    HIPCHK(hipMemcpy(b->d_ptrs, &b->d_dev, sizeof(BoxDev*), hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&b->d_st, sizeof(BoxState)));
+    HIPCHK(hipHostMalloc(&b->h_st, sizeof(BoxState), hipHostMallocDefault));
+    HIPCHK(hipMalloc(&b->d_ring, sizeof(Geom) * b->ring_len));
+    HIPCHK(hipHostMalloc(&b->h_ring, sizeof(Geom) * b->ring_len, hipHostMallocDefault));
+    int rc = alloc_atoms(b, c, false);
+    if (rc) return rc;
+    rc = alloc_pairs(b, (uint32_t)std::min<uint64_t>(std::max<uint64_t>(8 * (uint64_t)n, 4096), 0xFFFFFFF0u), false);
+    if (rc) return rc;
+    BoxDev& d = b->h_dev;
+    // locals (the primary buffers are the ones named without the a- prefix)
+    auto up = [&](void* dst, const void* src, size_t bytes) {
+        return bytes ? hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) : hipSuccess;
+    };
+    std::vector<double> inv(n);
+    for (size_t i = 0; i < n; ++i) inv[i] = 1.0 / b->hmass[i];
+    std::vector<int32_t> zeros_i(n, 0);
+    HIPCHK(up(d.x, b->hx.data(), n * 8)); HIPCHK(up(d.y, b->hy.data(), n * 8)); HIPCHK(up(d.z, b->hz.data(), n * 8));
+    HIPCHK(up(d.vx, b->hvx.data(), n * 8)); HIPCHK(up(d.vy, b->hvy.data(), n * 8)); HIPCHK(up(d.vz, b->hvz.data(), n * 8));
+    HIPCHK(up(d.fx, b->hfx.data(), n * 8)); HIPCHK(up(d.fy, b->hfy.data(), n * 8)); HIPCHK(up(d.fz, b->hfz.data(), n * 8));
+    HIPCHK(up(d.mass, b->hmass.data(), n * 8)); HIPCHK(up(d.inv_mass, inv.data(), n * 8));
+    HIPCHK(up(d.rad, b->hrad.data(), n * 8));
+    HIPCHK(up(d.tag, b->htag.data(), n * 4)); HIPCHK(up(d.type, b->htype.data(), n * 4));
+    HIPCHK(up(d.origin, zeros_i.data(), n * 4));
+    HIPCHK(up(d.imx, b->himx.data(), n * 4)); HIPCHK(up(d.imy, b->himy.data(), n * 4)); HIPCHK(up(d.imz, b->himz.data(), n * 4));
+    for (size_t i = 0; i < b->cols.size(); ++i)
+        HIPCHK(up(d.cols[i].data, b->cols[i].data.data(), n * b->cols[i].width * 8));
+    d.st = b->d_st;
+    d.geom_ring = b->d_ring;
+    d.ring_len = b->ring_len;
+    d.ring_stride = 1;
+    d.harmonic_k = b->harmonic_k;
+    d.consumer = b->consumer ? 1 : 0;
+    BoxState st{};
+    st.nlocal = (uint32_t)n;
+    st.state = 0;  // CurrentState(FullRebuild), neighbor.rs:545
+    st.ring_pos = 0;
+    std::memcpy(b->h_st, &st, sizeof(st));
+    HIPCHK(hipMemcpy(b->d_st, &st, sizeof(BoxState), hipMemcpyHostToDevice));
+    b->ring_pos = 0;
+    b->L.maxgrid = 4096;
+    rc = push_desc(b);
+    if (rc) return rc;
+    rebuild_tiles(b);
+    b->setup_done = true;
+    rc = upload_geom_now(b);
+    if (rc) return rc;
+    // host copies no longer needed
+    b->hx.clear(); b->hy.clear(); b->hz.clear(); b->hvx.clear(); b->hvy.clear(); b->hvz.clear();
+    b->hfx.clear(); b->hfy.clear(); b->hfz.clear(); b->hmass.clear(); b->htag.clear(); b->htype.clear();
+    b->himx.clear(); b->himy.clear(); b->himz.clear();
+    return 0;
+}
+
+int soil_hip_set_consumer(soil_hip_box* b, int enabled, double k) {
+    if (!b) return set_error(SOIL_HIP_EINVAL, "null box");
+    if (b->setup_done) {
+        int rc = pull_state(b);
+        if (rc) return rc;
+        if (b->h_st->rebuilds != 0)
+            return set_error(SOIL_HIP_ESTATE, "enable the consumer before the first neighbor build");
+    }
+    b->consumer = enabled != 0;
+    b->harmonic_k = k;
+    if (b->setup_done) {
+        b->h_dev.consumer = b->consumer ? 1 : 0;
+        b->h_dev.harmonic_k = k;
+        // patch the two fields in place (the rest of the device descriptor may hold swapped pointers)
+        HIPCHK(hipStreamSynchronize(b->stream));
+        HIPCHK(hipMemcpy(&b->d_dev->harmonic_k, &b->h_dev.harmonic_k, sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(&b->d_dev->consumer, &b->h_dev.consumer, sizeof(int32_t), hipMemcpyHostToDevice));
+    }
+    return 0;
+}
+
+// ── per-phase systems ──────────────────────────────────────────────────────
+#define PHASE_PROLOGUE(b)              \
+    do {                               \
+        int rc_ = check_box(b, true);  \
+        if (rc_) return rc_;           \
+    } while (0)
+
+int soil_hip_apply_deform(soil_hip_box* b) {
+    PHASE_PROLOGUE(b);
+    b->model.apply_deform();
+    int rc = upload_geom_now(b);
+    if (rc) return rc;
+    phase_deform(b->L);
+    return 0;
+}
+
+int soil_hip_initial_integration(soil_hip_box* b) {
+    PHASE_PROLOGUE(b);
+    int rc = upload_geom_now(b);
+    if (rc) return rc;
+    phase_initial(b->L);
+    return 0;
+}
+
+int soil_hip_decide_rebuild(soil_hip_box* b, int* full_rebuild) {
+    PHASE_PROLOGUE(b);
+    int rc = upload_geom_now(b);
+    if (rc) return rc;
+    // decide_rebuild removes nothing: keep ghosts until remove_ghost_atoms
+    phase_decide_only(b->L);
+    rc = pull_state(b);
+    if (rc) return rc;
+    if (full_rebuild) *full_rebuild = b->h_st->state == 0 ? 1 : 0;
+    return 0;
+}
+
+int soil_hip_remove_ghost_atoms(soil_hip_box* b) {
+    PHASE_PROLOGUE(b);
+    phase_remove_ghosts(b->L);
+    return 0;
+}
+
+int soil_hip_zero_all_forces(soil_hip_box* b) {
+    PHASE_PROLOGUE(b);
+    phase_zero(b->L);
+    return 0;
+}
+
+int soil_hip_pbc(soil_hip_box* b) {
+    PHASE_PROLOGUE(b);
+    int rc = upload_geom_now(b);
+    if (rc) return rc;
+    phase_pbc(b->L);
+    return 0;
+}
+
+int soil_hip_sort_atoms_by_bin(soil_hip_box* b) {
+    PHASE_PROLOGUE(b);
+    b->model.sort_prepare();
+    int rc = upload_geom_now(b);
+    if (rc) return rc;
+    phase_sort(b->L);
+    return 0;
+}
+
+int soil_hip_borders(soil_hip_box* b) {
+    PHASE_PROLOGUE(b);
+    int rc = upload_geom_now(b);
+    if (rc) return rc;
+    phase_set_rebuild_now(b->L);
+    phase_borders(b->L);
+    rc = pull_state(b);
+    if (rc) return rc;
+    return check_err_bits(b->h_st->err);
+}
+
+int soil_hip_forward_comm_borders(soil_hip_box* b) {
+    PHASE_PROLOGUE(b);
+    int rc = upload_geom_now(b);
+    if (rc) return rc;
+    phase_set_rebuild_now(b->L);
+    phase_forward(b->L);
+    return 0;
+}
+
+int soil_hip_bin_neighbor_list(soil_hip_box* b) {
+    PHASE_PROLOGUE(b);
+    int rc = upload_geom_now(b);
+    if (rc) return rc;
+    phase_set_rebuild_now(b->L);
+    for (int attempt = 0; attempt < 3; ++attempt) {
+        phase_build(b->L, nullptr, nullptr, nullptr);
+        rc = pull_state(b);
+        if (rc) return rc;
+        if (!(b->h_st->err & ERR_PAIR_CAPACITY)) break;
+        uint32_t np = b->h_st->npairs;
+        rc = alloc_pairs(b, np + np / 2 + 4096, false);
+        if (rc) return rc;
+        b->h_st->err &= ~ERR_PAIR_CAPACITY;
+        b->h_st->rebuilds -= 1;
+        b->h_st->state = 0;
+        HIPCHK(hipMemcpy(b->d_st, b->h_st, sizeof(BoxState), hipMemcpyHostToDevice));
+        rc = push_desc(b);
+        if (rc) return rc;
+        rebuild_tiles(b);
+    }
+    return check_err_bits(b->h_st->err);
+}
+
+int soil_hip_harmonic_force(soil_hip_box* b) {
+    PHASE_PROLOGUE(b);
+    if (!b->consumer) return set_error(SOIL_HIP_ESTATE, "consumer disabled (soil_hip_set_consumer)");
+    phase_force(b->L);
+    return 0;
+}
+
+int soil_hip_reverse_send_force(soil_hip_box* b) {
+    PHASE_PROLOGUE(b);
+    if (b->model.newton) phase_reverse(b->L);
+    return 0;
+}
+
+int soil_hip_final_integration(soil_hip_box* b) {
+    PHASE_PROLOGUE(b);
+    phase_final(b->L);
+    return 0;
+}
+
+int soil_hip_step(soil_hip_box* b, int nsteps) {
+    PHASE_PROLOGUE(b);
+    if (nsteps < 0) return set_error(SOIL_HIP_EINVAL, "nsteps < 0");
+    HIPCHK(hipSetDevice(b->ctx->device));
+    return step_plain(b, nsteps);
+}
+
+int soil_hip_sync(soil_hip_box* b) {
+    if (!b) return set_error(SOIL_HIP_EINVAL, "null box");
+    int rc = pull_state(b);
+    if (rc) return rc;
+    if (b->timing) resolve_timing(b);
+    return check_err_bits(b->h_st->err);
+}
+
+int soil_hip_set_tilt(soil_hip_box* b, double xy, double bvx) {
+    PHASE_PROLOGUE(b);
+    b->model.set_tilt(xy, bvx);
+    return 0;
+}
+
+// ── readback ────────────────────────────────────────────────────────────────
+int soil_hip_get_counts(soil_hip_box* b, uint64_t* out) {
+    if (!b || !out) return set_error(SOIL_HIP_EINVAL, "null argument");
+    if (!b->setup_done) {
+        std::memset(out, 0, 8 * sizeof(uint64_t));
+        out[0] = b->nlocal;
+        return 0;
+    }
+    int rc = pull_state(b);
+    if (rc) return rc;
+    out[0] = b->h_st->nlocal;
+    out[1] = b->h_st->nghost;
+    out[2] = b->h_st->npairs;
+    out[3] = b->model.ncells;
+    out[4] = b->h_st->rebuilds;
+    out[5] = b->h_st->steps;
+    out[6] = b->model.stencil_fwd.size();
+    out[7] = b->h_st->state;
+    return 0;
+}
+
+int soil_hip_get_csr(soil_hip_box* b, uint32_t* offsets, size_t cap_off, uint32_t* indices, size_t cap_idx,
+                     size_t* n_idx) {
+    int rc = check_box(b, true);
+    if (rc) return rc;
+    rc = pull_state(b);
+    if (rc) return rc;
+    rc = pull_desc(b);
+    if (rc) return rc;
+    const size_t nl = b->h_st->nlocal, np = b->h_st->npairs;
+    if (n_idx) *n_idx = np;
+    if (offsets) {
+        if (cap_off < nl + 1) return set_error(SOIL_HIP_ECAPACITY, "offsets needs %zu", nl + 1);
+        HIPCHK(hipMemcpy(offsets, b->h_dev.offsets, (nl + 1) * 4, hipMemcpyDeviceToHost));
+    }
+    if (indices) {
+        if (cap_idx < np) return set_error(SOIL_HIP_ECAPACITY, "indices needs %zu", np);
+        if (np) HIPCHK(hipMemcpy(indices, b->h_dev.indices, np * 4, hipMemcpyDeviceToHost));
+    }
+    return 0;
+}
+
+int soil_hip_get_atoms(soil_hip_box* b, size_t cap, double* pos, double* vel, double* force, double* radius,
+                       double* mass, double* inv_mass, uint32_t* tag, int32_t* origin, int32_t* image) {
+    int rc = check_box(b, true);
+    if (rc) return rc;
+    rc = pull_state(b);
+    if (rc) return rc;
+    rc = pull_desc(b);
+    if (rc) return rc;
+    const size_t n = (size_t)b->h_st->nlocal + b->h_st->nghost;
+    if (cap < n) return set_error(SOIL_HIP_ECAPACITY, "atoms needs %zu", n);
+    const BoxDev& d = b->h_dev;
+    std::vector<double> t0(n), t1(n), t2(n);
+    auto get3d = [&](double* out, const double* a, const double* bb, const double* c) -> int {
+        HIPCHK(hipMemcpy(t0.data(), a, n * 8, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(t1.data(), bb, n * 8, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(t2.data(), c, n * 8, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; ++i) {
+            out[3 * i] = t0[i];
+            out[3 * i + 1] = t1[i];
+            out[3 * i + 2] = t2[i];
+        }
+        return 0;
+    };
+    if (n == 0) return 0;
+    if (pos && (rc = get3d(pos, d.x, d.y, d.z))) return rc;
+    if (vel && (rc = get3d(vel, d.vx, d.vy, d.vz))) return rc;
+    if (force && (rc = get3d(force, d.fx, d.fy, d.fz))) return rc;
+    if (radius) HIPCHK(hipMemcpy(radius, d.rad, n * 8, hipMemcpyDeviceToHost));
+    if (mass) HIPCHK(hipMemcpy(mass, d.mass, n * 8, hipMemcpyDeviceToHost));
+    if (inv_mass) HIPCHK(hipMemcpy(inv_mass, d.inv_mass, n * 8, hipMemcpyDeviceToHost));
+    if (tag) HIPCHK(hipMemcpy(tag, d.tag, n * 4, hipMemcpyDeviceToHost));
+    if (origin) HIPCHK(hipMemcpy(origin, d.origin, n * 4, hipMemcpyDeviceToHost));
+    if (image) {
+        std::vector<int32_t> a(n), bb(n), c(n);
+        HIPCHK(hipMemcpy(a.data(), d.imx, n * 4, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(bb.data(), d.imy, n * 4, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(c.data(), d.imz, n * 4, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; ++i) {
+            image[3 * i] = a[i];
+            image[3 * i + 1] = bb[i];
+            image[3 * i + 2] = c[i];
+        }
+    }
+    return 0;
+}
+
+int soil_hip_get_column(soil_hip_box* b, int col, double* out, size_t cap_atoms) {
+    int rc = check_box(b, true);
+    if (rc) return rc;
+    if (col < 0 || col >= (int)b->cols.size()) return set_error(SOIL_HIP_EINVAL, "bad column");
+    rc = pull_state(b);
+    if (rc) return rc;
+    rc = pull_desc(b);
+    if (rc) return rc;
+    const size_t n = (size_t)b->h_st->nlocal + b->h_st->nghost;
+    if (cap_atoms < n) return set_error(SOIL_HIP_ECAPACITY, "column needs %zu atoms", n);
+    if (n) HIPCHK(hipMemcpy(out, b->h_dev.cols[col].data, n * b->cols[col].width * 8, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int soil_hip_get_bins(soil_hip_box* b, uint32_t* bin_start, size_t cap_cells1, uint32_t* sorted_atoms,
+                      uint32_t* sorted_idx, uint32_t* atom_cell, size_t cap_atoms) {
+    int rc = check_box(b, true);
+    if (rc) return rc;
+    rc = pull_state(b);
+    if (rc) return rc;
+    rc = pull_desc(b);
+    if (rc) return rc;
+    const size_t n = (size_t)b->h_st->nlocal + b->h_st->nghost;
+    const size_t c1 = (size_t)b->last_geom.ncells + 1;
+    if (bin_start) {
+        if (cap_cells1 < c1) return set_error(SOIL_HIP_ECAPACITY, "bin_start needs %zu", c1);
+        HIPCHK(hipMemcpy(bin_start, b->h_dev.bin_start, c1 * 4, hipMemcpyDeviceToHost));
+    }
+    if ((sorted_atoms || sorted_idx || atom_cell) && cap_atoms < n)
+        return set_error(SOIL_HIP_ECAPACITY, "atom arrays need %zu", n);
+    if (n == 0) return 0;
+    if (sorted_atoms) HIPCHK(hipMemcpy(sorted_atoms, b->h_dev.sorted, n * 4, hipMemcpyDeviceToHost));
+    if (sorted_idx) HIPCHK(hipMemcpy(sorted_idx, b->h_dev.sidx, n * 4, hipMemcpyDeviceToHost));
+    if (atom_cell) HIPCHK(hipMemcpy(atom_cell, b->h_dev.cell, n * 4, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int soil_hip_get_swaps(soil_hip_box* b, double* out, size_t cap_rows, size_t* nrows) {
+    int rc = check_box(b, true);
+    if (rc) return rc;
+    rc = pull_state(b);
+    if (rc) return rc;
+    if (nrows) *nrows = SOIL_NSWAP;
+    if (cap_rows < SOIL_NSWAP) return set_error(SOIL_HIP_ECAPACITY, "swaps need %d rows", SOIL_NSWAP);
+    const Geom& g = b->last_geom;
+    for (int s = 0; s < SOIL_NSWAP; ++s) {
+        double* o = out + 11 * s;
+        const int d = s / 2;
+        o[0] = b->h_st->swap_start[s];
+        o[1] = b->h_st->swap_count[s];
+        o[2] = b->h_st->swap_count[s];
+        o[3] = g.periodic[d] ? 0.0 : -1.0;
+        o[4] = g.periodic[d] ? 0.0 : -1.0;
+        for (int k = 0; k < 3; ++k) {
+            o[5 + k] = g.swap_shift[s][k];
+            o[8 + k] = g.swap_vshift[s][k];
+        }
+    }
+    return 0;
+}
+
+int soil_hip_get_grid(soil_hip_box* b, double* out) {
+    if (!b || !out) return set_error(SOIL_HIP_EINVAL, "null argument");
+    const HostModel& m = b->model;
+    for (int k = 0; k < 3; ++k) {
+        out[k] = m.nbin[k];
+        out[3 + k] = m.bin_size[k];
+        out[6 + k] = m.bin_origin[k];
+        out[12 + k] = m.tilt[k];
+        out[15 + k] = m.bvel[k];
+    }
+    out[9] = m.ghost_cutoff;
+    out[10] = m.cached_max_cutoff;
+    out[11] = m.bin_min_size;
+    out[18] = m.cached_min_skin;
+    out[19] = m.uniform ? 1.0 : 0.0;
+    out[20] = m.ucut_sq;
+    return 0;
+}
+
+int soil_hip_get_stencil(soil_hip_box* b, int32_t* out, size_t cap, size_t* n) {
+    if (!b) return set_error(SOIL_HIP_EINVAL, "null box");
+    const auto& s = b->model.stencil_fwd;
+    if (n) *n = s.size();
+    if (out) {
+        if (cap < s.size()) return set_error(SOIL_HIP_ECAPACITY, "stencil needs %zu", s.size());
+        for (size_t i = 0; i < s.size(); ++i) out[i] = s[i];
+    }
+    return 0;
+}
+
+// ── timing ─────────────────────────────────────────────────────────────────
+int soil_hip_set_timing(soil_hip_box* b, int enabled) {
+    int rc = check_box(b, true);
+    if (rc) return rc;
+    rc = pull_state(b);
+    if (rc) return rc;
+    resolve_timing(b);
+    b->timing = enabled != 0;
+    b->t_builds = b->t_csr_ms = b->t_fill_ms = 0.0;
+    b->bytes_at_enable = b->h_st->csr_bytes;
+    return 0;
+}
+
+int soil_hip_get_timing(soil_hip_box* b, double* out4) {
+    int rc = check_box(b, true);
+    if (rc) return rc;
+    rc = pull_state(b);
+    if (rc) return rc;
+    resolve_timing(b);
+    out4[0] = b->t_builds;
+    out4[1] = b->t_csr_ms;
+    out4[2] = (double)(b->h_st->csr_bytes - b->bytes_at_enable);
+    out4[3] = b->t_fill_ms;
+    return 0;
+}
+
+}  // extern "C"

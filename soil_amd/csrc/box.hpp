@@ -1,0 +1,109 @@
+// Host-side box / context / batch objects.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "geometry.hpp"
+#include "kernels.hpp"
+#include "soil_hip.h"
+
+struct soil_hip_ctx {
+    int device = 0;
+};
+
+namespace soil_hip {
+
+struct HostCol {
+    int width;
+    uint32_t flags;
+    std::vector<double> data;  // locals, uploaded at setup
+};
+
+struct TileTables {
+    std::vector<Tile> box, local, local1, atom, atom1, ghost, cell, pair, border, scan_cell, scan_local, scan_atom;
+};
+
+struct Caps {
+    uint32_t local = 0, atoms = 0, cells = 0, pairs = 0, tiles = 0;
+};
+
+void build_tiles(TileTables& tt, uint32_t box_index, const Caps& c);
+void upload_tiles(const TileTables& tt, Launch& L, void** arena);
+
+}  // namespace soil_hip
+
+struct soil_hip_box;
+namespace soil_hip {
+int alloc_atoms(soil_hip_box* b, const Caps& nc, bool keep);
+int alloc_pairs(soil_hip_box* b, uint32_t cap_pairs, bool keep);
+int push_desc(soil_hip_box* b);
+int pull_desc(soil_hip_box* b);
+int pull_state(soil_hip_box* b);
+int rebuild_tiles(soil_hip_box* b);
+int check_err_bits(uint32_t err);
+}  // namespace soil_hip
+
+struct soil_hip_box {
+    soil_hip_ctx* ctx = nullptr;
+    soil_hip_box_desc desc{};
+    soil_hip::HostModel model;
+    std::vector<soil_hip::HostCol> cols;
+    // host copies of the uploaded locals (SoA), consumed by setup
+    std::vector<double> hx, hy, hz, hvx, hvy, hvz, hfx, hfy, hfz, hmass, hrad;
+    std::vector<uint32_t> htag, htype;
+    std::vector<int32_t> himx, himy, himz;
+    size_t nlocal = 0;
+    bool setup_done = false;
+    bool in_batch = false;
+    // device
+    hipStream_t stream = nullptr;
+    soil_hip::Caps caps;
+    void* arena_atoms = nullptr;
+    void* arena_pairs = nullptr;
+    void* arena_tiles = nullptr;
+    soil_hip::BoxDev h_dev{};        // host mirror of the device descriptor
+    soil_hip::BoxDev* d_dev = nullptr;
+    soil_hip::BoxDev** d_ptrs = nullptr;  // one-entry pointer array for single-box launches
+    soil_hip::BoxState* d_st = nullptr;
+    soil_hip::BoxState* h_st = nullptr;   // pinned
+    soil_hip::Geom* d_ring = nullptr;
+    soil_hip::Geom* h_ring = nullptr;     // pinned
+    uint32_t ring_len = 64;
+    uint32_t ring_pos = 0;                // host mirror of st.ring_pos
+    soil_hip::Geom last_geom{};
+    soil_hip::Launch L;
+    bool consumer = false;
+    double harmonic_k = 0.0;
+    // timing
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    struct Pending { hipEvent_t e0, e1, e2; };
+    std::vector<Pending> pending;
+    double t_builds = 0.0, t_csr_ms = 0.0, t_fill_ms = 0.0;
+    uint64_t bytes_at_enable = 0;
+};
+
+struct soil_hip_batch {
+    soil_hip_ctx* ctx = nullptr;
+    std::vector<soil_hip_box*> boxes;
+    hipStream_t stream = nullptr;
+    soil_hip::BoxDev** d_ptrs = nullptr;
+    void* arena_tiles = nullptr;
+    soil_hip::Launch L;
+    soil_hip::Geom* d_ring = nullptr;  // [nbox][K]
+    soil_hip::Geom* h_ring = nullptr;  // pinned [2][nbox][K/2] staging halves
+    uint32_t K = 64;
+    uint64_t steps_done = 0;
+    hipEvent_t half_done[2] = {nullptr, nullptr};
+    bool half_pending[2] = {false, false};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    std::vector<const soil_hip::Geom*> saved_rings;
+    std::vector<uint32_t> saved_ring_len;
+    bool timing = false;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    double t_ms = 0.0;
+    uint64_t bytes_at_enable = 0;
+};

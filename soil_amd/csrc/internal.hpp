@@ -1,0 +1,131 @@
+// Internal structures of the soil_hip library (host + device).
+//
+// Device data layout per box (all SoA, f64 geometry):
+//   atoms   [0, nlocal) locals, [nlocal, nlocal+nghost) ghosts, capacity cap_atoms
+//           x y z vx vy vz fx fy fz mass inv_mass rad : f64
+//           tag type : u32, origin : i32, imx imy imz : i32, gswap : u8 (ghost swap id)
+//   alt_*   second buffers of the permuted columns (spatial sort gathers into
+//           them, then the device-side descriptor swaps the pointers)
+//   bins    bin_start[ncells+1], counts, sorted atoms, sorted idx, spos (x,y,z,r)
+//   csr     offsets[nlocal+1], indices[cap_pairs]
+//   consumer transpose: tstart[n+1], trow[P], row_of[P]
+// The descriptor BoxDev lives in device memory and is the authority for
+// pointers; kernels read it at run time, so a captured graph stays valid when
+// the spatial sort swaps column buffers.
+#pragma once
+#include <cstddef>
+#include <cstdint>
+
+#define SOIL_MAX_STENCIL 320
+#define SOIL_MAX_COLS 8
+#define SOIL_NSWAP 6
+
+namespace soil_hip {
+
+// error bits raised on device
+enum : uint32_t {
+    ERR_PAIR_CAPACITY = 1u,
+    ERR_GHOST_CAPACITY = 2u,
+    ERR_ATOM_LEFT_BOX = 4u,   // non-periodic removal (not supported on device)
+    ERR_STENCIL = 8u,
+};
+
+// Host-computed geometry of one box at one step (apply_deform + recompute_bins
+// are functions of the step count only, so the host precomputes them).
+struct Geom {
+    double lo[3], hi[3], size[3];
+    double tilt[3], bvel[3];
+    double bin_size[3], inv_bin[3], bin_origin[3];
+    double pbc_box[3], inv_pbc_box[3], half_pbc_box[3];
+    double thr_lo[3], thr_hi[3];          // border selection thresholds (real or lamda)
+    double swap_shift[SOIL_NSWAP][3];
+    double swap_vshift[SOIL_NSWAP][3];
+    double dt;
+    double shear_rate, yc;
+    double skin_fraction, sf_sq, disp_thr_sq;
+    double ucut_sq;
+    double ghost_cut;
+    uint64_t every;
+    int32_t nbin[3];
+    uint32_t ncells;
+    int32_t periodic[3];
+    int32_t triclinic;
+    int32_t pbc_all;
+    int32_t check;
+    int32_t uniform, newton, has_self;
+    int32_t nstencil;
+    int32_t init_shear, clear_last, bounds_changed, periodic_sort;
+    int32_t disjoint[3];
+    int32_t stencil[SOIL_MAX_STENCIL];
+};
+
+// Device-written per-box counters and flags (host reads them back).
+struct BoxState {
+    uint32_t nlocal, nghost, npairs, state;     // state: 0 FullRebuild, 1 CommunicateOnly
+    uint32_t last_valid;                        // last_build_pos holds nlocal positions
+    uint32_t exceed;                            // displacement flag of this step
+    uint32_t do_sort, not_identity;
+    uint64_t steps_since_build;
+    uint64_t rebuilds, steps;
+    uint32_t swap_start[SOIL_NSWAP], swap_count[SOIL_NSWAP];
+    uint32_t rebuild_now;                       // this step takes the FullRebuild path
+    uint32_t err;
+    uint32_t ring_pos;                          // geometry slot of the current step
+    uint32_t pad;
+    uint64_t csr_bytes, csr_builds;             // algorithmic bytes of CSR builds (timing)
+};
+
+struct ColDev {
+    double* data;
+    double* alt;
+    int32_t width;
+    uint32_t flags;
+};
+
+struct BoxDev {
+    double *x, *y, *z, *vx, *vy, *vz, *fx, *fy, *fz, *mass, *inv_mass, *rad;
+    uint32_t *tag, *type;
+    int32_t *origin, *imx, *imy, *imz;
+    // spatial-sort alternates (same capacity)
+    double *ax, *ay, *az, *avx, *avy, *avz, *amass, *ainv_mass, *arad;
+    uint32_t *atag, *atype;
+    int32_t *aorigin, *aimx, *aimy, *aimz;
+    uint8_t* gswap;
+    double *lbx, *lby, *lbz;
+    // sort / bin scratch
+    uint32_t* cell;       // per atom cell (locals: spatial sort result reused by the build)
+    uint32_t* slot;       // atomic arrival slot within a cell
+    uint32_t* tmp;        // unordered scatter
+    uint32_t* sorted;     // bin_sorted_atoms (or spatial-sort perm)
+    uint32_t* sidx;       // bin_atom_sorted_idx
+    double* spos;         // 4 f64 per sorted slot: x y z r
+    uint32_t* bin_start;  // cap_cells + 1
+    uint32_t* counts;     // cap_cells + 1
+    uint32_t* scan_part;  // per-tile partial sums (scan scratch)
+    // CSR
+    uint32_t* offsets;    // cap_local + 1
+    uint32_t* indices;    // cap_pairs
+    // consumer transpose
+    uint32_t* tcount;     // cap_atoms + 1
+    uint32_t* tstart;     // cap_atoms + 1
+    uint32_t* tslot;      // cap_pairs
+    uint32_t* ttmp;       // cap_pairs
+    uint32_t* trow;       // cap_pairs
+    uint32_t* row_of;     // cap_pairs
+    // border compaction scratch: per tile counts of the two swaps
+    uint32_t* btile;      // 2 * max_tiles
+    ColDev cols[SOIL_MAX_COLS];
+    int32_t ncols;
+    uint32_t cap_local, cap_atoms, cap_cells, cap_pairs, cap_tiles;
+    BoxState* st;
+    const Geom* geom_ring;  // per-step geometry ring (host-written)
+    uint32_t ring_len;
+    uint32_t ring_stride;   // Geom records between consecutive slots of this box
+    double harmonic_k;
+    int32_t consumer;
+    int32_t pad;
+};
+
+int set_error(int code, const char* fmt, ...);
+
+}  // namespace soil_hip

@@ -1,0 +1,1091 @@
+// HIP kernels for gfx950 (wave64). Every kernel walks a tile table
+// (box, first item) with a grid-stride loop, so one launch covers one box (the
+// plain path) or a whole ensemble (the batched path) with the same code.
+//
+// Arithmetic follows the reference expression for expression (f64, no
+// contraction: the library is built with -ffp-contract=off; the only fused
+// multiply-adds are explicit fma() where the reference writes mul_add).
+// Integer outputs are order-exact: every sort is a stable counting sort
+// (atomic arrival slots, then an in-segment rank by index), every ghost append
+// is an ordered wave-ballot compaction.
+
+#include <hip/hip_runtime.h>
+#include "kernels.hpp"
+
+namespace soil_hip {
+
+#define FOR_TILES(t) for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x)
+#define KARGS BoxDev* const* __restrict__ boxes, const Tile* __restrict__ tiles, uint32_t ntiles
+
+__device__ __forceinline__ const Geom& geom_of(const BoxDev& b) {
+    return b.geom_ring[(size_t)b.st->ring_pos * b.ring_stride];
+}
+
+// neighbor.rs:915-922 / 1065-1076: floor, saturating `as i32`, clamp.
+__device__ __forceinline__ int cell_coord(double v, int n) {
+    double f = floor(v);
+    if (!(f >= 0.0)) return 0;
+    if (f >= (double)(n - 1)) return n - 1;
+    return (int)f;
+}
+
+// domain.rs:211-217
+__device__ __forceinline__ void x2lamda(const Geom& g, double x, double y, double z, double& lx, double& ly,
+                                        double& lz) {
+    double xy = g.tilt[0], xz = g.tilt[1], yz = g.tilt[2];
+    lz = (z - g.lo[2]) / g.size[2];
+    ly = (y - g.lo[1] - yz * lz) / g.size[1];
+    lx = (x - g.lo[0] - xy * ly - xz * lz) / g.size[0];
+}
+
+// domain.rs:222-229
+__device__ __forceinline__ void lamda2x(const Geom& g, double lx, double ly, double lz, double& x, double& y,
+                                        double& z) {
+    double xy = g.tilt[0], xz = g.tilt[1], yz = g.tilt[2];
+    x = g.lo[0] + g.size[0] * lx + xy * ly + xz * lz;
+    y = g.lo[1] + g.size[1] * ly + yz * lz;
+    z = g.lo[2] + g.size[2] * lz;
+}
+
+__device__ __forceinline__ uint32_t cell_of(const Geom& g, double x, double y, double z) {
+    double px = x, py = y, pz = z;
+    if (g.triclinic) x2lamda(g, x, y, z, px, py, pz);
+    int cx = cell_coord((px - g.bin_origin[0]) * g.inv_bin[0], g.nbin[0]);
+    int cy = cell_coord((py - g.bin_origin[1]) * g.inv_bin[1], g.nbin[1]);
+    int cz = cell_coord((pz - g.bin_origin[2]) * g.inv_bin[2], g.nbin[2]);
+    return (uint32_t)(cx * g.nbin[1] * g.nbin[2] + cy * g.nbin[2] + cz);
+}
+
+__device__ __forceinline__ uint32_t lane_prefix(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// Block-wide exclusive scan of one u32 per thread (BLK threads, wave64).
+__device__ uint32_t block_excl_scan(uint32_t v, uint32_t* lds, uint32_t& total) {
+    const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= (uint32_t)o) x += y;
+    }
+    if (lane == 63u) lds[wid] = x;
+    __syncthreads();
+    uint32_t wpre = 0, tot = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < BLK / 64; ++w) {
+        uint32_t s = lds[w];
+        if (w < wid) wpre += s;
+        tot += s;
+    }
+    __syncthreads();
+    total = tot;
+    return wpre + x - v;
+}
+
+// ── integration: soil_verlet/src/lib.rs:131-194 ───────────────────────────
+__global__ void __launch_bounds__(BLK) k_initial(KARGS) {
+    FOR_TILES(t) {
+        const Tile tl = tiles[t];
+        const BoxDev& b = *boxes[tl.box];
+        const uint32_t i = tl.start + threadIdx.x;
+        if (i >= b.st->nlocal) continue;
+        const Geom& g = geom_of(b);
+        const double dt = g.dt;
+        const double h = 0.5 * dt * b.inv_mass[i];
+        const double v0 = b.vx[i] + h * b.fx[i];
+        const double v1 = b.vy[i] + h * b.fy[i];
+        const double v2 = b.vz[i] + h * b.fz[i];
+        b.vx[i] = v0;
+        b.vy[i] = v1;
+        b.vz[i] = v2;
+        b.x[i] = b.x[i] + v0 * dt;
+        b.y[i] = b.y[i] + v1 * dt;
+        b.z[i] = b.z[i] + v2 * dt;
+    }
+}
+
+__global__ void __launch_bounds__(BLK) k_final(KARGS) {
+    FOR_TILES(t) {
+        const Tile tl = tiles[t];
+        const BoxDev& b = *boxes[tl.box];
+        const uint32_t i = tl.start + threadIdx.x;
+        if (i >= b.st->nlocal) continue;
+        const Geom& g = geom_of(b);
+        const double h = 0.5 * g.dt * b.inv_mass[i];
+        b.vx[i] = b.vx[i] + h * b.fx[i];
+        b.vy[i] = b.vy[i] + h * b.fy[i];
+        b.vz[i] = b.vz[i] + h * b.fz[i];
+    }
+}
+
+// soil_deform/src/lib.rs:568-574: one-time streaming profile v_x += γ̇ (y - y_c)
+__global__ void __launch_bounds__(BLK) k_deform_init(KARGS) {
+    FOR_TILES(t) {
+        const Tile tl = tiles[t];
+        const BoxDev& b = *boxes[tl.box];
+        const Geom& g = geom_of(b);
+        if (!g.init_shear) continue;
+        const uint32_t i = tl.start + threadIdx.x;
+        if (i >= b.st->nlocal) continue;
+        b.vx[i] += g.shear_rate * (b.y[i] - g.yc);
+    }
+}
+
+// ── rebuild decision: neighbor.rs:693-822 ─────────────────────────────────
+__global__ void __launch_bounds__(BLK) k_disp(KARGS) {
+    FOR_TILES(t) {
+        const Tile tl = tiles[t];
+        const BoxDev& b = *boxes[tl.box];
+        BoxState* st = b.st;
+        const Geom& g = geom_of(b);
+        if (st->state != 1u || !st->last_valid || g.clear_last) continue;
+        if (g.every != 0 && !g.check) continue;
+        const uint32_t i = tl.start + threadIdx.x;
+        if (i >= st->nlocal) continue;
+        double dx = b.x[i] - b.lbx[i];
+        double dy = b.y[i] - b.lby[i];
+        double dz = b.z[i] - b.lbz[i];
+        bool ex;
+        if (g.pbc_all) {
+            dx -= g.pbc_box[0] * round(dx * g.inv_pbc_box[0]);
+            dy -= g.pbc_box[1] * round(dy * g.inv_pbc_box[1]);
+            dz -= g.pbc_box[2] * round(dz * g.inv_pbc_box[2]);
+            ex = fma(dx, dx, fma(dy, dy, dz * dz)) > g.disp_thr_sq;
+        } else {
+            if (g.periodic[0]) {
+                if (dx > g.half_pbc_box[0]) dx -= g.pbc_box[0];
+                else if (dx < -g.half_pbc_box[0]) dx += g.pbc_box[0];
+            }
+            if (g.periodic[1]) {
+                if (dy > g.half_pbc_box[1]) dy -= g.pbc_box[1];
+                else if (dy < -g.half_pbc_box[1]) dy += g.pbc_box[1];
+            }
+            if (g.periodic[2]) {
+                if (dz > g.half_pbc_box[2]) dz -= g.pbc_box[2];
+                else if (dz < -g.half_pbc_box[2]) dz += g.pbc_box[2];
+            }
+            ex = dx * dx + dy * dy + dz * dz > g.disp_thr_sq;
+        }
+        if (__ballot(ex) != 0ull && (threadIdx.x & 63u) == 0u) atomicOr(&st->exceed, 1u);
+    }
+}
+
+// decide_rebuild + remove_ghost_atoms (one thread per box)
+__global__ void k_decide(KARGS) {
+    FOR_TILES(t) {
+        if (threadIdx.x != 0) continue;
+        const BoxDev& b = *boxes[tiles[t].box];
+        BoxState* st = b.st;
+        const Geom& g = geom_of(b);
+        if (g.clear_last) st->last_valid = 0;
+        if (st->state == 1u) {
+            bool need;
+            if (!st->last_valid || st->nlocal == 0) need = true;
+            else if (g.every == 0) need = st->exceed != 0;
+            else if (g.check) need = st->steps_since_build >= g.every || st->exceed != 0;
+            else need = st->steps_since_build >= g.every;
+            if (need) st->state = 0u;
+            else st->steps_since_build += 1;
+        }
+        st->exceed = 0;
+    }
+}
+
+// remove_ghost_atoms (atom.rs:625-629), FullRebuild only
+__global__ void k_remove_ghosts(KARGS) {
+    FOR_TILES(t) {
+        if (threadIdx.x != 0) continue;
+        BoxState* st = boxes[tiles[t].box]->st;
+        if (st->state == 0u) st->nghost = 0;
+    }
+}
+
+// run_if(in_state(FullRebuild)) for the per-phase entry points
+__global__ void k_set_rebuild_now(KARGS) {
+    FOR_TILES(t) {
+        if (threadIdx.x != 0) continue;
+        BoxState* st = boxes[tiles[t].box]->st;
+        st->rebuild_now = st->state == 0u ? 1u : 0u;
+    }
+}
+
+// zero_all_forces: atom.rs:631-635 (force + #[zero] columns of all atoms)
+__global__ void __launch_bounds__(BLK) k_zero(KARGS) {
+    FOR_TILES(t) {
+        const Tile tl = tiles[t];
+        const BoxDev& b = *boxes[tl.box];
+        const uint32_t i = tl.start + threadIdx.x;
+        if (i >= b.st->nlocal + b.st->nghost) continue;
+        b.fx[i] = 0.0;
+        b.fy[i] = 0.0;
+        b.fz[i] = 0.0;
+        for (int c = 0; c < b.ncols; ++c)
+            if (b.cols[c].flags & 4u)
+                for (int w = 0; w < b.cols[c].width; ++w) b.cols[c].data[(size_t)i * b.cols[c].width + w] = 0.0;
+    }
+}
+
+// pbc / wrap_periodic / pbc_triclinic: domain.rs:594-729 (FullRebuild only)
+__global__ void __launch_bounds__(BLK) k_pbc(KARGS) {
+    FOR_TILES(t) {
+        const Tile tl = tiles[t];
+        const BoxDev& b = *boxes[tl.box];
+        BoxState* st = b.st;
+        if (st->state != 0u) continue;
+        const uint32_t i = tl.start + threadIdx.x;
+        if (i >= st->nlocal) continue;
+        const Geom& g = geom_of(b);
+        if (g.triclinic) {
+            double l[3];
+            x2lamda(g, b.x[i], b.y[i], b.z[i], l[0], l[1], l[2]);
+            int delta[3] = {0, 0, 0};
+            bool gone = false;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                if (g.periodic[k]) {
+                    if (l[k] < 0.0) { l[k] += 1.0; delta[k] = -1; }
+                    else if (l[k] >= 1.0) { l[k] -= 1.0; delta[k] = 1; }
+                } else if (l[k] < 0.0 || l[k] >= 1.0) {
+                    gone = true;
+                }
+            }
+            if (gone) { atomicOr(&st->err, ERR_ATOM_LEFT_BOX); continue; }
+            double x, y, z;
+            lamda2x(g, l[0], l[1], l[2], x, y, z);
+            b.x[i] = x;
+            b.y[i] = y;
+            b.z[i] = z;
+            b.imx[i] += delta[0];
+            b.imy[i] += delta[1];
+            b.imz[i] += delta[2];
+            if (delta[1] != 0) {
+                const double s = (double)delta[1];
+                b.vx[i] -= s * g.bvel[0];
+                b.vy[i] -= s * g.bvel[1];
+                b.vz[i] -= s * g.bvel[2];
+            }
+            continue;
+        }
+        double* p[3] = {b.x, b.y, b.z};
+        int32_t* im[3] = {b.imx, b.imy, b.imz};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            double v = p[k][i];
+            if (g.periodic[k]) {
+                const double high = g.lo[k] + g.size[k];
+                if (v < g.lo[k]) { p[k][i] = v + g.size[k]; im[k][i] -= 1; }
+                else if (v >= high) { p[k][i] = v - g.size[k]; im[k][i] += 1; }
+            } else if (v < g.lo[k] || v >= g.hi[k]) {
+                atomicOr(&st->err, ERR_ATOM_LEFT_BOX);
+            }
+        }
+    }
+}
+
+// sort_atoms_by_bin bookkeeping before the sort (one thread per box)
+__global__ void k_pre_sort(KARGS) {
+    FOR_TILES(t) {
+        if (threadIdx.x != 0) continue;
+        const BoxDev& b = *boxes[tiles[t].box];
+        BoxState* st = b.st;
+        const Geom& g = geom_of(b);
+        if (g.bounds_changed) st->state = 0u;
+        st->do_sort = (st->nlocal > 0 && g.ncells > 0 && (g.periodic_sort || st->state == 0u)) ? 1u : 0u;
+        st->not_identity = 0;
+    }
+}
+
+// after the sort: a periodic sort that moved atoms forces a rebuild (neighbor.rs:989-996)
+__global__ void k_post_sort(KARGS) {
+    FOR_TILES(t) {
+        if (threadIdx.x != 0) continue;
+        const BoxDev& b = *boxes[tiles[t].box];
+        BoxState* st = b.st;
+        const Geom& g = geom_of(b);
+        if (g.periodic_sort && st->do_sort && st->not_identity) {
+            st->last_valid = 0;
+            st->state = 0u;
+        }
+        st->rebuild_now = st->state == 0u ? 1u : 0u;
+    }
+}
+
+// ── stable counting sort by key ────────────────────────────────────────────
+// SPATIAL: locals by cell (perm into `sorted`), BUILD: locals+ghosts by cell,
+// TRANSPOSE: CSR entries by column (consumer gather lists).
+template <int W>
+struct SortView {
+    uint32_t n, nkeys;
+    bool active;
+    uint32_t *key, *slot, *counts, *start, *tmp, *out;
+};
+
+template <int W>
+__device__ __forceinline__ SortView<W> sort_view(const BoxDev& b) {
+    SortView<W> v;
+    const BoxState* st = b.st;
+    const Geom& g = geom_of(b);
+    if (W == SORT_SPATIAL) {
+        v.n = st->nlocal; v.nkeys = g.ncells; v.active = st->do_sort != 0;
+        v.key = b.cell; v.slot = b.slot; v.counts = b.counts; v.start = b.bin_start; v.tmp = b.tmp; v.out = b.sorted;
+    } else if (W == SORT_BUILD) {
+        v.n = st->nlocal + st->nghost; v.nkeys = g.ncells;
+        v.active = st->rebuild_now != 0 && (st->err & (ERR_GHOST_CAPACITY)) == 0;
+        v.key = b.cell; v.slot = b.slot; v.counts = b.counts; v.start = b.bin_start; v.tmp = b.tmp; v.out = b.sorted;
+    } else {
+        v.n = st->npairs; v.nkeys = st->nlocal + st->nghost;
+        v.active = st->rebuild_now != 0 && b.consumer && (st->err & (ERR_PAIR_CAPACITY | ERR_GHOST_CAPACITY)) == 0;
+        v.key = b.indices; v.slot = b.tslot; v.counts = b.tcount; v.start = b.tstart; v.tmp = b.ttmp; v.out = b.trow;
+    }
+    return v;
+}
+
+template <int W>
+__global__ void __launch_bounds__(BLK) k_sort_zero(KARGS) {
+    FOR_TILES(t) {
+        const Tile tl = tiles[t];
+        const BoxDev& b = *boxes[tl.box];
+        SortView<W> v = sort_view<W>(b);
+        if (!v.active) continue;
+        const uint32_t c = tl.start + threadIdx.x;
+        if (c <= v.nkeys) v.counts[c] = 0u;
+    }
+}
+
+template <int W>
+__global__ void __launch_bounds__(BLK) k_sort_count(KARGS) {
+    FOR_TILES(t) {
+        const Tile tl = tiles[t];
+        const BoxDev& b = *boxes[tl.box];
+        SortView<W> v = sort_view<W>(b);
+        if (!v.active) continue;
+        const uint32_t i = tl.start + threadIdx.x;
+        if (i >= v.n) continue;
+        uint32_t key;
+        if (W == SORT_TRANSPOSE) {
+            key = v.key[i];
+        } else {
+            key = cell_of(geom_of(b), b.x[i], b.y[i], b.z[i]);
+            v.key[i] = key;
+        }
+        v.slot[i] = atomicAdd(&v.counts[key], 1u);
+    }
+}
+
+template <int W>
+__global__ void __launch_bounds__(BLK) k_sort_scatter(KARGS) {
+    FOR_TILES(t) {
+        const Tile tl = tiles[t];
+        const BoxDev& b = *boxes[tl.box];
+        SortView<W> v = sort_view<W>(b);
+        if (!v.active) continue;
+        const uint32_t i = tl.start + threadIdx.x;
+        if (i >= v.n) continue;
+        const uint32_t key = v.key[i];
+        v.tmp[v.start[key] + v.slot[i]] = i;
+    }
+}
+
+template <int W>
+__global__ void __launch_bounds__(BLK) k_sort_rank(KARGS) {
+    FOR_TILES(t) {
+        const Tile tl = tiles[t];
+        const BoxDev& b = *boxes[tl.box];
+        SortView<W> v = sort_view<W>(b);
+        if (!v.active) continue;
+        const uint32_t i = tl.start + threadIdx.x;
+        if (i >= v.n) continue;
+        const uint32_t key = v.key[i];
+        const uint32_t s = v.start[key], e = v.start[key + 1];
+        uint32_t r = 0;
+        if (e - s > 1u)
+            for (uint32_t m = s; m < e; ++m) r += v.tmp[m] < i ? 1u : 0u;
+        const uint32_t dst = s + r;
+        v.out[dst] = i;
+        if (W == SORT_BUILD) {
+            b.sidx[i] = dst;
+            double* sp = b.spos + 4 * (size_t)dst;
+            sp[0] = b.x[i];
+            sp[1] = b.y[i];
+            sp[2] = b.z[i];
+            sp[3] = b.rad[i];
+        }
+        if (W == SORT_SPATIAL) {
+            if (__ballot(dst != i) != 0ull && (threadIdx.x & 63u) == 0u) atomicOr(&b.st->not_identity, 1u);
+        }
+    }
+}
+
+// atom.rs:479-490 apply_permutation on every permuted column (gather into the
+// alternates; forces and #[zero] columns are zero at this point of the step)
+__global__ void __launch_bounds__(BLK) k_permute(KARGS) {
+    FOR_TILES(t) {
+        const Tile tl = tiles[t];
+        const BoxDev& b = *boxes[tl.box];
+        const BoxState* st = b.st;
+        if (!st->do_sort || !st->not_identity) continue;
+        const uint32_t k = tl.start + threadIdx.x;
+        if (k >= st->nlocal) continue;
+        const uint32_t p = b.sorted[k];
+        b.ax[k] = b.x[p];
+        b.ay[k] = b.y[p];
+        b.az[k] = b.z[p];
+        b.avx[k] = b.vx[p];
+        b.avy[k] = b.vy[p];
+        b.avz[k] = b.vz[p];
+        b.amass[k] = b.mass[p];
+        b.ainv_mass[k] = b.inv_mass[p];
+        b.arad[k] = b.rad[p];
+        b.atag[k] = b.tag[p];
+        b.atype[k] = b.type[p];
+        b.aorigin[k] = b.origin[p];
+        b.aimx[k] = b.imx[p];
+        b.aimy[k] = b.imy[p];
+        b.aimz[k] = b.imz[p];
+        for (int c = 0; c < b.ncols; ++c) {
+            const ColDev& cd = b.cols[c];
+            if (cd.flags & 4u) continue;
+            for (int w = 0; w < cd.width; ++w) cd.alt[(size_t)k * cd.width + w] = cd.data[(size_t)p * cd.width + w];
+        }
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void swap_ptr(T*& a, T*& b) {
+    T* t = a;
+    a = b;
+    b = t;
+}
+
+__global__ void k_swap_ptrs(KARGS) {
+    FOR_TILES(t) {
+        if (threadIdx.x != 0) continue;
+        BoxDev& b = *boxes[tiles[t].box];
+        if (!b.st->do_sort || !b.st->not_identity) continue;
+        swap_ptr(b.x, b.ax); swap_ptr(b.y, b.ay); swap_ptr(b.z, b.az);
+        swap_ptr(b.vx, b.avx); swap_ptr(b.vy, b.avy); swap_ptr(b.vz, b.avz);
+        swap_ptr(b.mass, b.amass); swap_ptr(b.inv_mass, b.ainv_mass); swap_ptr(b.rad, b.arad);
+        swap_ptr(b.tag, b.atag); swap_ptr(b.type, b.atype); swap_ptr(b.origin, b.aorigin);
+        swap_ptr(b.imx, b.aimx); swap_ptr(b.imy, b.aimy); swap_ptr(b.imz, b.aimz);
+        for (int c = 0; c < b.ncols; ++c)
+            if (!(b.cols[c].flags & 4u)) swap_ptr(b.cols[c].data, b.cols[c].alt);
+    }
+}
+
+// ── exclusive scans (reduce tiles -> scan tile sums per box -> down-sweep) ──
+template <int W>
+__device__ __forceinline__ void scan_view(const BoxDev& b, uint32_t& n, bool& active, uint32_t*& in, uint32_t*& out) {
+    const BoxState* st = b.st;
+    if (W == SCAN_CELLS_SPATIAL || W == SCAN_CELLS_BUILD) {
+        n = geom_of(b).ncells + 1;
+        active = W == SCAN_CELLS_SPATIAL ? st->do_sort != 0
+                                         : (st->rebuild_now != 0 && (st->err & ERR_GHOST_CAPACITY) == 0);
+        in = b.counts;
+        out = b.bin_start;
+    } else if (W == SCAN_CSR) {
+        n = st->nlocal + 1;
+        active = st->rebuild_now != 0 && (st->err & ERR_GHOST_CAPACITY) == 0;
+        in = b.offsets;
+        out = b.offsets;
+    } else {
+        n = st->nlocal + st->nghost + 1;
+        active = st->rebuild_now != 0 && b.consumer && (st->err & (ERR_PAIR_CAPACITY | ERR_GHOST_CAPACITY)) == 0;
+        in = b.tcount;
+        out = b.tstart;
+    }
+}
+
+template <int W>
+__global__ void __launch_bounds__(BLK) k_scan_reduce(KARGS) {
+    __shared__ uint32_t lds[BLK / 64];
+    FOR_TILES(t) {
+        const Tile tl = tiles[t];
+        const BoxDev& b = *boxes[tl.box];
+        uint32_t n;
+        bool active;
+        uint32_t *in, *out;
+        scan_view<W>(b, n, active, in, out);
+        if (!active || tl.start >= n) continue;  // block-uniform
+        uint32_t s = 0;
+        const uint32_t base = tl.start + threadIdx.x * SCAN_PER_THREAD;
+#pragma unroll
+        for (int k = 0; k < SCAN_PER_THREAD; ++k)
+            if (base + k < n) s += in[base + k];
+        uint32_t tot;
+        block_excl_scan(s, lds, tot);
+        if (threadIdx.x == 0) b.scan_part[tl.start / SCAN_TILE] = tot;
+    }
+}
+
+template <int W>
+__global__ void __launch_bounds__(BLK) k_scan_parts(KARGS) {
+    __shared__ uint32_t lds[BLK / 64];
+    FOR_TILES(t) {
+        const BoxDev& b = *boxes[tiles[t].box];
+        uint32_t n;
+        bool active;
+        uint32_t *in, *out;
+        scan_view<W>(b, n, active, in, out);
+        if (!active) continue;
+        const uint32_t nparts = (n + SCAN_TILE - 1) / SCAN_TILE;
+        uint32_t carry = 0;
+        for (uint32_t base = 0; base < nparts; base += BLK * SCAN_PER_THREAD) {
+            uint32_t v[SCAN_PER_THREAD];
+            uint32_t s = 0;
+            const uint32_t i0 = base + threadIdx.x * SCAN_PER_THREAD;
+#pragma unroll
+            for (int k = 0; k < SCAN_PER_THREAD; ++k) {
+                v[k] = i0 + k < nparts ? b.scan_part[i0 + k] : 0u;
+                s += v[k];
+            }
+            uint32_t tot;
+            uint32_t pre = block_excl_scan(s, lds, tot) + carry;
+#pragma unroll
+            for (int k = 0; k < SCAN_PER_THREAD; ++k) {
+                if (i0 + k < nparts) b.scan_part[i0 + k] = pre;
+                pre += v[k];
+            }
+            carry += tot;
+        }
+        if (threadIdx.x == 0) {
+            BoxState* st = b.st;
+            if (W == SCAN_CSR) {
+                st->npairs = carry;
+                if (carry > b.cap_pairs) atomicOr(&st->err, ERR_PAIR_CAPACITY);
+            }
+        }
+    }
+}
+
+template <int W>
+__global__ void __launch_bounds__(BLK) k_scan_down(KARGS) {
+    __shared__ uint32_t lds[BLK / 64];
+    FOR_TILES(t) {
+        const Tile tl = tiles[t];
+        const BoxDev& b = *boxes[tl.box];
+        uint32_t n;
+        bool active;
+        uint32_t *in, *out;
+        scan_view<W>(b, n, active, in, out);
+        if (!active || tl.start >= n) continue;
+        uint32_t v[SCAN_PER_THREAD];
+        uint32_t s = 0;
+        const uint32_t base = tl.start + threadIdx.x * SCAN_PER_THREAD;
+#pragma unroll
+        for (int k = 0; k < SCAN_PER_THREAD; ++k) {
+            v[k] = base + k < n ? in[base + k] : 0u;
+            s += v[k];
+        }
+        uint32_t tot;
+        uint32_t pre = block_excl_scan(s, lds, tot) + b.scan_part[tl.start / SCAN_TILE];
+#pragma unroll
+        for (int k = 0; k < SCAN_PER_THREAD; ++k) {
+            if (base + k < n) out[base + k] = pre;
+            pre += v[k];
+        }
+    }
+}
+
+// ── borders: comm.rs:682-860 + pack_border_atoms :288-358 ──────────────────
+__device__ __forceinline__ void border_preds(const Geom& g, const BoxDev& b, uint32_t i, int d, bool& p0, bool& p1) {
+    double v;
+    if (g.triclinic) {
+        double lx, ly, lz;
+        x2lamda(g, b.x[i], b.y[i], b.z[i], lx, ly, lz);
+        v = d == 0 ? lx : (d == 1 ? ly : lz);
+    } else {
+        v = d == 0 ? b.x[i] : (d == 1 ? b.y[i] : b.z[i]);
+    }
+    p0 = v < g.thr_lo[d];
+    p1 = v >= g.thr_hi[d];
+}
+
+__global__ void __launch_bounds__(BLK) k_border_count(KARGS, int d) {
+    __shared__ uint32_t lds[BLK / 64];
+    FOR_TILES(t) {
+        const Tile tl = tiles[t];
+        const BoxDev& b = *boxes[tl.box];
+        const BoxState* st = b.st;
+        const Geom& g = geom_of(b);
+        if (!st->rebuild_now || !g.periodic[d]) continue;
+        const uint32_t scan_end = st->nlocal + (d == 0 ? 0u : st->nghost);
+        if (tl.start >= scan_end) continue;
+        uint32_t c0 = 0, c1 = 0;
+        for (int r = 0; r < BORDER_TILE / BLK; ++r) {
+            const uint32_t i = tl.start + r * BLK + threadIdx.x;
+            if (i < scan_end) {
+                bool p0, p1;
+                border_preds(g, b, i, d, p0, p1);
+                c0 += p0;
+                c1 += p1;
+            }
+        }
+        uint32_t tot0, tot1;
+        block_excl_scan(c0, lds, tot0);
+        block_excl_scan(c1, lds, tot1);
+        if (threadIdx.x == 0) {
+            const uint32_t tix = tl.start / BORDER_TILE;
+            b.btile[2 * tix] = tot0;
+            b.btile[2 * tix + 1] = tot1;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(BLK) k_border_scan(KARGS, int d) {
+    __shared__ uint32_t lds[BLK / 64];
+    FOR_TILES(t) {
+        const BoxDev& b = *boxes[tiles[t].box];
+        BoxState* st = b.st;
+        const Geom& g = geom_of(b);
+        if (!st->rebuild_now) continue;
+        const uint32_t nghost0 = d == 0 ? 0u : st->nghost;  // borders truncates old ghosts first
+        const uint32_t scan_end = st->nlocal + nghost0;
+        uint32_t c0 = 0, c1 = 0;
+        if (g.periodic[d] && !(st->err & ERR_GHOST_CAPACITY)) {
+            const uint32_t ntl = (scan_end + BORDER_TILE - 1) / BORDER_TILE;
+            for (uint32_t base = 0; base < ntl; base += BLK) {
+                const uint32_t k = base + threadIdx.x;
+                const uint32_t v0 = k < ntl ? b.btile[2 * k] : 0u;
+                const uint32_t v1 = k < ntl ? b.btile[2 * k + 1] : 0u;
+                uint32_t tot0, tot1;
+                const uint32_t e0 = block_excl_scan(v0, lds, tot0) + c0;
+                const uint32_t e1 = block_excl_scan(v1, lds, tot1) + c1;
+                if (k < ntl) {
+                    b.btile[2 * k] = e0;
+                    b.btile[2 * k + 1] = e1;
+                }
+                c0 += tot0;
+                c1 += tot1;
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            if ((size_t)scan_end + c0 + c1 > b.cap_atoms) {
+                atomicOr(&st->err, ERR_GHOST_CAPACITY);
+                c0 = c1 = 0;
+            }
+            st->swap_start[2 * d] = scan_end;
+            st->swap_count[2 * d] = c0;
+            st->swap_start[2 * d + 1] = scan_end + c0;
+            st->swap_count[2 * d + 1] = c1;
+            st->nghost = nghost0 + c0 + c1;
+        }
+        __syncthreads();
+    }
+}
+
+// pack_border + unpack_atom (atom.rs:525-571): ghost g is a copy of atom i shifted by swap s
+__device__ __forceinline__ void write_ghost(const BoxDev& b, const Geom& g, uint32_t dst, uint32_t i, int s) {
+    b.x[dst] = b.x[i] + g.swap_shift[s][0];
+    b.y[dst] = b.y[i] + g.swap_shift[s][1];
+    b.z[dst] = b.z[i] + g.swap_shift[s][2];
+    b.vx[dst] = b.vx[i] + g.swap_vshift[s][0];
+    b.vy[dst] = b.vy[i] + g.swap_vshift[s][1];
+    b.vz[dst] = b.vz[i] + g.swap_vshift[s][2];
+    b.fx[dst] = b.fx[i];
+    b.fy[dst] = b.fy[i];
+    b.fz[dst] = b.fz[i];
+    const double m = b.mass[i];
+    b.mass[dst] = m;
+    b.inv_mass[dst] = 1.0 / m;
+    b.rad[dst] = b.rad[i];
+    b.tag[dst] = b.tag[i];
+    b.type[dst] = b.type[i];
+    b.origin[dst] = (int32_t)i;
+    b.imx[dst] = b.imx[i];
+    b.imy[dst] = b.imy[i];
+    b.imz[dst] = b.imz[i];
+    b.gswap[dst] = (uint8_t)s;
+    for (int c = 0; c < b.ncols; ++c) {
+        const ColDev& cd = b.cols[c];
+        for (int w = 0; w < cd.width; ++w) cd.data[(size_t)dst * cd.width + w] = cd.data[(size_t)i * cd.width + w];
+    }
+}
+
+__global__ void __launch_bounds__(BLK) k_border_write(KARGS, int d) {
+    __shared__ uint32_t wc[2][BLK / 64];
+    FOR_TILES(t) {
+        const Tile tl = tiles[t];
+        const BoxDev& b = *boxes[tl.box];
+        const BoxState* st = b.st;
+        const Geom& g = geom_of(b);
+        if (!st->rebuild_now || !g.periodic[d] || (st->err & ERR_GHOST_CAPACITY)) continue;
+        const uint32_t scan_end = st->swap_start[2 * d];
+        if (tl.start >= scan_end) continue;
+        const uint32_t tix = tl.start / BORDER_TILE;
+        uint32_t run0 = st->swap_start[2 * d] + b.btile[2 * tix];
+        uint32_t run1 = st->swap_start[2 * d + 1] + b.btile[2 * tix + 1];
+        const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+        for (int r = 0; r < BORDER_TILE / BLK; ++r) {
+            const uint32_t i = tl.start + r * BLK + threadIdx.x;
+            bool p0 = false, p1 = false;
+            if (i < scan_end) border_preds(g, b, i, d, p0, p1);
+            const uint64_t m0 = __ballot(p0), m1 = __ballot(p1);
+            if (lane == 0) {
+                wc[0][wid] = (uint32_t)__popcll(m0);
+                wc[1][wid] = (uint32_t)__popcll(m1);
+            }
+            __syncthreads();
+            uint32_t o0 = run0, o1 = run1;
+            for (uint32_t w = 0; w < wid; ++w) {
+                o0 += wc[0][w];
+                o1 += wc[1][w];
+            }
+            if (p0) write_ghost(b, g, o0 + lane_prefix(m0), i, 2 * d);
+            if (p1) write_ghost(b, g, o1 + lane_prefix(m1), i, 2 * d + 1);
+            for (uint32_t w = 0; w < BLK / 64; ++w) {
+                run0 += wc[0][w];
+                run1 += wc[1][w];
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// ── forward comm: comm.rs:400-432, 496-592 ─────────────────────────────────
+// A ghost refreshes from the chain ghost -> ... -> local root, adding the
+// fixed swap offsets root-first, which equals the reference's swap-by-swap order.
+__global__ void __launch_bounds__(BLK) k_forward(KARGS) {
+    FOR_TILES(t) {
+        const Tile tl = tiles[t];
+        const BoxDev& b = *boxes[tl.box];
+        const BoxState* st = b.st;
+        if (st->rebuild_now) continue;
+        const uint32_t nl = st->nlocal;
+        const uint32_t gi = nl + tl.start + threadIdx.x;
+        if (gi >= nl + st->nghost) continue;
+        const Geom& g = geom_of(b);
+        int sw[3];
+        int depth = 0;
+        uint32_t cur = gi;
+        while (cur >= nl && depth < 3) {
+            sw[depth++] = b.gswap[cur];
+            cur = (uint32_t)b.origin[cur];
+        }
+        double px = b.x[cur], py = b.y[cur], pz = b.z[cur];
+        double qx = b.vx[cur], qy = b.vy[cur], qz = b.vz[cur];
+        for (int k = depth - 1; k >= 0; --k) {
+            const int s = sw[k];
+            px = px + g.swap_shift[s][0];
+            py = py + g.swap_shift[s][1];
+            pz = pz + g.swap_shift[s][2];
+            qx = qx + g.swap_vshift[s][0];
+            qy = qy + g.swap_vshift[s][1];
+            qz = qz + g.swap_vshift[s][2];
+        }
+        b.x[gi] = px;
+        b.y[gi] = py;
+        b.z[gi] = pz;
+        b.vx[gi] = qx;
+        b.vy[gi] = qy;
+        b.vz[gi] = qz;
+        for (int c = 0; c < b.ncols; ++c) {
+            const ColDev& cd = b.cols[c];
+            if (!(cd.flags & 1u)) continue;
+            for (int w = 0; w < cd.width; ++w) cd.data[(size_t)gi * cd.width + w] = cd.data[(size_t)cur * cd.width + w];
+        }
+    }
+}
+
+// ── reverse comm: comm.rs:867-878, 938-1024, one launch per swap ───────────
+__global__ void __launch_bounds__(BLK) k_reverse(KARGS, int s) {
+    FOR_TILES(t) {
+        const Tile tl = tiles[t];
+        const BoxDev& b = *boxes[tl.box];
+        const BoxState* st = b.st;
+        const Geom& g = geom_of(b);
+        if (!g.newton || (st->err & ERR_GHOST_CAPACITY)) continue;
+        const uint32_t k = tl.start + threadIdx.x;
+        if (k >= st->swap_count[s]) continue;
+        const uint32_t gi = st->swap_start[s] + k;
+        const uint32_t o = (uint32_t)b.origin[gi];
+        b.fx[o] += b.fx[gi];
+        b.fy[o] += b.fy[gi];
+        b.fz[o] += b.fz[gi];
+        for (int c = 0; c < b.ncols; ++c) {
+            const ColDev& cd = b.cols[c];
+            if (!(cd.flags & 2u)) continue;
+            for (int w = 0; w < cd.width; ++w) cd.data[(size_t)o * cd.width + w] += cd.data[(size_t)gi * cd.width + w];
+        }
+    }
+}
+
+// ── neighbor build: neighbor.rs:1013-1319 ──────────────────────────────────
+__global__ void k_build_prep(KARGS) {
+    FOR_TILES(t) {
+        if (threadIdx.x != 0) continue;
+        const BoxDev& b = *boxes[tiles[t].box];
+        BoxState* st = b.st;
+        if (!st->rebuild_now) continue;
+        st->state = 1u;  // CommunicateOnly
+        st->steps_since_build = 0;
+        st->last_valid = 1;
+        st->rebuilds += 1;
+    }
+}
+
+// save_build_positions (neighbor.rs:669-690); radius extrema are fixed per box
+__global__ void __launch_bounds__(BLK) k_save_build(KARGS) {
+    FOR_TILES(t) {
+        const Tile tl = tiles[t];
+        const BoxDev& b = *boxes[tl.box];
+        if (!b.st->rebuild_now) continue;
+        const uint32_t i = tl.start + threadIdx.x;
+        if (i >= b.st->nlocal) continue;
+        b.lbx[i] = b.x[i];
+        b.lby[i] = b.y[i];
+        b.lbz[i] = b.z[i];
+    }
+}
+
+template <bool FILL>
+__device__ __forceinline__ uint32_t csr_row(const BoxDev& b, const Geom& g, uint32_t i, uint32_t* out) {
+    const uint32_t c = b.cell[i];
+    const double xi = b.x[i], yi = b.y[i], zi = b.z[i], si = b.rad[i];
+    const bool uniform = g.uniform != 0;
+    const double ucut = g.ucut_sq, sfsq = g.sf_sq;
+    uint32_t n = 0;
+    auto scan = [&](uint32_t m0, uint32_t m1, uint32_t skip) {
+        for (uint32_t m = m0; m < m1; ++m) {
+            const double* sp = b.spos + 4 * (size_t)m;
+            const double dx = sp[0] - xi;
+            const double dy = sp[1] - yi;
+            const double dz = sp[2] - zi;
+            const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+            bool acc;
+            if (uniform) {
+                acc = r2 < ucut;
+            } else {
+                const double sum = si + sp[3];
+                acc = r2 < sum * sum * sfsq;
+            }
+            if (acc) {
+                const uint32_t j = b.sorted[m];
+                if (j != skip) {
+                    if (FILL) out[n] = j;
+                    ++n;
+                }
+            }
+        }
+    };
+    if (g.has_self) {
+        if (g.newton) scan(b.sidx[i] + 1, b.bin_start[c + 1], 0xffffffffu);
+        else scan(b.bin_start[c], b.bin_start[c + 1], i);
+    }
+    for (int k = 0; k < g.nstencil; ++k) {
+        const uint32_t cc = (uint32_t)((int32_t)c + g.stencil[k]);
+        if (cc >= g.ncells) continue;  // only reachable by atoms far outside the box
+        scan(b.bin_start[cc], b.bin_start[cc + 1], 0xffffffffu);
+    }
+    return n;
+}
+
+__global__ void __launch_bounds__(BLK) k_csr_count(KARGS) {
+    FOR_TILES(t) {
+        const Tile tl = tiles[t];
+        const BoxDev& b = *boxes[tl.box];
+        const BoxState* st = b.st;
+        if (!st->rebuild_now || (st->err & ERR_GHOST_CAPACITY)) continue;
+        const uint32_t i = tl.start + threadIdx.x;
+        const uint32_t nl = st->nlocal;
+        if (i > nl) continue;
+        if (i == nl) {
+            b.offsets[i] = 0u;
+            continue;
+        }
+        b.offsets[i] = csr_row<false>(b, geom_of(b), i, nullptr);
+    }
+}
+
+__global__ void __launch_bounds__(BLK) k_csr_fill(KARGS) {
+    FOR_TILES(t) {
+        const Tile tl = tiles[t];
+        const BoxDev& b = *boxes[tl.box];
+        const BoxState* st = b.st;
+        if (!st->rebuild_now || (st->err & (ERR_GHOST_CAPACITY | ERR_PAIR_CAPACITY))) continue;
+        const uint32_t i = tl.start + threadIdx.x;
+        if (i >= st->nlocal) continue;
+        csr_row<true>(b, geom_of(b), i, b.indices + b.offsets[i]);
+    }
+}
+
+// algorithmic bytes of this build's CSR phase, accumulated on device
+__global__ void k_csr_account(KARGS) {
+    FOR_TILES(t) {
+        if (threadIdx.x != 0) continue;
+        const BoxDev& b = *boxes[tiles[t].box];
+        BoxState* st = b.st;
+        if (!st->rebuild_now || (st->err & (ERR_GHOST_CAPACITY | ERR_PAIR_CAPACITY))) continue;
+        const Geom& g = geom_of(b);
+        const uint64_t na = (uint64_t)st->nlocal + st->nghost;
+        const uint64_t per = 24u + (g.uniform ? 0u : 8u) + 4u;
+        st->csr_bytes += per * na + 4u * ((uint64_t)g.ncells + 1u) + 4u * ((uint64_t)st->nlocal + 1u) +
+                         4u * (uint64_t)st->npairs;
+        st->csr_builds += 1;
+    }
+}
+
+// consumer: row of each CSR entry
+__global__ void __launch_bounds__(BLK) k_row_of(KARGS) {
+    FOR_TILES(t) {
+        const Tile tl = tiles[t];
+        const BoxDev& b = *boxes[tl.box];
+        const BoxState* st = b.st;
+        if (!st->rebuild_now || !b.consumer || (st->err & (ERR_GHOST_CAPACITY | ERR_PAIR_CAPACITY))) continue;
+        const uint32_t i = tl.start + threadIdx.x;
+        if (i >= st->nlocal) continue;
+        for (uint32_t e = b.offsets[i]; e < b.offsets[i + 1]; ++e) b.row_of[e] = i;
+    }
+}
+
+// Harmonic soft-sphere test consumer (not reference code): deterministic
+// gather, -F(a,j) over a's own row then +F(i,a) over entries (i,a) ascending.
+__device__ __forceinline__ void pair_force(const BoxDev& b, uint32_t i, uint32_t j, double k, double& fx, double& fy,
+                                           double& fz) {
+    const double dx = b.x[j] - b.x[i];
+    const double dy = b.y[j] - b.y[i];
+    const double dz = b.z[j] - b.z[i];
+    const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+    const double rs = b.rad[i] + b.rad[j];
+    if (r2 < rs * rs && r2 > 0.0) {
+        const double dist = sqrt(r2);
+        const double s = k * (rs - dist) / dist;
+        fx = s * dx;
+        fy = s * dy;
+        fz = s * dz;
+    } else {
+        fx = 0.0;
+        fy = 0.0;
+        fz = 0.0;
+    }
+}
+
+__global__ void __launch_bounds__(BLK) k_harmonic(KARGS) {
+    FOR_TILES(t) {
+        const Tile tl = tiles[t];
+        const BoxDev& b = *boxes[tl.box];
+        const BoxState* st = b.st;
+        if (!b.consumer || st->rebuilds == 0 || (st->err & (ERR_GHOST_CAPACITY | ERR_PAIR_CAPACITY))) continue;
+        const uint32_t a = tl.start + threadIdx.x;
+        const uint32_t nl = st->nlocal;
+        if (a >= nl + st->nghost) continue;
+        const double k = b.harmonic_k;
+        double ax = 0.0, ay = 0.0, az = 0.0;
+        if (a < nl) {
+            for (uint32_t e = b.offsets[a]; e < b.offsets[a + 1]; ++e) {
+                double fx, fy, fz;
+                pair_force(b, a, b.indices[e], k, fx, fy, fz);
+                ax = ax - fx;
+                ay = ay - fy;
+                az = az - fz;
+            }
+        }
+        for (uint32_t q = b.tstart[a]; q < b.tstart[a + 1]; ++q) {
+            const uint32_t e = b.trow[q];
+            double fx, fy, fz;
+            pair_force(b, b.row_of[e], a, k, fx, fy, fz);
+            ax = ax + fx;
+            ay = ay + fy;
+            az = az + fz;
+        }
+        b.fx[a] += ax;
+        b.fy[a] += ay;
+        b.fz[a] += az;
+    }
+}
+
+__global__ void k_step_end(KARGS) {
+    FOR_TILES(t) {
+        if (threadIdx.x != 0) continue;
+        const BoxDev& b = *boxes[tiles[t].box];
+        BoxState* st = b.st;
+        st->steps += 1;
+        st->ring_pos = (st->ring_pos + 1u) % b.ring_len;
+    }
+}
+
+// ── launch helpers ─────────────────────────────────────────────────────────
+template <typename... A>
+static void launch(void (*k)(BoxDev* const*, const Tile*, uint32_t, A...), const TileSet& ts, BoxDev* const* boxes,
+                   hipStream_t s, uint32_t maxgrid, A... args) {
+    if (ts.n == 0) return;
+    const uint32_t grid = ts.n < maxgrid ? ts.n : maxgrid;
+    hipLaunchKernelGGL(k, dim3(grid), dim3(BLK), 0, s, boxes, (const Tile*)ts.d, ts.n, args...);
+}
+
+template <int W>
+static void run_scan(const Launch& L, const TileSet& scan_tiles) {
+    launch(k_scan_reduce<W>, scan_tiles, L.boxes, L.stream, L.maxgrid);
+    launch(k_scan_parts<W>, L.t_box, L.boxes, L.stream, L.maxgrid);
+    launch(k_scan_down<W>, scan_tiles, L.boxes, L.stream, L.maxgrid);
+}
+
+template <int W, int SW>
+static void run_sort(const Launch& L, const TileSet& zero_tiles, const TileSet& item_tiles, const TileSet& scan_tiles) {
+    launch(k_sort_zero<W>, zero_tiles, L.boxes, L.stream, L.maxgrid);
+    launch(k_sort_count<W>, item_tiles, L.boxes, L.stream, L.maxgrid);
+    run_scan<SW>(L, scan_tiles);
+    launch(k_sort_scatter<W>, item_tiles, L.boxes, L.stream, L.maxgrid);
+    launch(k_sort_rank<W>, item_tiles, L.boxes, L.stream, L.maxgrid);
+}
+
+void phase_deform(const Launch& L) { launch(k_deform_init, L.t_local, L.boxes, L.stream, L.maxgrid); }
+void phase_initial(const Launch& L) { launch(k_initial, L.t_local, L.boxes, L.stream, L.maxgrid); }
+void phase_decide(const Launch& L) {
+    launch(k_disp, L.t_local, L.boxes, L.stream, L.maxgrid);
+    launch(k_decide, L.t_box, L.boxes, L.stream, L.maxgrid);
+    launch(k_remove_ghosts, L.t_box, L.boxes, L.stream, L.maxgrid);
+}
+void phase_decide_only(const Launch& L) {
+    launch(k_disp, L.t_local, L.boxes, L.stream, L.maxgrid);
+    launch(k_decide, L.t_box, L.boxes, L.stream, L.maxgrid);
+}
+void phase_remove_ghosts(const Launch& L) { launch(k_remove_ghosts, L.t_box, L.boxes, L.stream, L.maxgrid); }
+void phase_set_rebuild_now(const Launch& L) { launch(k_set_rebuild_now, L.t_box, L.boxes, L.stream, L.maxgrid); }
+void phase_sort_flags(const Launch& L) {
+    launch(k_pre_sort, L.t_box, L.boxes, L.stream, L.maxgrid);
+    launch(k_post_sort, L.t_box, L.boxes, L.stream, L.maxgrid);
+}
+void phase_zero(const Launch& L) { launch(k_zero, L.t_atom, L.boxes, L.stream, L.maxgrid); }
+void phase_pbc(const Launch& L) { launch(k_pbc, L.t_local, L.boxes, L.stream, L.maxgrid); }
+void phase_sort(const Launch& L) {
+    launch(k_pre_sort, L.t_box, L.boxes, L.stream, L.maxgrid);
+    run_sort<SORT_SPATIAL, SCAN_CELLS_SPATIAL>(L, L.t_cell, L.t_local, L.t_scan_cell);
+    launch(k_permute, L.t_local, L.boxes, L.stream, L.maxgrid);
+    launch(k_swap_ptrs, L.t_box, L.boxes, L.stream, L.maxgrid);
+    launch(k_post_sort, L.t_box, L.boxes, L.stream, L.maxgrid);
+}
+void phase_borders(const Launch& L) {
+    for (int d = 0; d < 3; ++d) {
+        launch(k_border_count, L.t_border, L.boxes, L.stream, L.maxgrid, d);
+        launch(k_border_scan, L.t_box, L.boxes, L.stream, L.maxgrid, d);
+        launch(k_border_write, L.t_border, L.boxes, L.stream, L.maxgrid, d);
+    }
+}
+void phase_forward(const Launch& L) { launch(k_forward, L.t_ghost, L.boxes, L.stream, L.maxgrid); }
+void phase_build(const Launch& L, hipEvent_t ev0, hipEvent_t ev1, hipEvent_t ev2) {
+    launch(k_build_prep, L.t_box, L.boxes, L.stream, L.maxgrid);
+    launch(k_save_build, L.t_local, L.boxes, L.stream, L.maxgrid);
+    run_sort<SORT_BUILD, SCAN_CELLS_BUILD>(L, L.t_cell, L.t_atom, L.t_scan_cell);
+    if (ev0) (void)hipEventRecord(ev0, L.stream);
+    launch(k_csr_count, L.t_local1, L.boxes, L.stream, L.maxgrid);
+    run_scan<SCAN_CSR>(L, L.t_scan_local);
+    if (ev1) (void)hipEventRecord(ev1, L.stream);
+    launch(k_csr_fill, L.t_local, L.boxes, L.stream, L.maxgrid);
+    if (ev2) (void)hipEventRecord(ev2, L.stream);
+    launch(k_csr_account, L.t_box, L.boxes, L.stream, L.maxgrid);
+    // consumer gather lists (transpose of the half list)
+    run_sort<SORT_TRANSPOSE, SCAN_TRANSPOSE>(L, L.t_atom1, L.t_pair, L.t_scan_atom);
+    launch(k_row_of, L.t_local, L.boxes, L.stream, L.maxgrid);
+}
+void phase_force(const Launch& L) { launch(k_harmonic, L.t_atom, L.boxes, L.stream, L.maxgrid); }
+void phase_reverse(const Launch& L) {
+    // rounds last -> first, lo swap then hi swap (comm.rs:965-1021)
+    const int order[6] = {4, 5, 2, 3, 0, 1};
+    for (int k = 0; k < 6; ++k) launch(k_reverse, L.t_ghost, L.boxes, L.stream, L.maxgrid, order[k]);
+}
+void phase_final(const Launch& L) { launch(k_final, L.t_local, L.boxes, L.stream, L.maxgrid); }
+void phase_step_end(const Launch& L) { launch(k_step_end, L.t_box, L.boxes, L.stream, L.maxgrid); }
+
+}  // namespace soil_hip

@@ -1,0 +1,53 @@
+// Launch-side declarations of the kernels (tile tables and phase launchers).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "internal.hpp"
+
+namespace soil_hip {
+
+constexpr uint32_t BLK = 256;              // threads per block (4 waves of 64)
+constexpr int SCAN_PER_THREAD = 8;
+constexpr uint32_t SCAN_TILE = BLK * SCAN_PER_THREAD;
+constexpr uint32_t BORDER_TILE = 1024;     // atoms per border-compaction tile
+
+enum { SORT_SPATIAL = 0, SORT_BUILD = 1, SORT_TRANSPOSE = 2 };
+enum { SCAN_CELLS_SPATIAL = 0, SCAN_CELLS_BUILD = 1, SCAN_CSR = 2, SCAN_TRANSPOSE = 3 };
+
+struct Tile {
+    uint32_t box, start;
+};
+
+struct TileSet {
+    void* d = nullptr;
+    uint32_t n = 0;
+};
+
+// One launch context: a set of boxes (device array of descriptor pointers),
+// the stream, and the tile tables of each item space.
+struct Launch {
+    BoxDev* const* boxes = nullptr;
+    hipStream_t stream = nullptr;
+    uint32_t maxgrid = 4096;
+    TileSet t_box, t_local, t_local1, t_atom, t_atom1, t_ghost, t_cell, t_pair, t_border;
+    TileSet t_scan_cell, t_scan_local, t_scan_atom;
+};
+
+void phase_deform(const Launch& L);
+void phase_initial(const Launch& L);
+void phase_decide(const Launch& L);
+void phase_decide_only(const Launch& L);
+void phase_remove_ghosts(const Launch& L);
+void phase_set_rebuild_now(const Launch& L);
+void phase_sort_flags(const Launch& L);
+void phase_zero(const Launch& L);
+void phase_pbc(const Launch& L);
+void phase_sort(const Launch& L);
+void phase_borders(const Launch& L);
+void phase_forward(const Launch& L);
+void phase_build(const Launch& L, hipEvent_t ev0, hipEvent_t ev1, hipEvent_t ev2);
+void phase_force(const Launch& L);
+void phase_reverse(const Launch& L);
+void phase_final(const Launch& L);
+void phase_step_end(const Launch& L);
+
+}  // namespace soil_hip

@@ -1,0 +1,169 @@
+"""GPU parity: the HIP path against the CPU oracle on the same seeded inputs.
+
+Integer outputs (bins, ghost lists, CSR offsets and indices, swap tables,
+rebuild counts) must be bit-identical at every compared step. Without a force
+consumer the trajectory is pure f64 add/mul/div and must be bit-identical too;
+with the harmonic consumer positions and velocities are held to 1e-12
+relative (the consumer takes a square root).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import soil_amd
+from paritylib import assert_same, make_pair
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return soil_amd.Context(0)
+
+
+def small(seed, n, rmin, rmax, phi, aspect=(1, 1, 1), nblobs=0, blob_frac=0.0, sigma=0.0, shear=0.0):
+    return soil_amd.gen_custom(seed, n, rmin, rmax, phi, aspect, nblobs, blob_frac, sigma, 0.1, shear)
+
+
+def run_compare(ctx, gen, nsteps, every, k=None, exact=True, desc_mod=None, columns=(), col_data=()):
+    d, pos, vel, mass, rad = gen
+    if desc_mod:
+        desc_mod(d)
+    g, r = make_pair(ctx, d, pos, vel, mass, rad, k=k, columns=columns, col_data=col_data)
+    done = 0
+    while done < nsteps:
+        s = min(every, nsteps - done)
+        g.step(s)
+        r.step(s)
+        done += s
+        print(f"compare step {done}", flush=True)
+        assert_same(g, r, exact_state=exact, what=f"step {done}")
+    return g, r
+
+
+def test_c1_full_config_bit_exact_substrate(ctx):
+    # C1 (4,000 monodisperse spheres, phi 0.30), 100 steps, no force: bit-exact everything
+    print("case: C1 substrate only", flush=True)
+    run_compare(ctx, soil_amd.gen_box(1), 100, 10)
+
+
+def test_c1_full_config_with_consumer(ctx):
+    print("case: C1 + harmonic k=1000", flush=True)
+    run_compare(ctx, soil_amd.gen_box(1), 100, 20, k=1000.0, exact=False)
+
+
+def test_polydisperse_dense_with_consumer(ctx):
+    # C2 shape at reduced size: r~U[0.4,0.6], phi 0.5, trigger-driven rebuilds
+    print("case: polydisperse phi=0.5 n=6000", flush=True)
+    g, r = run_compare(ctx, small(101, 6000, 0.4, 0.6, 0.5), 150, 25, k=1000.0, exact=False)
+    assert g.counts()["rebuilds"] > 3
+
+
+def test_lees_edwards_clustered(ctx):
+    # C3 shape at reduced size: 3:1:1, LE shear, Gaussian blobs over uniform background
+    print("case: LE clustered n=12000", flush=True)
+    run_compare(ctx, small(202, 12000, 0.45, 0.55, 0.4, (3, 1, 1), 16, 0.5, 2.0, 0.1), 40, 10, k=1000.0,
+                exact=False)
+
+
+def test_lees_edwards_substrate_bit_exact_with_flip(ctx):
+    # high strain rate so the tilt wraps (box flip) within the run
+    print("case: LE flip n=3000", flush=True)
+
+    def mod(d):
+        d.dt = 1e-2
+        d.shear_rate = 2.0
+        d.has_shear = 1
+    run_compare(ctx, small(303, 3000, 0.4, 0.6, 0.3), 60, 5, desc_mod=mod)
+
+
+def test_periodic_sort_and_every_check(ctx):
+    print("case: sort_every=7 every=5 check=0", flush=True)
+
+    def mod(d):
+        d.sort_every = 7
+        d.every = 5
+        d.check = 0
+    run_compare(ctx, small(404, 2500, 0.4, 0.6, 0.35), 40, 1, desc_mod=mod)
+
+    def mod2(d):
+        d.sort_every = 3
+        d.every = 4
+        d.check = 1
+    run_compare(ctx, small(405, 2500, 0.5, 0.5, 0.35), 30, 1, k=1000.0, exact=False, desc_mod=mod2)
+
+
+def test_full_list_newton_off(ctx):
+    print("case: newton off", flush=True)
+
+    def mod(d):
+        d.newton = 0
+    run_compare(ctx, small(505, 2000, 0.4, 0.6, 0.3), 20, 5, desc_mod=mod)
+
+
+def test_atomdata_columns_forward_reverse_zero(ctx):
+    # #[forward] overwrite on ghosts, #[reverse] += onto owners, #[zero] reset each step
+    print("case: AtomData columns", flush=True)
+    gen = small(606, 2000, 0.4, 0.6, 0.35)
+    n = len(gen[1])
+    rng = np.random.default_rng(6)
+    cols = ((3, soil_amd._ffi.COL_FORWARD), (1, soil_amd._ffi.COL_REVERSE | soil_amd._ffi.COL_ZERO), (2, 0))
+    data = (rng.normal(size=3 * n), np.zeros(n), rng.normal(size=2 * n))
+    g, r = run_compare(ctx, gen, 12, 3, columns=cols, col_data=data)
+    for c in range(3):
+        assert np.array_equal(g.column(c), r.column(c)), f"column {c}"
+
+
+def test_tiny_and_single_atom_boxes(ctx):
+    print("case: tiny boxes", flush=True)
+    run_compare(ctx, small(707, 1, 0.5, 0.5, 0.01), 5, 1)
+    run_compare(ctx, small(708, 20, 0.4, 0.6, 0.05), 10, 2)
+
+
+def test_phase_by_phase_matches_oracle(ctx):
+    # the per-system entry points follow the reference systems one at a time
+    print("case: per-phase", flush=True)
+    d, pos, vel, mass, rad = small(808, 3000, 0.4, 0.6, 0.4)
+    g, r = make_pair(ctx, d, pos, vel, mass, rad, k=1000.0)
+    for step in range(3):
+        g.apply_deform(); r.phase("apply_deform")
+        g.initial_integration(); r.phase("initial_integration")
+        if step > 0:
+            fr = g.decide_rebuild(); r.phase("decide_rebuild")
+            assert fr == (r.state == 0)
+        if r.state == 0:
+            g.remove_ghost_atoms(); r.phase("remove_ghost_atoms")
+        g.zero_all_forces(); r.phase("zero_all_forces")
+        if r.state == 0:
+            g.pbc(); r.phase("pbc")
+        g.sort_atoms_by_bin(); r.phase("sort_atoms_by_bin")
+        if r.state == 0:
+            g.borders(); r.phase("borders")
+            g.bin_neighbor_list(); r.phase("bin_neighbor_list")
+        else:
+            g.forward_comm_borders(); r.phase("forward_comm_borders")
+        g.harmonic_force(); r.phase("harmonic_force")
+        g.reverse_send_force(); r.phase("reverse_send_force")
+        g.final_integration(); r.phase("final_integration")
+        cg, cr = g.counts(), r.counts()
+        assert (cg["nghost"], cg["npairs"]) == (cr["nghost"], cr["npairs"])
+        ag, ar = g.atoms(), r.atoms()
+        assert np.max(np.abs(ag["pos"] - ar["pos"])) <= 1e-12 * np.max(np.abs(ar["pos"]))
+        og, ig = g.csr()
+        orr, ir = r.csr()
+        assert np.array_equal(og, orr) and np.array_equal(ig, ir)
+
+
+def test_sqrt_and_division_match_host(ctx):
+    # the consumer's sqrt and division: one step on a box whose pairs overlap
+    # heavily; forces must agree with the oracle to the last few ulps
+    print("case: consumer forces", flush=True)
+    d, pos, vel, mass, rad = small(909, 3000, 0.4, 0.6, 0.6)
+    g, r = make_pair(ctx, d, pos, vel, mass, rad, k=1000.0)
+    g.step(1)
+    r.step(1)
+    fg, fr = g.atoms()["force"], r.atoms()["force"]
+    err = np.abs(fg - fr) / np.maximum(np.abs(fr), 1e-300)
+    print("max rel force err", float(err.max()), "exact fraction", float(np.mean(fg == fr)), flush=True)
+    assert float(err.max()) <= 1e-12

@@ -1,0 +1,64 @@
+"""CPU checks of the product library: it loads, exports every header symbol,
+and its host-side generator is deterministic. No GPU call is made here."""
+import os
+import re
+
+import numpy as np
+
+import soil_amd
+from soil_amd import _ffi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols():
+    text = open(os.path.join(ROOT, "include", "soil_hip.h")).read()
+    return sorted(set(re.findall(r"\b(soil_hip_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_header_symbol():
+    lib = _ffi.lib()
+    syms = header_symbols()
+    assert len(syms) >= 40
+    for s in syms:
+        assert hasattr(lib, s), f"missing export {s}"
+
+
+def test_ffi_signatures_cover_header():
+    assert set(header_symbols()) <= set(_ffi._SIGS)
+
+
+def test_generator_is_deterministic_and_matches_closed_form():
+    d1, p1, v1, m1, r1 = soil_amd.gen_box(1)
+    d2, p2, v2, m2, r2 = soil_amd.gen_box(1)
+    assert np.array_equal(p1, p2) and np.array_equal(v1, v2)
+    # C1: 4000 spheres d=1 at phi=0.30 -> L = cbrt(4000 * pi/6 / 0.3)
+    L = (4000 * np.pi / 6 / 0.3) ** (1 / 3)
+    assert abs(d1.hi[0] - L) < 1e-9
+    assert np.all((p1 >= 0) & (p1 < L))
+    assert abs(v1.std() - 0.1) < 0.005
+    assert np.all(r1 == 0.5) and np.all(m1 == 1.0)
+    assert d1.skin_fraction == 1.1 and d1.dt == 1e-3 and d1.has_shear == 0
+
+
+def test_ensemble_parameters_in_range():
+    ns = []
+    for k in range(32):
+        d, p, v, m, r = soil_amd.gen_box(4, k)
+        ns.append(len(p))
+        assert 1000 <= len(p) <= 64000
+        assert 0.0 < d.shear_rate <= 0.2 and d.has_shear == 1
+        phi = np.sum(4 / 3 * np.pi * r ** 3) / np.prod(np.array(d.hi) - np.array(d.lo))
+        assert 0.2 - 1e-9 <= phi <= 0.55 + 1e-9
+    assert len(set(ns)) > 20
+
+
+def test_missing_device_is_an_error_not_a_fallback():
+    if soil_amd.device_count() > 0:
+        return
+    try:
+        soil_amd.Context(0)
+    except _ffi.SoilHipError as e:
+        assert e.code == _ffi.EDEVICE
+    else:
+        raise AssertionError("context creation must fail without a device")
