@@ -161,6 +161,11 @@ int soil_hip_get_swaps(soil_hip_box* box, double* out, size_t cap_rows, size_t* 
  * bin_min_size, tilt[3], boundary_vel[3], cached_min_skin, uniform, uniform_cutoff_sq */
 int soil_hip_get_grid(soil_hip_box* box, double* out21);
 int soil_hip_get_stencil(soil_hip_box* box, int32_t* out, size_t cap, size_t* n);
+/* Host only (no device): grid record and stencil (forward if newton, else full)
+ * that neighbor_setup + compute_bin_grid(_triclinic) (neighbor.rs:311-495,
+ * 604-665) give a box with these radii at tilt xy. */
+int soil_hip_grid_for(const soil_hip_box_desc* desc, const double* radius, size_t n, double tilt_xy, double* out21,
+                      int32_t* stencil, size_t cap, size_t* nstencil);
 
 /* ── batched ensemble (many boxes, one launch per phase, hipGraph step) ─── */
 /* Boxes must be set up and not used individually while in the batch. */

@@ -95,6 +95,7 @@ _SIGS = {
     "soil_hip_get_swaps": ([P, DP, SZ, SZP], C.c_int),
     "soil_hip_get_grid": ([P, DP], C.c_int),
     "soil_hip_get_stencil": ([P, I32P, SZ, SZP], C.c_int),
+    "soil_hip_grid_for": ([C.POINTER(BoxDesc), DP, SZ, C.c_double, DP, I32P, SZ, SZP], C.c_int),
     "soil_hip_batch_create": ([P, C.POINTER(P), C.c_int, C.POINTER(P)], C.c_int),
     "soil_hip_batch_destroy": ([P], C.c_int),
     "soil_hip_batch_autotune": ([P], C.c_int),

@@ -140,11 +140,9 @@ static int batch_run(soil_hip_batch* bt, int nsteps, bool use_graph) {
     return 0;
 }
 
-}  // namespace soil_hip
-
-extern "C" {
-
-int soil_hip_batch_create(soil_hip_ctx* ctx, soil_hip_box** boxes, int nboxes, soil_hip_batch** out) {
+// Groups set-up boxes into a batch: repoints every box at the batch's geometry
+// ring (one small descriptor patch per box) and builds the combined tile tables.
+int batch_make(soil_hip_ctx* ctx, soil_hip_box** boxes, int nboxes, soil_hip_batch** out) {
     if (!ctx || !boxes || nboxes <= 0 || !out) return set_error(SOIL_HIP_EINVAL, "bad arguments");
     for (int k = 0; k < nboxes; ++k) {
         if (!boxes[k] || !boxes[k]->setup_done) return set_error(SOIL_HIP_ESTATE, "box %d not set up", k);
@@ -167,20 +165,20 @@ int soil_hip_batch_create(soil_hip_ctx* ctx, soil_hip_box** boxes, int nboxes, s
     std::vector<BoxDev*> ptrs(nb);
     for (uint32_t k = 0; k < nb; ++k) {
         soil_hip_box* b = boxes[k];
-        int rc = pull_desc(b);
-        if (rc) return rc;
-        rc = pull_state(b);
-        if (rc) return rc;
+        HIPCHK(hipStreamSynchronize(b->stream));
         bt->saved_rings.push_back(b->h_dev.geom_ring);
         bt->saved_ring_len.push_back(b->h_dev.ring_len);
         // repoint the box at the batch ring: slot j of box k is record j*nb + k
-        b->h_dev.geom_ring = bt->d_ring + k;
-        b->h_dev.ring_len = bt->K;
-        b->h_dev.ring_stride = nb;
-        rc = push_desc(b);
-        if (rc) return rc;
-        b->h_st->ring_pos = 0;
-        HIPCHK(hipMemcpy(&b->d_st->ring_pos, &b->h_st->ring_pos, 4, hipMemcpyHostToDevice));
+        const Geom* ring = bt->d_ring + k;
+        const uint32_t len = bt->K, stride = nb, zero = 0;
+        HIPCHK(hipMemcpyAsync(&b->d_dev->geom_ring, &ring, sizeof(ring), hipMemcpyHostToDevice, bt->stream));
+        HIPCHK(hipMemcpyAsync(&b->d_dev->ring_len, &len, 4, hipMemcpyHostToDevice, bt->stream));
+        HIPCHK(hipMemcpyAsync(&b->d_dev->ring_stride, &stride, 4, hipMemcpyHostToDevice, bt->stream));
+        HIPCHK(hipMemcpyAsync(&b->d_st->ring_pos, &zero, 4, hipMemcpyHostToDevice, bt->stream));
+        HIPCHK(hipStreamSynchronize(bt->stream));
+        b->h_dev.geom_ring = ring;
+        b->h_dev.ring_len = len;
+        b->h_dev.ring_stride = stride;
         b->in_batch = true;
         ptrs[k] = b->d_dev;
     }
@@ -192,14 +190,15 @@ int soil_hip_batch_create(soil_hip_ctx* ctx, soil_hip_box** boxes, int nboxes, s
     return 0;
 }
 
-int soil_hip_batch_destroy(soil_hip_batch* bt) {
+// Releases a batch. With `restore`, every box goes back to its own ring and
+// can be stepped alone again.
+int batch_free(soil_hip_batch* bt, bool restore) {
     if (!bt) return 0;
     hipStreamSynchronize(bt->stream);
     drop_graph(bt);
     for (size_t k = 0; k < bt->boxes.size(); ++k) {
         soil_hip_box* b = bt->boxes[k];
-        if (pull_desc(b) == 0 && pull_state(b) == 0) {
-            // hand the box back on its own ring, positioned after the batch's steps
+        if (restore && pull_desc(b) == 0 && pull_state(b) == 0) {
             b->h_dev.geom_ring = bt->saved_rings[k];
             b->h_dev.ring_len = bt->saved_ring_len[k];
             b->h_dev.ring_stride = 1;
@@ -225,16 +224,29 @@ int soil_hip_batch_destroy(soil_hip_batch* bt) {
     return 0;
 }
 
-int soil_hip_batch_step(soil_hip_batch* bt, int nsteps, int use_graph) {
+int batch_step(soil_hip_batch* bt, int nsteps, bool use_graph) {
     if (!bt || nsteps < 0) return set_error(SOIL_HIP_EINVAL, "bad arguments");
-    HIPCHK(hipSetDevice(bt->ctx->device));
     if (use_graph && !bt->graph_exec) {
         int rc = capture_graph(bt);
         if (rc) return rc;
     }
-    int rc = batch_run(bt, nsteps, use_graph != 0);
-    if (rc) return rc;
-    return 0;
+    return batch_run(bt, nsteps, use_graph);
+}
+
+}  // namespace soil_hip
+
+extern "C" {
+
+int soil_hip_batch_create(soil_hip_ctx* ctx, soil_hip_box** boxes, int nboxes, soil_hip_batch** out) {
+    return batch_make(ctx, boxes, nboxes, out);
+}
+
+int soil_hip_batch_destroy(soil_hip_batch* bt) { return batch_free(bt, true); }
+
+int soil_hip_batch_step(soil_hip_batch* bt, int nsteps, int use_graph) {
+    if (!bt || nsteps < 0) return set_error(SOIL_HIP_EINVAL, "bad arguments");
+    HIPCHK(hipSetDevice(bt->ctx->device));
+    return batch_step(bt, nsteps, use_graph != 0);
 }
 
 int soil_hip_batch_sync(soil_hip_batch* bt) {

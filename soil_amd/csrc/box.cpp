@@ -112,6 +112,7 @@ void carve_atoms(Carver& c, BoxDev& d, const Caps& k, const std::vector<HostCol>
     c.take(d.spos, 4 * A);
     c.take(d.bin_start, C1); c.take(d.counts, C1);
     c.take(d.scan_part, std::max(C1, A + 1) / SCAN_TILE + 2);
+    c.take(d.scell, k.local); c.take(d.sstart, C1); c.take(d.perm, k.local);
     c.take(d.offsets, L1);
     c.take(d.tcount, A + 1); c.take(d.tstart, A + 1);
     c.take(d.btile, 2 * (size_t)k.tiles + 2);
@@ -614,6 +615,7 @@ int soil_hip_setup(soil_hip_box* b) {
     st.nlocal = (uint32_t)n;
     st.state = 0;  // CurrentState(FullRebuild), neighbor.rs:545
     st.ring_pos = 0;
+    st.step_limit = ~0ull;
     std::memcpy(b->h_st, &st, sizeof(st));
     HIPCHK(hipMemcpy(b->d_st, &st, sizeof(BoxState), hipMemcpyHostToDevice));
     b->ring_pos = 0;
@@ -976,6 +978,39 @@ int soil_hip_get_stencil(soil_hip_box* b, int32_t* out, size_t cap, size_t* n) {
     if (out) {
         if (cap < s.size()) return set_error(SOIL_HIP_ECAPACITY, "stencil needs %zu", s.size());
         for (size_t i = 0; i < s.size(); ++i) out[i] = s[i];
+    }
+    return 0;
+}
+
+// Host-only: the grid and forward stencil a box with these radii gets at tilt xy.
+int soil_hip_grid_for(const soil_hip_box_desc* desc, const double* radius, size_t n, double tilt_xy, double* out21,
+                      int32_t* stencil, size_t cap, size_t* nstencil) {
+    if (!desc || !out21 || (!radius && n)) return set_error(SOIL_HIP_EINVAL, "null argument");
+    HostModel m;
+    m.init(*desc);
+    m.setup(radius, n);
+    if (tilt_xy != 0.0 || desc->has_shear) {
+        m.set_tilt(tilt_xy, 0.0);
+        m.sort_prepare();
+    }
+    for (int k = 0; k < 3; ++k) {
+        out21[k] = m.nbin[k];
+        out21[3 + k] = m.bin_size[k];
+        out21[6 + k] = m.bin_origin[k];
+        out21[12 + k] = m.tilt[k];
+        out21[15 + k] = m.bvel[k];
+    }
+    out21[9] = m.ghost_cutoff;
+    out21[10] = m.cached_max_cutoff;
+    out21[11] = m.bin_min_size;
+    out21[18] = m.cached_min_skin;
+    out21[19] = m.uniform ? 1.0 : 0.0;
+    out21[20] = m.ucut_sq;
+    const auto& s = m.newton ? m.stencil_fwd : m.stencil_full;
+    if (nstencil) *nstencil = s.size();
+    if (stencil) {
+        if (cap < s.size()) return set_error(SOIL_HIP_ECAPACITY, "stencil needs %zu", s.size());
+        for (size_t i = 0; i < s.size(); ++i) stencil[i] = s[i];
     }
     return 0;
 }

@@ -35,6 +35,7 @@ void upload_tiles(const TileTables& tt, Launch& L, void** arena);
 }  // namespace soil_hip
 
 struct soil_hip_box;
+struct soil_hip_batch;
 namespace soil_hip {
 int alloc_atoms(soil_hip_box* b, const Caps& nc, bool keep);
 int alloc_pairs(soil_hip_box* b, uint32_t cap_pairs, bool keep);
@@ -43,6 +44,9 @@ int pull_desc(soil_hip_box* b);
 int pull_state(soil_hip_box* b);
 int rebuild_tiles(soil_hip_box* b);
 int check_err_bits(uint32_t err);
+int batch_make(soil_hip_ctx* ctx, soil_hip_box** boxes, int nboxes, soil_hip_batch** out);
+int batch_free(soil_hip_batch* bt, bool restore);
+int batch_step(soil_hip_batch* bt, int nsteps, bool use_graph);
 }  // namespace soil_hip
 
 struct soil_hip_box {

@@ -73,6 +73,7 @@ struct BoxState {
     uint32_t ring_pos;                          // geometry slot of the current step
     uint32_t pad;
     uint64_t csr_bytes, csr_builds;             // algorithmic bytes of CSR builds (timing)
+    uint64_t step_limit;                        // the box stops (every kernel skips it) at this step count
 };
 
 struct ColDev {
@@ -102,6 +103,11 @@ struct BoxDev {
     uint32_t* bin_start;  // cap_cells + 1
     uint32_t* counts;     // cap_cells + 1
     uint32_t* scan_part;  // per-tile partial sums (scan scratch)
+    // spatial sort (sort_atoms_by_bin) keeps its own arrays, like the reference's
+    // sort_cell / sort_counts / sort_perm, so the last build's bins stay readable
+    uint32_t* scell;      // cap_local
+    uint32_t* sstart;     // cap_cells + 1
+    uint32_t* perm;       // cap_local
     // CSR
     uint32_t* offsets;    // cap_local + 1
     uint32_t* indices;    // cap_pairs

@@ -17,6 +17,9 @@ namespace soil_hip {
 #define FOR_TILES(t) for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x)
 #define KARGS BoxDev* const* __restrict__ boxes, const Tile* __restrict__ tiles, uint32_t ntiles
 
+// A box whose step count reached its limit is frozen: every phase skips it.
+__device__ __forceinline__ bool live(const BoxDev& b) { return b.st->steps < b.st->step_limit; }
+
 __device__ __forceinline__ const Geom& geom_of(const BoxDev& b) {
     return b.geom_ring[(size_t)b.st->ring_pos * b.ring_stride];
 }
@@ -88,6 +91,7 @@ __global__ void __launch_bounds__(BLK) k_initial(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
         const BoxDev& b = *boxes[tl.box];
+        if (!live(b)) continue;
         const uint32_t i = tl.start + threadIdx.x;
         if (i >= b.st->nlocal) continue;
         const Geom& g = geom_of(b);
@@ -109,6 +113,7 @@ __global__ void __launch_bounds__(BLK) k_final(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
         const BoxDev& b = *boxes[tl.box];
+        if (!live(b)) continue;
         const uint32_t i = tl.start + threadIdx.x;
         if (i >= b.st->nlocal) continue;
         const Geom& g = geom_of(b);
@@ -124,6 +129,7 @@ __global__ void __launch_bounds__(BLK) k_deform_init(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
         const BoxDev& b = *boxes[tl.box];
+        if (!live(b)) continue;
         const Geom& g = geom_of(b);
         if (!g.init_shear) continue;
         const uint32_t i = tl.start + threadIdx.x;
@@ -137,6 +143,7 @@ __global__ void __launch_bounds__(BLK) k_disp(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
         const BoxDev& b = *boxes[tl.box];
+        if (!live(b)) continue;
         BoxState* st = b.st;
         const Geom& g = geom_of(b);
         if (st->state != 1u || !st->last_valid || g.clear_last) continue;
@@ -176,6 +183,7 @@ __global__ void k_decide(KARGS) {
     FOR_TILES(t) {
         if (threadIdx.x != 0) continue;
         const BoxDev& b = *boxes[tiles[t].box];
+        if (!live(b)) continue;
         BoxState* st = b.st;
         const Geom& g = geom_of(b);
         if (g.clear_last) st->last_valid = 0;
@@ -196,7 +204,9 @@ __global__ void k_decide(KARGS) {
 __global__ void k_remove_ghosts(KARGS) {
     FOR_TILES(t) {
         if (threadIdx.x != 0) continue;
-        BoxState* st = boxes[tiles[t].box]->st;
+        const BoxDev& bb = *boxes[tiles[t].box];
+        if (!live(bb)) continue;
+        BoxState* st = bb.st;
         if (st->state == 0u) st->nghost = 0;
     }
 }
@@ -205,7 +215,9 @@ __global__ void k_remove_ghosts(KARGS) {
 __global__ void k_set_rebuild_now(KARGS) {
     FOR_TILES(t) {
         if (threadIdx.x != 0) continue;
-        BoxState* st = boxes[tiles[t].box]->st;
+        const BoxDev& bb = *boxes[tiles[t].box];
+        if (!live(bb)) continue;
+        BoxState* st = bb.st;
         st->rebuild_now = st->state == 0u ? 1u : 0u;
     }
 }
@@ -215,6 +227,7 @@ __global__ void __launch_bounds__(BLK) k_zero(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
         const BoxDev& b = *boxes[tl.box];
+        if (!live(b)) continue;
         const uint32_t i = tl.start + threadIdx.x;
         if (i >= b.st->nlocal + b.st->nghost) continue;
         b.fx[i] = 0.0;
@@ -231,6 +244,7 @@ __global__ void __launch_bounds__(BLK) k_pbc(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
         const BoxDev& b = *boxes[tl.box];
+        if (!live(b)) continue;
         BoxState* st = b.st;
         if (st->state != 0u) continue;
         const uint32_t i = tl.start + threadIdx.x;
@@ -288,6 +302,7 @@ __global__ void k_pre_sort(KARGS) {
     FOR_TILES(t) {
         if (threadIdx.x != 0) continue;
         const BoxDev& b = *boxes[tiles[t].box];
+        if (!live(b)) continue;
         BoxState* st = b.st;
         const Geom& g = geom_of(b);
         if (g.bounds_changed) st->state = 0u;
@@ -301,6 +316,7 @@ __global__ void k_post_sort(KARGS) {
     FOR_TILES(t) {
         if (threadIdx.x != 0) continue;
         const BoxDev& b = *boxes[tiles[t].box];
+        if (!live(b)) continue;
         BoxState* st = b.st;
         const Geom& g = geom_of(b);
         if (g.periodic_sort && st->do_sort && st->not_identity) {
@@ -328,7 +344,7 @@ __device__ __forceinline__ SortView<W> sort_view(const BoxDev& b) {
     const Geom& g = geom_of(b);
     if (W == SORT_SPATIAL) {
         v.n = st->nlocal; v.nkeys = g.ncells; v.active = st->do_sort != 0;
-        v.key = b.cell; v.slot = b.slot; v.counts = b.counts; v.start = b.bin_start; v.tmp = b.tmp; v.out = b.sorted;
+        v.key = b.scell; v.slot = b.slot; v.counts = b.counts; v.start = b.sstart; v.tmp = b.tmp; v.out = b.perm;
     } else if (W == SORT_BUILD) {
         v.n = st->nlocal + st->nghost; v.nkeys = g.ncells;
         v.active = st->rebuild_now != 0 && (st->err & (ERR_GHOST_CAPACITY)) == 0;
@@ -346,6 +362,7 @@ __global__ void __launch_bounds__(BLK) k_sort_zero(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
         const BoxDev& b = *boxes[tl.box];
+        if (!live(b)) continue;
         SortView<W> v = sort_view<W>(b);
         if (!v.active) continue;
         const uint32_t c = tl.start + threadIdx.x;
@@ -358,6 +375,7 @@ __global__ void __launch_bounds__(BLK) k_sort_count(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
         const BoxDev& b = *boxes[tl.box];
+        if (!live(b)) continue;
         SortView<W> v = sort_view<W>(b);
         if (!v.active) continue;
         const uint32_t i = tl.start + threadIdx.x;
@@ -378,6 +396,7 @@ __global__ void __launch_bounds__(BLK) k_sort_scatter(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
         const BoxDev& b = *boxes[tl.box];
+        if (!live(b)) continue;
         SortView<W> v = sort_view<W>(b);
         if (!v.active) continue;
         const uint32_t i = tl.start + threadIdx.x;
@@ -392,6 +411,7 @@ __global__ void __launch_bounds__(BLK) k_sort_rank(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
         const BoxDev& b = *boxes[tl.box];
+        if (!live(b)) continue;
         SortView<W> v = sort_view<W>(b);
         if (!v.active) continue;
         const uint32_t i = tl.start + threadIdx.x;
@@ -423,11 +443,12 @@ __global__ void __launch_bounds__(BLK) k_permute(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
         const BoxDev& b = *boxes[tl.box];
+        if (!live(b)) continue;
         const BoxState* st = b.st;
         if (!st->do_sort || !st->not_identity) continue;
         const uint32_t k = tl.start + threadIdx.x;
         if (k >= st->nlocal) continue;
-        const uint32_t p = b.sorted[k];
+        const uint32_t p = b.perm[k];
         b.ax[k] = b.x[p];
         b.ay[k] = b.y[p];
         b.az[k] = b.z[p];
@@ -462,6 +483,7 @@ __global__ void k_swap_ptrs(KARGS) {
     FOR_TILES(t) {
         if (threadIdx.x != 0) continue;
         BoxDev& b = *boxes[tiles[t].box];
+        if (!live(b)) continue;
         if (!b.st->do_sort || !b.st->not_identity) continue;
         swap_ptr(b.x, b.ax); swap_ptr(b.y, b.ay); swap_ptr(b.z, b.az);
         swap_ptr(b.vx, b.avx); swap_ptr(b.vy, b.avy); swap_ptr(b.vz, b.avz);
@@ -482,7 +504,7 @@ __device__ __forceinline__ void scan_view(const BoxDev& b, uint32_t& n, bool& ac
         active = W == SCAN_CELLS_SPATIAL ? st->do_sort != 0
                                          : (st->rebuild_now != 0 && (st->err & ERR_GHOST_CAPACITY) == 0);
         in = b.counts;
-        out = b.bin_start;
+        out = W == SCAN_CELLS_SPATIAL ? b.sstart : b.bin_start;
     } else if (W == SCAN_CSR) {
         n = st->nlocal + 1;
         active = st->rebuild_now != 0 && (st->err & ERR_GHOST_CAPACITY) == 0;
@@ -502,6 +524,7 @@ __global__ void __launch_bounds__(BLK) k_scan_reduce(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
         const BoxDev& b = *boxes[tl.box];
+        if (!live(b)) continue;
         uint32_t n;
         bool active;
         uint32_t *in, *out;
@@ -523,6 +546,7 @@ __global__ void __launch_bounds__(BLK) k_scan_parts(KARGS) {
     __shared__ uint32_t lds[BLK / 64];
     FOR_TILES(t) {
         const BoxDev& b = *boxes[tiles[t].box];
+        if (!live(b)) continue;
         uint32_t n;
         bool active;
         uint32_t *in, *out;
@@ -564,6 +588,7 @@ __global__ void __launch_bounds__(BLK) k_scan_down(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
         const BoxDev& b = *boxes[tl.box];
+        if (!live(b)) continue;
         uint32_t n;
         bool active;
         uint32_t *in, *out;
@@ -606,6 +631,7 @@ __global__ void __launch_bounds__(BLK) k_border_count(KARGS, int d) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
         const BoxDev& b = *boxes[tl.box];
+        if (!live(b)) continue;
         const BoxState* st = b.st;
         const Geom& g = geom_of(b);
         if (!st->rebuild_now || !g.periodic[d]) continue;
@@ -636,6 +662,7 @@ __global__ void __launch_bounds__(BLK) k_border_scan(KARGS, int d) {
     __shared__ uint32_t lds[BLK / 64];
     FOR_TILES(t) {
         const BoxDev& b = *boxes[tiles[t].box];
+        if (!live(b)) continue;
         BoxState* st = b.st;
         const Geom& g = geom_of(b);
         if (!st->rebuild_now) continue;
@@ -708,6 +735,7 @@ __global__ void __launch_bounds__(BLK) k_border_write(KARGS, int d) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
         const BoxDev& b = *boxes[tl.box];
+        if (!live(b)) continue;
         const BoxState* st = b.st;
         const Geom& g = geom_of(b);
         if (!st->rebuild_now || !g.periodic[d] || (st->err & ERR_GHOST_CAPACITY)) continue;
@@ -750,6 +778,7 @@ __global__ void __launch_bounds__(BLK) k_forward(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
         const BoxDev& b = *boxes[tl.box];
+        if (!live(b)) continue;
         const BoxState* st = b.st;
         if (st->rebuild_now) continue;
         const uint32_t nl = st->nlocal;
@@ -793,6 +822,7 @@ __global__ void __launch_bounds__(BLK) k_reverse(KARGS, int s) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
         const BoxDev& b = *boxes[tl.box];
+        if (!live(b)) continue;
         const BoxState* st = b.st;
         const Geom& g = geom_of(b);
         if (!g.newton || (st->err & ERR_GHOST_CAPACITY)) continue;
@@ -816,6 +846,7 @@ __global__ void k_build_prep(KARGS) {
     FOR_TILES(t) {
         if (threadIdx.x != 0) continue;
         const BoxDev& b = *boxes[tiles[t].box];
+        if (!live(b)) continue;
         BoxState* st = b.st;
         if (!st->rebuild_now) continue;
         st->state = 1u;  // CommunicateOnly
@@ -830,6 +861,7 @@ __global__ void __launch_bounds__(BLK) k_save_build(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
         const BoxDev& b = *boxes[tl.box];
+        if (!live(b)) continue;
         if (!b.st->rebuild_now) continue;
         const uint32_t i = tl.start + threadIdx.x;
         if (i >= b.st->nlocal) continue;
@@ -885,6 +917,7 @@ __global__ void __launch_bounds__(BLK) k_csr_count(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
         const BoxDev& b = *boxes[tl.box];
+        if (!live(b)) continue;
         const BoxState* st = b.st;
         if (!st->rebuild_now || (st->err & ERR_GHOST_CAPACITY)) continue;
         const uint32_t i = tl.start + threadIdx.x;
@@ -902,6 +935,7 @@ __global__ void __launch_bounds__(BLK) k_csr_fill(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
         const BoxDev& b = *boxes[tl.box];
+        if (!live(b)) continue;
         const BoxState* st = b.st;
         if (!st->rebuild_now || (st->err & (ERR_GHOST_CAPACITY | ERR_PAIR_CAPACITY))) continue;
         const uint32_t i = tl.start + threadIdx.x;
@@ -915,6 +949,7 @@ __global__ void k_csr_account(KARGS) {
     FOR_TILES(t) {
         if (threadIdx.x != 0) continue;
         const BoxDev& b = *boxes[tiles[t].box];
+        if (!live(b)) continue;
         BoxState* st = b.st;
         if (!st->rebuild_now || (st->err & (ERR_GHOST_CAPACITY | ERR_PAIR_CAPACITY))) continue;
         const Geom& g = geom_of(b);
@@ -931,6 +966,7 @@ __global__ void __launch_bounds__(BLK) k_row_of(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
         const BoxDev& b = *boxes[tl.box];
+        if (!live(b)) continue;
         const BoxState* st = b.st;
         if (!st->rebuild_now || !b.consumer || (st->err & (ERR_GHOST_CAPACITY | ERR_PAIR_CAPACITY))) continue;
         const uint32_t i = tl.start + threadIdx.x;
@@ -965,6 +1001,7 @@ __global__ void __launch_bounds__(BLK) k_harmonic(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
         const BoxDev& b = *boxes[tl.box];
+        if (!live(b)) continue;
         const BoxState* st = b.st;
         if (!b.consumer || st->rebuilds == 0 || (st->err & (ERR_GHOST_CAPACITY | ERR_PAIR_CAPACITY))) continue;
         const uint32_t a = tl.start + threadIdx.x;
@@ -999,6 +1036,7 @@ __global__ void k_step_end(KARGS) {
     FOR_TILES(t) {
         if (threadIdx.x != 0) continue;
         const BoxDev& b = *boxes[tiles[t].box];
+        if (!live(b)) continue;
         BoxState* st = b.st;
         st->steps += 1;
         st->ring_pos = (st->ring_pos + 1u) % b.ring_len;

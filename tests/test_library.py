@@ -62,3 +62,36 @@ def test_missing_device_is_an_error_not_a_fallback():
         assert e.code == _ffi.EDEVICE
     else:
         raise AssertionError("context creation must fail without a device")
+
+
+def _grid_for(desc, rad, xy):
+    import ctypes as C
+    out = np.zeros(21)
+    st = np.zeros(4096, dtype=np.int32)
+    n = C.c_size_t(0)
+    rad = np.ascontiguousarray(rad, dtype=np.float64)
+    _ffi.check(_ffi.lib().soil_hip_grid_for(C.byref(desc), rad.ctypes.data_as(_ffi.DP), rad.size, xy,
+                                            out.ctypes.data_as(_ffi.DP), st.ctypes.data_as(_ffi.I32P), st.size,
+                                            C.byref(n)))
+    return out, st[: n.value]
+
+
+def test_host_geometry_model_matches_oracle_grids():
+    # the product's host geometry (grid, ghost cut, stencil) against the oracle's, orthogonal and tilted
+    from oracle_ref import RefBox
+    for cfg, tilts in ((1, [0.0]), (4, [0.0, 0.3, -2.0, 5.5])):
+        for box_id in range(3):
+            d, pos, vel, mass, rad = soil_amd.gen_box(cfg, box_id) if cfg == 4 else soil_amd.gen_box(cfg)
+            for xy in tilts:
+                d.has_shear = 0
+                r = RefBox(d)
+                r.add_atoms(pos, mass, rad)
+                r.setup()
+                if xy != 0.0:
+                    r.set_tilt(xy, 0.0)
+                    r.phase("sort_atoms_by_bin")
+                g_ref = r.grid()
+                s_ref = r.stencil(True)
+                g, s = _grid_for(d, rad, xy)
+                assert np.array_equal(g[:12], g_ref[:12]), (cfg, box_id, xy)
+                assert np.array_equal(s, s_ref), (cfg, box_id, xy)
