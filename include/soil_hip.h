@@ -186,11 +186,44 @@ int soil_hip_get_timing(soil_hip_box* box, double* out4);
 int soil_hip_batch_set_timing(soil_hip_batch* batch, int enabled);
 int soil_hip_batch_get_timing(soil_hip_batch* batch, double* out4);
 
+/* ── served stream (continuous batching of simulation jobs) ─────────────── */
+/* A server admits jobs (one box each) against an HBM budget (GB) and a slot
+ * limit, keeps their state resident, and steps all admitted boxes together in
+ * the batched form; membership changes at chunk boundaries of at most
+ * `chunk_steps` steps. No reference counterpart: SOIL runs one App per process. */
+int soil_hip_server_create(soil_hip_ctx* ctx, double budget_gb, int max_slots, int chunk_steps, int use_graph,
+                           soil_hip_server** out);
+int soil_hip_server_start(soil_hip_server* server);
+/* Queues a job (copies the inputs); the harmonic consumer with constant k acts as its force. */
+int soil_hip_server_submit(soil_hip_server* server, const soil_hip_box_desc* desc, size_t n, const double* pos,
+                           const double* vel, const double* mass, const double* radius, int nsteps, double k,
+                           int keep_result, int* job_id);
+/* Waits for every submitted job, then stops the loop. */
+int soil_hip_server_drain(soil_hip_server* server);
+/* out20: jobs_done, jobs_failed, particle_steps, busy_s, wall_s, step_p50_ms, step_p99_ms,
+ * completion_p50_s, completion_p99_s, wait_p50_s, wait_p99_s, mean_active_jobs, reconfigs,
+ * reconfig_s, budget_gb, chunks, step_mean_ms, completion_max_s, wait_max_s, server_steps */
+int soil_hip_server_stats(soil_hip_server* server, double* out20);
+/* Final local positions/velocities of a finished job submitted with keep_result. */
+int soil_hip_server_result(soil_hip_server* server, int job_id, double* pos, double* vel, size_t cap, size_t* n);
+int soil_hip_server_destroy(soil_hip_server* server);
+/* Benchmark driver: njobs jobs of the served-stream workload (config 5) arrive
+ * as a Poisson process of `rate` jobs/s and are submitted by a client thread. */
+int soil_hip_server_bench_stream(soil_hip_ctx* ctx, int njobs, double rate, uint64_t seed, double budget_gb,
+                                 int max_slots, int chunk_steps, int use_graph, double* stats_out20);
+
 /* ── synthetic inputs (host only; SplitMix64 + Box–Muller) ─────────────── */
 /* Writes a box description and atoms for `config` (1..5) / `box_id`.
  * With pos == NULL only the desc and *n are produced (query size). */
 int soil_hip_gen_box(int config, int box_id, uint64_t seed, soil_hip_box_desc* desc, size_t* n, double* pos,
                      double* vel, double* mass, double* radius);
+/* Any box: n spheres with radii U[rmin,rmax] at packing fraction phi in a box
+ * of edge ratios aspect[3], optional Gaussian blobs, N(0, vel_sigma) velocities. */
+int soil_hip_gen_custom(uint64_t seed, size_t n, double rmin, double rmax, double phi, const double* aspect,
+                        int nblobs, double blob_frac, double sigma, double vel_sigma, double shear_rate,
+                        soil_hip_box_desc* desc, double* pos, double* vel, double* mass, double* radius);
+/* Served-stream arrivals: Poisson times (s) of rate `rate` and steps ~ U{500..5000}. */
+int soil_hip_gen_arrivals(uint64_t seed, int njobs, double rate, double* arrival_s, int32_t* steps);
 
 #ifdef __cplusplus
 }

@@ -109,11 +109,18 @@ _SIGS = {
     "soil_hip_gen_custom": ([C.c_uint64, SZ, C.c_double, C.c_double, C.c_double, DP, C.c_int, C.c_double,
                              C.c_double, C.c_double, C.c_double, C.POINTER(BoxDesc), DP, DP, DP, DP], C.c_int),
     "soil_hip_gen_arrivals": ([C.c_uint64, C.c_int, C.c_double, DP, I32P], C.c_int),
+    "soil_hip_server_create": ([P, C.c_double, C.c_int, C.c_int, C.c_int, C.POINTER(P)], C.c_int),
+    "soil_hip_server_start": ([P], C.c_int),
+    "soil_hip_server_submit": ([P, C.POINTER(BoxDesc), SZ, DP, DP, DP, DP, C.c_int, C.c_double, C.c_int,
+                                C.POINTER(C.c_int)], C.c_int),
+    "soil_hip_server_drain": ([P], C.c_int),
+    "soil_hip_server_stats": ([P, DP], C.c_int),
+    "soil_hip_server_result": ([P, C.c_int, DP, DP, SZ, SZP], C.c_int),
+    "soil_hip_server_destroy": ([P], C.c_int),
+    "soil_hip_server_bench_stream": ([P, C.c_int, C.c_double, C.c_uint64, C.c_double, C.c_int, C.c_int, C.c_int,
+                                      DP], C.c_int),
 }
 
-# Symbols of include/soil_hip.h that must be exported (checked by the CPU tests).
-HEADER_SYMBOLS = sorted(k for k in _SIGS if not k.startswith("soil_hip_gen_custom")
-                        and not k.startswith("soil_hip_gen_arrivals"))
 
 
 def lib():

@@ -17,6 +17,8 @@ CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(OUT_DIR, "libsoil_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+HOSTCXX = os.environ.get("HOSTCXX", "g++")
+ROCM_INCLUDE = "/opt/rocm/include"
 ARCH = os.environ.get("SOIL_HIP_ARCH", "gfx950")
 
 DEVICE_SOURCES = ["kernels.hip"]
@@ -26,11 +28,13 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "
           "-Wno-unused-function", "-I", os.path.join(ROOT, "include"), "-I", CSRC]
 DEVICE_FLAGS = ["-x", "hip", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
                 "-packed-fp32-ops", "-munsafe-fp-atomics"]
-HOST_FLAGS = ["-D__HIP_PLATFORM_AMD__"]
+HOST_FLAGS = ["-D__HIP_PLATFORM_AMD__", "-I", ROCM_INCLUDE, "-pthread"]
 
 
 def _compile(src, obj, device):
-    cmd = [HIPCC] + COMMON + (DEVICE_FLAGS if device else HOST_FLAGS) + ["-c", src, "-o", obj]
+    # device sources go through hipcc for gfx950; host-only sources through the host compiler
+    cc = [HIPCC] if device else [HOSTCXX]
+    cmd = cc + COMMON + (DEVICE_FLAGS if device else HOST_FLAGS) + ["-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
@@ -49,7 +53,7 @@ def build(verbose=False):
     with ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
         objs = list(ex.map(lambda j: _compile(*j), jobs))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

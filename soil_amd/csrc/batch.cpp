@@ -42,10 +42,10 @@ static void launch_step(const Launch& L) {
     phase_decide(L);
     phase_zero(L);
     phase_pbc(L);
-    phase_sort(L);
+    phase_sort_fused(L);
     phase_borders(L);
     phase_forward(L);
-    phase_build(L, nullptr, nullptr, nullptr);
+    phase_build_fused(L, nullptr, nullptr, nullptr);
     phase_force(L);
     phase_reverse(L);
     phase_final(L);
@@ -81,7 +81,7 @@ static int batch_check(soil_hip_batch* bt, int* grown) {
         if ((uint64_t)st.nghost * 10 > ghost_cap * 8) {
             Caps nc = b->caps;
             nc.atoms = (uint32_t)std::min<uint64_t>((uint64_t)b->caps.local + 2 * ghost_cap + 4096, 0x7FFFFFFFull);
-            nc.tiles = (nc.atoms + BORDER_TILE - 1) / BORDER_TILE;
+            nc.tiles = (nc.atoms + 63) / 64;
             rc = alloc_atoms(b, nc, true);
             if (rc) return rc;
             *grown = 1;

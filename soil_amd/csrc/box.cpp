@@ -107,6 +107,7 @@ void carve_atoms(Carver& c, BoxDev& d, const Caps& k, const std::vector<HostCol>
     c.take(d.atag, A); c.take(d.atype, A); c.take(d.aorigin, A);
     c.take(d.aimx, A); c.take(d.aimy, A); c.take(d.aimz, A);
     c.take(d.gswap, A);
+    c.take(d.xr, 4 * A);
     c.take(d.lbx, k.local); c.take(d.lby, k.local); c.take(d.lbz, k.local);
     c.take(d.cell, A); c.take(d.slot, A); c.take(d.tmp, A); c.take(d.sorted, A); c.take(d.sidx, A);
     c.take(d.spos, 4 * A);
@@ -115,7 +116,8 @@ void carve_atoms(Carver& c, BoxDev& d, const Caps& k, const std::vector<HostCol>
     c.take(d.scell, k.local); c.take(d.sstart, C1); c.take(d.perm, k.local);
     c.take(d.offsets, L1);
     c.take(d.tcount, A + 1); c.take(d.tstart, A + 1);
-    c.take(d.btile, 2 * (size_t)k.tiles + 2);
+    c.take(d.btile, 2 * ((size_t)k.atoms / BLK + 2));
+    c.take(d.bwave, 2 * ((size_t)k.atoms / 64 + 8));
     for (size_t i = 0; i < cols.size(); ++i) {
         c.take(d.cols[i].data, A * cols[i].width);
         c.take(d.cols[i].alt, A * cols[i].width);
@@ -191,6 +193,7 @@ int alloc_atoms(soil_hip_box* b, const Caps& nc, bool keep) {
         cp(nd.tag, od.tag, n * 4); cp(nd.type, od.type, n * 4); cp(nd.origin, od.origin, n * 4);
         cp(nd.imx, od.imx, n * 4); cp(nd.imy, od.imy, n * 4); cp(nd.imz, od.imz, n * 4);
         cp(nd.gswap, od.gswap, n);
+        cp(nd.xr, od.xr, n * 32);
         cp(nd.lbx, od.lbx, nl * 8); cp(nd.lby, od.lby, nl * 8); cp(nd.lbz, od.lbz, nl * 8);
         cp(nd.offsets, od.offsets, (nl + 1) * 4);
         cp(nd.tstart, od.tstart, (n + 1) * 4);
@@ -332,7 +335,7 @@ int rebuild_tail(soil_hip_box* b) {
         uint64_t ghost_cap = (uint64_t)(b->caps.atoms - b->caps.local);
         ghost_cap = ghost_cap * 2 + 4096;
         nc.atoms = (uint32_t)(b->caps.local + ghost_cap);
-        nc.tiles = ceil_div(nc.atoms, BORDER_TILE);
+        nc.tiles = ceil_div(nc.atoms, 64);
         rc = alloc_atoms(b, nc, true);
         if (rc) return rc;
         b->h_st->err &= ~ERR_GHOST_CAPACITY;
@@ -352,9 +355,8 @@ int rebuild_tail(soil_hip_box* b) {
     // build, split at the CSR scan so the pair capacity is known before the fill
     {
         Launch L = b->L;
-        // run the build with the fill suppressed? The kernels themselves check capacity:
-        // phase_build runs count, scan (flags capacity), fill (skipped on overflow).
-        phase_build(L, e0, e1, e2);
+        // count, scan (flags the capacity), fill (skipped on overflow)
+        phase_build_fused(L, e0, e1, e2);
     }
     int rc = pull_state(b);
     if (rc) return rc;
@@ -410,7 +412,7 @@ int step_plain(soil_hip_box* b, int nsteps) {
         phase_zero(b->L);
         if (rebuild) {
             phase_pbc(b->L);
-            phase_sort(b->L);
+            phase_sort_fused(b->L);
             rc = rebuild_tail(b);
             if (rc) return rc;
         } else {
@@ -427,6 +429,27 @@ int step_plain(soil_hip_box* b, int nsteps) {
     return 0;
 }
 
+}  // namespace soil_hip
+
+namespace soil_hip {
+// Device bytes a box will hold after setup (its arenas at their initial capacities).
+size_t box_bytes_estimate(soil_hip_box* b) {
+    HostModel m = b->model;
+    m.setup(b->hrad.data(), b->nlocal);
+    Caps c;
+    c.local = (uint32_t)b->nlocal;
+    const uint64_t ghost = m.ghost_estimate(b->nlocal);
+    c.atoms = (uint32_t)std::min<uint64_t>(b->nlocal + ghost + ghost / 2 + 1024, 0x7FFFFFFFull);
+    c.cells = (uint32_t)m.max_cells();
+    c.tiles = ceil_div(c.atoms, 64);
+    c.pairs = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(8 * (uint64_t)b->nlocal, 4096), 0xFFFFFFF0u);
+    BoxDev d{};
+    Carver a{nullptr, 0, true};
+    carve_atoms(a, d, c, b->cols);
+    Carver p{nullptr, 0, true};
+    carve_pairs(p, d, c);
+    return a.off + p.off + sizeof(Geom) * b->ring_len + 4096;
+}
 }  // namespace soil_hip
 
 // ─────────────────────────────── C ABI ─────────────────────────────────────
@@ -569,7 +592,7 @@ int soil_hip_setup(soil_hip_box* b) {
     const uint64_t ghost = b->model.ghost_estimate(n);
     c.atoms = (uint32_t)std::min<uint64_t>(n + ghost + ghost / 2 + 1024, 0x7FFFFFFFull);
     c.cells = (uint32_t)b->model.max_cells();
-    c.tiles = ceil_div(c.atoms, BORDER_TILE);
+    c.tiles = ceil_div(c.atoms, 64);
     b->h_dev = BoxDev{};
     b->h_dev.ncols = (int32_t)b->cols.size();
     for (size_t i = 0; i < b->cols.size(); ++i) {
@@ -605,6 +628,16 @@ int soil_hip_setup(soil_hip_box* b) {
     HIPCHK(up(d.imx, b->himx.data(), n * 4)); HIPCHK(up(d.imy, b->himy.data(), n * 4)); HIPCHK(up(d.imz, b->himz.data(), n * 4));
     for (size_t i = 0; i < b->cols.size(); ++i)
         HIPCHK(up(d.cols[i].data, b->cols[i].data.data(), n * b->cols[i].width * 8));
+    {
+        std::vector<double> xr(4 * n);
+        for (size_t i = 0; i < n; ++i) {
+            xr[4 * i] = b->hx[i];
+            xr[4 * i + 1] = b->hy[i];
+            xr[4 * i + 2] = b->hz[i];
+            xr[4 * i + 3] = b->hrad[i];
+        }
+        HIPCHK(up(d.xr, xr.data(), n * 32));
+    }
     d.st = b->d_st;
     d.geom_ring = b->d_ring;
     d.ring_len = b->ring_len;

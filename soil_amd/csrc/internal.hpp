@@ -92,6 +92,7 @@ struct BoxDev {
     uint32_t *atag, *atype;
     int32_t *aorigin, *aimx, *aimy, *aimz;
     uint8_t* gswap;
+    double* xr;           // AoS mirror (x, y, z, rad) per atom for neighbor gathers
     double *lbx, *lby, *lbz;
     // sort / bin scratch
     uint32_t* cell;       // per atom cell (locals: spatial sort result reused by the build)
@@ -119,7 +120,8 @@ struct BoxDev {
     uint32_t* trow;       // cap_pairs
     uint32_t* row_of;     // cap_pairs
     // border compaction scratch: per tile counts of the two swaps
-    uint32_t* btile;      // 2 * max_tiles
+    uint32_t* btile;      // 2 per tile of BLK atoms (swap totals, then exclusive prefixes)
+    uint32_t* bwave;      // 2 per wave of 64 atoms (prefix inside the tile)
     ColDev cols[SOIL_MAX_COLS];
     int32_t ncols;
     uint32_t cap_local, cap_atoms, cap_cells, cap_pairs, cap_tiles;

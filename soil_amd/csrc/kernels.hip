@@ -14,7 +14,24 @@
 
 namespace soil_hip {
 
-#define FOR_TILES(t) for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x)
+// Tile iteration. Blocks are dispatched round-robin over the 8 XCDs. Each
+// window of gridDim.x consecutive tiles is cut into 8 contiguous chunks, one
+// per XCD (block b takes chunk b % 8), so the neighbor cells of a tile sit in
+// the same XCD's L2 while every XCD still gets work when only a prefix of a
+// capacity-sized tile table is active.
+struct TileIter {
+    uint32_t t, hi, step;
+    __device__ __forceinline__ explicit TileIter(uint32_t n) {
+        const uint32_t g = gridDim.x;
+        hi = n;
+        step = g;
+        if ((g & 7u) == 0u) t = (blockIdx.x & 7u) * (g >> 3) + (blockIdx.x >> 3);
+        else t = blockIdx.x;
+    }
+};
+#define FOR_TILES(t) \
+    for (TileIter it_(ntiles); it_.t < it_.hi; it_.t += it_.step) \
+        if (const uint32_t t = it_.t; true)
 #define KARGS BoxDev* const* __restrict__ boxes, const Tile* __restrict__ tiles, uint32_t ntiles
 
 // A box whose step count reached its limit is frozen: every phase skips it.
@@ -57,6 +74,24 @@ __device__ __forceinline__ uint32_t cell_of(const Geom& g, double x, double y, d
     int cy = cell_coord((py - g.bin_origin[1]) * g.inv_bin[1], g.nbin[1]);
     int cz = cell_coord((pz - g.bin_origin[2]) * g.inv_bin[2], g.nbin[2]);
     return (uint32_t)(cx * g.nbin[1] * g.nbin[2] + cy * g.nbin[2] + cz);
+}
+
+// Sets *flag to 1 if any active lane of the wave has pred; one atomic per wave
+// at most, and none once the flag is already set.
+__device__ __forceinline__ void wave_flag(uint32_t* flag, bool pred) {
+    const uint64_t m = __ballot(pred);
+    if (m == 0ull) return;
+    const uint64_t act = __ballot(1);
+    if ((uint32_t)__lane_id() == (uint32_t)__builtin_ctzll(act)) {
+        if (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) atomicOr(flag, 1u);
+    }
+}
+
+__device__ __forceinline__ void put_xr(const BoxDev& b, uint32_t i, double x, double y, double z) {
+    double* p = b.xr + 4 * (size_t)i;
+    p[0] = x;
+    p[1] = y;
+    p[2] = z;
 }
 
 __device__ __forceinline__ uint32_t lane_prefix(uint64_t mask) {
@@ -103,9 +138,13 @@ __global__ void __launch_bounds__(BLK) k_initial(KARGS) {
         b.vx[i] = v0;
         b.vy[i] = v1;
         b.vz[i] = v2;
-        b.x[i] = b.x[i] + v0 * dt;
-        b.y[i] = b.y[i] + v1 * dt;
-        b.z[i] = b.z[i] + v2 * dt;
+        const double x = b.x[i] + v0 * dt;
+        const double y = b.y[i] + v1 * dt;
+        const double z = b.z[i] + v2 * dt;
+        b.x[i] = x;
+        b.y[i] = y;
+        b.z[i] = z;
+        put_xr(b, i, x, y, z);
     }
 }
 
@@ -174,11 +213,12 @@ __global__ void __launch_bounds__(BLK) k_disp(KARGS) {
             }
             ex = dx * dx + dy * dy + dz * dz > g.disp_thr_sq;
         }
-        if (__ballot(ex) != 0ull && (threadIdx.x & 63u) == 0u) atomicOr(&st->exceed, 1u);
+        wave_flag(&st->exceed, ex);
     }
 }
 
-// decide_rebuild + remove_ghost_atoms (one thread per box)
+// decide_rebuild (+ remove_ghost_atoms when REMOVE), one thread per box
+template <bool REMOVE>
 __global__ void k_decide(KARGS) {
     FOR_TILES(t) {
         if (threadIdx.x != 0) continue;
@@ -197,6 +237,7 @@ __global__ void k_decide(KARGS) {
             else st->steps_since_build += 1;
         }
         st->exceed = 0;
+        if (REMOVE && st->state == 0u) st->nghost = 0;
     }
 }
 
@@ -270,6 +311,7 @@ __global__ void __launch_bounds__(BLK) k_pbc(KARGS) {
             b.x[i] = x;
             b.y[i] = y;
             b.z[i] = z;
+            put_xr(b, i, x, y, z);
             b.imx[i] += delta[0];
             b.imy[i] += delta[1];
             b.imz[i] += delta[2];
@@ -294,6 +336,7 @@ __global__ void __launch_bounds__(BLK) k_pbc(KARGS) {
                 atomicOr(&st->err, ERR_ATOM_LEFT_BOX);
             }
         }
+        put_xr(b, i, b.x[i], b.y[i], b.z[i]);
     }
 }
 
@@ -311,21 +354,12 @@ __global__ void k_pre_sort(KARGS) {
     }
 }
 
-// after the sort: a periodic sort that moved atoms forces a rebuild (neighbor.rs:989-996)
-__global__ void k_post_sort(KARGS) {
-    FOR_TILES(t) {
-        if (threadIdx.x != 0) continue;
-        const BoxDev& b = *boxes[tiles[t].box];
-        if (!live(b)) continue;
-        BoxState* st = b.st;
-        const Geom& g = geom_of(b);
-        if (g.periodic_sort && st->do_sort && st->not_identity) {
-            st->last_valid = 0;
-            st->state = 0u;
-        }
-        st->rebuild_now = st->state == 0u ? 1u : 0u;
-    }
-}
+// after the sort: a periodic sort that moved atoms forces a rebuild
+// (neighbor.rs:989-996). FUSED also swaps the permuted column buffers and does
+// the build's bookkeeping (bin_neighbor_list sets CommunicateOnly and resets the
+// rebuild counters, neighbor.rs:1023-1024 / 669-690).
+template <bool FUSED>
+__global__ void k_post_sort(KARGS);
 
 // ── stable counting sort by key ────────────────────────────────────────────
 // SPATIAL: locals by cell (perm into `sorted`), BUILD: locals+ghosts by cell,
@@ -379,15 +413,36 @@ __global__ void __launch_bounds__(BLK) k_sort_count(KARGS) {
         SortView<W> v = sort_view<W>(b);
         if (!v.active) continue;
         const uint32_t i = tl.start + threadIdx.x;
-        if (i >= v.n) continue;
-        uint32_t key;
         if (W == SORT_TRANSPOSE) {
-            key = v.key[i];
-        } else {
+            if (i >= v.n) continue;
+            const uint32_t key = v.key[i];
+            v.slot[i] = atomicAdd(&v.counts[key], 1u);
+            continue;
+        }
+        // Cell sorts: the input is nearly cell-ordered (locals were sorted at the
+        // last rebuild, ghosts follow their sources), so lanes form runs of equal
+        // keys; one atomic per run hands out consecutive slots.
+        const bool act = i < v.n;
+        uint32_t key = 0xffffffffu;
+        if (act) {
             key = cell_of(geom_of(b), b.x[i], b.y[i], b.z[i]);
             v.key[i] = key;
         }
-        v.slot[i] = atomicAdd(&v.counts[key], 1u);
+        const uint32_t lane = __lane_id();
+        const uint32_t prev = __shfl_up(key, 1, 64);
+        const bool head = act && (lane == 0u || prev != key);
+        const uint64_t heads = __ballot(head);
+        const uint32_t nact = (uint32_t)__popcll(__ballot(act));
+        const uint64_t upto = lane == 63u ? ~0ull : ((2ull << lane) - 1ull);
+        const uint32_t leader = 63u - (uint32_t)__builtin_clzll((heads & upto) | 1ull);
+        uint32_t base = 0;
+        if (head) {
+            const uint64_t above = heads & ~((2ull << lane) - 1ull);
+            const uint32_t next = (lane == 63u || above == 0ull) ? nact : (uint32_t)__builtin_ctzll(above);
+            base = atomicAdd(&v.counts[key], next - lane);
+        }
+        base = __shfl(base, leader, 64);
+        if (act) v.slot[i] = base + (lane - leader);
     }
 }
 
@@ -432,7 +487,7 @@ __global__ void __launch_bounds__(BLK) k_sort_rank(KARGS) {
             sp[3] = b.rad[i];
         }
         if (W == SORT_SPATIAL) {
-            if (__ballot(dst != i) != 0ull && (threadIdx.x & 63u) == 0u) atomicOr(&b.st->not_identity, 1u);
+            wave_flag(&b.st->not_identity, dst != i);
         }
     }
 }
@@ -449,15 +504,23 @@ __global__ void __launch_bounds__(BLK) k_permute(KARGS) {
         const uint32_t k = tl.start + threadIdx.x;
         if (k >= st->nlocal) continue;
         const uint32_t p = b.perm[k];
-        b.ax[k] = b.x[p];
-        b.ay[k] = b.y[p];
-        b.az[k] = b.z[p];
+        const double px = b.x[p], py = b.y[p], pz = b.z[p], pr = b.rad[p];
+        b.ax[k] = px;
+        b.ay[k] = py;
+        b.az[k] = pz;
+        {
+            double* q = b.xr + 4 * (size_t)k;
+            q[0] = px;
+            q[1] = py;
+            q[2] = pz;
+            q[3] = pr;
+        }
         b.avx[k] = b.vx[p];
         b.avy[k] = b.vy[p];
         b.avz[k] = b.vz[p];
         b.amass[k] = b.mass[p];
         b.ainv_mass[k] = b.inv_mass[p];
-        b.arad[k] = b.rad[p];
+        b.arad[k] = pr;
         b.atag[k] = b.tag[p];
         b.atype[k] = b.type[p];
         b.aorigin[k] = b.origin[p];
@@ -492,6 +555,37 @@ __global__ void k_swap_ptrs(KARGS) {
         swap_ptr(b.imx, b.aimx); swap_ptr(b.imy, b.aimy); swap_ptr(b.imz, b.aimz);
         for (int c = 0; c < b.ncols; ++c)
             if (!(b.cols[c].flags & 4u)) swap_ptr(b.cols[c].data, b.cols[c].alt);
+    }
+}
+
+template <bool FUSED>
+__global__ void k_post_sort(KARGS) {
+    FOR_TILES(t) {
+        if (threadIdx.x != 0) continue;
+        BoxDev& b = *boxes[tiles[t].box];
+        if (!live(b)) continue;
+        BoxState* st = b.st;
+        const Geom& g = geom_of(b);
+        if (FUSED && st->do_sort && st->not_identity) {
+            swap_ptr(b.x, b.ax); swap_ptr(b.y, b.ay); swap_ptr(b.z, b.az);
+            swap_ptr(b.vx, b.avx); swap_ptr(b.vy, b.avy); swap_ptr(b.vz, b.avz);
+            swap_ptr(b.mass, b.amass); swap_ptr(b.inv_mass, b.ainv_mass); swap_ptr(b.rad, b.arad);
+            swap_ptr(b.tag, b.atag); swap_ptr(b.type, b.atype); swap_ptr(b.origin, b.aorigin);
+            swap_ptr(b.imx, b.aimx); swap_ptr(b.imy, b.aimy); swap_ptr(b.imz, b.aimz);
+            for (int c = 0; c < b.ncols; ++c)
+                if (!(b.cols[c].flags & 4u)) swap_ptr(b.cols[c].data, b.cols[c].alt);
+        }
+        if (g.periodic_sort && st->do_sort && st->not_identity) {
+            st->last_valid = 0;
+            st->state = 0u;
+        }
+        st->rebuild_now = st->state == 0u ? 1u : 0u;
+        if (FUSED && st->rebuild_now) {
+            st->state = 1u;
+            st->steps_since_build = 0;
+            st->last_valid = 1;
+            st->rebuilds += 1;
+        }
     }
 }
 
@@ -576,7 +670,16 @@ __global__ void __launch_bounds__(BLK) k_scan_parts(KARGS) {
             BoxState* st = b.st;
             if (W == SCAN_CSR) {
                 st->npairs = carry;
-                if (carry > b.cap_pairs) atomicOr(&st->err, ERR_PAIR_CAPACITY);
+                if (carry > b.cap_pairs) {
+                    atomicOr(&st->err, ERR_PAIR_CAPACITY);
+                } else {
+                    const Geom& g = geom_of(b);
+                    const uint64_t na = (uint64_t)st->nlocal + st->nghost;
+                    const uint64_t per = 24u + (g.uniform ? 0u : 8u) + 4u;
+                    st->csr_bytes += per * na + 4u * ((uint64_t)g.ncells + 1u) + 4u * ((uint64_t)st->nlocal + 1u) +
+                                     4u * (uint64_t)carry;
+                    st->csr_builds += 1;
+                }
             }
         }
     }
@@ -626,8 +729,10 @@ __device__ __forceinline__ void border_preds(const Geom& g, const BoxDev& b, uin
     p1 = v >= g.thr_hi[d];
 }
 
+// Per-wave (64 consecutive atoms) counts of the two swaps' selections, and
+// per-tile (BLK atoms) totals for the scan.
 __global__ void __launch_bounds__(BLK) k_border_count(KARGS, int d) {
-    __shared__ uint32_t lds[BLK / 64];
+    __shared__ uint32_t wc[2][BLK / 64];
     FOR_TILES(t) {
         const Tile tl = tiles[t];
         const BoxDev& b = *boxes[tl.box];
@@ -637,27 +742,32 @@ __global__ void __launch_bounds__(BLK) k_border_count(KARGS, int d) {
         if (!st->rebuild_now || !g.periodic[d]) continue;
         const uint32_t scan_end = st->nlocal + (d == 0 ? 0u : st->nghost);
         if (tl.start >= scan_end) continue;
-        uint32_t c0 = 0, c1 = 0;
-        for (int r = 0; r < BORDER_TILE / BLK; ++r) {
-            const uint32_t i = tl.start + r * BLK + threadIdx.x;
-            if (i < scan_end) {
-                bool p0, p1;
-                border_preds(g, b, i, d, p0, p1);
-                c0 += p0;
-                c1 += p1;
+        const uint32_t i = tl.start + threadIdx.x;
+        bool p0 = false, p1 = false;
+        if (i < scan_end) border_preds(g, b, i, d, p0, p1);
+        const uint64_t m0 = __ballot(p0), m1 = __ballot(p1);
+        const uint32_t wid = threadIdx.x >> 6;
+        if ((threadIdx.x & 63u) == 0u) {
+            wc[0][wid] = (uint32_t)__popcll(m0);
+            wc[1][wid] = (uint32_t)__popcll(m1);
+        }
+        __syncthreads();
+        if (threadIdx.x < 2u) {
+            const uint32_t k = threadIdx.x;
+            uint32_t run = 0;
+            const uint32_t w0 = tl.start >> 6;
+#pragma unroll
+            for (uint32_t w = 0; w < BLK / 64; ++w) {
+                b.bwave[2 * (w0 + w) + k] = run;  // exclusive prefix of the wave inside its tile
+                run += wc[k][w];
             }
+            b.btile[2 * (tl.start / BLK) + k] = run;
         }
-        uint32_t tot0, tot1;
-        block_excl_scan(c0, lds, tot0);
-        block_excl_scan(c1, lds, tot1);
-        if (threadIdx.x == 0) {
-            const uint32_t tix = tl.start / BORDER_TILE;
-            b.btile[2 * tix] = tot0;
-            b.btile[2 * tix + 1] = tot1;
-        }
+        __syncthreads();
     }
 }
 
+// Exclusive scan of the per-tile totals (one block per box); sets the swap table.
 __global__ void __launch_bounds__(BLK) k_border_scan(KARGS, int d) {
     __shared__ uint32_t lds[BLK / 64];
     FOR_TILES(t) {
@@ -670,17 +780,29 @@ __global__ void __launch_bounds__(BLK) k_border_scan(KARGS, int d) {
         const uint32_t scan_end = st->nlocal + nghost0;
         uint32_t c0 = 0, c1 = 0;
         if (g.periodic[d] && !(st->err & ERR_GHOST_CAPACITY)) {
-            const uint32_t ntl = (scan_end + BORDER_TILE - 1) / BORDER_TILE;
-            for (uint32_t base = 0; base < ntl; base += BLK) {
-                const uint32_t k = base + threadIdx.x;
-                const uint32_t v0 = k < ntl ? b.btile[2 * k] : 0u;
-                const uint32_t v1 = k < ntl ? b.btile[2 * k + 1] : 0u;
+            const uint32_t nt = (scan_end + BLK - 1) / BLK;
+            constexpr int PER = 8;
+            for (uint32_t base = 0; base < nt; base += BLK * PER) {
+                uint32_t v0[PER], v1[PER], s0 = 0, s1 = 0;
+                const uint32_t k0 = base + threadIdx.x * PER;
+#pragma unroll
+                for (int q = 0; q < PER; ++q) {
+                    v0[q] = k0 + q < nt ? b.btile[2 * (k0 + q)] : 0u;
+                    v1[q] = k0 + q < nt ? b.btile[2 * (k0 + q) + 1] : 0u;
+                    s0 += v0[q];
+                    s1 += v1[q];
+                }
                 uint32_t tot0, tot1;
-                const uint32_t e0 = block_excl_scan(v0, lds, tot0) + c0;
-                const uint32_t e1 = block_excl_scan(v1, lds, tot1) + c1;
-                if (k < ntl) {
-                    b.btile[2 * k] = e0;
-                    b.btile[2 * k + 1] = e1;
+                uint32_t e0 = block_excl_scan(s0, lds, tot0) + c0;
+                uint32_t e1 = block_excl_scan(s1, lds, tot1) + c1;
+#pragma unroll
+                for (int q = 0; q < PER; ++q) {
+                    if (k0 + q < nt) {
+                        b.btile[2 * (k0 + q)] = e0;
+                        b.btile[2 * (k0 + q) + 1] = e1;
+                    }
+                    e0 += v0[q];
+                    e1 += v1[q];
                 }
                 c0 += tot0;
                 c1 += tot1;
@@ -704,9 +826,19 @@ __global__ void __launch_bounds__(BLK) k_border_scan(KARGS, int d) {
 
 // pack_border + unpack_atom (atom.rs:525-571): ghost g is a copy of atom i shifted by swap s
 __device__ __forceinline__ void write_ghost(const BoxDev& b, const Geom& g, uint32_t dst, uint32_t i, int s) {
-    b.x[dst] = b.x[i] + g.swap_shift[s][0];
-    b.y[dst] = b.y[i] + g.swap_shift[s][1];
-    b.z[dst] = b.z[i] + g.swap_shift[s][2];
+    const double gx = b.x[i] + g.swap_shift[s][0];
+    const double gy = b.y[i] + g.swap_shift[s][1];
+    const double gz = b.z[i] + g.swap_shift[s][2];
+    b.x[dst] = gx;
+    b.y[dst] = gy;
+    b.z[dst] = gz;
+    {
+        double* q = b.xr + 4 * (size_t)dst;
+        q[0] = gx;
+        q[1] = gy;
+        q[2] = gz;
+        q[3] = b.rad[i];
+    }
     b.vx[dst] = b.vx[i] + g.swap_vshift[s][0];
     b.vy[dst] = b.vy[i] + g.swap_vshift[s][1];
     b.vz[dst] = b.vz[i] + g.swap_vshift[s][2];
@@ -730,8 +862,9 @@ __device__ __forceinline__ void write_ghost(const BoxDev& b, const Geom& g, uint
     }
 }
 
+// Ordered append: each wave writes its selected atoms at its scanned offset
+// plus the lane's rank among the selected lanes (scan order is preserved).
 __global__ void __launch_bounds__(BLK) k_border_write(KARGS, int d) {
-    __shared__ uint32_t wc[2][BLK / 64];
     FOR_TILES(t) {
         const Tile tl = tiles[t];
         const BoxDev& b = *boxes[tl.box];
@@ -741,33 +874,17 @@ __global__ void __launch_bounds__(BLK) k_border_write(KARGS, int d) {
         if (!st->rebuild_now || !g.periodic[d] || (st->err & ERR_GHOST_CAPACITY)) continue;
         const uint32_t scan_end = st->swap_start[2 * d];
         if (tl.start >= scan_end) continue;
-        const uint32_t tix = tl.start / BORDER_TILE;
-        uint32_t run0 = st->swap_start[2 * d] + b.btile[2 * tix];
-        uint32_t run1 = st->swap_start[2 * d + 1] + b.btile[2 * tix + 1];
-        const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
-        for (int r = 0; r < BORDER_TILE / BLK; ++r) {
-            const uint32_t i = tl.start + r * BLK + threadIdx.x;
-            bool p0 = false, p1 = false;
-            if (i < scan_end) border_preds(g, b, i, d, p0, p1);
-            const uint64_t m0 = __ballot(p0), m1 = __ballot(p1);
-            if (lane == 0) {
-                wc[0][wid] = (uint32_t)__popcll(m0);
-                wc[1][wid] = (uint32_t)__popcll(m1);
-            }
-            __syncthreads();
-            uint32_t o0 = run0, o1 = run1;
-            for (uint32_t w = 0; w < wid; ++w) {
-                o0 += wc[0][w];
-                o1 += wc[1][w];
-            }
-            if (p0) write_ghost(b, g, o0 + lane_prefix(m0), i, 2 * d);
-            if (p1) write_ghost(b, g, o1 + lane_prefix(m1), i, 2 * d + 1);
-            for (uint32_t w = 0; w < BLK / 64; ++w) {
-                run0 += wc[0][w];
-                run1 += wc[1][w];
-            }
-            __syncthreads();
-        }
+        const uint32_t i = tl.start + threadIdx.x;
+        bool p0 = false, p1 = false;
+        if (i < scan_end) border_preds(g, b, i, d, p0, p1);
+        const uint64_t m0 = __ballot(p0), m1 = __ballot(p1);
+        if (!(p0 || p1)) continue;
+        const uint32_t w = i >> 6, tix = i / BLK;
+        if (p0)
+            write_ghost(b, g, st->swap_start[2 * d] + b.btile[2 * tix] + b.bwave[2 * w] + lane_prefix(m0), i, 2 * d);
+        if (p1)
+            write_ghost(b, g, st->swap_start[2 * d + 1] + b.btile[2 * tix + 1] + b.bwave[2 * w + 1] + lane_prefix(m1),
+                        i, 2 * d + 1);
     }
 }
 
@@ -806,6 +923,7 @@ __global__ void __launch_bounds__(BLK) k_forward(KARGS) {
         b.x[gi] = px;
         b.y[gi] = py;
         b.z[gi] = pz;
+        put_xr(b, gi, px, py, pz);
         b.vx[gi] = qx;
         b.vy[gi] = qy;
         b.vz[gi] = qz;
@@ -871,25 +989,71 @@ __global__ void __launch_bounds__(BLK) k_save_build(KARGS) {
     }
 }
 
-template <bool FILL>
-__device__ __forceinline__ uint32_t csr_row(const BoxDev& b, const Geom& g, uint32_t i, uint32_t* out) {
-    const uint32_t c = b.cell[i];
-    const double xi = b.x[i], yi = b.y[i], zi = b.z[i], si = b.rad[i];
-    const bool uniform = g.uniform != 0;
+// One CSR row (neighbor.rs:1175-1304): the self cell after atom i, then the
+// forward stencil cells in stencil order, each cell in bin order. The cell
+// ranges of up to CSR_RK stencil slots sit in a per-thread LDS table and the
+// candidates of all of them are walked as one flat sequence, so a wave runs
+// max(per-lane candidates) iterations instead of a sum of per-cell maxima.
+constexpr int CSR_RK = 16;
+
+template <bool FILL, bool UNIFORM>
+__device__ __forceinline__ uint32_t csr_row(const BoxDev& b, const Geom& g, uint32_t i, bool act, uint32_t* out,
+                                            uint2 (*rng)[BLK]) {
+    const uint32_t tid = threadIdx.x;
+    uint32_t c = 0;
+    double xi = 0.0, yi = 0.0, zi = 0.0, si = 0.0;
+    if (act) {
+        c = b.cell[i];
+        const double4 pi = ((const double4*)b.xr)[i];
+        xi = pi.x; yi = pi.y; zi = pi.z; si = pi.w;
+    }
     const double ucut = g.ucut_sq, sfsq = g.sf_sq;
+    const double4* __restrict__ sp4 = (const double4*)b.spos;
+    const uint32_t skip = g.newton ? 0xffffffffu : i;
+    const int total = (g.has_self ? 1 : 0) + g.nstencil;
+    const uint32_t ncell = g.ncells;
     uint32_t n = 0;
-    auto scan = [&](uint32_t m0, uint32_t m1, uint32_t skip) {
-        for (uint32_t m = m0; m < m1; ++m) {
-            const double* sp = b.spos + 4 * (size_t)m;
-            const double dx = sp[0] - xi;
-            const double dy = sp[1] - yi;
-            const double dz = sp[2] - zi;
+    for (int kb = 0; kb < total; kb += CSR_RK) {
+        const int cnt = act ? (total - kb < CSR_RK ? total - kb : CSR_RK) : 0;
+#pragma unroll
+        for (int q = 0; q < CSR_RK; ++q) {
+            uint32_t rs = 0u, re = 0u;
+            if (q < cnt) {
+                const int k = kb + q - (g.has_self ? 1 : 0);
+                if (k < 0) {  // self cell
+                    rs = g.newton ? b.sidx[i] + 1 : b.bin_start[c];
+                    re = b.bin_start[c + 1];
+                } else {
+                    const uint32_t cc = (uint32_t)((int32_t)c + g.stencil[k]);
+                    if (cc < ncell) {  // only atoms far outside the box reach past the grid
+                        rs = b.bin_start[cc];
+                        re = b.bin_start[cc + 1];
+                    }
+                }
+            }
+            rng[q][tid] = make_uint2(rs, re);
+        }
+        int q = 0;
+        uint2 r = rng[0][tid];
+        uint32_t m = r.x, e = r.y;
+        while (true) {
+            while (m >= e) {
+                if (++q >= cnt) break;
+                r = rng[q][tid];
+                m = r.x;
+                e = r.y;
+            }
+            if (q >= cnt) break;
+            const double4 p = sp4[m];
+            const double dx = p.x - xi;
+            const double dy = p.y - yi;
+            const double dz = p.z - zi;
             const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
             bool acc;
-            if (uniform) {
+            if (UNIFORM) {
                 acc = r2 < ucut;
             } else {
-                const double sum = si + sp[3];
+                const double sum = si + p.w;
                 acc = r2 < sum * sum * sfsq;
             }
             if (acc) {
@@ -899,21 +1063,14 @@ __device__ __forceinline__ uint32_t csr_row(const BoxDev& b, const Geom& g, uint
                     ++n;
                 }
             }
+            ++m;
         }
-    };
-    if (g.has_self) {
-        if (g.newton) scan(b.sidx[i] + 1, b.bin_start[c + 1], 0xffffffffu);
-        else scan(b.bin_start[c], b.bin_start[c + 1], i);
-    }
-    for (int k = 0; k < g.nstencil; ++k) {
-        const uint32_t cc = (uint32_t)((int32_t)c + g.stencil[k]);
-        if (cc >= g.ncells) continue;  // only reachable by atoms far outside the box
-        scan(b.bin_start[cc], b.bin_start[cc + 1], 0xffffffffu);
     }
     return n;
 }
 
 __global__ void __launch_bounds__(BLK) k_csr_count(KARGS) {
+    __shared__ uint2 rng[CSR_RK][BLK];
     FOR_TILES(t) {
         const Tile tl = tiles[t];
         const BoxDev& b = *boxes[tl.box];
@@ -927,11 +1084,14 @@ __global__ void __launch_bounds__(BLK) k_csr_count(KARGS) {
             b.offsets[i] = 0u;
             continue;
         }
-        b.offsets[i] = csr_row<false>(b, geom_of(b), i, nullptr);
+        const Geom& g = geom_of(b);
+        b.offsets[i] = g.uniform ? csr_row<false, true>(b, g, i, true, nullptr, rng)
+                                 : csr_row<false, false>(b, g, i, true, nullptr, rng);
     }
 }
 
 __global__ void __launch_bounds__(BLK) k_csr_fill(KARGS) {
+    __shared__ uint2 rng[CSR_RK][BLK];
     FOR_TILES(t) {
         const Tile tl = tiles[t];
         const BoxDev& b = *boxes[tl.box];
@@ -940,24 +1100,10 @@ __global__ void __launch_bounds__(BLK) k_csr_fill(KARGS) {
         if (!st->rebuild_now || (st->err & (ERR_GHOST_CAPACITY | ERR_PAIR_CAPACITY))) continue;
         const uint32_t i = tl.start + threadIdx.x;
         if (i >= st->nlocal) continue;
-        csr_row<true>(b, geom_of(b), i, b.indices + b.offsets[i]);
-    }
-}
-
-// algorithmic bytes of this build's CSR phase, accumulated on device
-__global__ void k_csr_account(KARGS) {
-    FOR_TILES(t) {
-        if (threadIdx.x != 0) continue;
-        const BoxDev& b = *boxes[tiles[t].box];
-        if (!live(b)) continue;
-        BoxState* st = b.st;
-        if (!st->rebuild_now || (st->err & (ERR_GHOST_CAPACITY | ERR_PAIR_CAPACITY))) continue;
         const Geom& g = geom_of(b);
-        const uint64_t na = (uint64_t)st->nlocal + st->nghost;
-        const uint64_t per = 24u + (g.uniform ? 0u : 8u) + 4u;
-        st->csr_bytes += per * na + 4u * ((uint64_t)g.ncells + 1u) + 4u * ((uint64_t)st->nlocal + 1u) +
-                         4u * (uint64_t)st->npairs;
-        st->csr_builds += 1;
+        uint32_t* out = b.indices + b.offsets[i];
+        if (g.uniform) csr_row<true, true>(b, g, i, true, out, rng);
+        else csr_row<true, false>(b, g, i, true, out, rng);
     }
 }
 
@@ -977,13 +1123,13 @@ __global__ void __launch_bounds__(BLK) k_row_of(KARGS) {
 
 // Harmonic soft-sphere test consumer (not reference code): deterministic
 // gather, -F(a,j) over a's own row then +F(i,a) over entries (i,a) ascending.
-__device__ __forceinline__ void pair_force(const BoxDev& b, uint32_t i, uint32_t j, double k, double& fx, double& fy,
+__device__ __forceinline__ void pair_force(const double4& pi, const double4& pj, double k, double& fx, double& fy,
                                            double& fz) {
-    const double dx = b.x[j] - b.x[i];
-    const double dy = b.y[j] - b.y[i];
-    const double dz = b.z[j] - b.z[i];
+    const double dx = pj.x - pi.x;
+    const double dy = pj.y - pi.y;
+    const double dz = pj.z - pi.z;
     const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-    const double rs = b.rad[i] + b.rad[j];
+    const double rs = pi.w + pj.w;
     if (r2 < rs * rs && r2 > 0.0) {
         const double dist = sqrt(r2);
         const double s = k * (rs - dist) / dist;
@@ -1008,11 +1154,13 @@ __global__ void __launch_bounds__(BLK) k_harmonic(KARGS) {
         const uint32_t nl = st->nlocal;
         if (a >= nl + st->nghost) continue;
         const double k = b.harmonic_k;
+        const double4* xr = (const double4*)b.xr;
+        const double4 pa = xr[a];
         double ax = 0.0, ay = 0.0, az = 0.0;
         if (a < nl) {
             for (uint32_t e = b.offsets[a]; e < b.offsets[a + 1]; ++e) {
                 double fx, fy, fz;
-                pair_force(b, a, b.indices[e], k, fx, fy, fz);
+                pair_force(pa, xr[b.indices[e]], k, fx, fy, fz);
                 ax = ax - fx;
                 ay = ay - fy;
                 az = az - fz;
@@ -1021,7 +1169,7 @@ __global__ void __launch_bounds__(BLK) k_harmonic(KARGS) {
         for (uint32_t q = b.tstart[a]; q < b.tstart[a + 1]; ++q) {
             const uint32_t e = b.trow[q];
             double fx, fy, fz;
-            pair_force(b, b.row_of[e], a, k, fx, fy, fz);
+            pair_force(xr[b.row_of[e]], pa, k, fx, fy, fz);
             ax = ax + fx;
             ay = ay + fy;
             az = az + fz;
@@ -1043,12 +1191,26 @@ __global__ void k_step_end(KARGS) {
     }
 }
 
+// Copies each box's error bits and list sizes to out[4*k..]: err, nghost, npairs, steps
+__global__ void k_collect(KARGS, uint32_t* out) {
+    FOR_TILES(t) {
+        if (threadIdx.x != 0) continue;
+        const uint32_t k = tiles[t].box;
+        const BoxState* st = boxes[k]->st;
+        out[4 * k] = st->err;
+        out[4 * k + 1] = st->nghost;
+        out[4 * k + 2] = st->npairs;
+        out[4 * k + 3] = (uint32_t)st->steps;
+    }
+}
+
 // ── launch helpers ─────────────────────────────────────────────────────────
 template <typename... A>
 static void launch(void (*k)(BoxDev* const*, const Tile*, uint32_t, A...), const TileSet& ts, BoxDev* const* boxes,
                    hipStream_t s, uint32_t maxgrid, A... args) {
     if (ts.n == 0) return;
-    const uint32_t grid = ts.n < maxgrid ? ts.n : maxgrid;
+    uint32_t grid = ts.n < maxgrid ? ts.n : maxgrid;
+    if (grid >= 8) grid = (grid + 7u) & ~7u;
     hipLaunchKernelGGL(k, dim3(grid), dim3(BLK), 0, s, boxes, (const Tile*)ts.d, ts.n, args...);
 }
 
@@ -1072,18 +1234,17 @@ void phase_deform(const Launch& L) { launch(k_deform_init, L.t_local, L.boxes, L
 void phase_initial(const Launch& L) { launch(k_initial, L.t_local, L.boxes, L.stream, L.maxgrid); }
 void phase_decide(const Launch& L) {
     launch(k_disp, L.t_local, L.boxes, L.stream, L.maxgrid);
-    launch(k_decide, L.t_box, L.boxes, L.stream, L.maxgrid);
-    launch(k_remove_ghosts, L.t_box, L.boxes, L.stream, L.maxgrid);
+    launch(k_decide<true>, L.t_box, L.boxes, L.stream, L.maxgrid);
 }
 void phase_decide_only(const Launch& L) {
     launch(k_disp, L.t_local, L.boxes, L.stream, L.maxgrid);
-    launch(k_decide, L.t_box, L.boxes, L.stream, L.maxgrid);
+    launch(k_decide<false>, L.t_box, L.boxes, L.stream, L.maxgrid);
 }
 void phase_remove_ghosts(const Launch& L) { launch(k_remove_ghosts, L.t_box, L.boxes, L.stream, L.maxgrid); }
 void phase_set_rebuild_now(const Launch& L) { launch(k_set_rebuild_now, L.t_box, L.boxes, L.stream, L.maxgrid); }
 void phase_sort_flags(const Launch& L) {
     launch(k_pre_sort, L.t_box, L.boxes, L.stream, L.maxgrid);
-    launch(k_post_sort, L.t_box, L.boxes, L.stream, L.maxgrid);
+    launch(k_post_sort<false>, L.t_box, L.boxes, L.stream, L.maxgrid);
 }
 void phase_zero(const Launch& L) { launch(k_zero, L.t_atom, L.boxes, L.stream, L.maxgrid); }
 void phase_pbc(const Launch& L) { launch(k_pbc, L.t_local, L.boxes, L.stream, L.maxgrid); }
@@ -1092,18 +1253,29 @@ void phase_sort(const Launch& L) {
     run_sort<SORT_SPATIAL, SCAN_CELLS_SPATIAL>(L, L.t_cell, L.t_local, L.t_scan_cell);
     launch(k_permute, L.t_local, L.boxes, L.stream, L.maxgrid);
     launch(k_swap_ptrs, L.t_box, L.boxes, L.stream, L.maxgrid);
-    launch(k_post_sort, L.t_box, L.boxes, L.stream, L.maxgrid);
+    launch(k_post_sort<false>, L.t_box, L.boxes, L.stream, L.maxgrid);
+}
+// sort + permute + (when rebuilding) the build's bookkeeping in one per-box kernel
+void phase_sort_fused(const Launch& L) {
+    launch(k_pre_sort, L.t_box, L.boxes, L.stream, L.maxgrid);
+    run_sort<SORT_SPATIAL, SCAN_CELLS_SPATIAL>(L, L.t_cell, L.t_local, L.t_scan_cell);
+    launch(k_permute, L.t_local, L.boxes, L.stream, L.maxgrid);
+    launch(k_post_sort<true>, L.t_box, L.boxes, L.stream, L.maxgrid);
 }
 void phase_borders(const Launch& L) {
     for (int d = 0; d < 3; ++d) {
-        launch(k_border_count, L.t_border, L.boxes, L.stream, L.maxgrid, d);
+        launch(k_border_count, L.t_atom, L.boxes, L.stream, L.maxgrid, d);
         launch(k_border_scan, L.t_box, L.boxes, L.stream, L.maxgrid, d);
-        launch(k_border_write, L.t_border, L.boxes, L.stream, L.maxgrid, d);
+        launch(k_border_write, L.t_atom, L.boxes, L.stream, L.maxgrid, d);
     }
 }
 void phase_forward(const Launch& L) { launch(k_forward, L.t_ghost, L.boxes, L.stream, L.maxgrid); }
 void phase_build(const Launch& L, hipEvent_t ev0, hipEvent_t ev1, hipEvent_t ev2) {
     launch(k_build_prep, L.t_box, L.boxes, L.stream, L.maxgrid);
+    phase_build_fused(L, ev0, ev1, ev2);
+}
+// the build after phase_sort_fused (which already did the bookkeeping)
+void phase_build_fused(const Launch& L, hipEvent_t ev0, hipEvent_t ev1, hipEvent_t ev2) {
     launch(k_save_build, L.t_local, L.boxes, L.stream, L.maxgrid);
     run_sort<SORT_BUILD, SCAN_CELLS_BUILD>(L, L.t_cell, L.t_atom, L.t_scan_cell);
     if (ev0) (void)hipEventRecord(ev0, L.stream);
@@ -1112,7 +1284,6 @@ void phase_build(const Launch& L, hipEvent_t ev0, hipEvent_t ev1, hipEvent_t ev2
     if (ev1) (void)hipEventRecord(ev1, L.stream);
     launch(k_csr_fill, L.t_local, L.boxes, L.stream, L.maxgrid);
     if (ev2) (void)hipEventRecord(ev2, L.stream);
-    launch(k_csr_account, L.t_box, L.boxes, L.stream, L.maxgrid);
     // consumer gather lists (transpose of the half list)
     run_sort<SORT_TRANSPOSE, SCAN_TRANSPOSE>(L, L.t_atom1, L.t_pair, L.t_scan_atom);
     launch(k_row_of, L.t_local, L.boxes, L.stream, L.maxgrid);
@@ -1125,5 +1296,6 @@ void phase_reverse(const Launch& L) {
 }
 void phase_final(const Launch& L) { launch(k_final, L.t_local, L.boxes, L.stream, L.maxgrid); }
 void phase_step_end(const Launch& L) { launch(k_step_end, L.t_box, L.boxes, L.stream, L.maxgrid); }
+void phase_collect(const Launch& L, uint32_t* out) { launch(k_collect, L.t_box, L.boxes, L.stream, L.maxgrid, out); }
 
 }  // namespace soil_hip

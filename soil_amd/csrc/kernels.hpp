@@ -42,12 +42,15 @@ void phase_sort_flags(const Launch& L);
 void phase_zero(const Launch& L);
 void phase_pbc(const Launch& L);
 void phase_sort(const Launch& L);
+void phase_sort_fused(const Launch& L);
 void phase_borders(const Launch& L);
 void phase_forward(const Launch& L);
 void phase_build(const Launch& L, hipEvent_t ev0, hipEvent_t ev1, hipEvent_t ev2);
+void phase_build_fused(const Launch& L, hipEvent_t ev0, hipEvent_t ev1, hipEvent_t ev2);
 void phase_force(const Launch& L);
 void phase_reverse(const Launch& L);
 void phase_final(const Launch& L);
 void phase_step_end(const Launch& L);
+void phase_collect(const Launch& L, uint32_t* out);
 
 }  // namespace soil_hip

@@ -1,0 +1,510 @@
+// Served stream: simulation jobs (one box each) arrive at any time, are
+// admitted against an HBM budget and a slot limit, stay resident on the
+// device, and step together in the batched form (one launch per phase for all
+// admitted boxes). Composition changes at chunk boundaries: a job that reaches
+// its step count freezes on device (its step limit) and is retired at the end
+// of the chunk; queued jobs that fit join the batch there. New boxes are set
+// up (allocated and uploaded) on the host thread while the current chunk runs
+// on the device.
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "box.hpp"
+
+using namespace soil_hip;
+
+#define HIPCHK(call)                                                                                \
+    do {                                                                                            \
+        hipError_t e_ = (call);                                                                     \
+        if (e_ != hipSuccess) return set_error(SOIL_HIP_EDEVICE, "%s: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+namespace soil_hip {
+size_t box_bytes_estimate(soil_hip_box* b);
+}
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct Job {
+    int id = 0;
+    soil_hip_box_desc desc{};
+    std::vector<double> pos, vel, mass, rad;
+    int steps = 0;
+    double k = 1000.0;
+    bool keep_result = false;
+    double t_arrive = 0, t_admit = 0, t_done = 0;
+    soil_hip_box* box = nullptr;
+    size_t bytes = 0;
+    int done_steps = 0;  // steps this job has taken on the device
+    bool failed = false;
+    std::vector<double> out_pos, out_vel;
+    size_t n = 0;
+};
+
+double pct(std::vector<double> v, double p) {
+    if (v.empty()) return 0.0;
+    std::sort(v.begin(), v.end());
+    const double idx = p * (double)(v.size() - 1);
+    const size_t lo = (size_t)std::floor(idx), hi = std::min(lo + 1, v.size() - 1);
+    return v[lo] + (v[hi] - v[lo]) * (idx - (double)lo);
+}
+
+}  // namespace
+
+struct soil_hip_server {
+    soil_hip_ctx* ctx = nullptr;
+    size_t budget = 0;
+    int max_slots = 256;
+    int chunk = 16;
+    bool use_graph = true;
+    Clock::time_point t0;
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<std::unique_ptr<Job>> pending;
+    std::vector<std::unique_ptr<Job>> active;   // in the batch (or ready to join)
+    std::vector<std::unique_ptr<Job>> done;
+    soil_hip_batch* batch = nullptr;
+    uint64_t batch_steps = 0;                   // steps run by the current batch
+    uint64_t server_steps = 0;
+    size_t used = 0;
+    int next_id = 0;
+    std::thread worker;
+    std::atomic<bool> running{false};
+    std::atomic<bool> draining{false};
+    int error = 0;
+    std::string error_msg;
+    // statistics
+    std::vector<double> step_ms;                // one sample per (active job, step)
+    double particle_steps = 0.0;
+    double busy_s = 0.0;
+    uint64_t reconfigs = 0;
+    double reconfig_s = 0.0;
+    double active_sum = 0.0;
+    uint64_t chunks = 0;
+
+    double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
+};
+
+namespace {
+
+int make_job_box(soil_hip_server* sv, Job* j) {
+    soil_hip_box* b = nullptr;
+    int rc = soil_hip_box_create(sv->ctx, &j->desc, &b);
+    if (rc) return rc;
+    rc = soil_hip_upload(b, j->n, j->pos.data(), j->vel.data(), nullptr, j->mass.data(), j->rad.data(), nullptr,
+                         nullptr, nullptr);
+    if (!rc) rc = soil_hip_set_consumer(b, 1, j->k);
+    if (rc) {
+        soil_hip_box_destroy(b);
+        return rc;
+    }
+    j->bytes = box_bytes_estimate(b);
+    j->box = b;
+    return 0;
+}
+
+// Finishes jobs whose step count is reached: copies out results, frees boxes.
+// The batch is released first (its membership changes and the boxes it points
+// at are about to be destroyed); the loop builds the next one.
+int retire(soil_hip_server* sv) {
+    bool any = false;
+    for (auto& j : sv->active) any = any || j->failed || j->done_steps >= j->steps;
+    if (!any) return 0;
+    if (sv->batch) {
+        batch_free(sv->batch, false);
+        sv->batch = nullptr;
+    }
+    int err = 0;
+    std::vector<std::unique_ptr<Job>> keep;
+    for (auto& j : sv->active) {
+        if (!j->failed && j->done_steps < j->steps) {
+            keep.push_back(std::move(j));
+            continue;
+        }
+        j->t_done = sv->now();
+        if (j->keep_result && !j->failed) {
+            uint64_t c[8];
+            int rc = soil_hip_get_counts(j->box, c);
+            const size_t na = rc ? 0 : c[0] + c[1];
+            std::vector<double> p(3 * na), v(3 * na);
+            if (!rc)
+                rc = soil_hip_get_atoms(j->box, na, p.data(), v.data(), nullptr, nullptr, nullptr, nullptr, nullptr,
+                                        nullptr, nullptr);
+            if (rc) {
+                j->failed = true;
+                err = rc;
+            } else {
+                j->out_pos.assign(p.begin(), p.begin() + 3 * j->n);
+                j->out_vel.assign(v.begin(), v.begin() + 3 * j->n);
+            }
+        }
+        sv->used -= j->bytes;
+        soil_hip_box_destroy(j->box);
+        j->box = nullptr;
+        std::lock_guard<std::mutex> lk(sv->mu);
+        sv->done.push_back(std::move(j));
+    }
+    sv->active.swap(keep);
+    return err;
+}
+
+// Takes queued jobs that fit the budget and the slot table, in arrival order,
+// and sets their boxes up on the device (allocation + upload on the box's own
+// stream, so it overlaps a running chunk).
+void admit(soil_hip_server* sv, std::vector<std::unique_ptr<Job>>& incoming) {
+    while (true) {
+        std::unique_ptr<Job> j;
+        {
+            std::lock_guard<std::mutex> lk(sv->mu);
+            if (sv->pending.empty() || (int)(sv->active.size() + incoming.size()) >= sv->max_slots) return;
+            Job* f = sv->pending.front().get();
+            if (!f->box && make_job_box(sv, f)) {
+                f->failed = true;
+                f->t_done = sv->now();
+                sv->done.push_back(std::move(sv->pending.front()));
+                sv->pending.pop_front();
+                continue;
+            }
+            if (sv->used + f->bytes > sv->budget) return;  // FIFO: wait for memory to free up
+            sv->used += f->bytes;
+            j = std::move(sv->pending.front());
+            sv->pending.pop_front();
+        }
+        j->t_admit = sv->now();
+        int rc = soil_hip_setup(j->box);
+        if (!rc) {
+            const uint64_t limit = (uint64_t)j->steps;  // the box freezes on device after its own steps
+            if (hipMemcpyAsync(&j->box->d_st->step_limit, &limit, 8, hipMemcpyHostToDevice, j->box->stream) !=
+                    hipSuccess ||
+                hipStreamSynchronize(j->box->stream) != hipSuccess)
+                rc = SOIL_HIP_EDEVICE;
+        }
+        if (rc) {
+            j->failed = true;
+            j->t_done = sv->now();
+            sv->used -= j->bytes;
+            soil_hip_box_destroy(j->box);
+            j->box = nullptr;
+            std::lock_guard<std::mutex> lk(sv->mu);
+            sv->done.push_back(std::move(j));
+            continue;
+        }
+        incoming.push_back(std::move(j));
+    }
+}
+
+int rebatch(soil_hip_server* sv) {
+    auto t = Clock::now();
+    if (sv->batch) {
+        batch_free(sv->batch, false);
+        sv->batch = nullptr;
+    }
+    std::vector<soil_hip_box*> boxes;
+    for (auto& j : sv->active) boxes.push_back(j->box);
+    if (!boxes.empty()) {
+        int rc = batch_make(sv->ctx, boxes.data(), (int)boxes.size(), &sv->batch);
+        if (rc) return rc;
+    }
+    sv->reconfigs += 1;
+    sv->reconfig_s += std::chrono::duration<double>(Clock::now() - t).count();
+    return 0;
+}
+
+bool same_members(soil_hip_server* sv) {
+    const size_t nb = sv->batch ? sv->batch->boxes.size() : 0;
+    if (nb != sv->active.size()) return false;
+    for (size_t k = 0; k < nb; ++k)
+        if (sv->batch->boxes[k] != sv->active[k]->box) return false;
+    return true;
+}
+
+void fail(soil_hip_server* sv, int rc) {
+    sv->error = rc;
+    sv->error_msg = soil_hip_last_error();
+}
+
+void loop(soil_hip_server* sv) {
+    hipSetDevice(sv->ctx->device);
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    uint32_t* d_col = nullptr;
+    uint32_t* h_col = nullptr;
+    size_t col_cap = 0;
+    std::vector<std::unique_ptr<Job>> incoming;
+    while (true) {
+        int rc = retire(sv);
+        if (rc) { fail(sv, rc); break; }
+        for (auto& j : incoming) sv->active.push_back(std::move(j));
+        incoming.clear();
+        if (sv->active.empty()) {
+            admit(sv, incoming);
+            if (!incoming.empty()) continue;
+            std::unique_lock<std::mutex> lk(sv->mu);
+            if (sv->pending.empty() && (sv->draining.load() || !sv->running.load())) break;
+            sv->cv.wait_for(lk, std::chrono::milliseconds(1));
+            continue;
+        }
+        if (!same_members(sv)) {
+            rc = rebatch(sv);
+            if (rc) { fail(sv, rc); break; }
+            const size_t need = 4 * sv->active.size();
+            if (need > col_cap) {
+                if (d_col) hipFree(d_col);
+                if (h_col) hipHostFree(h_col);
+                col_cap = need * 2;
+                hipMalloc(&d_col, col_cap * 4);
+                hipHostMalloc(&h_col, col_cap * 4, hipHostMallocDefault);
+            }
+        }
+        // a chunk ends when the first active job reaches its step count
+        int left = 1 << 30;
+        double nparticles = 0.0;
+        for (auto& j : sv->active) {
+            left = std::min(left, j->steps - j->done_steps);
+            nparticles += (double)j->n;
+        }
+        const int n = std::max(1, std::min(sv->chunk, left));
+        hipEventRecord(e0, sv->batch->stream);
+        rc = batch_step(sv->batch, n, sv->use_graph);
+        if (rc) { fail(sv, rc); break; }
+        phase_collect(sv->batch->L, d_col);
+        hipMemcpyAsync(h_col, d_col, 4 * sv->active.size() * 4, hipMemcpyDeviceToHost, sv->batch->stream);
+        hipEventRecord(e1, sv->batch->stream);
+        // overlaps the chunk: admit queued jobs and set their boxes up
+        admit(sv, incoming);
+        hipEventSynchronize(e1);
+        sv->batch->half_pending[0] = sv->batch->half_pending[1] = false;
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, e0, e1);
+        sv->busy_s += ms * 1e-3;
+        sv->server_steps += (uint64_t)n;
+        sv->chunks += 1;
+        sv->active_sum += (double)sv->active.size();
+        sv->particle_steps += nparticles * n;
+        const double per = ms / n;
+        for (size_t k = 0; k < sv->active.size(); ++k) {
+            Job* j = sv->active[k].get();
+            j->done_steps = std::min(j->steps, j->done_steps + n);
+            for (int q = 0; q < n; ++q) sv->step_ms.push_back(per);
+            if (h_col[4 * k]) {  // device error bits: the job's results are invalid
+                j->failed = true;
+                check_err_bits(h_col[4 * k]);
+            }
+            if (h_col[4 * k + 3] != (uint32_t)j->done_steps) j->failed = true;  // device/host step drift
+        }
+    }
+    for (auto& j : incoming) sv->active.push_back(std::move(j));
+    if (sv->batch) {
+        batch_free(sv->batch, false);
+        sv->batch = nullptr;
+    }
+    if (d_col) hipFree(d_col);
+    if (h_col) hipHostFree(h_col);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+}
+
+}  // namespace
+
+extern "C" {
+
+int soil_hip_server_create(soil_hip_ctx* ctx, double budget_gb, int max_slots, int chunk_steps, int use_graph,
+                           soil_hip_server** out) {
+    if (!ctx || !out || budget_gb <= 0 || max_slots < 1 || chunk_steps < 1)
+        return set_error(SOIL_HIP_EINVAL, "bad arguments");
+    soil_hip_server* sv = new soil_hip_server();
+    sv->ctx = ctx;
+    sv->budget = (size_t)(budget_gb * 1e9);
+    sv->max_slots = max_slots;
+    sv->chunk = chunk_steps;
+    sv->use_graph = use_graph != 0;
+    sv->t0 = Clock::now();
+    *out = sv;
+    return 0;
+}
+
+int soil_hip_server_start(soil_hip_server* sv) {
+    if (!sv) return set_error(SOIL_HIP_EINVAL, "null server");
+    if (sv->running.load()) return set_error(SOIL_HIP_ESTATE, "already running");
+    sv->running = true;
+    sv->draining = false;
+    sv->worker = std::thread(loop, sv);
+    return 0;
+}
+
+int soil_hip_server_submit(soil_hip_server* sv, const soil_hip_box_desc* desc, size_t n, const double* pos,
+                           const double* vel, const double* mass, const double* radius, int nsteps, double k,
+                           int keep_result, int* job_id) {
+    if (!sv || !desc || !pos || !mass || !radius || nsteps < 1) return set_error(SOIL_HIP_EINVAL, "bad arguments");
+    auto j = std::make_unique<Job>();
+    j->desc = *desc;
+    j->n = n;
+    j->pos.assign(pos, pos + 3 * n);
+    j->vel = vel ? std::vector<double>(vel, vel + 3 * n) : std::vector<double>(3 * n, 0.0);
+    j->mass.assign(mass, mass + n);
+    j->rad.assign(radius, radius + n);
+    j->steps = nsteps;
+    j->k = k;
+    j->keep_result = keep_result != 0;
+    std::lock_guard<std::mutex> lk(sv->mu);
+    j->t_arrive = sv->now();
+    j->id = sv->next_id++;
+    if (job_id) *job_id = j->id;
+    sv->pending.push_back(std::move(j));
+    sv->cv.notify_one();
+    return 0;
+}
+
+// Waits until every submitted job is finished, then stops the loop.
+int soil_hip_server_drain(soil_hip_server* sv) {
+    if (!sv) return set_error(SOIL_HIP_EINVAL, "null server");
+    sv->draining = true;
+    sv->cv.notify_one();
+    if (sv->worker.joinable()) sv->worker.join();
+    sv->running = false;
+    if (sv->error) return set_error(sv->error, "%s", sv->error_msg.c_str());
+    return 0;
+}
+
+// out[0..19]: jobs_done, jobs_failed, particle_steps, busy_s, wall_s, step_p50_ms, step_p99_ms,
+// completion_p50_s, completion_p99_s, wait_p50_s, wait_p99_s, mean_active_jobs, reconfigs,
+// reconfig_s, peak_bytes_budget_gb, chunks, step_mean_ms, completion_max_s, wait_max_s, server_steps
+int soil_hip_server_stats(soil_hip_server* sv, double* out) {
+    if (!sv || !out) return set_error(SOIL_HIP_EINVAL, "null argument");
+    std::lock_guard<std::mutex> lk(sv->mu);
+    std::vector<double> comp, wait;
+    int failed = 0;
+    double tmax = 0.0, tmin = 1e300;
+    for (auto& j : sv->done) {
+        if (j->failed) {
+            ++failed;
+            continue;
+        }
+        comp.push_back(j->t_done - j->t_arrive);
+        wait.push_back(j->t_admit - j->t_arrive);
+        tmax = std::max(tmax, j->t_done);
+        tmin = std::min(tmin, j->t_arrive);
+    }
+    std::memset(out, 0, 20 * sizeof(double));
+    out[0] = (double)comp.size();
+    out[1] = failed;
+    out[2] = sv->particle_steps;
+    out[3] = sv->busy_s;
+    out[4] = comp.empty() ? 0.0 : tmax - tmin;
+    out[5] = pct(sv->step_ms, 0.50);
+    out[6] = pct(sv->step_ms, 0.99);
+    out[7] = pct(comp, 0.50);
+    out[8] = pct(comp, 0.99);
+    out[9] = pct(wait, 0.50);
+    out[10] = pct(wait, 0.99);
+    out[11] = sv->chunks ? sv->active_sum / (double)sv->chunks : 0.0;
+    out[12] = (double)sv->reconfigs;
+    out[13] = sv->reconfig_s;
+    out[14] = (double)sv->budget / 1e9;
+    out[15] = (double)sv->chunks;
+    double sum = 0.0;
+    for (double v : sv->step_ms) sum += v;
+    out[16] = sv->step_ms.empty() ? 0.0 : sum / (double)sv->step_ms.size();
+    out[17] = comp.empty() ? 0.0 : *std::max_element(comp.begin(), comp.end());
+    out[18] = wait.empty() ? 0.0 : *std::max_element(wait.begin(), wait.end());
+    out[19] = (double)sv->server_steps;
+    return 0;
+}
+
+// Final local positions/velocities (AoS-of-3) of a finished job submitted with keep_result.
+int soil_hip_server_result(soil_hip_server* sv, int job_id, double* pos, double* vel, size_t cap, size_t* n) {
+    if (!sv) return set_error(SOIL_HIP_EINVAL, "null server");
+    std::lock_guard<std::mutex> lk(sv->mu);
+    for (auto& j : sv->done) {
+        if (j->id != job_id) continue;
+        if (j->failed) return set_error(SOIL_HIP_EDEVICE, "job %d failed", job_id);
+        if (!j->keep_result) return set_error(SOIL_HIP_ESTATE, "job %d was submitted without keep_result", job_id);
+        if (n) *n = j->n;
+        if (cap < j->n) return set_error(SOIL_HIP_ECAPACITY, "result needs %zu atoms", j->n);
+        if (pos) std::memcpy(pos, j->out_pos.data(), j->out_pos.size() * 8);
+        if (vel) std::memcpy(vel, j->out_vel.data(), j->out_vel.size() * 8);
+        return 0;
+    }
+    return set_error(SOIL_HIP_ESTATE, "job %d not finished", job_id);
+}
+
+int soil_hip_server_destroy(soil_hip_server* sv) {
+    if (!sv) return 0;
+    if (sv->worker.joinable()) {
+        sv->draining = true;
+        sv->running = false;
+        sv->cv.notify_one();
+        sv->worker.join();
+    }
+    for (auto& j : sv->active)
+        if (j->box) soil_hip_box_destroy(j->box);
+    for (auto& j : sv->pending)
+        if (j->box) soil_hip_box_destroy(j->box);
+    delete sv;
+    return 0;
+}
+
+// Benchmark driver: `njobs` jobs of the served-stream workload arrive as a
+// Poisson process of rate `rate` (jobs/s); a client thread submits each at its
+// arrival time while the server runs. Inputs are generated before the clock
+// starts. Returns the server statistics (see soil_hip_server_stats).
+int soil_hip_server_bench_stream(soil_hip_ctx* ctx, int njobs, double rate, uint64_t seed, double budget_gb,
+                                 int max_slots, int chunk_steps, int use_graph, double* stats_out) {
+    if (!ctx || njobs < 1 || rate <= 0 || !stats_out) return set_error(SOIL_HIP_EINVAL, "bad arguments");
+    std::vector<double> arrival(njobs);
+    std::vector<int32_t> steps(njobs);
+    int rc = soil_hip_gen_arrivals(seed, njobs, rate, arrival.data(), steps.data());
+    if (rc) return rc;
+    struct In {
+        soil_hip_box_desc d;
+        size_t n;
+        std::vector<double> pos, vel, mass, rad;
+    };
+    std::vector<In> ins(njobs);
+    for (int j = 0; j < njobs; ++j) {
+        In& in = ins[j];
+        rc = soil_hip_gen_box(5, j, seed, &in.d, &in.n, nullptr, nullptr, nullptr, nullptr);
+        if (rc) return rc;
+        in.pos.resize(3 * in.n);
+        in.vel.resize(3 * in.n);
+        in.mass.resize(in.n);
+        in.rad.resize(in.n);
+        rc = soil_hip_gen_box(5, j, seed, &in.d, &in.n, in.pos.data(), in.vel.data(), in.mass.data(), in.rad.data());
+        if (rc) return rc;
+    }
+    soil_hip_server* sv = nullptr;
+    rc = soil_hip_server_create(ctx, budget_gb, max_slots, chunk_steps, use_graph, &sv);
+    if (rc) return rc;
+    rc = soil_hip_server_start(sv);
+    if (rc) return rc;
+    std::thread client([&]() {
+        const auto start = Clock::now();
+        for (int j = 0; j < njobs; ++j) {
+            std::this_thread::sleep_until(start + std::chrono::duration_cast<Clock::duration>(
+                                                      std::chrono::duration<double>(arrival[j])));
+            const In& in = ins[j];
+            soil_hip_server_submit(sv, &in.d, in.n, in.pos.data(), in.vel.data(), in.mass.data(), in.rad.data(),
+                                   steps[j], 1000.0, 0, nullptr);
+        }
+    });
+    client.join();
+    rc = soil_hip_server_drain(sv);
+    int rc2 = soil_hip_server_stats(sv, stats_out);
+    soil_hip_server_destroy(sv);
+    return rc ? rc : rc2;
+}
+
+}  // extern "C"
