@@ -172,6 +172,8 @@ def main():
     ap.add_argument("--parts", default="plain,batched,cpu")
     ap.add_argument("--batched-steps", type=int, default=200)
     ap.add_argument("--plain-ensemble-steps", type=int, default=10)
+    ap.add_argument("--probe-copy-mb", type=int, default=0,
+                    help="also run the calibration copy kernel over this many MB (traffic counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each box or job shards "
@@ -204,6 +206,13 @@ def main():
             "parallelism": "1 GPU",
         },
     }
+    if args.probe_copy_mb:
+        import ctypes as C
+        ms = C.c_double(0.0)
+        soil_amd._ffi.check(soil_amd._ffi.lib().soil_hip_probe_copy(ctx.h, args.probe_copy_mb * 1000000, 20,
+                                                                   C.byref(ms)))
+        out["probe_copy"] = {"bytes": args.probe_copy_mb * 1000000, "ms": ms.value,
+                             "GBps": 2 * args.probe_copy_mb * 1e6 / (ms.value * 1e-3) / 1e9}
     if "plain" in parts:
         log("plain: 1M box")
         res, d = run_plain(ctx, args.steps, args.warmup)

@@ -212,6 +212,10 @@ int soil_hip_server_destroy(soil_hip_server* server);
 int soil_hip_server_bench_stream(soil_hip_ctx* ctx, int njobs, double rate, uint64_t seed, double budget_gb,
                                  int max_slots, int chunk_steps, int use_graph, double* stats_out20);
 
+/* Streams `bytes` through a plain copy kernel `reps` times; *ms = time per
+ * launch. Calibrates HBM traffic counters against a known byte count. */
+int soil_hip_probe_copy(soil_hip_ctx* ctx, size_t bytes, int reps, double* ms);
+
 /* ── synthetic inputs (host only; SplitMix64 + Box–Muller) ─────────────── */
 /* Writes a box description and atoms for `config` (1..5) / `box_id`.
  * With pos == NULL only the desc and *n are produced (query size). */

@@ -109,6 +109,7 @@ _SIGS = {
     "soil_hip_gen_custom": ([C.c_uint64, SZ, C.c_double, C.c_double, C.c_double, DP, C.c_int, C.c_double,
                              C.c_double, C.c_double, C.c_double, C.POINTER(BoxDesc), DP, DP, DP, DP], C.c_int),
     "soil_hip_gen_arrivals": ([C.c_uint64, C.c_int, C.c_double, DP, I32P], C.c_int),
+    "soil_hip_probe_copy": ([P, SZ, C.c_int, DP], C.c_int),
     "soil_hip_server_create": ([P, C.c_double, C.c_int, C.c_int, C.c_int, C.POINTER(P)], C.c_int),
     "soil_hip_server_start": ([P], C.c_int),
     "soil_hip_server_submit": ([P, C.POINTER(BoxDesc), SZ, DP, DP, DP, DP, C.c_int, C.c_double, C.c_int,

@@ -9,6 +9,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <cstdlib>
 
 using namespace soil_hip;
 
@@ -116,6 +117,7 @@ void carve_atoms(Carver& c, BoxDev& d, const Caps& k, const std::vector<HostCol>
     c.take(d.scell, k.local); c.take(d.sstart, C1); c.take(d.perm, k.local);
     c.take(d.offsets, L1);
     c.take(d.tcount, A + 1); c.take(d.tstart, A + 1);
+    c.take(d.ctile, (size_t)k.local / BLK + 2);
     c.take(d.btile, 2 * ((size_t)k.atoms / BLK + 2));
     c.take(d.bwave, 2 * ((size_t)k.atoms / 64 + 8));
     for (size_t i = 0; i < cols.size(); ++i) {
@@ -161,6 +163,7 @@ int rebuild_tiles(soil_hip_box* b) {
     upload_tiles(tt, b->L, &b->arena_tiles);
     b->L.boxes = b->d_ptrs;
     b->L.stream = b->stream;
+    b->L.csr_mode = b->h_dev.csr_mode;
     return 0;
 }
 
@@ -644,6 +647,11 @@ int soil_hip_setup(soil_hip_box* b) {
     d.ring_stride = 1;
     d.harmonic_k = b->harmonic_k;
     d.consumer = b->consumer ? 1 : 0;
+    {
+        const char* m = getenv("SOIL_HIP_CSR_MODE");
+        d.csr_mode = m ? atoi(m) : 2;
+        if (d.csr_mode < 0 || d.csr_mode > 2) d.csr_mode = 2;
+    }
     BoxState st{};
     st.nlocal = (uint32_t)n;
     st.state = 0;  // CurrentState(FullRebuild), neighbor.rs:545
@@ -1045,6 +1053,33 @@ int soil_hip_grid_for(const soil_hip_box_desc* desc, const double* radius, size_
         if (cap < s.size()) return set_error(SOIL_HIP_ECAPACITY, "stencil needs %zu", s.size());
         for (size_t i = 0; i < s.size(); ++i) stencil[i] = s[i];
     }
+    return 0;
+}
+
+// Copy kernel of known size (calibrates HBM traffic counters); returns ms per launch.
+int soil_hip_probe_copy(soil_hip_ctx* ctx, size_t bytes, int reps, double* ms) {
+    if (!ctx || bytes < 8 || reps < 1) return set_error(SOIL_HIP_EINVAL, "bad arguments");
+    HIPCHK(hipSetDevice(ctx->device));
+    const size_t n = bytes / 8;
+    double *a = nullptr, *b = nullptr;
+    HIPCHK(hipMalloc(&a, n * 8));
+    HIPCHK(hipMalloc(&b, n * 8));
+    HIPCHK(hipMemset(a, 0, n * 8));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    copy_probe(a, b, n, nullptr);
+    HIPCHK(hipEventRecord(e0, nullptr));
+    for (int r = 0; r < reps; ++r) copy_probe(a, b, n, nullptr);
+    HIPCHK(hipEventRecord(e1, nullptr));
+    HIPCHK(hipEventSynchronize(e1));
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, e0, e1));
+    if (ms) *ms = t / reps;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(a);
+    hipFree(b);
     return 0;
 }
 

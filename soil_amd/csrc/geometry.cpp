@@ -294,6 +294,26 @@ int HostModel::fill(Geom& g) const {
         }
         g.nstencil = n;
     }
+    {
+        // ascending-offset order of the stencil entries plus the self cell (entry nstencil)
+        std::vector<std::pair<int, int>> o;
+        for (int k = 0; k < g.nstencil; ++k) o.push_back({g.stencil[k], k});
+        o.push_back({0, g.nstencil});
+        std::stable_sort(o.begin(), o.end(), [](const std::pair<int, int>& a, const std::pair<int, int>& b) {
+            return a.first < b.first;
+        });
+        for (size_t k = 0; k < o.size(); ++k) g.sorder[k] = (int16_t)o[k].second;
+    }
+    g.nruns = 0;
+    for (int k = 0; k < g.nstencil; ++k) {
+        if (g.nruns > 0 && g.stencil[k] == g.run_off[g.nruns - 1] + g.run_len[g.nruns - 1]) {
+            g.run_len[g.nruns - 1] += 1;
+        } else {
+            g.run_off[g.nruns] = g.stencil[k];
+            g.run_len[g.nruns] = 1;
+            g.nruns += 1;
+        }
+    }
     g.init_shear = ev_init_shear ? 1 : 0;
     g.clear_last = ev_clear_last ? 1 : 0;
     g.bounds_changed = ev_bounds_changed ? 1 : 0;

@@ -16,7 +16,7 @@
 #include <cstddef>
 #include <cstdint>
 
-#define SOIL_MAX_STENCIL 320
+#define SOIL_MAX_STENCIL 256
 #define SOIL_MAX_COLS 8
 #define SOIL_NSWAP 6
 
@@ -57,6 +57,13 @@ struct Geom {
     int32_t init_shear, clear_last, bounds_changed, periodic_sort;
     int32_t disjoint[3];
     int32_t stencil[SOIL_MAX_STENCIL];
+    // stencil entries in ascending offset order; entry nstencil is the self cell (offset 0)
+    int16_t sorder[SOIL_MAX_STENCIL + 1];
+    // the stencil as runs of consecutive cells (offsets o, o+1, ... in stencil order):
+    // their atoms are contiguous in the bin-sorted arrays
+    int32_t nruns;
+    int32_t run_off[SOIL_MAX_STENCIL];
+    int16_t run_len[SOIL_MAX_STENCIL];
 };
 
 // Device-written per-box counters and flags (host reads them back).
@@ -120,6 +127,7 @@ struct BoxDev {
     uint32_t* trow;       // cap_pairs
     uint32_t* row_of;     // cap_pairs
     // border compaction scratch: per tile counts of the two swaps
+    uint32_t* ctile;      // CSR: per local tile, entries staged in the temp region (or overflow mark)
     uint32_t* btile;      // 2 per tile of BLK atoms (swap totals, then exclusive prefixes)
     uint32_t* bwave;      // 2 per wave of 64 atoms (prefix inside the tile)
     ColDev cols[SOIL_MAX_COLS];
@@ -131,7 +139,7 @@ struct BoxDev {
     uint32_t ring_stride;   // Geom records between consecutive slots of this box
     double harmonic_k;
     int32_t consumer;
-    int32_t pad;
+    int32_t csr_mode;     // CSR strategy (bit-neutral): 0 LDS tiles, 1 two global passes, 2 one pass + row slots
 };
 
 int set_error(int code, const char* fmt, ...);

@@ -989,121 +989,339 @@ __global__ void __launch_bounds__(BLK) k_save_build(KARGS) {
     }
 }
 
-// One CSR row (neighbor.rs:1175-1304): the self cell after atom i, then the
-// forward stencil cells in stencil order, each cell in bin order. The cell
-// ranges of up to CSR_RK stencil slots sit in a per-thread LDS table and the
-// candidates of all of them are walked as one flat sequence, so a wave runs
-// max(per-lane candidates) iterations instead of a sum of per-cell maxima.
-constexpr int CSR_RK = 16;
+// ── CSR build: neighbor.rs:1175-1304 ──────────────────────────────────────
+// Row i = accepted candidates of the self cell after atom i (newton; the whole
+// cell but i without newton), then of each stencil cell in stencil order, each
+// cell in bin order. The tests are the reference's: r2 = fma(dx,dx,fma(dy,dy,dz*dz))
+// against (2 r sf)^2 (uniform) or (ri+rj)^2 sf^2.
 
+template <bool UNIFORM>
+__device__ __forceinline__ bool accept_pair(double xi, double yi, double zi, double si, const double4& q, double ucut,
+                                            double sfsq) {
+    const double dx = q.x - xi;
+    const double dy = q.y - yi;
+    const double dz = q.z - zi;
+    const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+    if (UNIFORM) return r2 < ucut;
+    const double sum = si + q.w;
+    return r2 < sum * sum * sfsq;
+}
+
+// Global-memory row walk, used by tiles whose neighborhood does not fit LDS.
 template <bool FILL, bool UNIFORM>
-__device__ __forceinline__ uint32_t csr_row(const BoxDev& b, const Geom& g, uint32_t i, bool act, uint32_t* out,
-                                            uint2 (*rng)[BLK]) {
-    const uint32_t tid = threadIdx.x;
-    uint32_t c = 0;
-    double xi = 0.0, yi = 0.0, zi = 0.0, si = 0.0;
-    if (act) {
-        c = b.cell[i];
-        const double4 pi = ((const double4*)b.xr)[i];
-        xi = pi.x; yi = pi.y; zi = pi.z; si = pi.w;
-    }
-    const double ucut = g.ucut_sq, sfsq = g.sf_sq;
+__device__ uint32_t csr_row_g(const BoxDev& b, const Geom& g, uint32_t i, uint32_t* out, uint32_t out_cap = 0xffffffffu) {
+    const uint32_t c = b.cell[i];
+    const double4 pi = ((const double4*)b.xr)[i];
     const double4* __restrict__ sp4 = (const double4*)b.spos;
     const uint32_t skip = g.newton ? 0xffffffffu : i;
-    const int total = (g.has_self ? 1 : 0) + g.nstencil;
-    const uint32_t ncell = g.ncells;
     uint32_t n = 0;
-    for (int kb = 0; kb < total; kb += CSR_RK) {
-        const int cnt = act ? (total - kb < CSR_RK ? total - kb : CSR_RK) : 0;
+    const uint32_t cap = out_cap;
+    auto take = [&](uint32_t m) {
+        const uint32_t j = b.sorted[m];
+        if (j != skip) {
+            if (FILL && n < cap) out[n] = j;
+            ++n;
+        }
+    };
+    auto scan = [&](uint32_t m0, uint32_t m1) {
+        uint32_t m = m0;
+        for (; m + 1 < m1; m += 2) {  // two candidate loads in flight
+            const double4 q0 = sp4[m], q1 = sp4[m + 1];
+            const bool a0 = accept_pair<UNIFORM>(pi.x, pi.y, pi.z, pi.w, q0, g.ucut_sq, g.sf_sq);
+            const bool a1 = accept_pair<UNIFORM>(pi.x, pi.y, pi.z, pi.w, q1, g.ucut_sq, g.sf_sq);
+            if (a0) take(m);
+            if (a1) take(m + 1);
+        }
+        if (m < m1 && accept_pair<UNIFORM>(pi.x, pi.y, pi.z, pi.w, sp4[m], g.ucut_sq, g.sf_sq)) take(m);
+    };
+    // self cell, merged with a first run that starts at the next cell (newton)
+    int r0 = 0;
+    if (g.has_self) {
+        if (g.newton) {
+            uint32_t cend = c + 1;
+            if (g.nruns > 0 && g.run_off[0] == 1) {
+                cend = c + 1 + (uint32_t)g.run_len[0];
+                r0 = 1;
+            }
+            if (cend > g.ncells) cend = g.ncells;
+            scan(b.sidx[i] + 1, b.bin_start[cend]);
+        } else {
+            scan(b.bin_start[c], b.bin_start[c + 1]);
+        }
+    }
+    const int nr = g.nruns;
+    const int32_t ncell = (int32_t)g.ncells;
+    for (int k0 = r0; k0 < nr; k0 += 8) {
+        uint32_t rs[8], re[8];
 #pragma unroll
-        for (int q = 0; q < CSR_RK; ++q) {
-            uint32_t rs = 0u, re = 0u;
-            if (q < cnt) {
-                const int k = kb + q - (g.has_self ? 1 : 0);
-                if (k < 0) {  // self cell
-                    rs = g.newton ? b.sidx[i] + 1 : b.bin_start[c];
-                    re = b.bin_start[c + 1];
-                } else {
-                    const uint32_t cc = (uint32_t)((int32_t)c + g.stencil[k]);
-                    if (cc < ncell) {  // only atoms far outside the box reach past the grid
-                        rs = b.bin_start[cc];
-                        re = b.bin_start[cc + 1];
-                    }
+        for (int q = 0; q < 8; ++q) {
+            const int k = k0 + q;
+            rs[q] = 0u;
+            re[q] = 0u;
+            if (k < nr) {
+                int32_t c0 = (int32_t)c + g.run_off[k];
+                int32_t c1 = c0 + g.run_len[k];
+                if (c0 < 0) c0 = 0;  // only atoms far outside the box reach past the grid
+                if (c1 > ncell) c1 = ncell;
+                if (c0 < c1) {
+                    rs[q] = b.bin_start[c0];
+                    re[q] = b.bin_start[c1];
                 }
             }
-            rng[q][tid] = make_uint2(rs, re);
         }
-        int q = 0;
-        uint2 r = rng[0][tid];
-        uint32_t m = r.x, e = r.y;
-        while (true) {
-            while (m >= e) {
-                if (++q >= cnt) break;
-                r = rng[q][tid];
-                m = r.x;
-                e = r.y;
-            }
-            if (q >= cnt) break;
-            const double4 p = sp4[m];
-            const double dx = p.x - xi;
-            const double dy = p.y - yi;
-            const double dz = p.z - zi;
-            const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-            bool acc;
-            if (UNIFORM) {
-                acc = r2 < ucut;
-            } else {
-                const double sum = si + p.w;
-                acc = r2 < sum * sum * sfsq;
-            }
-            if (acc) {
-                const uint32_t j = b.sorted[m];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) scan(rs[q], re[q]);
+    }
+    return n;
+}
+
+// LDS staging of one tile of BLK consecutive (cell-sorted) locals: the union
+// of the cell ranges [c_lo + o, c_hi + o] over the stencil offsets o (and 0),
+// merged in ascending order, copied from the bin-sorted arrays.
+constexpr int CSR_LA = 2048;  // staged atoms
+constexpr int CSR_LC = 1280;  // staged cell-table entries
+constexpr int CSR_LR = 48;    // merged ranges
+constexpr uint32_t CSR_OVERFLOW = 0xffffffffu;
+
+struct CsrLds {
+    double4 pos[CSR_LA];
+    uint32_t j[CSR_LA];
+    uint32_t ct[CSR_LC];
+    int32_t rcs[CSR_LR], rce[CSR_LR];
+    uint32_t rct[CSR_LR], rbase[CSR_LR], rlds[CSR_LR], rlen[CSR_LR];
+    uint8_t kr[SOIL_MAX_STENCIL + 1];
+    uint32_t nr, ok, c_lo, c_hi, unsorted;
+    uint32_t scan[BLK / 64];
+};
+
+template <bool FILL, bool UNIFORM>
+__device__ __forceinline__ uint32_t csr_row_lds(const BoxDev& b, const Geom& g, const CsrLds& L, uint32_t i,
+                                                uint32_t c, const double4& pi, uint32_t* out) {
+    const uint32_t skip = g.newton ? 0xffffffffu : i;
+    const int ns = g.nstencil;
+    uint32_t n = 0;
+    for (int e = g.has_self ? -1 : 0; e < ns; ++e) {
+        const int kk = e < 0 ? ns : e;
+        const uint32_t r = L.kr[kk];
+        if (r == 255u) continue;
+        const int32_t cc = (int32_t)c + (e < 0 ? 0 : g.stencil[e]);
+        if (cc < L.rcs[r] || cc > L.rce[r]) continue;
+        const uint32_t tix = L.rct[r] + (uint32_t)(cc - L.rcs[r]);
+        uint32_t a0 = L.ct[tix];
+        const uint32_t a1 = L.ct[tix + 1];
+        if (e < 0 && g.newton) a0 = b.sidx[i] - L.rbase[r] + L.rlds[r] + 1u;
+        for (uint32_t a = a0; a < a1; ++a) {
+            if (accept_pair<UNIFORM>(pi.x, pi.y, pi.z, pi.w, L.pos[a], g.ucut_sq, g.sf_sq)) {
+                const uint32_t j = L.j[a];
                 if (j != skip) {
                     if (FILL) out[n] = j;
                     ++n;
                 }
             }
-            ++m;
         }
     }
     return n;
 }
 
+// Pass 1: per tile, stage the neighborhood, count every row (offsets[i]) and
+// emit the rows into the tile's slot of the temp region when they fit.
+template <bool UNIFORM>
+__device__ void csr_tile(const BoxDev& b, const Geom& g, CsrLds& L, uint32_t i0) {
+    const BoxState* st = b.st;
+    const uint32_t nl = st->nlocal;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t tix = i0 / BLK;
+    const uint32_t ntl = (b.cap_local + BLK - 1) / BLK;
+    const uint32_t tcap = b.cap_pairs / (ntl ? ntl : 1u);
+    if (i0 >= nl) {  // only the CSR end marker
+        if (tid == 0 && i0 == nl) {
+            b.offsets[nl] = 0u;
+            b.ctile[tix] = 0u;
+        }
+        return;
+    }
+    const uint32_t tn = nl - i0 < BLK ? nl - i0 : BLK;
+    const uint32_t i = i0 + tid;
+    const bool act = tid < tn;
+    const uint32_t c = act ? b.cell[i] : 0u;
+    if (tid == 0) {
+        L.unsorted = 0u;
+        L.ok = 1u;
+        L.c_lo = c;
+        L.c_hi = b.cell[i0 + tn - 1];
+    }
+    if (tn < BLK && tid == tn && i0 + tn == nl) b.offsets[nl] = 0u;
+    __syncthreads();
+    if (act && tid + 1 < tn && c > b.cell[i + 1]) L.unsorted = 1u;
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t ok = L.unsorted ? 0u : 1u;
+        const int32_t ncell = (int32_t)g.ncells;
+        const int32_t lo0 = (int32_t)L.c_lo, hi0 = (int32_t)L.c_hi;
+        uint32_t nr = 0, ctot = 0;
+        for (int q = 0; ok && q <= g.nstencil; ++q) {
+            const int kk = g.sorder[q];
+            const int32_t o = kk == g.nstencil ? 0 : g.stencil[kk];
+            int32_t lo = lo0 + o, hi = hi0 + o;
+            if (lo < 0) lo = 0;
+            if (hi > ncell - 1) hi = ncell - 1;
+            if (lo > hi) {
+                L.kr[kk] = 255;
+                continue;
+            }
+            if (nr > 0 && lo <= L.rce[nr - 1] + 1) {
+                if (hi > L.rce[nr - 1]) L.rce[nr - 1] = hi;
+            } else {
+                if (nr == (uint32_t)CSR_LR) { ok = 0; break; }
+                L.rcs[nr] = lo;
+                L.rce[nr] = hi;
+                ++nr;
+            }
+            L.kr[kk] = (uint8_t)(nr - 1);
+        }
+        for (uint32_t r = 0; ok && r < nr; ++r) {
+            L.rct[r] = ctot;
+            ctot += (uint32_t)(L.rce[r] - L.rcs[r]) + 2u;
+        }
+        if (ctot > (uint32_t)CSR_LC) ok = 0;
+        L.nr = ok ? nr : 0u;
+        L.ok = ok;
+    }
+    __syncthreads();
+    if (tid < L.nr) {
+        const uint32_t s0 = b.bin_start[L.rcs[tid]];
+        L.rbase[tid] = s0;
+        L.rlen[tid] = b.bin_start[L.rce[tid] + 1] - s0;
+    }
+    __syncthreads();
+    if (tid == 0 && L.ok) {
+        uint32_t tot = 0;
+        for (uint32_t r = 0; r < L.nr; ++r) {
+            L.rlds[r] = tot;
+            tot += L.rlen[r];
+        }
+        if (tot > (uint32_t)CSR_LA) L.ok = 0;
+    }
+    __syncthreads();
+    const double4 pi = act ? ((const double4*)b.xr)[i] : make_double4(0.0, 0.0, 0.0, 0.0);
+    if (!L.ok) {  // neighborhood too large for LDS: rows from global memory, filled in pass 2
+        if (act) b.offsets[i] = csr_row_g<false, UNIFORM>(b, g, i, nullptr);
+        if (tid == 0) b.ctile[tix] = CSR_OVERFLOW;
+        __syncthreads();
+        return;
+    }
+    const double4* __restrict__ sp4 = (const double4*)b.spos;
+    for (uint32_t r = 0; r < L.nr; ++r) {
+        const uint32_t base = L.rbase[r], dst = L.rlds[r], len = L.rlen[r];
+        for (uint32_t x = tid; x < len; x += BLK) {
+            L.pos[dst + x] = sp4[base + x];
+            L.j[dst + x] = b.sorted[base + x];
+        }
+        const uint32_t ncr = (uint32_t)(L.rce[r] - L.rcs[r]) + 2u;
+        for (uint32_t x = tid; x < ncr; x += BLK) L.ct[L.rct[r] + x] = b.bin_start[L.rcs[r] + x] - base + dst;
+    }
+    __syncthreads();
+    const uint32_t cnt = act ? csr_row_lds<false, UNIFORM>(b, g, L, i, c, pi, nullptr) : 0u;
+    uint32_t tot;
+    const uint32_t lo = block_excl_scan(cnt, L.scan, tot);
+    if (act) b.offsets[i] = cnt;
+    if (tot <= tcap) {
+        if (act && cnt) csr_row_lds<true, UNIFORM>(b, g, L, i, c, pi, b.ttmp + (size_t)tix * tcap + lo);
+        if (tid == 0) b.ctile[tix] = tot;
+    } else if (tid == 0) {
+        b.ctile[tix] = CSR_OVERFLOW;
+    }
+    __syncthreads();
+}
+
 __global__ void __launch_bounds__(BLK) k_csr_count(KARGS) {
-    __shared__ uint2 rng[CSR_RK][BLK];
+    __shared__ CsrLds L;
     FOR_TILES(t) {
         const Tile tl = tiles[t];
         const BoxDev& b = *boxes[tl.box];
         if (!live(b)) continue;
         const BoxState* st = b.st;
         if (!st->rebuild_now || (st->err & ERR_GHOST_CAPACITY)) continue;
+        const Geom& g = geom_of(b);
+        if (g.uniform) csr_tile<true>(b, g, L, tl.start);
+        else csr_tile<false>(b, g, L, tl.start);
+    }
+}
+
+// Global-walk count (no LDS): mode 1 counts only; mode 2 also keeps rows of up
+// to `slot` entries in a per-row slot of the temp region for the fill pass.
+__global__ void __launch_bounds__(BLK) k_csr_count_g(KARGS) {
+    FOR_TILES(t) {
+        const Tile tl = tiles[t];
+        const BoxDev& b = *boxes[tl.box];
+        if (!live(b)) continue;
+        const BoxState* st = b.st;
+        if (!st->rebuild_now || (st->err & ERR_GHOST_CAPACITY)) continue;
+        const Geom& g = geom_of(b);
         const uint32_t i = tl.start + threadIdx.x;
         const uint32_t nl = st->nlocal;
         if (i > nl) continue;
         if (i == nl) {
-            b.offsets[i] = 0u;
+            b.offsets[nl] = 0u;
             continue;
         }
-        const Geom& g = geom_of(b);
-        b.offsets[i] = g.uniform ? csr_row<false, true>(b, g, i, true, nullptr, rng)
-                                 : csr_row<false, false>(b, g, i, true, nullptr, rng);
+        const uint32_t slot = b.csr_mode == 2 ? b.cap_pairs / (b.cap_local ? b.cap_local : 1u) : 0u;
+        uint32_t n;
+        if (slot) {
+            uint32_t* out = b.ttmp + (size_t)i * slot;
+            n = g.uniform ? csr_row_g<true, true>(b, g, i, out, slot) : csr_row_g<true, false>(b, g, i, out, slot);
+        } else {
+            n = g.uniform ? csr_row_g<false, true>(b, g, i, nullptr) : csr_row_g<false, false>(b, g, i, nullptr);
+        }
+        b.offsets[i] = n;
     }
 }
 
+// Pass 2 (after the offset scan): move each tile's rows from the temp region
+// to their place (one contiguous copy), or walk overflowed tiles' rows again.
 __global__ void __launch_bounds__(BLK) k_csr_fill(KARGS) {
-    __shared__ uint2 rng[CSR_RK][BLK];
     FOR_TILES(t) {
         const Tile tl = tiles[t];
         const BoxDev& b = *boxes[tl.box];
         if (!live(b)) continue;
         const BoxState* st = b.st;
         if (!st->rebuild_now || (st->err & (ERR_GHOST_CAPACITY | ERR_PAIR_CAPACITY))) continue;
-        const uint32_t i = tl.start + threadIdx.x;
-        if (i >= st->nlocal) continue;
-        const Geom& g = geom_of(b);
-        uint32_t* out = b.indices + b.offsets[i];
-        if (g.uniform) csr_row<true, true>(b, g, i, true, out, rng);
-        else csr_row<true, false>(b, g, i, true, out, rng);
+        const uint32_t nl = st->nlocal;
+        const uint32_t i0 = tl.start;
+        if (i0 >= nl) continue;
+        if (b.csr_mode != 0) {
+            const uint32_t i = i0 + threadIdx.x;
+            if (i >= nl) continue;
+            const Geom& g = geom_of(b);
+            const uint32_t off = b.offsets[i], n = b.offsets[i + 1] - off;
+            const uint32_t slot = b.csr_mode == 2 ? b.cap_pairs / (b.cap_local ? b.cap_local : 1u) : 0u;
+            uint32_t* out = b.indices + off;
+            if (n <= slot) {
+                const uint32_t* src = b.ttmp + (size_t)i * slot;
+                for (uint32_t k = 0; k < n; ++k) out[k] = src[k];
+            } else if (g.uniform) {
+                csr_row_g<true, true>(b, g, i, out);
+            } else {
+                csr_row_g<true, false>(b, g, i, out);
+            }
+            continue;
+        }
+        const uint32_t tix = i0 / BLK;
+        const uint32_t ct = b.ctile[tix];
+        if (ct == CSR_OVERFLOW) {
+            const uint32_t i = i0 + threadIdx.x;
+            if (i >= nl) continue;
+            const Geom& g = geom_of(b);
+            uint32_t* out = b.indices + b.offsets[i];
+            if (g.uniform) csr_row_g<true, true>(b, g, i, out);
+            else csr_row_g<true, false>(b, g, i, out);
+            continue;
+        }
+        const uint32_t ntl = (b.cap_local + BLK - 1) / BLK;
+        const uint32_t tcap = b.cap_pairs / (ntl ? ntl : 1u);
+        const uint32_t* src = b.ttmp + (size_t)tix * tcap;
+        uint32_t* dst = b.indices + b.offsets[i0];
+        for (uint32_t x = threadIdx.x; x < ct; x += BLK) dst[x] = src[x];
     }
 }
 
@@ -1204,6 +1422,17 @@ __global__ void k_collect(KARGS, uint32_t* out) {
     }
 }
 
+// Streams n doubles from src to dst: a copy of known size that calibrates the
+// HBM traffic counters (FETCH_SIZE, WRITE_SIZE) against algorithmic bytes.
+__global__ void __launch_bounds__(BLK) k_copy_probe(const double* __restrict__ src, double* __restrict__ dst,
+                                                    size_t n) {
+    for (size_t i = (size_t)blockIdx.x * BLK + threadIdx.x; i < n; i += (size_t)gridDim.x * BLK) dst[i] = src[i];
+}
+
+void copy_probe(const double* src, double* dst, size_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_copy_probe, dim3(4096), dim3(BLK), 0, s, src, dst, n);
+}
+
 // ── launch helpers ─────────────────────────────────────────────────────────
 template <typename... A>
 static void launch(void (*k)(BoxDev* const*, const Tile*, uint32_t, A...), const TileSet& ts, BoxDev* const* boxes,
@@ -1279,7 +1508,8 @@ void phase_build_fused(const Launch& L, hipEvent_t ev0, hipEvent_t ev1, hipEvent
     launch(k_save_build, L.t_local, L.boxes, L.stream, L.maxgrid);
     run_sort<SORT_BUILD, SCAN_CELLS_BUILD>(L, L.t_cell, L.t_atom, L.t_scan_cell);
     if (ev0) (void)hipEventRecord(ev0, L.stream);
-    launch(k_csr_count, L.t_local1, L.boxes, L.stream, L.maxgrid);
+    if (L.csr_mode == 0) launch(k_csr_count, L.t_local1, L.boxes, L.stream, L.maxgrid);  // tiles + end marker
+    else launch(k_csr_count_g, L.t_local1, L.boxes, L.stream, L.maxgrid);
     run_scan<SCAN_CSR>(L, L.t_scan_local);
     if (ev1) (void)hipEventRecord(ev1, L.stream);
     launch(k_csr_fill, L.t_local, L.boxes, L.stream, L.maxgrid);

@@ -28,6 +28,7 @@ struct Launch {
     BoxDev* const* boxes = nullptr;
     hipStream_t stream = nullptr;
     uint32_t maxgrid = 4096;
+    int csr_mode = 2;
     TileSet t_box, t_local, t_local1, t_atom, t_atom1, t_ghost, t_cell, t_pair, t_border;
     TileSet t_scan_cell, t_scan_local, t_scan_atom;
 };
@@ -52,5 +53,6 @@ void phase_reverse(const Launch& L);
 void phase_final(const Launch& L);
 void phase_step_end(const Launch& L);
 void phase_collect(const Launch& L, uint32_t* out);
+void copy_probe(const double* src, double* dst, size_t n, hipStream_t s);
 
 }  // namespace soil_hip
