@@ -1006,10 +1006,13 @@ void apply_deform(soil_ref_box* b) {
     b->neighbor.last_build_pos.clear();
 }
 
-// Harmonic soft-sphere test consumer (not reference code). Deterministic
-// gather: each atom a sums, in order, -F(a,j) over its own CSR row and then
-// +F(i,a) over every entry (i,a) of the list in ascending entry order.
-// F(i,j) = k (ri+rj-d)/d * (xj - xi) for 0 < d < ri+rj.
+// Harmonic soft-sphere test consumer (not reference code). For each CSR pair
+// (i, j): F = k (ri+rj-d)/d * (xj - xi) for 0 < d < ri+rj, evaluated once in
+// f64, then quantized to q = round_half_even(F * 2^32) per component;
+// atom i gets -q, atom j gets +q, accumulated in int64 (exact, so the order
+// of accumulation cannot matter), and force += acc * 2^-32 (exact).
+const double kFixScale = 4294967296.0;  // 2^32
+
 inline void pair_force(const Atom& a, size_t i, size_t j, double k, double F[3]) {
     double dx = a.pos[3 * j] - a.pos[3 * i];
     double dy = a.pos[3 * j + 1] - a.pos[3 * i + 1];
@@ -1030,34 +1033,20 @@ void harmonic_force(soil_ref_box* b, double k) {
     const Neighbor& nb = b->neighbor;
     size_t n_all = a.len();
     size_t nlocal = a.nlocal;
-    size_t P = nb.neighbor_indices.size();
-    // transpose: column -> ascending entry positions
-    std::vector<uint32_t> tstart(n_all + 1, 0), trow(P), cursor;
-    std::vector<uint32_t> row_of(P);
+    std::vector<int64_t> acc(3 * n_all, 0);
     for (size_t i = 0; i < nlocal; ++i)
-        for (uint32_t e = nb.neighbor_offsets[i]; e < nb.neighbor_offsets[i + 1]; ++e) row_of[e] = (uint32_t)i;
-    for (size_t e = 0; e < P; ++e) tstart[nb.neighbor_indices[e] + 1] += 1;
-    for (size_t c = 0; c < n_all; ++c) tstart[c + 1] += tstart[c];
-    cursor.assign(tstart.begin(), tstart.end() - 1);
-    for (size_t e = 0; e < P; ++e) trow[cursor[nb.neighbor_indices[e]]++] = (uint32_t)e;
-    for (size_t ai = 0; ai < n_all; ++ai) {
-        double acc[3] = {0.0, 0.0, 0.0};
-        if (ai < nlocal)
-            for (uint32_t e = nb.neighbor_offsets[ai]; e < nb.neighbor_offsets[ai + 1]; ++e) {
-                double F[3];
-                pair_force(a, ai, nb.neighbor_indices[e], k, F);
-                acc[0] = acc[0] - F[0]; acc[1] = acc[1] - F[1]; acc[2] = acc[2] - F[2];
-            }
-        for (uint32_t t = tstart[ai]; t < tstart[ai + 1]; ++t) {
-            uint32_t e = trow[t];
+        for (uint32_t e = nb.neighbor_offsets[i]; e < nb.neighbor_offsets[i + 1]; ++e) {
+            size_t j = nb.neighbor_indices[e];
             double F[3];
-            pair_force(a, row_of[e], ai, k, F);
-            acc[0] = acc[0] + F[0]; acc[1] = acc[1] + F[1]; acc[2] = acc[2] + F[2];
+            pair_force(a, i, j, k, F);
+            for (int c = 0; c < 3; ++c) {
+                int64_t q = (int64_t)std::llrint(F[c] * kFixScale);
+                acc[3 * i + c] -= q;
+                acc[3 * j + c] += q;
+            }
         }
-        a.force[3 * ai] += acc[0];
-        a.force[3 * ai + 1] += acc[1];
-        a.force[3 * ai + 2] += acc[2];
-    }
+    for (size_t ai = 0; ai < n_all; ++ai)
+        for (int c = 0; c < 3; ++c) a.force[3 * ai + c] += (double)acc[3 * ai + c] * (1.0 / kFixScale);
 }
 
 // One timestep in the phase order of schedule.rs:22-52 with the plugin

@@ -47,7 +47,6 @@ void build_tiles(TileTables& tt, uint32_t b, const Caps& c) {
     push_tiles(tt.atom1, b, (uint64_t)c.atoms + 1, BLK);
     push_tiles(tt.ghost, b, (uint64_t)c.atoms - c.local, BLK);
     push_tiles(tt.cell, b, (uint64_t)c.cells + 1, BLK);
-    push_tiles(tt.pair, b, c.pairs, BLK);
     push_tiles(tt.border, b, c.atoms, BORDER_TILE);
     push_tiles(tt.scan_cell, b, (uint64_t)c.cells + 1, SCAN_TILE);
     push_tiles(tt.scan_local, b, (uint64_t)c.local + 1, SCAN_TILE);
@@ -57,10 +56,10 @@ void build_tiles(TileTables& tt, uint32_t b, const Caps& c) {
 // Tiles of one space are interleaved across boxes in XCD-friendly order: the
 // tables keep box-major order; kernels stride over them with the grid.
 void upload_tiles(const TileTables& tt, Launch& L, void** arena) {
-    const std::vector<Tile>* lists[12] = {&tt.box, &tt.local, &tt.local1, &tt.atom, &tt.atom1, &tt.ghost,
-                                          &tt.cell, &tt.pair, &tt.border, &tt.scan_cell, &tt.scan_local, &tt.scan_atom};
-    TileSet* sets[12] = {&L.t_box, &L.t_local, &L.t_local1, &L.t_atom, &L.t_atom1, &L.t_ghost,
-                         &L.t_cell, &L.t_pair, &L.t_border, &L.t_scan_cell, &L.t_scan_local, &L.t_scan_atom};
+    const std::vector<Tile>* lists[11] = {&tt.box,   &tt.local,  &tt.local1,    &tt.atom,       &tt.atom1,   &tt.ghost,
+                                          &tt.cell,  &tt.border, &tt.scan_cell, &tt.scan_local, &tt.scan_atom};
+    TileSet* sets[11] = {&L.t_box,  &L.t_local,  &L.t_local1,    &L.t_atom,       &L.t_atom1,   &L.t_ghost,
+                         &L.t_cell, &L.t_border, &L.t_scan_cell, &L.t_scan_local, &L.t_scan_atom};
     size_t total = 0;
     for (auto* l : lists) total += l->size();
     if (*arena) hipFree(*arena);
@@ -69,7 +68,7 @@ void upload_tiles(const TileTables& tt, Launch& L, void** arena) {
     hipMalloc(arena, total * sizeof(Tile));
     Tile* base = (Tile*)*arena;
     size_t off = 0;
-    for (int k = 0; k < 12; ++k) {
+    for (int k = 0; k < 11; ++k) {
         sets[k]->d = base + off;
         sets[k]->n = (uint32_t)lists[k]->size();
         if (!lists[k]->empty()) hipMemcpy(base + off, lists[k]->data(), lists[k]->size() * sizeof(Tile), hipMemcpyHostToDevice);
@@ -128,7 +127,7 @@ void carve_atoms(Carver& c, BoxDev& d, const Caps& k, const std::vector<HostCol>
 
 void carve_pairs(Carver& c, BoxDev& d, const Caps& k) {
     const size_t P = k.pairs;
-    c.take(d.indices, P); c.take(d.tslot, P); c.take(d.ttmp, P); c.take(d.trow, P); c.take(d.row_of, P);
+    c.take(d.indices, P); c.take(d.ttmp, P); c.take(d.tslot, P); c.take(d.tsrc, P);
 }
 
 int alloc_arena(void** arena, size_t bytes) {
@@ -235,18 +234,16 @@ int alloc_pairs(soil_hip_box* b, uint32_t cap_pairs, bool keep) {
         const size_t np = std::min<size_t>(b->h_st->npairs, cap_pairs);
         if (np) {
             hipMemcpyAsync(nd.indices, b->h_dev.indices, np * 4, hipMemcpyDeviceToDevice, b->stream);
-            hipMemcpyAsync(nd.trow, b->h_dev.trow, np * 4, hipMemcpyDeviceToDevice, b->stream);
-            hipMemcpyAsync(nd.row_of, b->h_dev.row_of, np * 4, hipMemcpyDeviceToDevice, b->stream);
+            hipMemcpyAsync(nd.tsrc, b->h_dev.tsrc, np * 4, hipMemcpyDeviceToDevice, b->stream);
         }
         HIPCHK(hipStreamSynchronize(b->stream));
     }
     if (b->arena_pairs) hipFree(b->arena_pairs);
     b->arena_pairs = arena;
     b->h_dev.indices = nd.indices;
-    b->h_dev.tslot = nd.tslot;
     b->h_dev.ttmp = nd.ttmp;
-    b->h_dev.trow = nd.trow;
-    b->h_dev.row_of = nd.row_of;
+    b->h_dev.tslot = nd.tslot;
+    b->h_dev.tsrc = nd.tsrc;
     b->h_dev.cap_pairs = cap_pairs;
     b->caps.pairs = cap_pairs;
     return 0;

@@ -22,7 +22,7 @@ struct HostCol {
 };
 
 struct TileTables {
-    std::vector<Tile> box, local, local1, atom, atom1, ghost, cell, pair, border, scan_cell, scan_local, scan_atom;
+    std::vector<Tile> box, local, local1, atom, atom1, ghost, cell, border, scan_cell, scan_local, scan_atom;
 };
 
 struct Caps {

@@ -8,7 +8,7 @@
 //           them, then the device-side descriptor swaps the pointers)
 //   bins    bin_start[ncells+1], counts, sorted atoms, sorted idx, spos (x,y,z,r)
 //   csr     offsets[nlocal+1], indices[cap_pairs]
-//   consumer transpose: tstart[n+1], trow[P], row_of[P]
+//   consumer transpose: tstart[n+1], tsrc[P]
 // The descriptor BoxDev lives in device memory and is the authority for
 // pointers; kernels read it at run time, so a captured graph stays valid when
 // the spatial sort swaps column buffers.
@@ -119,13 +119,12 @@ struct BoxDev {
     // CSR
     uint32_t* offsets;    // cap_local + 1
     uint32_t* indices;    // cap_pairs
-    // consumer transpose
+    uint32_t* ttmp;       // cap_pairs: CSR temp region (rows before placement)
+    // consumer: incoming pairs per atom (transpose of the half list, any order)
     uint32_t* tcount;     // cap_atoms + 1
     uint32_t* tstart;     // cap_atoms + 1
-    uint32_t* tslot;      // cap_pairs
-    uint32_t* ttmp;       // cap_pairs
-    uint32_t* trow;       // cap_pairs
-    uint32_t* row_of;     // cap_pairs
+    uint32_t* tslot;      // cap_pairs: arrival slot of each CSR entry in its column
+    uint32_t* tsrc;       // cap_pairs: row atom of each incoming pair, grouped by column
     // border compaction scratch: per tile counts of the two swaps
     uint32_t* ctile;      // CSR: per local tile, entries staged in the temp region (or overflow mark)
     uint32_t* btile;      // 2 per tile of BLK atoms (swap totals, then exclusive prefixes)

@@ -362,8 +362,7 @@ template <bool FUSED>
 __global__ void k_post_sort(KARGS);
 
 // ── stable counting sort by key ────────────────────────────────────────────
-// SPATIAL: locals by cell (perm into `sorted`), BUILD: locals+ghosts by cell,
-// TRANSPOSE: CSR entries by column (consumer gather lists).
+// SPATIAL: locals by cell (perm into `sorted`), BUILD: locals+ghosts by cell.
 template <int W>
 struct SortView {
     uint32_t n, nkeys;
@@ -383,10 +382,6 @@ __device__ __forceinline__ SortView<W> sort_view(const BoxDev& b) {
         v.n = st->nlocal + st->nghost; v.nkeys = g.ncells;
         v.active = st->rebuild_now != 0 && (st->err & (ERR_GHOST_CAPACITY)) == 0;
         v.key = b.cell; v.slot = b.slot; v.counts = b.counts; v.start = b.bin_start; v.tmp = b.tmp; v.out = b.sorted;
-    } else {
-        v.n = st->npairs; v.nkeys = st->nlocal + st->nghost;
-        v.active = st->rebuild_now != 0 && b.consumer && (st->err & (ERR_PAIR_CAPACITY | ERR_GHOST_CAPACITY)) == 0;
-        v.key = b.indices; v.slot = b.tslot; v.counts = b.tcount; v.start = b.tstart; v.tmp = b.ttmp; v.out = b.trow;
     }
     return v;
 }
@@ -413,12 +408,6 @@ __global__ void __launch_bounds__(BLK) k_sort_count(KARGS) {
         SortView<W> v = sort_view<W>(b);
         if (!v.active) continue;
         const uint32_t i = tl.start + threadIdx.x;
-        if (W == SORT_TRANSPOSE) {
-            if (i >= v.n) continue;
-            const uint32_t key = v.key[i];
-            v.slot[i] = atomicAdd(&v.counts[key], 1u);
-            continue;
-        }
         // Cell sorts: the input is nearly cell-ordered (locals were sorted at the
         // last rebuild, ghosts follow their sources), so lanes form runs of equal
         // keys; one atomic per run hands out consecutive slots.
@@ -589,6 +578,12 @@ __global__ void k_post_sort(KARGS) {
     }
 }
 
+// consumer transpose is rebuilt with the CSR, for boxes that have a consumer
+__device__ __forceinline__ bool transpose_active(const BoxDev& b) {
+    const BoxState* st = b.st;
+    return st->rebuild_now != 0 && b.consumer && (st->err & (ERR_PAIR_CAPACITY | ERR_GHOST_CAPACITY)) == 0;
+}
+
 // ── exclusive scans (reduce tiles -> scan tile sums per box -> down-sweep) ──
 template <int W>
 __device__ __forceinline__ void scan_view(const BoxDev& b, uint32_t& n, bool& active, uint32_t*& in, uint32_t*& out) {
@@ -606,7 +601,7 @@ __device__ __forceinline__ void scan_view(const BoxDev& b, uint32_t& n, bool& ac
         out = b.offsets;
     } else {
         n = st->nlocal + st->nghost + 1;
-        active = st->rebuild_now != 0 && b.consumer && (st->err & (ERR_PAIR_CAPACITY | ERR_GHOST_CAPACITY)) == 0;
+        active = transpose_active(b);
         in = b.tcount;
         out = b.tstart;
     }
@@ -982,6 +977,7 @@ __global__ void __launch_bounds__(BLK) k_save_build(KARGS) {
         if (!live(b)) continue;
         if (!b.st->rebuild_now) continue;
         const uint32_t i = tl.start + threadIdx.x;
+        if (b.consumer && i <= b.st->nlocal + b.st->nghost) b.tcount[i] = 0u;
         if (i >= b.st->nlocal) continue;
         b.lbx[i] = b.x[i];
         b.lby[i] = b.y[i];
@@ -1325,22 +1321,37 @@ __global__ void __launch_bounds__(BLK) k_csr_fill(KARGS) {
     }
 }
 
-// consumer: row of each CSR entry
-__global__ void __launch_bounds__(BLK) k_row_of(KARGS) {
+// ── consumer transpose: incoming pairs of every atom ─────────────────────
+// The harmonic consumer sums exact integers, so the order of a column's
+// entries is free: arrival slots from atomics place them directly.
+__global__ void __launch_bounds__(BLK) k_tcount(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
         const BoxDev& b = *boxes[tl.box];
-        if (!live(b)) continue;
-        const BoxState* st = b.st;
-        if (!st->rebuild_now || !b.consumer || (st->err & (ERR_GHOST_CAPACITY | ERR_PAIR_CAPACITY))) continue;
+        if (!live(b) || !transpose_active(b)) continue;
         const uint32_t i = tl.start + threadIdx.x;
-        if (i >= st->nlocal) continue;
-        for (uint32_t e = b.offsets[i]; e < b.offsets[i + 1]; ++e) b.row_of[e] = i;
+        if (i >= b.st->nlocal) continue;
+        for (uint32_t e = b.offsets[i]; e < b.offsets[i + 1]; ++e) b.tslot[e] = atomicAdd(&b.tcount[b.indices[e]], 1u);
     }
 }
 
-// Harmonic soft-sphere test consumer (not reference code): deterministic
-// gather, -F(a,j) over a's own row then +F(i,a) over entries (i,a) ascending.
+__global__ void __launch_bounds__(BLK) k_tplace(KARGS) {
+    FOR_TILES(t) {
+        const Tile tl = tiles[t];
+        const BoxDev& b = *boxes[tl.box];
+        if (!live(b) || !transpose_active(b)) continue;
+        const uint32_t i = tl.start + threadIdx.x;
+        if (i >= b.st->nlocal) continue;
+        for (uint32_t e = b.offsets[i]; e < b.offsets[i + 1]; ++e) b.tsrc[b.tstart[b.indices[e]] + b.tslot[e]] = i;
+    }
+}
+
+// Harmonic soft-sphere test consumer (not reference code). For a CSR pair
+// (i, j), F = k (ri+rj-d)/d * (xj - xi) when 0 < d < ri+rj, quantized to
+// q = round_half_even(F * 2^32); i receives -q and j receives +q. Each atom
+// sums its row and its incoming pairs in int64 (exact, so any order gives the
+// same sum) and adds sum * 2^-32 (exact) to its f64 force. Both sides evaluate
+// F with the row atom first, so they round identically.
 __device__ __forceinline__ void pair_force(const double4& pi, const double4& pj, double k, double& fx, double& fy,
                                            double& fz) {
     const double dx = pj.x - pi.x;
@@ -1361,6 +1372,8 @@ __device__ __forceinline__ void pair_force(const double4& pi, const double4& pj,
     }
 }
 
+constexpr double kFixScale = 4294967296.0;  // 2^32
+
 __global__ void __launch_bounds__(BLK) k_harmonic(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
@@ -1369,32 +1382,31 @@ __global__ void __launch_bounds__(BLK) k_harmonic(KARGS) {
         const BoxState* st = b.st;
         if (!b.consumer || st->rebuilds == 0 || (st->err & (ERR_GHOST_CAPACITY | ERR_PAIR_CAPACITY))) continue;
         const uint32_t a = tl.start + threadIdx.x;
-        const uint32_t nl = st->nlocal;
-        if (a >= nl + st->nghost) continue;
+        if (a >= st->nlocal + st->nghost) continue;
         const double k = b.harmonic_k;
         const double4* xr = (const double4*)b.xr;
         const double4 pa = xr[a];
-        double ax = 0.0, ay = 0.0, az = 0.0;
-        if (a < nl) {
+        long long ax = 0, ay = 0, az = 0;
+        if (a < st->nlocal) {
             for (uint32_t e = b.offsets[a]; e < b.offsets[a + 1]; ++e) {
                 double fx, fy, fz;
                 pair_force(pa, xr[b.indices[e]], k, fx, fy, fz);
-                ax = ax - fx;
-                ay = ay - fy;
-                az = az - fz;
+                ax -= __double2ll_rn(fx * kFixScale);
+                ay -= __double2ll_rn(fy * kFixScale);
+                az -= __double2ll_rn(fz * kFixScale);
             }
         }
-        for (uint32_t q = b.tstart[a]; q < b.tstart[a + 1]; ++q) {
-            const uint32_t e = b.trow[q];
+        for (uint32_t e = b.tstart[a]; e < b.tstart[a + 1]; ++e) {
             double fx, fy, fz;
-            pair_force(xr[b.row_of[e]], pa, k, fx, fy, fz);
-            ax = ax + fx;
-            ay = ay + fy;
-            az = az + fz;
+            pair_force(xr[b.tsrc[e]], pa, k, fx, fy, fz);
+            ax += __double2ll_rn(fx * kFixScale);
+            ay += __double2ll_rn(fy * kFixScale);
+            az += __double2ll_rn(fz * kFixScale);
         }
-        b.fx[a] += ax;
-        b.fy[a] += ay;
-        b.fz[a] += az;
+        const double inv = 1.0 / kFixScale;
+        b.fx[a] = b.fx[a] + (double)ax * inv;
+        b.fy[a] = b.fy[a] + (double)ay * inv;
+        b.fz[a] = b.fz[a] + (double)az * inv;
     }
 }
 
@@ -1505,7 +1517,7 @@ void phase_build(const Launch& L, hipEvent_t ev0, hipEvent_t ev1, hipEvent_t ev2
 }
 // the build after phase_sort_fused (which already did the bookkeeping)
 void phase_build_fused(const Launch& L, hipEvent_t ev0, hipEvent_t ev1, hipEvent_t ev2) {
-    launch(k_save_build, L.t_local, L.boxes, L.stream, L.maxgrid);
+    launch(k_save_build, L.t_atom1, L.boxes, L.stream, L.maxgrid);
     run_sort<SORT_BUILD, SCAN_CELLS_BUILD>(L, L.t_cell, L.t_atom, L.t_scan_cell);
     if (ev0) (void)hipEventRecord(ev0, L.stream);
     if (L.csr_mode == 0) launch(k_csr_count, L.t_local1, L.boxes, L.stream, L.maxgrid);  // tiles + end marker
@@ -1514,9 +1526,10 @@ void phase_build_fused(const Launch& L, hipEvent_t ev0, hipEvent_t ev1, hipEvent
     if (ev1) (void)hipEventRecord(ev1, L.stream);
     launch(k_csr_fill, L.t_local, L.boxes, L.stream, L.maxgrid);
     if (ev2) (void)hipEventRecord(ev2, L.stream);
-    // consumer gather lists (transpose of the half list)
-    run_sort<SORT_TRANSPOSE, SCAN_TRANSPOSE>(L, L.t_atom1, L.t_pair, L.t_scan_atom);
-    launch(k_row_of, L.t_local, L.boxes, L.stream, L.maxgrid);
+    // consumer transpose (tcount was cleared by k_save_build)
+    launch(k_tcount, L.t_local, L.boxes, L.stream, L.maxgrid);
+    run_scan<SCAN_TRANSPOSE>(L, L.t_scan_atom);
+    launch(k_tplace, L.t_local, L.boxes, L.stream, L.maxgrid);
 }
 void phase_force(const Launch& L) { launch(k_harmonic, L.t_atom, L.boxes, L.stream, L.maxgrid); }
 void phase_reverse(const Launch& L) {

@@ -10,7 +10,7 @@ constexpr int SCAN_PER_THREAD = 8;
 constexpr uint32_t SCAN_TILE = BLK * SCAN_PER_THREAD;
 constexpr uint32_t BORDER_TILE = 1024;     // atoms per border-compaction tile
 
-enum { SORT_SPATIAL = 0, SORT_BUILD = 1, SORT_TRANSPOSE = 2 };
+enum { SORT_SPATIAL = 0, SORT_BUILD = 1 };
 enum { SCAN_CELLS_SPATIAL = 0, SCAN_CELLS_BUILD = 1, SCAN_CSR = 2, SCAN_TRANSPOSE = 3 };
 
 struct Tile {
@@ -29,7 +29,7 @@ struct Launch {
     hipStream_t stream = nullptr;
     uint32_t maxgrid = 4096;
     int csr_mode = 2;
-    TileSet t_box, t_local, t_local1, t_atom, t_atom1, t_ghost, t_cell, t_pair, t_border;
+    TileSet t_box, t_local, t_local1, t_atom, t_atom1, t_ghost, t_cell, t_border;
     TileSet t_scan_cell, t_scan_local, t_scan_atom;
 };
 
