@@ -43,7 +43,7 @@ def test_batch_matches_oracle_per_box(ctx, graph):
             batch.close()
         for i, (g, r) in enumerate(pairs):
             if batch_close:
-                assert_same(g, r, exact_state=False, what=f"box {i}")
+                assert_same(g, r, what=f"box {i}")
             else:
                 # inside a batch, compare through the batch-synchronised state only
                 pass
@@ -51,7 +51,7 @@ def test_batch_matches_oracle_per_box(ctx, graph):
     for g, r in pairs:
         g.step(5)
         r.step(5)
-        assert_same(g, r, exact_state=False, what="after batch")
+        assert_same(g, r, what="after batch")
 
 
 def test_batch_autotune_does_not_change_state(ctx):
@@ -67,4 +67,4 @@ def test_batch_autotune_does_not_change_state(ctx):
     batch.close()
     for g, r in pairs:
         r.step(10)
-        assert_same(g, r, exact_state=False, what="autotune")
+        assert_same(g, r, what="autotune")
