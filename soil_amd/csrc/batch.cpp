@@ -39,15 +39,14 @@ static int batch_tiles(soil_hip_batch* bt) {
 
 static void launch_step(const Launch& L) {
     phase_deform(L);
-    phase_initial(L);
-    phase_decide(L);
-    phase_zero(L);
+    phase_initial_decide(L);
+    phase_zero_fused(L);
     phase_pbc(L);
     phase_sort_fused(L);
     phase_borders(L);
     phase_forward(L);
     phase_build_fused(L, nullptr, nullptr, nullptr);
-    phase_force(L);
+    phase_force_fused(L);
     phase_reverse(L);
     phase_final(L);
     phase_step_end(L);
@@ -127,6 +126,22 @@ static int batch_run(soil_hip_batch* bt, int nsteps, bool use_graph) {
                 bt->boxes[k]->last_geom = row[k];
             }
         }
+        // reverse rounds: split a dim's swaps once any box can select an atom for both
+        bool resplit = false;
+        for (uint32_t j = 0; j < run; ++j) {
+            const Geom* row = bt->h_ring + (size_t)(slot0 + j) * nb;
+            for (uint32_t k = 0; k < nb; ++k)
+                for (int d = 0; d < 3; ++d)
+                    if (!row[k].disjoint[d] && !bt->L.rev_split[d]) {
+                        bt->L.rev_split[d] = 1;
+                        resplit = true;
+                    }
+        }
+        if (resplit && bt->graph_exec) {
+            HIPCHK(hipStreamSynchronize(bt->stream));
+            int rc = capture_graph(bt);
+            if (rc) return rc;
+        }
         HIPCHK(hipMemcpyAsync(bt->d_ring + (size_t)slot0 * nb, bt->h_ring + (size_t)slot0 * nb,
                               sizeof(Geom) * nb * run, hipMemcpyHostToDevice, bt->stream));
         for (uint32_t j = 0; j < run; ++j) {
@@ -186,6 +201,7 @@ int batch_make(soil_hip_ctx* ctx, soil_hip_box** boxes, int nboxes, soil_hip_bat
     HIPCHK(hipMalloc(&bt->d_ptrs, sizeof(BoxDev*) * nb));
     HIPCHK(hipMemcpy(bt->d_ptrs, ptrs.data(), sizeof(BoxDev*) * nb, hipMemcpyHostToDevice));
     bt->L.maxgrid = 4096;
+    for (int d = 0; d < 3; ++d) bt->L.rev_split[d] = 0;  // raised by batch_run when a box needs it
     batch_tiles(bt);
     *out = bt;
     return 0;

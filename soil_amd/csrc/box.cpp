@@ -403,13 +403,14 @@ int step_plain(soil_hip_box* b, int nsteps) {
         b->last_geom = g;
         HIPCHK(hipMemcpyAsync(b->d_ring + b->ring_pos, &g, sizeof(Geom), hipMemcpyHostToDevice, b->stream));
         phase_deform(b->L);
-        phase_initial(b->L);
-        phase_decide(b->L);
+        phase_initial_decide(b->L);
         int rc = pull_state(b);
         if (rc) return rc;
         if (b->h_st->err) return check_err_bits(b->h_st->err);
         const bool rebuild = b->h_st->state == 0 || g.periodic_sort || g.bounds_changed;
-        phase_zero(b->L);
+        bool zero_cols = false;
+        for (const auto& c : b->cols) zero_cols |= (c.flags & SOIL_HIP_COL_ZERO) != 0;
+        if (!b->consumer || zero_cols) phase_zero_fused(b->L);
         if (rebuild) {
             phase_pbc(b->L);
             phase_sort_fused(b->L);
@@ -419,8 +420,11 @@ int step_plain(soil_hip_box* b, int nsteps) {
             phase_sort_flags(b->L);
             phase_forward(b->L);
         }
-        if (b->consumer) phase_force(b->L);
-        if (b->model.newton) phase_reverse(b->L);
+        if (b->consumer) phase_force_fused(b->L);
+        if (b->model.newton) {
+            for (int d = 0; d < 3; ++d) b->L.rev_split[d] = g.disjoint[d] ? 0 : 1;
+            phase_reverse(b->L);
+        }
         phase_final(b->L);
         phase_step_end(b->L);
         b->ring_pos = (b->ring_pos + 1) % b->ring_len;
@@ -810,7 +814,10 @@ int soil_hip_harmonic_force(soil_hip_box* b) {
 
 int soil_hip_reverse_send_force(soil_hip_box* b) {
     PHASE_PROLOGUE(b);
-    if (b->model.newton) phase_reverse(b->L);
+    if (b->model.newton) {
+        for (int d = 0; d < 3; ++d) b->L.rev_split[d] = 1;  // ordered rounds whatever the geometry
+        phase_reverse(b->L);
+    }
     return 0;
 }
 

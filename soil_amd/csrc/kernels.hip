@@ -122,32 +122,6 @@ __device__ uint32_t block_excl_scan(uint32_t v, uint32_t* lds, uint32_t& total) 
 }
 
 // ── integration: soil_verlet/src/lib.rs:131-194 ───────────────────────────
-__global__ void __launch_bounds__(BLK) k_initial(KARGS) {
-    FOR_TILES(t) {
-        const Tile tl = tiles[t];
-        const BoxDev& b = *boxes[tl.box];
-        if (!live(b)) continue;
-        const uint32_t i = tl.start + threadIdx.x;
-        if (i >= b.st->nlocal) continue;
-        const Geom& g = geom_of(b);
-        const double dt = g.dt;
-        const double h = 0.5 * dt * b.inv_mass[i];
-        const double v0 = b.vx[i] + h * b.fx[i];
-        const double v1 = b.vy[i] + h * b.fy[i];
-        const double v2 = b.vz[i] + h * b.fz[i];
-        b.vx[i] = v0;
-        b.vy[i] = v1;
-        b.vz[i] = v2;
-        const double x = b.x[i] + v0 * dt;
-        const double y = b.y[i] + v1 * dt;
-        const double z = b.z[i] + v2 * dt;
-        b.x[i] = x;
-        b.y[i] = y;
-        b.z[i] = z;
-        put_xr(b, i, x, y, z);
-    }
-}
-
 __global__ void __launch_bounds__(BLK) k_final(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
@@ -162,7 +136,6 @@ __global__ void __launch_bounds__(BLK) k_final(KARGS) {
         b.vz[i] = b.vz[i] + h * b.fz[i];
     }
 }
-
 // soil_deform/src/lib.rs:568-574: one-time streaming profile v_x += γ̇ (y - y_c)
 __global__ void __launch_bounds__(BLK) k_deform_init(KARGS) {
     FOR_TILES(t) {
@@ -177,7 +150,74 @@ __global__ void __launch_bounds__(BLK) k_deform_init(KARGS) {
     }
 }
 
-// ── rebuild decision: neighbor.rs:693-822 ─────────────────────────────────
+// displacement_exceeded (neighbor.rs:693-758) applies in CommunicateOnly with
+// a valid last build, unless `every` is set without `check`
+__device__ __forceinline__ bool disp_active(const Geom& g, const BoxState* st) {
+    return st->state == 1u && st->last_valid && !g.clear_last && (g.every == 0 || g.check);
+}
+
+__device__ __forceinline__ bool disp_exceeds(const Geom& g, const BoxDev& b, uint32_t i, double x, double y,
+                                             double z) {
+    double dx = x - b.lbx[i];
+    double dy = y - b.lby[i];
+    double dz = z - b.lbz[i];
+    if (g.pbc_all) {
+        dx -= g.pbc_box[0] * round(dx * g.inv_pbc_box[0]);
+        dy -= g.pbc_box[1] * round(dy * g.inv_pbc_box[1]);
+        dz -= g.pbc_box[2] * round(dz * g.inv_pbc_box[2]);
+        return fma(dx, dx, fma(dy, dy, dz * dz)) > g.disp_thr_sq;
+    }
+    if (g.periodic[0]) {
+        if (dx > g.half_pbc_box[0]) dx -= g.pbc_box[0];
+        else if (dx < -g.half_pbc_box[0]) dx += g.pbc_box[0];
+    }
+    if (g.periodic[1]) {
+        if (dy > g.half_pbc_box[1]) dy -= g.pbc_box[1];
+        else if (dy < -g.half_pbc_box[1]) dy += g.pbc_box[1];
+    }
+    if (g.periodic[2]) {
+        if (dz > g.half_pbc_box[2]) dz -= g.pbc_box[2];
+        else if (dz < -g.half_pbc_box[2]) dz += g.pbc_box[2];
+    }
+    return dx * dx + dy * dy + dz * dz > g.disp_thr_sq;
+}
+
+// initial_integration (soil_verlet/src/lib.rs:131-162). With DISP the
+// displacement check of the following decide_rebuild runs on the new
+// positions while they are in registers (nothing between the two systems
+// changes positions or the state it depends on).
+template <bool DISP>
+__global__ void __launch_bounds__(BLK) k_initial(KARGS) {
+    FOR_TILES(t) {
+        const Tile tl = tiles[t];
+        const BoxDev& b = *boxes[tl.box];
+        if (!live(b)) continue;
+        BoxState* st = b.st;
+        const Geom& g = geom_of(b);
+        const uint32_t i = tl.start + threadIdx.x;
+        const bool act = i < st->nlocal;
+        bool ex = false;
+        if (act) {
+            const double dt = g.dt;
+            const double h = 0.5 * dt * b.inv_mass[i];
+            const double v0 = b.vx[i] + h * b.fx[i];
+            const double v1 = b.vy[i] + h * b.fy[i];
+            const double v2 = b.vz[i] + h * b.fz[i];
+            b.vx[i] = v0;
+            b.vy[i] = v1;
+            b.vz[i] = v2;
+            const double x = b.x[i] + v0 * dt;
+            const double y = b.y[i] + v1 * dt;
+            const double z = b.z[i] + v2 * dt;
+            b.x[i] = x;
+            b.y[i] = y;
+            b.z[i] = z;
+            put_xr(b, i, x, y, z);
+            if (DISP && disp_active(g, st)) ex = disp_exceeds(g, b, i, x, y, z);
+        }
+        if (DISP) wave_flag(&st->exceed, ex);
+    }
+}
 __global__ void __launch_bounds__(BLK) k_disp(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
@@ -185,39 +225,12 @@ __global__ void __launch_bounds__(BLK) k_disp(KARGS) {
         if (!live(b)) continue;
         BoxState* st = b.st;
         const Geom& g = geom_of(b);
-        if (st->state != 1u || !st->last_valid || g.clear_last) continue;
-        if (g.every != 0 && !g.check) continue;
+        if (!disp_active(g, st)) continue;
         const uint32_t i = tl.start + threadIdx.x;
         if (i >= st->nlocal) continue;
-        double dx = b.x[i] - b.lbx[i];
-        double dy = b.y[i] - b.lby[i];
-        double dz = b.z[i] - b.lbz[i];
-        bool ex;
-        if (g.pbc_all) {
-            dx -= g.pbc_box[0] * round(dx * g.inv_pbc_box[0]);
-            dy -= g.pbc_box[1] * round(dy * g.inv_pbc_box[1]);
-            dz -= g.pbc_box[2] * round(dz * g.inv_pbc_box[2]);
-            ex = fma(dx, dx, fma(dy, dy, dz * dz)) > g.disp_thr_sq;
-        } else {
-            if (g.periodic[0]) {
-                if (dx > g.half_pbc_box[0]) dx -= g.pbc_box[0];
-                else if (dx < -g.half_pbc_box[0]) dx += g.pbc_box[0];
-            }
-            if (g.periodic[1]) {
-                if (dy > g.half_pbc_box[1]) dy -= g.pbc_box[1];
-                else if (dy < -g.half_pbc_box[1]) dy += g.pbc_box[1];
-            }
-            if (g.periodic[2]) {
-                if (dz > g.half_pbc_box[2]) dz -= g.pbc_box[2];
-                else if (dz < -g.half_pbc_box[2]) dz += g.pbc_box[2];
-            }
-            ex = dx * dx + dy * dy + dz * dz > g.disp_thr_sq;
-        }
-        wave_flag(&st->exceed, ex);
+        wave_flag(&st->exceed, disp_exceeds(g, b, i, b.x[i], b.y[i], b.z[i]));
     }
 }
-
-// decide_rebuild (+ remove_ghost_atoms when REMOVE), one thread per box
 template <bool REMOVE>
 __global__ void k_decide(KARGS) {
     FOR_TILES(t) {
@@ -264,6 +277,9 @@ __global__ void k_set_rebuild_now(KARGS) {
 }
 
 // zero_all_forces: atom.rs:631-635 (force + #[zero] columns of all atoms)
+// zero_all_forces (atom.rs:631-635). In the fused step (CONSUMER_WRITES) a box
+// whose consumer then assigns every force skips the force columns here.
+template <bool CONSUMER_WRITES>
 __global__ void __launch_bounds__(BLK) k_zero(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
@@ -271,9 +287,11 @@ __global__ void __launch_bounds__(BLK) k_zero(KARGS) {
         if (!live(b)) continue;
         const uint32_t i = tl.start + threadIdx.x;
         if (i >= b.st->nlocal + b.st->nghost) continue;
-        b.fx[i] = 0.0;
-        b.fy[i] = 0.0;
-        b.fz[i] = 0.0;
+        if (!(CONSUMER_WRITES && b.consumer)) {
+            b.fx[i] = 0.0;
+            b.fy[i] = 0.0;
+            b.fz[i] = 0.0;
+        }
         for (int c = 0; c < b.ncols; ++c)
             if (b.cols[c].flags & 4u)
                 for (int w = 0; w < b.cols[c].width; ++w) b.cols[c].data[(size_t)i * b.cols[c].width + w] = 0.0;
@@ -931,7 +949,13 @@ __global__ void __launch_bounds__(BLK) k_forward(KARGS) {
 }
 
 // ── reverse comm: comm.rs:867-878, 938-1024, one launch per swap ───────────
-__global__ void __launch_bounds__(BLK) k_reverse(KARGS, int s) {
+// Round of dimension d. The lo swap's sources (x < lo+gc) and the hi swap's
+// (x >= hi-gc) are different atoms when the two selections cannot overlap
+// (Geom::disjoint), and a dim's sources all precede its own ghosts, so both
+// swaps of the dim then add onto distinct owners and run in one launch
+// (part 0). Otherwise part 0 is the lo swap and part 1 the hi swap, in the
+// reference order.
+__global__ void __launch_bounds__(BLK) k_reverse(KARGS, int d, int part) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
         const BoxDev& b = *boxes[tl.box];
@@ -939,8 +963,20 @@ __global__ void __launch_bounds__(BLK) k_reverse(KARGS, int s) {
         const BoxState* st = b.st;
         const Geom& g = geom_of(b);
         if (!g.newton || (st->err & ERR_GHOST_CAPACITY)) continue;
-        const uint32_t k = tl.start + threadIdx.x;
-        if (k >= st->swap_count[s]) continue;
+        const bool both = g.disjoint[d] != 0;
+        uint32_t k = tl.start + threadIdx.x;
+        int s;
+        if (part == 0) {
+            const uint32_t c0 = st->swap_count[2 * d];
+            const uint32_t n = both ? c0 + st->swap_count[2 * d + 1] : c0;
+            if (k >= n) continue;
+            s = k < c0 ? 2 * d : 2 * d + 1;
+            if (k >= c0) k -= c0;
+        } else {
+            if (both) continue;
+            s = 2 * d + 1;
+            if (k >= st->swap_count[s]) continue;
+        }
         const uint32_t gi = st->swap_start[s] + k;
         const uint32_t o = (uint32_t)b.origin[gi];
         b.fx[o] += b.fx[gi];
@@ -1374,15 +1410,25 @@ __device__ __forceinline__ void pair_force(const double4& pi, const double4& pj,
 
 constexpr double kFixScale = 4294967296.0;  // 2^32
 
+// ASSIGN (fused step, forces were not zeroed): f = sum * 2^-32, which equals
+// 0 + sum * 2^-32 bit for bit; otherwise f += sum * 2^-32.
+template <bool ASSIGN>
 __global__ void __launch_bounds__(BLK) k_harmonic(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
         const BoxDev& b = *boxes[tl.box];
-        if (!live(b)) continue;
+        if (!live(b) || !b.consumer) continue;
         const BoxState* st = b.st;
-        if (!b.consumer || st->rebuilds == 0 || (st->err & (ERR_GHOST_CAPACITY | ERR_PAIR_CAPACITY))) continue;
         const uint32_t a = tl.start + threadIdx.x;
         if (a >= st->nlocal + st->nghost) continue;
+        if (st->rebuilds == 0 || (st->err & (ERR_GHOST_CAPACITY | ERR_PAIR_CAPACITY))) {
+            if (ASSIGN) {
+                b.fx[a] = 0.0;
+                b.fy[a] = 0.0;
+                b.fz[a] = 0.0;
+            }
+            continue;
+        }
         const double k = b.harmonic_k;
         const double4* xr = (const double4*)b.xr;
         const double4 pa = xr[a];
@@ -1404,9 +1450,15 @@ __global__ void __launch_bounds__(BLK) k_harmonic(KARGS) {
             az += __double2ll_rn(fz * kFixScale);
         }
         const double inv = 1.0 / kFixScale;
-        b.fx[a] = b.fx[a] + (double)ax * inv;
-        b.fy[a] = b.fy[a] + (double)ay * inv;
-        b.fz[a] = b.fz[a] + (double)az * inv;
+        if (ASSIGN) {
+            b.fx[a] = (double)ax * inv;
+            b.fy[a] = (double)ay * inv;
+            b.fz[a] = (double)az * inv;
+        } else {
+            b.fx[a] = b.fx[a] + (double)ax * inv;
+            b.fy[a] = b.fy[a] + (double)ay * inv;
+            b.fz[a] = b.fz[a] + (double)az * inv;
+        }
     }
 }
 
@@ -1472,7 +1524,11 @@ static void run_sort(const Launch& L, const TileSet& zero_tiles, const TileSet& 
 }
 
 void phase_deform(const Launch& L) { launch(k_deform_init, L.t_local, L.boxes, L.stream, L.maxgrid); }
-void phase_initial(const Launch& L) { launch(k_initial, L.t_local, L.boxes, L.stream, L.maxgrid); }
+void phase_initial(const Launch& L) { launch(k_initial<false>, L.t_local, L.boxes, L.stream, L.maxgrid); }
+void phase_initial_decide(const Launch& L) {
+    launch(k_initial<true>, L.t_local, L.boxes, L.stream, L.maxgrid);
+    launch(k_decide<true>, L.t_box, L.boxes, L.stream, L.maxgrid);
+}
 void phase_decide(const Launch& L) {
     launch(k_disp, L.t_local, L.boxes, L.stream, L.maxgrid);
     launch(k_decide<true>, L.t_box, L.boxes, L.stream, L.maxgrid);
@@ -1487,7 +1543,8 @@ void phase_sort_flags(const Launch& L) {
     launch(k_pre_sort, L.t_box, L.boxes, L.stream, L.maxgrid);
     launch(k_post_sort<false>, L.t_box, L.boxes, L.stream, L.maxgrid);
 }
-void phase_zero(const Launch& L) { launch(k_zero, L.t_atom, L.boxes, L.stream, L.maxgrid); }
+void phase_zero(const Launch& L) { launch(k_zero<false>, L.t_atom, L.boxes, L.stream, L.maxgrid); }
+void phase_zero_fused(const Launch& L) { launch(k_zero<true>, L.t_atom, L.boxes, L.stream, L.maxgrid); }
 void phase_pbc(const Launch& L) { launch(k_pbc, L.t_local, L.boxes, L.stream, L.maxgrid); }
 void phase_sort(const Launch& L) {
     launch(k_pre_sort, L.t_box, L.boxes, L.stream, L.maxgrid);
@@ -1531,11 +1588,14 @@ void phase_build_fused(const Launch& L, hipEvent_t ev0, hipEvent_t ev1, hipEvent
     run_scan<SCAN_TRANSPOSE>(L, L.t_scan_atom);
     launch(k_tplace, L.t_local, L.boxes, L.stream, L.maxgrid);
 }
-void phase_force(const Launch& L) { launch(k_harmonic, L.t_atom, L.boxes, L.stream, L.maxgrid); }
+void phase_force(const Launch& L) { launch(k_harmonic<false>, L.t_atom, L.boxes, L.stream, L.maxgrid); }
+void phase_force_fused(const Launch& L) { launch(k_harmonic<true>, L.t_atom, L.boxes, L.stream, L.maxgrid); }
 void phase_reverse(const Launch& L) {
     // rounds last -> first, lo swap then hi swap (comm.rs:965-1021)
-    const int order[6] = {4, 5, 2, 3, 0, 1};
-    for (int k = 0; k < 6; ++k) launch(k_reverse, L.t_ghost, L.boxes, L.stream, L.maxgrid, order[k]);
+    for (int d = 2; d >= 0; --d) {
+        launch(k_reverse, L.t_ghost, L.boxes, L.stream, L.maxgrid, d, 0);
+        if (L.rev_split[d]) launch(k_reverse, L.t_ghost, L.boxes, L.stream, L.maxgrid, d, 1);
+    }
 }
 void phase_final(const Launch& L) { launch(k_final, L.t_local, L.boxes, L.stream, L.maxgrid); }
 void phase_step_end(const Launch& L) { launch(k_step_end, L.t_box, L.boxes, L.stream, L.maxgrid); }

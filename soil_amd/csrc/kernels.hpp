@@ -28,6 +28,9 @@ struct Launch {
     BoxDev* const* boxes = nullptr;
     hipStream_t stream = nullptr;
     uint32_t maxgrid = 4096;
+    // reverse comm: 1 = some box may have overlapping lo/hi selections in that
+    // dim, so its two swaps need separate ordered launches
+    uint8_t rev_split[3] = {1, 1, 1};
     int csr_mode = 2;
     TileSet t_box, t_local, t_local1, t_atom, t_atom1, t_ghost, t_cell, t_border;
     TileSet t_scan_cell, t_scan_local, t_scan_atom;
@@ -35,12 +38,14 @@ struct Launch {
 
 void phase_deform(const Launch& L);
 void phase_initial(const Launch& L);
+void phase_initial_decide(const Launch& L);  // initial + fused displacement check + decide
 void phase_decide(const Launch& L);
 void phase_decide_only(const Launch& L);
 void phase_remove_ghosts(const Launch& L);
 void phase_set_rebuild_now(const Launch& L);
 void phase_sort_flags(const Launch& L);
 void phase_zero(const Launch& L);
+void phase_zero_fused(const Launch& L);  // leaves forces of consumer boxes to phase_force_fused
 void phase_pbc(const Launch& L);
 void phase_sort(const Launch& L);
 void phase_sort_fused(const Launch& L);
@@ -49,6 +54,7 @@ void phase_forward(const Launch& L);
 void phase_build(const Launch& L, hipEvent_t ev0, hipEvent_t ev1, hipEvent_t ev2);
 void phase_build_fused(const Launch& L, hipEvent_t ev0, hipEvent_t ev1, hipEvent_t ev2);
 void phase_force(const Launch& L);
+void phase_force_fused(const Launch& L);  // assigns forces of consumer boxes
 void phase_reverse(const Launch& L);
 void phase_final(const Launch& L);
 void phase_step_end(const Launch& L);
