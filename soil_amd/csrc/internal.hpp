@@ -78,7 +78,7 @@ struct BoxState {
     uint32_t rebuild_now;                       // this step takes the FullRebuild path
     uint32_t err;
     uint32_t ring_pos;                          // geometry slot of the current step
-    uint32_t pad;
+    uint32_t local_cells_valid;                 // locals are in cell order from this step's sort (neighbor.rs:503,540)
     uint64_t csr_bytes, csr_builds;             // algorithmic bytes of CSR builds (timing)
     uint64_t step_limit;                        // the box stops (every kernel skips it) at this step count
 };
@@ -127,8 +127,7 @@ struct BoxDev {
     uint32_t* tsrc;       // cap_pairs: row atom of each incoming pair, grouped by column
     // border compaction scratch: per tile counts of the two swaps
     uint32_t* ctile;      // CSR: per local tile, entries staged in the temp region (or overflow mark)
-    uint32_t* btile;      // 2 per tile of BLK atoms (swap totals, then exclusive prefixes)
-    uint32_t* bwave;      // 2 per wave of 64 atoms (prefix inside the tile)
+    uint32_t* btile;      // 2 per border tile (swap totals, then exclusive prefixes)
     ColDev cols[SOIL_MAX_COLS];
     int32_t ncols;
     uint32_t cap_local, cap_atoms, cap_cells, cap_pairs, cap_tiles;

@@ -369,6 +369,7 @@ __global__ void k_pre_sort(KARGS) {
         if (g.bounds_changed) st->state = 0u;
         st->do_sort = (st->nlocal > 0 && g.ncells > 0 && (g.periodic_sort || st->state == 0u)) ? 1u : 0u;
         st->not_identity = 0;
+        st->local_cells_valid = 0;
     }
 }
 
@@ -381,10 +382,15 @@ __global__ void k_post_sort(KARGS);
 
 // ── stable counting sort by key ────────────────────────────────────────────
 // SPATIAL: locals by cell (perm into `sorted`), BUILD: locals+ghosts by cell.
+// BUILD with `fast`: the locals are already in cell order from this step's
+// spatial sort (the reference reuses sorted_local_cell, neighbor.rs:1036-1064):
+// `counts` still holds the local counts per cell and `sstart` their starts, so
+// only the ghosts are counted, scattered and ranked; a local k goes to
+// bin_start[c] + (k - sstart[c]).
 template <int W>
 struct SortView {
     uint32_t n, nkeys;
-    bool active;
+    bool active, fast;
     uint32_t *key, *slot, *counts, *start, *tmp, *out;
 };
 
@@ -393,12 +399,14 @@ __device__ __forceinline__ SortView<W> sort_view(const BoxDev& b) {
     SortView<W> v;
     const BoxState* st = b.st;
     const Geom& g = geom_of(b);
+    v.fast = false;
     if (W == SORT_SPATIAL) {
         v.n = st->nlocal; v.nkeys = g.ncells; v.active = st->do_sort != 0;
         v.key = b.scell; v.slot = b.slot; v.counts = b.counts; v.start = b.sstart; v.tmp = b.tmp; v.out = b.perm;
     } else if (W == SORT_BUILD) {
         v.n = st->nlocal + st->nghost; v.nkeys = g.ncells;
         v.active = st->rebuild_now != 0 && (st->err & (ERR_GHOST_CAPACITY)) == 0;
+        v.fast = st->local_cells_valid != 0;
         v.key = b.cell; v.slot = b.slot; v.counts = b.counts; v.start = b.bin_start; v.tmp = b.tmp; v.out = b.sorted;
     }
     return v;
@@ -411,7 +419,7 @@ __global__ void __launch_bounds__(BLK) k_sort_zero(KARGS) {
         const BoxDev& b = *boxes[tl.box];
         if (!live(b)) continue;
         SortView<W> v = sort_view<W>(b);
-        if (!v.active) continue;
+        if (!v.active || v.fast) continue;
         const uint32_t c = tl.start + threadIdx.x;
         if (c <= v.nkeys) v.counts[c] = 0u;
     }
@@ -429,7 +437,9 @@ __global__ void __launch_bounds__(BLK) k_sort_count(KARGS) {
         // Cell sorts: the input is nearly cell-ordered (locals were sorted at the
         // last rebuild, ghosts follow their sources), so lanes form runs of equal
         // keys; one atomic per run hands out consecutive slots.
-        const bool act = i < v.n;
+        const uint32_t first = v.fast ? b.st->nlocal : 0u;
+        if (v.fast && i < first) v.key[i] = b.scell[b.perm[i]];  // sorted_local_cell
+        const bool act = i >= first && i < v.n;
         uint32_t key = 0xffffffffu;
         if (act) {
             key = cell_of(geom_of(b), b.x[i], b.y[i], b.z[i]);
@@ -439,7 +449,8 @@ __global__ void __launch_bounds__(BLK) k_sort_count(KARGS) {
         const uint32_t prev = __shfl_up(key, 1, 64);
         const bool head = act && (lane == 0u || prev != key);
         const uint64_t heads = __ballot(head);
-        const uint32_t nact = (uint32_t)__popcll(__ballot(act));
+        const uint64_t actm = __ballot(act);
+        const uint32_t nact = actm ? 64u - (uint32_t)__builtin_clzll(actm) : 0u;  // one past the last active lane
         const uint64_t upto = lane == 63u ? ~0ull : ((2ull << lane) - 1ull);
         const uint32_t leader = 63u - (uint32_t)__builtin_clzll((heads & upto) | 1ull);
         uint32_t base = 0;
@@ -462,7 +473,7 @@ __global__ void __launch_bounds__(BLK) k_sort_scatter(KARGS) {
         SortView<W> v = sort_view<W>(b);
         if (!v.active) continue;
         const uint32_t i = tl.start + threadIdx.x;
-        if (i >= v.n) continue;
+        if (i >= v.n || (v.fast && i < b.st->nlocal)) continue;
         const uint32_t key = v.key[i];
         v.tmp[v.start[key] + v.slot[i]] = i;
     }
@@ -479,19 +490,29 @@ __global__ void __launch_bounds__(BLK) k_sort_rank(KARGS) {
         const uint32_t i = tl.start + threadIdx.x;
         if (i >= v.n) continue;
         const uint32_t key = v.key[i];
-        const uint32_t s = v.start[key], e = v.start[key + 1];
-        uint32_t r = 0;
-        if (e - s > 1u)
-            for (uint32_t m = s; m < e; ++m) r += v.tmp[m] < i ? 1u : 0u;
-        const uint32_t dst = s + r;
+        uint32_t s = v.start[key], e = v.start[key + 1];
+        uint32_t dst;
+        if (v.fast) {
+            const uint32_t ls = b.sstart[key], nloc = b.sstart[key + 1] - ls;
+            if (i < b.st->nlocal) {
+                dst = s + (i - ls);
+            } else {
+                s += nloc;  // the cell's ghosts follow its locals
+                uint32_t r = 0;
+                if (e - s > 1u)
+                    for (uint32_t m = s; m < e; ++m) r += v.tmp[m] < i ? 1u : 0u;
+                dst = s + r;
+            }
+        } else {
+            uint32_t r = 0;
+            if (e - s > 1u)
+                for (uint32_t m = s; m < e; ++m) r += v.tmp[m] < i ? 1u : 0u;
+            dst = s + r;
+        }
         v.out[dst] = i;
         if (W == SORT_BUILD) {
             b.sidx[i] = dst;
-            double* sp = b.spos + 4 * (size_t)dst;
-            sp[0] = b.x[i];
-            sp[1] = b.y[i];
-            sp[2] = b.z[i];
-            sp[3] = b.rad[i];
+            ((double4*)b.spos)[dst] = ((const double4*)b.xr)[i];  // bin_sorted_pos (+ radius)
         }
         if (W == SORT_SPATIAL) {
             wave_flag(&b.st->not_identity, dst != i);
@@ -586,6 +607,7 @@ __global__ void k_post_sort(KARGS) {
             st->last_valid = 0;
             st->state = 0u;
         }
+        if (st->do_sort) st->local_cells_valid = 1;
         st->rebuild_now = st->state == 0u ? 1u : 0u;
         if (FUSED && st->rebuild_now) {
             st->state = 1u;
@@ -682,6 +704,7 @@ __global__ void __launch_bounds__(BLK) k_scan_parts(KARGS) {
         if (threadIdx.x == 0) {
             BoxState* st = b.st;
             if (W == SCAN_CSR) {
+                st->local_cells_valid = 0;  // consumed by this build (neighbor.rs:596)
                 st->npairs = carry;
                 if (carry > b.cap_pairs) {
                     atomicOr(&st->err, ERR_PAIR_CAPACITY);
@@ -742,8 +765,14 @@ __device__ __forceinline__ void border_preds(const Geom& g, const BoxDev& b, uin
     p1 = v >= g.thr_hi[d];
 }
 
-// Per-wave (64 consecutive atoms) counts of the two swaps' selections, and
-// per-tile (BLK atoms) totals for the scan.
+// Border tiles: BORDER_TILE consecutive atoms per block; wave w reads
+// BORDER_PER chunks of 64 consecutive atoms (coalesced), chunk c of wave w
+// starting at tile + (w * BORDER_PER + c) * 64.
+__device__ __forceinline__ uint32_t border_atom(uint32_t tile_start, uint32_t wid, int c, uint32_t lane) {
+    return tile_start + (wid * BORDER_PER + (uint32_t)c) * 64u + lane;
+}
+
+// Per-tile totals of the two swaps' selections (scanned by k_border_scan).
 __global__ void __launch_bounds__(BLK) k_border_count(KARGS, int d) {
     __shared__ uint32_t wc[2][BLK / 64];
     FOR_TILES(t) {
@@ -755,26 +784,26 @@ __global__ void __launch_bounds__(BLK) k_border_count(KARGS, int d) {
         if (!st->rebuild_now || !g.periodic[d]) continue;
         const uint32_t scan_end = st->nlocal + (d == 0 ? 0u : st->nghost);
         if (tl.start >= scan_end) continue;
-        const uint32_t i = tl.start + threadIdx.x;
-        bool p0 = false, p1 = false;
-        if (i < scan_end) border_preds(g, b, i, d, p0, p1);
-        const uint64_t m0 = __ballot(p0), m1 = __ballot(p1);
-        const uint32_t wid = threadIdx.x >> 6;
-        if ((threadIdx.x & 63u) == 0u) {
-            wc[0][wid] = (uint32_t)__popcll(m0);
-            wc[1][wid] = (uint32_t)__popcll(m1);
+        const uint32_t wid = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+        uint32_t c0 = 0, c1 = 0;
+#pragma unroll
+        for (int c = 0; c < BORDER_PER; ++c) {
+            const uint32_t i = border_atom(tl.start, wid, c, lane);
+            bool p0 = false, p1 = false;
+            if (i < scan_end) border_preds(g, b, i, d, p0, p1);
+            c0 += (uint32_t)__popcll(__ballot(p0));
+            c1 += (uint32_t)__popcll(__ballot(p1));
+        }
+        if (lane == 0u) {
+            wc[0][wid] = c0;
+            wc[1][wid] = c1;
         }
         __syncthreads();
         if (threadIdx.x < 2u) {
-            const uint32_t k = threadIdx.x;
-            uint32_t run = 0;
-            const uint32_t w0 = tl.start >> 6;
+            uint32_t sum = 0;
 #pragma unroll
-            for (uint32_t w = 0; w < BLK / 64; ++w) {
-                b.bwave[2 * (w0 + w) + k] = run;  // exclusive prefix of the wave inside its tile
-                run += wc[k][w];
-            }
-            b.btile[2 * (tl.start / BLK) + k] = run;
+            for (uint32_t w = 0; w < BLK / 64; ++w) sum += wc[threadIdx.x][w];
+            b.btile[2 * (tl.start / BORDER_TILE) + threadIdx.x] = sum;
         }
         __syncthreads();
     }
@@ -793,7 +822,7 @@ __global__ void __launch_bounds__(BLK) k_border_scan(KARGS, int d) {
         const uint32_t scan_end = st->nlocal + nghost0;
         uint32_t c0 = 0, c1 = 0;
         if (g.periodic[d] && !(st->err & ERR_GHOST_CAPACITY)) {
-            const uint32_t nt = (scan_end + BLK - 1) / BLK;
+            const uint32_t nt = (scan_end + BORDER_TILE - 1) / BORDER_TILE;
             constexpr int PER = 8;
             for (uint32_t base = 0; base < nt; base += BLK * PER) {
                 uint32_t v0[PER], v1[PER], s0 = 0, s1 = 0;
@@ -875,9 +904,12 @@ __device__ __forceinline__ void write_ghost(const BoxDev& b, const Geom& g, uint
     }
 }
 
-// Ordered append: each wave writes its selected atoms at its scanned offset
-// plus the lane's rank among the selected lanes (scan order is preserved).
+// Ordered append. The block lists its tile's selected atoms in LDS in scan
+// order (lo swap first, then hi swap), then all lanes copy one ghost each
+// from the list: ghosts of a tile are consecutive, so the writes coalesce.
 __global__ void __launch_bounds__(BLK) k_border_write(KARGS, int d) {
+    __shared__ uint32_t wc[2][BLK / 64];
+    __shared__ uint32_t list[2 * BORDER_TILE];
     FOR_TILES(t) {
         const Tile tl = tiles[t];
         const BoxDev& b = *boxes[tl.box];
@@ -887,17 +919,52 @@ __global__ void __launch_bounds__(BLK) k_border_write(KARGS, int d) {
         if (!st->rebuild_now || !g.periodic[d] || (st->err & ERR_GHOST_CAPACITY)) continue;
         const uint32_t scan_end = st->swap_start[2 * d];
         if (tl.start >= scan_end) continue;
-        const uint32_t i = tl.start + threadIdx.x;
-        bool p0 = false, p1 = false;
-        if (i < scan_end) border_preds(g, b, i, d, p0, p1);
-        const uint64_t m0 = __ballot(p0), m1 = __ballot(p1);
-        if (!(p0 || p1)) continue;
-        const uint32_t w = i >> 6, tix = i / BLK;
-        if (p0)
-            write_ghost(b, g, st->swap_start[2 * d] + b.btile[2 * tix] + b.bwave[2 * w] + lane_prefix(m0), i, 2 * d);
-        if (p1)
-            write_ghost(b, g, st->swap_start[2 * d + 1] + b.btile[2 * tix + 1] + b.bwave[2 * w + 1] + lane_prefix(m1),
-                        i, 2 * d + 1);
+        const uint32_t wid = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+        uint64_t m0[BORDER_PER], m1[BORDER_PER];
+        uint32_t c0 = 0, c1 = 0;
+#pragma unroll
+        for (int c = 0; c < BORDER_PER; ++c) {
+            const uint32_t i = border_atom(tl.start, wid, c, lane);
+            bool p0 = false, p1 = false;
+            if (i < scan_end) border_preds(g, b, i, d, p0, p1);
+            m0[c] = __ballot(p0);
+            m1[c] = __ballot(p1);
+            c0 += (uint32_t)__popcll(m0[c]);
+            c1 += (uint32_t)__popcll(m1[c]);
+        }
+        if (lane == 0u) {
+            wc[0][wid] = c0;
+            wc[1][wid] = c1;
+        }
+        __syncthreads();
+        uint32_t w0 = 0, w1 = 0, t0 = 0, t1 = 0;
+#pragma unroll
+        for (uint32_t w = 0; w < BLK / 64; ++w) {
+            if (w < wid) {
+                w0 += wc[0][w];
+                w1 += wc[1][w];
+            }
+            t0 += wc[0][w];
+            t1 += wc[1][w];
+        }
+#pragma unroll
+        for (int c = 0; c < BORDER_PER; ++c) {
+            const uint32_t i = border_atom(tl.start, wid, c, lane);
+            if ((m0[c] >> lane) & 1ull) list[w0 + lane_prefix(m0[c])] = i;
+            if ((m1[c] >> lane) & 1ull) list[t0 + w1 + lane_prefix(m1[c])] = i;
+            w0 += (uint32_t)__popcll(m0[c]);
+            w1 += (uint32_t)__popcll(m1[c]);
+        }
+        __syncthreads();
+        const uint32_t tix = tl.start / BORDER_TILE;
+        const uint32_t base0 = st->swap_start[2 * d] + b.btile[2 * tix];
+        const uint32_t base1 = st->swap_start[2 * d + 1] + b.btile[2 * tix + 1];
+        for (uint32_t r = threadIdx.x; r < t0 + t1; r += BLK) {
+            const uint32_t i = list[r];
+            if (r < t0) write_ghost(b, g, base0 + r, i, 2 * d);
+            else write_ghost(b, g, base1 + (r - t0), i, 2 * d + 1);
+        }
+        __syncthreads();
     }
 }
 
@@ -1562,9 +1629,9 @@ void phase_sort_fused(const Launch& L) {
 }
 void phase_borders(const Launch& L) {
     for (int d = 0; d < 3; ++d) {
-        launch(k_border_count, L.t_atom, L.boxes, L.stream, L.maxgrid, d);
+        launch(k_border_count, L.t_border, L.boxes, L.stream, L.maxgrid, d);
         launch(k_border_scan, L.t_box, L.boxes, L.stream, L.maxgrid, d);
-        launch(k_border_write, L.t_atom, L.boxes, L.stream, L.maxgrid, d);
+        launch(k_border_write, L.t_border, L.boxes, L.stream, L.maxgrid, d);
     }
 }
 void phase_forward(const Launch& L) { launch(k_forward, L.t_ghost, L.boxes, L.stream, L.maxgrid); }

@@ -8,7 +8,8 @@ namespace soil_hip {
 constexpr uint32_t BLK = 256;              // threads per block (4 waves of 64)
 constexpr int SCAN_PER_THREAD = 8;
 constexpr uint32_t SCAN_TILE = BLK * SCAN_PER_THREAD;
-constexpr uint32_t BORDER_TILE = 1024;     // atoms per border-compaction tile
+constexpr uint32_t BORDER_PER = 4;                 // 64-atom chunks per wave in a border tile
+constexpr uint32_t BORDER_TILE = BLK * BORDER_PER;  // atoms per border-compaction tile
 
 enum { SORT_SPATIAL = 0, SORT_BUILD = 1 };
 enum { SCAN_CELLS_SPATIAL = 0, SCAN_CELLS_BUILD = 1, SCAN_CSR = 2, SCAN_TRANSPOSE = 3 };
