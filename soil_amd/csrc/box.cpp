@@ -117,7 +117,7 @@ void carve_atoms(Carver& c, BoxDev& d, const Caps& k, const std::vector<HostCol>
     c.take(d.offsets, L1);
     c.take(d.tcount, A + 1); c.take(d.tstart, A + 1);
     c.take(d.ctile, (size_t)k.local / BLK + 2);
-    c.take(d.btile, 2 * ((size_t)k.atoms / BORDER_TILE + 2));
+    c.take(d.bcnt, 6 * ((size_t)k.atoms / BORDER_TILE + 2));
     for (size_t i = 0; i < cols.size(); ++i) {
         c.take(d.cols[i].data, A * cols[i].width);
         c.take(d.cols[i].alt, A * cols[i].width);

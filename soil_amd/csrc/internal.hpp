@@ -127,7 +127,7 @@ struct BoxDev {
     uint32_t* tsrc;       // cap_pairs: row atom of each incoming pair, grouped by column
     // border compaction scratch: per tile counts of the two swaps
     uint32_t* ctile;      // CSR: per local tile, entries staged in the temp region (or overflow mark)
-    uint32_t* btile;      // 2 per border tile (swap totals, then exclusive prefixes)
+    uint32_t* bcnt;       // 6 per border tile: selections of the 6 swaps
     ColDev cols[SOIL_MAX_COLS];
     int32_t ncols;
     uint32_t cap_local, cap_atoms, cap_cells, cap_pairs, cap_tiles;

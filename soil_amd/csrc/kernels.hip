@@ -752,6 +752,16 @@ __global__ void __launch_bounds__(BLK) k_scan_down(KARGS) {
 }
 
 // ── borders: comm.rs:682-860 + pack_border_atoms :288-358 ──────────────────
+// selection value of a position along dim d: the coordinate, or lamda when triclinic
+__device__ __forceinline__ double border_coord(const Geom& g, double x, double y, double z, int d) {
+    if (g.triclinic) {
+        double lx, ly, lz;
+        x2lamda(g, x, y, z, lx, ly, lz);
+        return d == 0 ? lx : (d == 1 ? ly : lz);
+    }
+    return d == 0 ? x : (d == 1 ? y : z);
+}
+
 __device__ __forceinline__ void border_preds(const Geom& g, const BoxDev& b, uint32_t i, int d, bool& p0, bool& p1) {
     double v;
     if (g.triclinic) {
@@ -772,105 +782,69 @@ __device__ __forceinline__ uint32_t border_atom(uint32_t tile_start, uint32_t wi
     return tile_start + (wid * BORDER_PER + (uint32_t)c) * 64u + lane;
 }
 
-// Per-tile totals of the two swaps' selections (scanned by k_border_scan).
-__global__ void __launch_bounds__(BLK) k_border_count(KARGS, int d) {
-    __shared__ uint32_t wc[2][BLK / 64];
+// Selection counts per border tile for all three dims at once: bcnt[6 t + 2 d
+// + side] = atoms of tile t among the locals that swap (d, side) selects.
+// Tiles past the locals start at zero; each dim's write adds the ghosts it
+// creates to the later dims' counts (their scans include those ghosts).
+__global__ void __launch_bounds__(BLK) k_border_count(KARGS) {
+    __shared__ uint32_t wc[6][BLK / 64];
     FOR_TILES(t) {
         const Tile tl = tiles[t];
         const BoxDev& b = *boxes[tl.box];
         if (!live(b)) continue;
         const BoxState* st = b.st;
         const Geom& g = geom_of(b);
-        if (!st->rebuild_now || !g.periodic[d]) continue;
-        const uint32_t scan_end = st->nlocal + (d == 0 ? 0u : st->nghost);
-        if (tl.start >= scan_end) continue;
+        if (!st->rebuild_now) continue;
+        const uint32_t nl = st->nlocal;
+        const uint32_t tix = tl.start / BORDER_TILE;
+        if (tl.start >= nl) {  // block-uniform
+            if (threadIdx.x < 6u) b.bcnt[6 * tix + threadIdx.x] = 0u;
+            continue;
+        }
         const uint32_t wid = threadIdx.x >> 6, lane = threadIdx.x & 63u;
-        uint32_t c0 = 0, c1 = 0;
+        uint32_t c[6] = {0u, 0u, 0u, 0u, 0u, 0u};
 #pragma unroll
-        for (int c = 0; c < BORDER_PER; ++c) {
-            const uint32_t i = border_atom(tl.start, wid, c, lane);
-            bool p0 = false, p1 = false;
-            if (i < scan_end) border_preds(g, b, i, d, p0, p1);
-            c0 += (uint32_t)__popcll(__ballot(p0));
-            c1 += (uint32_t)__popcll(__ballot(p1));
+        for (int k = 0; k < BORDER_PER; ++k) {
+            const uint32_t i = border_atom(tl.start, wid, k, lane);
+            const bool in = i < nl;
+            double x = 0.0, y = 0.0, z = 0.0;
+            if (in) {
+                x = b.x[i];
+                y = b.y[i];
+                z = b.z[i];
+            }
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                bool p0 = false, p1 = false;
+                if (in && g.periodic[d]) {
+                    const double v = border_coord(g, x, y, z, d);
+                    p0 = v < g.thr_lo[d];
+                    p1 = v >= g.thr_hi[d];
+                }
+                c[2 * d] += (uint32_t)__popcll(__ballot(p0));
+                c[2 * d + 1] += (uint32_t)__popcll(__ballot(p1));
+            }
         }
-        if (lane == 0u) {
-            wc[0][wid] = c0;
-            wc[1][wid] = c1;
-        }
+        if (lane == 0u)
+#pragma unroll
+            for (int q = 0; q < 6; ++q) wc[q][wid] = c[q];
         __syncthreads();
-        if (threadIdx.x < 2u) {
+        if (threadIdx.x < 6u) {
             uint32_t sum = 0;
 #pragma unroll
             for (uint32_t w = 0; w < BLK / 64; ++w) sum += wc[threadIdx.x][w];
-            b.btile[2 * (tl.start / BORDER_TILE) + threadIdx.x] = sum;
-        }
-        __syncthreads();
-    }
-}
-
-// Exclusive scan of the per-tile totals (one block per box); sets the swap table.
-__global__ void __launch_bounds__(BLK) k_border_scan(KARGS, int d) {
-    __shared__ uint32_t lds[BLK / 64];
-    FOR_TILES(t) {
-        const BoxDev& b = *boxes[tiles[t].box];
-        if (!live(b)) continue;
-        BoxState* st = b.st;
-        const Geom& g = geom_of(b);
-        if (!st->rebuild_now) continue;
-        const uint32_t nghost0 = d == 0 ? 0u : st->nghost;  // borders truncates old ghosts first
-        const uint32_t scan_end = st->nlocal + nghost0;
-        uint32_t c0 = 0, c1 = 0;
-        if (g.periodic[d] && !(st->err & ERR_GHOST_CAPACITY)) {
-            const uint32_t nt = (scan_end + BORDER_TILE - 1) / BORDER_TILE;
-            constexpr int PER = 8;
-            for (uint32_t base = 0; base < nt; base += BLK * PER) {
-                uint32_t v0[PER], v1[PER], s0 = 0, s1 = 0;
-                const uint32_t k0 = base + threadIdx.x * PER;
-#pragma unroll
-                for (int q = 0; q < PER; ++q) {
-                    v0[q] = k0 + q < nt ? b.btile[2 * (k0 + q)] : 0u;
-                    v1[q] = k0 + q < nt ? b.btile[2 * (k0 + q) + 1] : 0u;
-                    s0 += v0[q];
-                    s1 += v1[q];
-                }
-                uint32_t tot0, tot1;
-                uint32_t e0 = block_excl_scan(s0, lds, tot0) + c0;
-                uint32_t e1 = block_excl_scan(s1, lds, tot1) + c1;
-#pragma unroll
-                for (int q = 0; q < PER; ++q) {
-                    if (k0 + q < nt) {
-                        b.btile[2 * (k0 + q)] = e0;
-                        b.btile[2 * (k0 + q) + 1] = e1;
-                    }
-                    e0 += v0[q];
-                    e1 += v1[q];
-                }
-                c0 += tot0;
-                c1 += tot1;
-            }
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            if ((size_t)scan_end + c0 + c1 > b.cap_atoms) {
-                atomicOr(&st->err, ERR_GHOST_CAPACITY);
-                c0 = c1 = 0;
-            }
-            st->swap_start[2 * d] = scan_end;
-            st->swap_count[2 * d] = c0;
-            st->swap_start[2 * d + 1] = scan_end + c0;
-            st->swap_count[2 * d + 1] = c1;
-            st->nghost = nghost0 + c0 + c1;
+            b.bcnt[6 * tix + threadIdx.x] = sum;
         }
         __syncthreads();
     }
 }
 
 // pack_border + unpack_atom (atom.rs:525-571): ghost g is a copy of atom i shifted by swap s
-__device__ __forceinline__ void write_ghost(const BoxDev& b, const Geom& g, uint32_t dst, uint32_t i, int s) {
-    const double gx = b.x[i] + g.swap_shift[s][0];
-    const double gy = b.y[i] + g.swap_shift[s][1];
-    const double gz = b.z[i] + g.swap_shift[s][2];
+__device__ __forceinline__ void write_ghost(const BoxDev& b, const Geom& g, uint32_t dst, uint32_t i, int s,
+                                            double& gx, double& gy, double& gz) {
+    gx = b.x[i] + g.swap_shift[s][0];
+    gy = b.y[i] + g.swap_shift[s][1];
+    gz = b.z[i] + g.swap_shift[s][2];
     b.x[dst] = gx;
     b.y[dst] = gy;
     b.z[dst] = gz;
@@ -904,22 +878,90 @@ __device__ __forceinline__ void write_ghost(const BoxDev& b, const Geom& g, uint
     }
 }
 
-// Ordered append. The block lists its tile's selected atoms in LDS in scan
-// order (lo swap first, then hi swap), then all lanes copy one ghost each
-// from the list: ghosts of a tile are consecutive, so the writes coalesce.
+// Dim d of borders (comm.rs:736-853) in one launch. Every block sums the
+// dim's per-tile counts (bcnt) for the totals and for its own tile's prefix,
+// lists its tile's selected atoms in LDS in scan order (lo swap, then hi swap)
+// and copies one ghost per lane from the list: a tile's ghosts are
+// consecutive, so the writes coalesce. The first tile's block writes the dim's
+// swap table. Ghosts created here are counted (LDS, then one atomic per tile)
+// into the selections of the later dims, whose scans include them.
 __global__ void __launch_bounds__(BLK) k_border_write(KARGS, int d) {
     __shared__ uint32_t wc[2][BLK / 64];
+    __shared__ uint32_t red[4][BLK / 64];
     __shared__ uint32_t list[2 * BORDER_TILE];
+    __shared__ uint32_t lc[16];
     FOR_TILES(t) {
         const Tile tl = tiles[t];
         const BoxDev& b = *boxes[tl.box];
         if (!live(b)) continue;
-        const BoxState* st = b.st;
+        BoxState* st = b.st;
         const Geom& g = geom_of(b);
-        if (!st->rebuild_now || !g.periodic[d] || (st->err & ERR_GHOST_CAPACITY)) continue;
-        const uint32_t scan_end = st->swap_start[2 * d];
-        if (tl.start >= scan_end) continue;
+        if (!st->rebuild_now) continue;
+        const uint32_t nl = st->nlocal;
+        // candidates: locals and the ghosts of the earlier dims (borders truncates old ghosts first)
+        const uint32_t scan_end = d == 0 ? nl : st->swap_start[2 * d - 1] + st->swap_count[2 * d - 1];
+        const uint32_t tix = tl.start / BORDER_TILE;
+        if (st->err & ERR_GHOST_CAPACITY) continue;
+        if (!g.periodic[d]) {
+            if (tix == 0 && threadIdx.x == 0) {
+                st->swap_start[2 * d] = scan_end;
+                st->swap_count[2 * d] = 0u;
+                st->swap_start[2 * d + 1] = scan_end;
+                st->swap_count[2 * d + 1] = 0u;
+                st->nghost = scan_end - nl;
+            }
+            continue;
+        }
+        // totals of the dim and this tile's exclusive prefixes
+        const uint32_t nt = (scan_end + BORDER_TILE - 1) / BORDER_TILE;
+        uint32_t pb0 = 0, pb1 = 0, tot0 = 0, tot1 = 0;
+        for (uint32_t k = threadIdx.x; k < nt; k += BLK) {
+            const uint32_t v0 = b.bcnt[6 * k + 2 * d], v1 = b.bcnt[6 * k + 2 * d + 1];
+            tot0 += v0;
+            tot1 += v1;
+            if (k < tix) {
+                pb0 += v0;
+                pb1 += v1;
+            }
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            pb0 += __shfl_xor(pb0, o, 64);
+            pb1 += __shfl_xor(pb1, o, 64);
+            tot0 += __shfl_xor(tot0, o, 64);
+            tot1 += __shfl_xor(tot1, o, 64);
+        }
         const uint32_t wid = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+        if (lane == 0u) {
+            red[0][wid] = pb0;
+            red[1][wid] = pb1;
+            red[2][wid] = tot0;
+            red[3][wid] = tot1;
+        }
+        if (threadIdx.x < 16u) lc[threadIdx.x] = 0u;
+        __syncthreads();
+        pb0 = pb1 = tot0 = tot1 = 0;
+#pragma unroll
+        for (uint32_t w = 0; w < BLK / 64; ++w) {
+            pb0 += red[0][w];
+            pb1 += red[1][w];
+            tot0 += red[2][w];
+            tot1 += red[3][w];
+        }
+        const bool over = (uint64_t)scan_end + tot0 + tot1 > (uint64_t)b.cap_atoms;
+        if (tix == 0 && threadIdx.x == 0) {
+            if (over) {
+                atomicOr(&st->err, ERR_GHOST_CAPACITY);
+            } else {
+                st->swap_start[2 * d] = scan_end;
+                st->swap_count[2 * d] = tot0;
+                st->swap_start[2 * d + 1] = scan_end + tot0;
+                st->swap_count[2 * d + 1] = tot1;
+                st->nghost = scan_end + tot0 + tot1 - nl;
+            }
+        }
+        __syncthreads();  // red[] is reused by the next tile
+        if (over || tl.start >= scan_end) continue;  // block-uniform
         uint64_t m0[BORDER_PER], m1[BORDER_PER];
         uint32_t c0 = 0, c1 = 0;
 #pragma unroll
@@ -956,13 +998,31 @@ __global__ void __launch_bounds__(BLK) k_border_write(KARGS, int d) {
             w1 += (uint32_t)__popcll(m1[c]);
         }
         __syncthreads();
-        const uint32_t tix = tl.start / BORDER_TILE;
-        const uint32_t base0 = st->swap_start[2 * d] + b.btile[2 * tix];
-        const uint32_t base1 = st->swap_start[2 * d + 1] + b.btile[2 * tix + 1];
+        const uint32_t base0 = scan_end + pb0;
+        const uint32_t base1 = scan_end + tot0 + pb1;
+        // a tile's lo (hi) ghosts span at most two border tiles: lc[range*8 + which*4 + later*2 + side]
+        const uint32_t ft0 = base0 / BORDER_TILE, ft1 = base1 / BORDER_TILE;
         for (uint32_t r = threadIdx.x; r < t0 + t1; r += BLK) {
             const uint32_t i = list[r];
-            if (r < t0) write_ghost(b, g, base0 + r, i, 2 * d);
-            else write_ghost(b, g, base1 + (r - t0), i, 2 * d + 1);
+            const bool lo = r < t0;
+            const uint32_t dst = lo ? base0 + r : base1 + (r - t0);
+            double gx, gy, gz;
+            write_ghost(b, g, dst, i, lo ? 2 * d : 2 * d + 1, gx, gy, gz);
+            for (int e = d + 1; e < 3; ++e) {
+                if (!g.periodic[e]) continue;
+                const double v = border_coord(g, gx, gy, gz, e);
+                const uint32_t which = dst / BORDER_TILE - (lo ? ft0 : ft1);
+                const uint32_t slot = (lo ? 0u : 8u) + which * 4u + (uint32_t)(e - d - 1) * 2u;
+                if (v < g.thr_lo[e]) atomicAdd(&lc[slot], 1u);
+                if (v >= g.thr_hi[e]) atomicAdd(&lc[slot + 1u], 1u);
+            }
+        }
+        __syncthreads();
+        if (d < 2 && threadIdx.x < 16u && lc[threadIdx.x]) {
+            const uint32_t q = threadIdx.x;
+            const uint32_t tile = ((q & 8u) ? ft1 : ft0) + ((q >> 2) & 1u);
+            const uint32_t e = (uint32_t)d + 1u + ((q >> 1) & 1u);
+            atomicAdd(&b.bcnt[6 * tile + 2 * e + (q & 1u)], lc[q]);
         }
         __syncthreads();
     }
@@ -1628,11 +1688,8 @@ void phase_sort_fused(const Launch& L) {
     launch(k_post_sort<true>, L.t_box, L.boxes, L.stream, L.maxgrid);
 }
 void phase_borders(const Launch& L) {
-    for (int d = 0; d < 3; ++d) {
-        launch(k_border_count, L.t_border, L.boxes, L.stream, L.maxgrid, d);
-        launch(k_border_scan, L.t_box, L.boxes, L.stream, L.maxgrid, d);
-        launch(k_border_write, L.t_border, L.boxes, L.stream, L.maxgrid, d);
-    }
+    launch(k_border_count, L.t_border, L.boxes, L.stream, L.maxgrid);
+    for (int d = 0; d < 3; ++d) launch(k_border_write, L.t_border, L.boxes, L.stream, L.maxgrid, d);
 }
 void phase_forward(const Launch& L) { launch(k_forward, L.t_ghost, L.boxes, L.stream, L.maxgrid); }
 void phase_build(const Launch& L, hipEvent_t ev0, hipEvent_t ev1, hipEvent_t ev2) {
