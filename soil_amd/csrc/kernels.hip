@@ -839,49 +839,81 @@ __global__ void __launch_bounds__(BLK) k_border_count(KARGS) {
     }
 }
 
-// pack_border + unpack_atom (atom.rs:525-571): ghost g is a copy of atom i shifted by swap s
-__device__ __forceinline__ void write_ghost(const BoxDev& b, const Geom& g, uint32_t dst, uint32_t i, int s,
-                                            double& gx, double& gy, double& gz) {
-    gx = b.x[i] + g.swap_shift[s][0];
-    gy = b.y[i] + g.swap_shift[s][1];
-    gz = b.z[i] + g.swap_shift[s][2];
+// pack_border + unpack_atom (atom.rs:525-571), split in two: the dim's kernel
+// writes what later dims and the chain need (position = source + shift, the
+// source index and the swap); k_ghost_fill copies the rest once all six swaps
+// are placed.
+__device__ __forceinline__ void place_ghost(const BoxDev& b, const Geom& g, uint32_t dst, uint32_t i, int s, double& gx,
+                                            double& gy, double& gz) {
+    // the source's {x, y, z, rad} record: for a ghost source that is the one
+    // placed by an earlier dim (its other fields are filled later)
+    const double4 p = ((const double4*)b.xr)[i];
+    gx = p.x + g.swap_shift[s][0];
+    gy = p.y + g.swap_shift[s][1];
+    gz = p.z + g.swap_shift[s][2];
     b.x[dst] = gx;
     b.y[dst] = gy;
     b.z[dst] = gz;
-    {
-        double* q = b.xr + 4 * (size_t)dst;
-        q[0] = gx;
-        q[1] = gy;
-        q[2] = gz;
-        q[3] = b.rad[i];
-    }
-    b.vx[dst] = b.vx[i] + g.swap_vshift[s][0];
-    b.vy[dst] = b.vy[i] + g.swap_vshift[s][1];
-    b.vz[dst] = b.vz[i] + g.swap_vshift[s][2];
-    b.fx[dst] = b.fx[i];
-    b.fy[dst] = b.fy[i];
-    b.fz[dst] = b.fz[i];
-    const double m = b.mass[i];
-    b.mass[dst] = m;
-    b.inv_mass[dst] = 1.0 / m;
-    b.rad[dst] = b.rad[i];
-    b.tag[dst] = b.tag[i];
-    b.type[dst] = b.type[i];
+    ((double4*)b.xr)[dst] = make_double4(gx, gy, gz, p.w);
     b.origin[dst] = (int32_t)i;
-    b.imx[dst] = b.imx[i];
-    b.imy[dst] = b.imy[i];
-    b.imz[dst] = b.imz[i];
     b.gswap[dst] = (uint8_t)s;
-    for (int c = 0; c < b.ncols; ++c) {
-        const ColDev& cd = b.cols[c];
-        for (int w = 0; w < cd.width; ++w) cd.data[(size_t)dst * cd.width + w] = cd.data[(size_t)i * cd.width + w];
+}
+
+// The remaining ghost fields. A ghost of a ghost copies its source, which
+// copied its own source, so every copied field equals the local at the root
+// of the chain; the velocity adds each swap's shift root-first, as the
+// swap-by-swap copies do; inv_mass = 1 / mass as in unpack_atom.
+__global__ void __launch_bounds__(BLK) k_ghost_fill(KARGS) {
+    FOR_TILES(t) {
+        const Tile tl = tiles[t];
+        const BoxDev& b = *boxes[tl.box];
+        if (!live(b)) continue;
+        const BoxState* st = b.st;
+        if (!st->rebuild_now || (st->err & ERR_GHOST_CAPACITY)) continue;
+        const uint32_t nl = st->nlocal;
+        const uint32_t gi = nl + tl.start + threadIdx.x;
+        if (gi >= nl + st->nghost) continue;
+        const Geom& g = geom_of(b);
+        int sw[3];
+        int depth = 0;
+        uint32_t r = gi;
+        while (r >= nl && depth < 3) {
+            sw[depth++] = b.gswap[r];
+            r = (uint32_t)b.origin[r];
+        }
+        double qx = b.vx[r], qy = b.vy[r], qz = b.vz[r];
+        for (int k = depth - 1; k >= 0; --k) {
+            const int s = sw[k];
+            qx = qx + g.swap_vshift[s][0];
+            qy = qy + g.swap_vshift[s][1];
+            qz = qz + g.swap_vshift[s][2];
+        }
+        b.vx[gi] = qx;
+        b.vy[gi] = qy;
+        b.vz[gi] = qz;
+        b.fx[gi] = b.fx[r];
+        b.fy[gi] = b.fy[r];
+        b.fz[gi] = b.fz[r];
+        const double m = b.mass[r];
+        b.mass[gi] = m;
+        b.inv_mass[gi] = 1.0 / m;
+        b.rad[gi] = b.rad[r];
+        b.tag[gi] = b.tag[r];
+        b.type[gi] = b.type[r];
+        b.imx[gi] = b.imx[r];
+        b.imy[gi] = b.imy[r];
+        b.imz[gi] = b.imz[r];
+        for (int c = 0; c < b.ncols; ++c) {
+            const ColDev& cd = b.cols[c];
+            for (int w = 0; w < cd.width; ++w) cd.data[(size_t)gi * cd.width + w] = cd.data[(size_t)r * cd.width + w];
+        }
     }
 }
 
 // Dim d of borders (comm.rs:736-853) in one launch. Every block sums the
 // dim's per-tile counts (bcnt) for the totals and for its own tile's prefix,
 // lists its tile's selected atoms in LDS in scan order (lo swap, then hi swap)
-// and copies one ghost per lane from the list: a tile's ghosts are
+// and places one ghost per lane from the list: a tile's ghosts are
 // consecutive, so the writes coalesce. The first tile's block writes the dim's
 // swap table. Ghosts created here are counted (LDS, then one atomic per tile)
 // into the selections of the later dims, whose scans include them.
@@ -1007,7 +1039,7 @@ __global__ void __launch_bounds__(BLK) k_border_write(KARGS, int d) {
             const bool lo = r < t0;
             const uint32_t dst = lo ? base0 + r : base1 + (r - t0);
             double gx, gy, gz;
-            write_ghost(b, g, dst, i, lo ? 2 * d : 2 * d + 1, gx, gy, gz);
+            place_ghost(b, g, dst, i, lo ? 2 * d : 2 * d + 1, gx, gy, gz);
             for (int e = d + 1; e < 3; ++e) {
                 if (!g.periodic[e]) continue;
                 const double v = border_coord(g, gx, gy, gz, e);
@@ -1690,6 +1722,7 @@ void phase_sort_fused(const Launch& L) {
 void phase_borders(const Launch& L) {
     launch(k_border_count, L.t_border, L.boxes, L.stream, L.maxgrid);
     for (int d = 0; d < 3; ++d) launch(k_border_write, L.t_border, L.boxes, L.stream, L.maxgrid, d);
+    launch(k_ghost_fill, L.t_ghost, L.boxes, L.stream, L.maxgrid);
 }
 void phase_forward(const Launch& L) { launch(k_forward, L.t_ghost, L.boxes, L.stream, L.maxgrid); }
 void phase_build(const Launch& L, hipEvent_t ev0, hipEvent_t ev1, hipEvent_t ev2) {
