@@ -389,7 +389,20 @@ int rebuild_tail(soil_hip_box* b) {
     return 0;
 }
 
+int step_plain_loop(soil_hip_box* b, int nsteps, bool& pending_final);
+
 int step_plain(soil_hip_box* b, int nsteps) {
+    // final_integration of the previous step, when it is carried into this
+    // step's initial_integration kernel; always launched before returning
+    bool pending_final = false;
+    int rc = step_plain_loop(b, nsteps, pending_final);
+    if (pending_final) phase_final(b->L);
+    if (rc) return rc;
+    if (b->timing) resolve_timing(b);
+    return 0;
+}
+
+int step_plain_loop(soil_hip_box* b, int nsteps, bool& pending_final) {
     for (int s = 0; s < nsteps; ++s) {
         b->model.apply_deform();
         b->model.sort_prepare();
@@ -401,8 +414,14 @@ int step_plain(soil_hip_box* b, int nsteps) {
         }
         b->last_geom = g;
         HIPCHK(hipMemcpyAsync(b->d_ring + b->ring_pos, &g, sizeof(Geom), hipMemcpyHostToDevice, b->stream));
+        if (pending_final && g.init_shear) {  // the one-time profile adds to the kicked velocities
+            phase_final(b->L);
+            pending_final = false;
+        }
         phase_deform(b->L);
-        phase_initial_decide(b->L);
+        if (pending_final) phase_final_initial_decide(b->L);
+        else phase_initial_decide(b->L);
+        pending_final = false;
         int rc = pull_state(b);
         if (rc) return rc;
         if (b->h_st->err) return check_err_bits(b->h_st->err);
@@ -424,11 +443,10 @@ int step_plain(soil_hip_box* b, int nsteps) {
             for (int d = 0; d < 3; ++d) b->L.rev_split[d] = g.disjoint[d] ? 0 : 1;
             phase_reverse(b->L);
         }
-        phase_final(b->L);
+        pending_final = true;  // launched with the next step's initial integration, or on return
         phase_step_end(b->L);
         b->ring_pos = (b->ring_pos + 1) % b->ring_len;
     }
-    if (b->timing) resolve_timing(b);
     return 0;
 }
 
@@ -646,6 +664,7 @@ int soil_hip_setup(soil_hip_box* b) {
     d.ring_len = b->ring_len;
     d.ring_stride = 1;
     d.harmonic_k = b->harmonic_k;
+    d.dt = b->model.dt;
     d.consumer = b->consumer ? 1 : 0;
     {
         const char* m = getenv("SOIL_HIP_CSR_MODE");

@@ -136,6 +136,7 @@ struct BoxDev {
     uint32_t ring_len;
     uint32_t ring_stride;   // Geom records between consecutive slots of this box
     double harmonic_k;
+    double dt;            // Atom::dt (constant per box)
     int32_t consumer;
     int32_t csr_mode;     // CSR strategy (bit-neutral): 0 LDS tiles, 1 two global passes, 2 one pass + row slots
 };

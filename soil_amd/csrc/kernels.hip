@@ -129,8 +129,9 @@ __global__ void __launch_bounds__(BLK) k_final(KARGS) {
         if (!live(b)) continue;
         const uint32_t i = tl.start + threadIdx.x;
         if (i >= b.st->nlocal) continue;
-        const Geom& g = geom_of(b);
-        const double h = 0.5 * g.dt * b.inv_mass[i];
+        // the box's dt, not the ring's: a final carried past the step's end must
+        // not read the next step's geometry slot
+        const double h = 0.5 * b.dt * b.inv_mass[i];
         b.vx[i] = b.vx[i] + h * b.fx[i];
         b.vy[i] = b.vy[i] + h * b.fy[i];
         b.vz[i] = b.vz[i] + h * b.fz[i];
@@ -186,7 +187,11 @@ __device__ __forceinline__ bool disp_exceeds(const Geom& g, const BoxDev& b, uin
 // displacement check of the following decide_rebuild runs on the new
 // positions while they are in registers (nothing between the two systems
 // changes positions or the state it depends on).
-template <bool DISP>
+// With FINAL the previous step's final_integration (v += h f) is applied
+// first, in registers: within a multi-step call nothing between the two
+// systems reads or writes velocities or forces (the host runs the final
+// separately when the one-time shear profile is due in between).
+template <bool DISP, bool FINAL>
 __global__ void __launch_bounds__(BLK) k_initial(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
@@ -200,9 +205,16 @@ __global__ void __launch_bounds__(BLK) k_initial(KARGS) {
         if (act) {
             const double dt = g.dt;
             const double h = 0.5 * dt * b.inv_mass[i];
-            const double v0 = b.vx[i] + h * b.fx[i];
-            const double v1 = b.vy[i] + h * b.fy[i];
-            const double v2 = b.vz[i] + h * b.fz[i];
+            const double fx = b.fx[i], fy = b.fy[i], fz = b.fz[i];
+            double u0 = b.vx[i], u1 = b.vy[i], u2 = b.vz[i];
+            if (FINAL) {
+                u0 = u0 + h * fx;
+                u1 = u1 + h * fy;
+                u2 = u2 + h * fz;
+            }
+            const double v0 = u0 + h * fx;
+            const double v1 = u1 + h * fy;
+            const double v2 = u2 + h * fz;
             b.vx[i] = v0;
             b.vy[i] = v1;
             b.vz[i] = v2;
@@ -1683,9 +1695,13 @@ static void run_sort(const Launch& L, const TileSet& zero_tiles, const TileSet& 
 }
 
 void phase_deform(const Launch& L) { launch(k_deform_init, L.t_local, L.boxes, L.stream, L.maxgrid); }
-void phase_initial(const Launch& L) { launch(k_initial<false>, L.t_local, L.boxes, L.stream, L.maxgrid); }
+void phase_initial(const Launch& L) { launch(k_initial<false, false>, L.t_local, L.boxes, L.stream, L.maxgrid); }
 void phase_initial_decide(const Launch& L) {
-    launch(k_initial<true>, L.t_local, L.boxes, L.stream, L.maxgrid);
+    launch(k_initial<true, false>, L.t_local, L.boxes, L.stream, L.maxgrid);
+    launch(k_decide<true>, L.t_box, L.boxes, L.stream, L.maxgrid);
+}
+void phase_final_initial_decide(const Launch& L) {
+    launch(k_initial<true, true>, L.t_local, L.boxes, L.stream, L.maxgrid);
     launch(k_decide<true>, L.t_box, L.boxes, L.stream, L.maxgrid);
 }
 void phase_decide(const Launch& L) {

@@ -40,6 +40,7 @@ struct Launch {
 void phase_deform(const Launch& L);
 void phase_initial(const Launch& L);
 void phase_initial_decide(const Launch& L);  // initial + fused displacement check + decide
+void phase_final_initial_decide(const Launch& L);  // previous step's final, then as above
 void phase_decide(const Launch& L);
 void phase_decide_only(const Launch& L);
 void phase_remove_ghosts(const Launch& L);
