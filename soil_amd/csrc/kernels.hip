@@ -355,18 +355,21 @@ __global__ void __launch_bounds__(BLK) k_pbc(KARGS) {
         }
         double* p[3] = {b.x, b.y, b.z};
         int32_t* im[3] = {b.imx, b.imy, b.imz};
+        double q[3];
+        bool moved = false;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             double v = p[k][i];
             if (g.periodic[k]) {
                 const double high = g.lo[k] + g.size[k];
-                if (v < g.lo[k]) { p[k][i] = v + g.size[k]; im[k][i] -= 1; }
-                else if (v >= high) { p[k][i] = v - g.size[k]; im[k][i] += 1; }
+                if (v < g.lo[k]) { v = v + g.size[k]; p[k][i] = v; im[k][i] -= 1; moved = true; }
+                else if (v >= high) { v = v - g.size[k]; p[k][i] = v; im[k][i] += 1; moved = true; }
             } else if (v < g.lo[k] || v >= g.hi[k]) {
                 atomicOr(&st->err, ERR_ATOM_LEFT_BOX);
             }
+            q[k] = v;
         }
-        put_xr(b, i, b.x[i], b.y[i], b.z[i]);
+        if (moved) put_xr(b, i, q[0], q[1], q[2]);  // xr already holds the unwrapped ones
     }
 }
 
@@ -382,6 +385,7 @@ __global__ void k_pre_sort(KARGS) {
         st->do_sort = (st->nlocal > 0 && g.ncells > 0 && (g.periodic_sort || st->state == 0u)) ? 1u : 0u;
         st->not_identity = 0;
         st->local_cells_valid = 0;
+        st->lb_fresh = 0;
     }
 }
 
@@ -534,6 +538,10 @@ __global__ void __launch_bounds__(BLK) k_sort_rank(KARGS) {
 
 // atom.rs:479-490 apply_permutation on every permuted column (gather into the
 // alternates; forces and #[zero] columns are zero at this point of the step)
+// FUSED (the step's sort, always followed by this step's build): also writes
+// the permuted positions as the build's last_build_pos (save_build_positions,
+// neighbor.rs:669-690), which k_save_build then skips.
+template <bool FUSED>
 __global__ void __launch_bounds__(BLK) k_permute(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
@@ -548,6 +556,11 @@ __global__ void __launch_bounds__(BLK) k_permute(KARGS) {
         b.ax[k] = px;
         b.ay[k] = py;
         b.az[k] = pz;
+        if (FUSED) {
+            b.lbx[k] = px;
+            b.lby[k] = py;
+            b.lbz[k] = pz;
+        }
         {
             double* q = b.xr + 4 * (size_t)k;
             q[0] = px;
@@ -620,6 +633,7 @@ __global__ void k_post_sort(KARGS) {
             st->state = 0u;
         }
         if (st->do_sort) st->local_cells_valid = 1;
+        if (FUSED && st->do_sort && st->not_identity) st->lb_fresh = 1;
         st->rebuild_now = st->state == 0u ? 1u : 0u;
         if (FUSED && st->rebuild_now) {
             st->state = 1u;
@@ -1185,7 +1199,7 @@ __global__ void __launch_bounds__(BLK) k_save_build(KARGS) {
         if (!b.st->rebuild_now) continue;
         const uint32_t i = tl.start + threadIdx.x;
         if (b.consumer && i <= b.st->nlocal + b.st->nghost) b.tcount[i] = 0u;
-        if (i >= b.st->nlocal) continue;
+        if (i >= b.st->nlocal || b.st->lb_fresh) continue;
         b.lbx[i] = b.x[i];
         b.lby[i] = b.y[i];
         b.lbz[i] = b.z[i];
@@ -1499,13 +1513,19 @@ __global__ void __launch_bounds__(BLK) k_csr_fill(KARGS) {
             const uint32_t off = b.offsets[i], n = b.offsets[i + 1] - off;
             const uint32_t slot = b.csr_mode == 2 ? b.cap_pairs / (b.cap_local ? b.cap_local : 1u) : 0u;
             uint32_t* out = b.indices + off;
+            const bool tr = transpose_active(b);
             if (n <= slot) {
                 const uint32_t* src = b.ttmp + (size_t)i * slot;
-                for (uint32_t k = 0; k < n; ++k) out[k] = src[k];
-            } else if (g.uniform) {
-                csr_row_g<true, true>(b, g, i, out);
+                for (uint32_t k = 0; k < n; ++k) {
+                    const uint32_t j = src[k];
+                    out[k] = j;
+                    if (tr) b.tslot[off + k] = atomicAdd(&b.tcount[j], 1u);  // consumer transpose slot
+                }
             } else {
-                csr_row_g<true, false>(b, g, i, out);
+                if (g.uniform) csr_row_g<true, true>(b, g, i, out);
+                else csr_row_g<true, false>(b, g, i, out);
+                if (tr)
+                    for (uint32_t k = 0; k < n; ++k) b.tslot[off + k] = atomicAdd(&b.tcount[out[k]], 1u);
             }
             continue;
         }
@@ -1724,7 +1744,7 @@ void phase_pbc(const Launch& L) { launch(k_pbc, L.t_local, L.boxes, L.stream, L.
 void phase_sort(const Launch& L) {
     launch(k_pre_sort, L.t_box, L.boxes, L.stream, L.maxgrid);
     run_sort<SORT_SPATIAL, SCAN_CELLS_SPATIAL>(L, L.t_cell, L.t_local, L.t_scan_cell);
-    launch(k_permute, L.t_local, L.boxes, L.stream, L.maxgrid);
+    launch(k_permute<false>, L.t_local, L.boxes, L.stream, L.maxgrid);
     launch(k_swap_ptrs, L.t_box, L.boxes, L.stream, L.maxgrid);
     launch(k_post_sort<false>, L.t_box, L.boxes, L.stream, L.maxgrid);
 }
@@ -1732,7 +1752,7 @@ void phase_sort(const Launch& L) {
 void phase_sort_fused(const Launch& L) {
     launch(k_pre_sort, L.t_box, L.boxes, L.stream, L.maxgrid);
     run_sort<SORT_SPATIAL, SCAN_CELLS_SPATIAL>(L, L.t_cell, L.t_local, L.t_scan_cell);
-    launch(k_permute, L.t_local, L.boxes, L.stream, L.maxgrid);
+    launch(k_permute<true>, L.t_local, L.boxes, L.stream, L.maxgrid);
     launch(k_post_sort<true>, L.t_box, L.boxes, L.stream, L.maxgrid);
 }
 void phase_borders(const Launch& L) {
@@ -1756,8 +1776,9 @@ void phase_build_fused(const Launch& L, hipEvent_t ev0, hipEvent_t ev1, hipEvent
     if (ev1) (void)hipEventRecord(ev1, L.stream);
     launch(k_csr_fill, L.t_local, L.boxes, L.stream, L.maxgrid);
     if (ev2) (void)hipEventRecord(ev2, L.stream);
-    // consumer transpose (tcount was cleared by k_save_build)
-    launch(k_tcount, L.t_local, L.boxes, L.stream, L.maxgrid);
+    // consumer transpose (tcount was cleared by k_save_build; modes 1 and 2
+    // take the arrival slots in k_csr_fill)
+    if (L.csr_mode == 0) launch(k_tcount, L.t_local, L.boxes, L.stream, L.maxgrid);
     run_scan<SCAN_TRANSPOSE>(L, L.t_scan_atom);
     launch(k_tplace, L.t_local, L.boxes, L.stream, L.maxgrid);
 }
