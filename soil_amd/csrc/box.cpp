@@ -418,7 +418,7 @@ int step_plain_loop(soil_hip_box* b, int nsteps, bool& pending_final) {
             phase_final(b->L);
             pending_final = false;
         }
-        phase_deform(b->L);
+        if (g.init_shear) phase_deform(b->L);  // the kernel acts only then
         if (pending_final) phase_final_initial_decide(b->L);
         else phase_initial_decide(b->L);
         pending_final = false;
@@ -435,8 +435,7 @@ int step_plain_loop(soil_hip_box* b, int nsteps, bool& pending_final) {
             rc = rebuild_tail(b);
             if (rc) return rc;
         } else {
-            phase_sort_flags(b->L);
-            phase_forward(b->L);
+            phase_forward(b->L);  // k_decide<true> left rebuild_now = 0 and the sort flags cleared
         }
         if (b->consumer) phase_force_fused(b->L);
         if (b->model.newton) {

@@ -262,7 +262,20 @@ __global__ void k_decide(KARGS) {
             else st->steps_since_build += 1;
         }
         st->exceed = 0;
-        if (REMOVE && st->state == 0u) st->nghost = 0;
+        if (REMOVE) {
+            if (st->state == 0u) st->nghost = 0;
+            // The fused step also does sort_atoms_by_bin's bookkeeping here
+            // (k_pre_sort): nothing before the sort changes its inputs. A
+            // bounds change sets FullRebuild only in k_post_sort, after pbc's
+            // run_if has seen the state decided here (neighbor.rs:837-841).
+            st->do_sort = (st->nlocal > 0 && g.ncells > 0 && (g.periodic_sort || st->state == 0u || g.bounds_changed))
+                              ? 1u
+                              : 0u;
+            st->not_identity = 0;
+            st->local_cells_valid = 0;
+            st->lb_fresh = 0;
+            st->rebuild_now = st->state == 0u ? 1u : 0u;  // final value set by k_post_sort when it runs
+        }
     }
 }
 
@@ -619,6 +632,7 @@ __global__ void k_post_sort(KARGS) {
         if (!live(b)) continue;
         BoxState* st = b.st;
         const Geom& g = geom_of(b);
+        if (g.bounds_changed) st->state = 0u;
         if (FUSED && st->do_sort && st->not_identity) {
             swap_ptr(b.x, b.ax); swap_ptr(b.y, b.ay); swap_ptr(b.z, b.az);
             swap_ptr(b.vx, b.avx); swap_ptr(b.vy, b.avy); swap_ptr(b.vz, b.avz);
@@ -1734,10 +1748,6 @@ void phase_decide_only(const Launch& L) {
 }
 void phase_remove_ghosts(const Launch& L) { launch(k_remove_ghosts, L.t_box, L.boxes, L.stream, L.maxgrid); }
 void phase_set_rebuild_now(const Launch& L) { launch(k_set_rebuild_now, L.t_box, L.boxes, L.stream, L.maxgrid); }
-void phase_sort_flags(const Launch& L) {
-    launch(k_pre_sort, L.t_box, L.boxes, L.stream, L.maxgrid);
-    launch(k_post_sort<false>, L.t_box, L.boxes, L.stream, L.maxgrid);
-}
 void phase_zero(const Launch& L) { launch(k_zero<false>, L.t_atom, L.boxes, L.stream, L.maxgrid); }
 void phase_zero_fused(const Launch& L) { launch(k_zero<true>, L.t_atom, L.boxes, L.stream, L.maxgrid); }
 void phase_pbc(const Launch& L) { launch(k_pbc, L.t_local, L.boxes, L.stream, L.maxgrid); }
@@ -1749,8 +1759,7 @@ void phase_sort(const Launch& L) {
     launch(k_post_sort<false>, L.t_box, L.boxes, L.stream, L.maxgrid);
 }
 // sort + permute + (when rebuilding) the build's bookkeeping in one per-box kernel
-void phase_sort_fused(const Launch& L) {
-    launch(k_pre_sort, L.t_box, L.boxes, L.stream, L.maxgrid);
+void phase_sort_fused(const Launch& L) {  // k_decide<true> did the pre-sort bookkeeping
     run_sort<SORT_SPATIAL, SCAN_CELLS_SPATIAL>(L, L.t_cell, L.t_local, L.t_scan_cell);
     launch(k_permute<true>, L.t_local, L.boxes, L.stream, L.maxgrid);
     launch(k_post_sort<true>, L.t_box, L.boxes, L.stream, L.maxgrid);

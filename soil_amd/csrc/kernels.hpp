@@ -45,7 +45,6 @@ void phase_decide(const Launch& L);
 void phase_decide_only(const Launch& L);
 void phase_remove_ghosts(const Launch& L);
 void phase_set_rebuild_now(const Launch& L);
-void phase_sort_flags(const Launch& L);
 void phase_zero(const Launch& L);
 void phase_zero_fused(const Launch& L);  // leaves forces of consumer boxes to phase_force_fused
 void phase_pbc(const Launch& L);
