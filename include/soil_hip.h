@@ -180,9 +180,10 @@ int soil_hip_batch_sync(soil_hip_batch* batch);
 
 /* ── kernel timing (HIP events around the CSR build of every rebuild) ──── */
 int soil_hip_set_timing(soil_hip_box* box, int enabled);
-/* out[0] = builds timed, out[1] = total ms of CSR count+fill, out[2] = total
- * algorithmic bytes of those builds, out[3] = total ms of the fill kernel only */
-int soil_hip_get_timing(soil_hip_box* box, double* out4);
+/* out[0] = builds timed, out[1] = total ms of the CSR build (count, offset
+ * scan, fill), out[2] = total algorithmic bytes of those builds, out[3] = total
+ * ms of the fill kernel, out[4] = total ms of the count kernel */
+int soil_hip_get_timing(soil_hip_box* box, double* out5);
 int soil_hip_batch_set_timing(soil_hip_batch* batch, int enabled);
 int soil_hip_batch_get_timing(soil_hip_batch* batch, double* out4);
 

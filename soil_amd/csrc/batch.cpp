@@ -45,7 +45,7 @@ static void launch_step(const Launch& L) {
     phase_sort_fused(L);
     phase_borders(L);
     phase_forward(L);
-    phase_build_fused(L, nullptr, nullptr, nullptr);
+    phase_build_fused(L, nullptr, nullptr, nullptr, nullptr);
     phase_force_fused(L);
     phase_reverse(L);
     phase_final(L);

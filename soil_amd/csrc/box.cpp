@@ -104,7 +104,7 @@ void carve_atoms(Carver& c, BoxDev& d, const Caps& k, const std::vector<HostCol>
     c.take(d.ax, A); c.take(d.ay, A); c.take(d.az, A);
     c.take(d.avx, A); c.take(d.avy, A); c.take(d.avz, A);
     c.take(d.amass, A); c.take(d.ainv_mass, A); c.take(d.arad, A);
-    c.take(d.atag, A); c.take(d.atype, A); c.take(d.aorigin, A);
+    c.take(d.atag, A); c.take(d.atype, A);
     c.take(d.aimx, A); c.take(d.aimy, A); c.take(d.aimz, A);
     c.take(d.gswap, A);
     c.take(d.xr, 4 * A);
@@ -294,13 +294,16 @@ void resolve_timing(soil_hip_box* b) {
     std::vector<soil_hip_box::Pending> keep;
     for (auto& p : b->pending) {
         if (hipEventQuery(p.e2) == hipSuccess) {
-            float ms_all = 0.f, ms_fill = 0.f;
+            float ms_all = 0.f, ms_fill = 0.f, ms_count = 0.f;
             hipEventElapsedTime(&ms_all, p.e0, p.e2);
             hipEventElapsedTime(&ms_fill, p.e1, p.e2);
+            hipEventElapsedTime(&ms_count, p.e0, p.ec);
             b->t_builds += 1.0;
             b->t_csr_ms += ms_all;
             b->t_fill_ms += ms_fill;
+            b->t_count_ms += ms_count;
             b->ev_pool.push_back(p.e0);
+            b->ev_pool.push_back(p.ec);
             b->ev_pool.push_back(p.e1);
             b->ev_pool.push_back(p.e2);
         } else {
@@ -345,9 +348,10 @@ int rebuild_tail(soil_hip_box* b) {
     }
     if (b->h_st->err & ERR_GHOST_CAPACITY) return set_error(SOIL_HIP_ECAPACITY, "ghost capacity growth failed");
     phase_forward(b->L);
-    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+    hipEvent_t e0 = nullptr, ec = nullptr, e1 = nullptr, e2 = nullptr;
     if (b->timing) {
         e0 = take_event(b);
+        ec = take_event(b);
         e1 = take_event(b);
         e2 = take_event(b);
     }
@@ -355,7 +359,7 @@ int rebuild_tail(soil_hip_box* b) {
     {
         Launch L = b->L;
         // count, scan (flags the capacity), fill (skipped on overflow)
-        phase_build_fused(L, e0, e1, e2);
+        phase_build_fused(L, e0, ec, e1, e2);
     }
     int rc = pull_state(b);
     if (rc) return rc;
@@ -374,7 +378,7 @@ int rebuild_tail(soil_hip_box* b) {
         b->h_st->rebuilds -= 1;
         b->h_st->state = 0;  // build_prep flips to CommunicateOnly again
         HIPCHK(hipMemcpy(b->d_st, b->h_st, sizeof(BoxState), hipMemcpyHostToDevice));
-        phase_build(b->L, nullptr, nullptr, nullptr);
+        phase_build(b->L, nullptr, nullptr, nullptr, nullptr);
         rc = pull_state(b);
         if (rc) return rc;
         if (b->h_st->err & ERR_PAIR_CAPACITY) return set_error(SOIL_HIP_ECAPACITY, "pair capacity growth failed");
@@ -385,7 +389,7 @@ int rebuild_tail(soil_hip_box* b) {
             e0 = nullptr;
         }
     }
-    if (e0) b->pending.push_back({e0, e1, e2});
+    if (e0) b->pending.push_back({e0, ec, e1, e2});
     return 0;
 }
 
@@ -804,7 +808,7 @@ int soil_hip_bin_neighbor_list(soil_hip_box* b) {
     if (rc) return rc;
     phase_set_rebuild_now(b->L);
     for (int attempt = 0; attempt < 3; ++attempt) {
-        phase_build(b->L, nullptr, nullptr, nullptr);
+        phase_build(b->L, nullptr, nullptr, nullptr, nullptr);
         rc = pull_state(b);
         if (rc) return rc;
         if (!(b->h_st->err & ERR_PAIR_CAPACITY)) break;
@@ -1112,21 +1116,23 @@ int soil_hip_set_timing(soil_hip_box* b, int enabled) {
     if (rc) return rc;
     resolve_timing(b);
     b->timing = enabled != 0;
-    b->t_builds = b->t_csr_ms = b->t_fill_ms = 0.0;
+    b->t_builds = b->t_csr_ms = b->t_fill_ms = b->t_count_ms = 0.0;
     b->bytes_at_enable = b->h_st->csr_bytes;
     return 0;
 }
 
-int soil_hip_get_timing(soil_hip_box* b, double* out4) {
+int soil_hip_get_timing(soil_hip_box* b, double* out5) {
     int rc = check_box(b, true);
     if (rc) return rc;
+    if (!out5) return set_error(SOIL_HIP_EINVAL, "null out");
     rc = pull_state(b);
     if (rc) return rc;
     resolve_timing(b);
-    out4[0] = b->t_builds;
-    out4[1] = b->t_csr_ms;
-    out4[2] = (double)(b->h_st->csr_bytes - b->bytes_at_enable);
-    out4[3] = b->t_fill_ms;
+    out5[0] = b->t_builds;
+    out5[1] = b->t_csr_ms;
+    out5[2] = (double)(b->h_st->csr_bytes - b->bytes_at_enable);
+    out5[3] = b->t_fill_ms;
+    out5[4] = b->t_count_ms;
     return 0;
 }
 

@@ -83,9 +83,9 @@ struct soil_hip_box {
     // timing
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
-    struct Pending { hipEvent_t e0, e1, e2; };
+    struct Pending { hipEvent_t e0, ec, e1, e2; };
     std::vector<Pending> pending;
-    double t_builds = 0.0, t_csr_ms = 0.0, t_fill_ms = 0.0;
+    double t_builds = 0.0, t_csr_ms = 0.0, t_fill_ms = 0.0, t_count_ms = 0.0;
     uint64_t bytes_at_enable = 0;
 };
 

@@ -99,7 +99,7 @@ struct BoxDev {
     // spatial-sort alternates (same capacity)
     double *ax, *ay, *az, *avx, *avy, *avz, *amass, *ainv_mass, *arad;
     uint32_t *atag, *atype;
-    int32_t *aorigin, *aimx, *aimy, *aimz;
+    int32_t *aimx, *aimy, *aimz;
     uint8_t* gswap;
     double* xr;           // AoS mirror (x, y, z, rad) per atom for neighbor gathers
     double *lbx, *lby, *lbz;

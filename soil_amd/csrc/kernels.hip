@@ -550,7 +550,8 @@ __global__ void __launch_bounds__(BLK) k_sort_rank(KARGS) {
 }
 
 // atom.rs:479-490 apply_permutation on every permuted column (gather into the
-// alternates; forces and #[zero] columns are zero at this point of the step)
+// alternates; forces and #[zero] columns are zero at this point of the step,
+// and a local's origin_index is always 0 on one rank, so those stay in place)
 // FUSED (the step's sort, always followed by this step's build): also writes
 // the permuted positions as the build's last_build_pos (save_build_positions,
 // neighbor.rs:669-690), which k_save_build then skips.
@@ -589,7 +590,6 @@ __global__ void __launch_bounds__(BLK) k_permute(KARGS) {
         b.arad[k] = pr;
         b.atag[k] = b.tag[p];
         b.atype[k] = b.type[p];
-        b.aorigin[k] = b.origin[p];
         b.aimx[k] = b.imx[p];
         b.aimy[k] = b.imy[p];
         b.aimz[k] = b.imz[p];
@@ -617,7 +617,7 @@ __global__ void k_swap_ptrs(KARGS) {
         swap_ptr(b.x, b.ax); swap_ptr(b.y, b.ay); swap_ptr(b.z, b.az);
         swap_ptr(b.vx, b.avx); swap_ptr(b.vy, b.avy); swap_ptr(b.vz, b.avz);
         swap_ptr(b.mass, b.amass); swap_ptr(b.inv_mass, b.ainv_mass); swap_ptr(b.rad, b.arad);
-        swap_ptr(b.tag, b.atag); swap_ptr(b.type, b.atype); swap_ptr(b.origin, b.aorigin);
+        swap_ptr(b.tag, b.atag); swap_ptr(b.type, b.atype);
         swap_ptr(b.imx, b.aimx); swap_ptr(b.imy, b.aimy); swap_ptr(b.imz, b.aimz);
         for (int c = 0; c < b.ncols; ++c)
             if (!(b.cols[c].flags & 4u)) swap_ptr(b.cols[c].data, b.cols[c].alt);
@@ -637,7 +637,7 @@ __global__ void k_post_sort(KARGS) {
             swap_ptr(b.x, b.ax); swap_ptr(b.y, b.ay); swap_ptr(b.z, b.az);
             swap_ptr(b.vx, b.avx); swap_ptr(b.vy, b.avy); swap_ptr(b.vz, b.avz);
             swap_ptr(b.mass, b.amass); swap_ptr(b.inv_mass, b.ainv_mass); swap_ptr(b.rad, b.arad);
-            swap_ptr(b.tag, b.atag); swap_ptr(b.type, b.atype); swap_ptr(b.origin, b.aorigin);
+            swap_ptr(b.tag, b.atag); swap_ptr(b.type, b.atype);
             swap_ptr(b.imx, b.aimx); swap_ptr(b.imy, b.aimy); swap_ptr(b.imz, b.aimz);
             for (int c = 0; c < b.ncols; ++c)
                 if (!(b.cols[c].flags & 4u)) swap_ptr(b.cols[c].data, b.cols[c].alt);
@@ -1770,17 +1770,18 @@ void phase_borders(const Launch& L) {
     launch(k_ghost_fill, L.t_ghost, L.boxes, L.stream, L.maxgrid);
 }
 void phase_forward(const Launch& L) { launch(k_forward, L.t_ghost, L.boxes, L.stream, L.maxgrid); }
-void phase_build(const Launch& L, hipEvent_t ev0, hipEvent_t ev1, hipEvent_t ev2) {
+void phase_build(const Launch& L, hipEvent_t ev0, hipEvent_t evc, hipEvent_t ev1, hipEvent_t ev2) {
     launch(k_build_prep, L.t_box, L.boxes, L.stream, L.maxgrid);
-    phase_build_fused(L, ev0, ev1, ev2);
+    phase_build_fused(L, ev0, evc, ev1, ev2);
 }
 // the build after phase_sort_fused (which already did the bookkeeping)
-void phase_build_fused(const Launch& L, hipEvent_t ev0, hipEvent_t ev1, hipEvent_t ev2) {
+void phase_build_fused(const Launch& L, hipEvent_t ev0, hipEvent_t evc, hipEvent_t ev1, hipEvent_t ev2) {
     launch(k_save_build, L.t_atom1, L.boxes, L.stream, L.maxgrid);
     run_sort<SORT_BUILD, SCAN_CELLS_BUILD>(L, L.t_cell, L.t_atom, L.t_scan_cell);
     if (ev0) (void)hipEventRecord(ev0, L.stream);
     if (L.csr_mode == 0) launch(k_csr_count, L.t_local1, L.boxes, L.stream, L.maxgrid);  // tiles + end marker
     else launch(k_csr_count_g, L.t_local1, L.boxes, L.stream, L.maxgrid);
+    if (evc) (void)hipEventRecord(evc, L.stream);
     run_scan<SCAN_CSR>(L, L.t_scan_local);
     if (ev1) (void)hipEventRecord(ev1, L.stream);
     launch(k_csr_fill, L.t_local, L.boxes, L.stream, L.maxgrid);

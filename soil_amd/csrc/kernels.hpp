@@ -52,8 +52,10 @@ void phase_sort(const Launch& L);
 void phase_sort_fused(const Launch& L);
 void phase_borders(const Launch& L);
 void phase_forward(const Launch& L);
-void phase_build(const Launch& L, hipEvent_t ev0, hipEvent_t ev1, hipEvent_t ev2);
-void phase_build_fused(const Launch& L, hipEvent_t ev0, hipEvent_t ev1, hipEvent_t ev2);
+// timing events (any may be null): ev0 before the CSR count, evc after it,
+// ev1 after the offset scan, ev2 after the fill
+void phase_build(const Launch& L, hipEvent_t ev0, hipEvent_t evc, hipEvent_t ev1, hipEvent_t ev2);
+void phase_build_fused(const Launch& L, hipEvent_t ev0, hipEvent_t evc, hipEvent_t ev1, hipEvent_t ev2);
 void phase_force(const Launch& L);
 void phase_force_fused(const Launch& L);  // assigns forces of consumer boxes
 void phase_reverse(const Launch& L);

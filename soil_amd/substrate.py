@@ -255,9 +255,9 @@ class Box:
         check(lib().soil_hip_set_timing(self.h, 1 if enabled else 0))
 
     def timing(self):
-        out = np.zeros(4)
+        out = np.zeros(5)
         check(lib().soil_hip_get_timing(self.h, _dp(out)))
-        return {"builds": out[0], "csr_ms": out[1], "csr_bytes": out[2], "fill_ms": out[3]}
+        return {"builds": out[0], "csr_ms": out[1], "csr_bytes": out[2], "fill_ms": out[3], "count_ms": out[4]}
 
 
 class Batch:
