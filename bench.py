@@ -39,6 +39,23 @@ def cpu_model():
     return platform.processor()
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_plain_1M.json")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this workload (copy-probe calibrated), or None."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            data = json.load(fh)
+    except (OSError, ValueError):
+        return None, None
+    for name, v in data.get("kernels", {}).items():
+        if kernel in name:
+            return v["read_bytes"] + v["write_bytes"], os.path.relpath(TRAFFIC_FILE, ROOT)
+    return None, None
+
+
 def run_plain(ctx, steps, warmup):
     import soil_amd
     d, pos, vel, mass, rad = soil_amd.gen_box(2)
@@ -222,21 +239,27 @@ def main():
         out["config"]["nghost"] = res["nghost"]
         out["config"]["npairs"] = res["npairs"]
         tm = res["csr"]
-        if tm["builds"] > 0 and tm["csr_ms"] > 0:
+        if tm["builds"] > 0 and tm["count_ms"] > 0:
             per_launch_bytes = tm["csr_bytes"] / tm["builds"]
-            ach = tm["csr_bytes"] / (tm["csr_ms"] * 1e-3) / 1e9
+            avg_count_ms = tm["count_ms"] / tm["builds"]
+            ach = per_launch_bytes / (avg_count_ms * 1e-3) / 1e9
+            traffic, traffic_src = measured_traffic("k_csr_count_g")
             out["roofline"] = {
-                "kernel": "CSR build (k_csr_count + CSR offset scan + k_csr_fill)",
+                "kernel": "k_csr_count_g (CSR count pass: candidate tests, row counts and row entries)",
                 "bound": "hbm",
                 "achieved": ach,
                 "peak": HBM_PEAK_GBS,
                 "unit": "GB/s",
                 "frac": ach / HBM_PEAK_GBS,
-                "traffic": None,
-                "algorithmic_bytes_per_build": per_launch_bytes,
-                "builds_timed": tm["builds"],
-                "avg_build_ms": tm["csr_ms"] / tm["builds"],
-                "avg_fill_ms": tm["fill_ms"] / tm["builds"],
+                "traffic": traffic,
+                "traffic_source": traffic_src,
+                "algorithmic_bytes_per_launch": per_launch_bytes,
+                "launches_timed": tm["builds"],
+                "avg_launch_ms": avg_count_ms,
+                "build_avg_ms": tm["csr_ms"] / tm["builds"],
+                "build_note": "whole CSR build = count + offset scan + fill; the fill also takes the consumer's "
+                              "transpose slots",
+                "fill_avg_ms": tm["fill_ms"] / tm["builds"],
             }
         log(f"plain: {out['value']:.3e} particle-steps/s, {res['ms_per_step']:.3f} ms/step")
     if "batched" in parts:
