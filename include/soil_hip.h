@@ -149,6 +149,8 @@ int soil_hip_get_csr(soil_hip_box* box, uint32_t* offsets, size_t cap_off, uint3
 int soil_hip_get_atoms(soil_hip_box* box, size_t cap, double* pos, double* vel, double* force, double* radius,
                        double* mass, double* inv_mass, uint32_t* tag, int32_t* origin, int32_t* image);
 int soil_hip_get_column(soil_hip_box* box, int col, double* out, size_t cap_atoms);
+/* Atom::atom_type of all atoms, locals then ghosts. */
+int soil_hip_get_types(soil_hip_box* box, uint32_t* type, size_t cap_atoms);
 /* Last build's bins: bin_start (ncells+1), bin_sorted_atoms, bin_atom_sorted_idx,
  * bin_atom_cell (nlocal+nghost each); neighbor.rs:149-161. */
 int soil_hip_get_bins(soil_hip_box* box, uint32_t* bin_start, size_t cap_cells1, uint32_t* sorted_atoms,
@@ -212,6 +214,48 @@ int soil_hip_server_destroy(soil_hip_server* server);
  * as a Poisson process of `rate` jobs/s and are submitted by a client thread. */
 int soil_hip_server_bench_stream(soil_hip_ctx* ctx, int njobs, double rate, uint64_t seed, double budget_gb,
                                  int max_slots, int chunk_steps, int use_graph, double* stats_out20);
+
+/* ── restart and dump files (soil_print) ───────────────────────────────── */
+/* RestartData (soil_print/src/lib.rs:317-398): bincode 1 as written by
+ * write_restart_inner (:1425-1445) or serde_json. Dump frames (:1272-1380):
+ * CSV text, little-endian binary or LAMMPS text of tag, type, pos, vel, force,
+ * radius (+ IsGhost). */
+#define SOIL_HIP_RESTART_BINCODE 0
+#define SOIL_HIP_RESTART_JSON 1
+#define SOIL_HIP_DUMP_TEXT 0
+#define SOIL_HIP_DUMP_BINARY 1
+#define SOIL_HIP_DUMP_LAMMPS 2
+
+/* Host view of one RestartData. pos/vel/force are AoS-of-3 (n*3). buf[k] holds
+ * buf_len[k] f64: AtomData extension k packed atom by atom (atom.rs:353-365). */
+typedef struct {
+    uint64_t natoms, total_cycle;
+    double dt;
+    size_t n;
+    uint32_t *tag, *type;
+    double *pos, *vel, *force, *mass, *radius;
+    size_t nbuf;
+    size_t* buf_len;
+    double** buf;
+} soil_hip_restart;
+
+/* Writes `r` to `path` (no reference counterpart for the call itself: it is
+ * write_restart_inner without the directory and step naming). */
+int soil_hip_restart_write(const char* path, int format, const soil_hip_restart* r);
+/* Reads `path`. Call with r->tag == NULL to get n, nbuf and (when buf_len is
+ * given with r->nbuf slots) the buffer lengths; then again with arrays of
+ * r->n atoms and r->nbuf buffers. ECAPACITY when they are too small. */
+int soil_hip_restart_read(const char* path, int format, soil_hip_restart* r);
+/* RestartData::from_atoms (lib.rs:352-380) of a box's locals and registered
+ * columns at `step`, written to `path`. */
+int soil_hip_box_write_restart(soil_hip_box* box, const char* path, int format, uint64_t step);
+/* read_restart (lib.rs:1527-1620) into a new box in place of soil_hip_upload:
+ * atoms, forces, AtomData columns (registered first, same widths) and dt;
+ * image flags start at zero. *step (may be NULL) receives total_cycle. */
+int soil_hip_box_read_restart(soil_hip_box* box, const char* path, int format, uint64_t* step);
+/* dump_atoms_inner (lib.rs:1138-1210) for one rank: `path_stem` + .csv / .bin
+ * / .lammpstrj; ghosts included (with IsGhost) when `ghost` is set. */
+int soil_hip_box_write_dump(soil_hip_box* box, const char* path_stem, int format, uint64_t step, int ghost);
 
 /* Streams `bytes` through a plain copy kernel `reps` times; *ms = time per
  * launch. Calibrates HBM traffic counters against a known byte count. */

@@ -59,6 +59,17 @@ U64P = C.POINTER(C.c_uint64)
 SZ = C.c_size_t
 SZP = C.POINTER(C.c_size_t)
 
+
+class Restart(C.Structure):
+    """soil_hip_restart (host view of one RestartData)."""
+    _fields_ = [("natoms", C.c_uint64), ("total_cycle", C.c_uint64), ("dt", C.c_double), ("n", C.c_size_t),
+                ("tag", U32P), ("type", U32P), ("pos", DP), ("vel", DP), ("force", DP), ("mass", DP),
+                ("radius", DP), ("nbuf", C.c_size_t), ("buf_len", SZP), ("buf", C.POINTER(DP))]
+
+
+RESTART_BINCODE, RESTART_JSON = 0, 1
+DUMP_TEXT, DUMP_BINARY, DUMP_LAMMPS = 0, 1, 2
+
 _SIGS = {
     "soil_hip_last_error": ([], C.c_char_p),
     "soil_hip_device_count": ([C.POINTER(C.c_int)], C.c_int),
@@ -91,6 +102,12 @@ _SIGS = {
     "soil_hip_get_csr": ([P, U32P, SZ, U32P, SZ, SZP], C.c_int),
     "soil_hip_get_atoms": ([P, SZ, DP, DP, DP, DP, DP, DP, U32P, I32P, I32P], C.c_int),
     "soil_hip_get_column": ([P, C.c_int, DP, SZ], C.c_int),
+    "soil_hip_get_types": ([P, U32P, SZ], C.c_int),
+    "soil_hip_restart_write": ([C.c_char_p, C.c_int, C.POINTER(Restart)], C.c_int),
+    "soil_hip_restart_read": ([C.c_char_p, C.c_int, C.POINTER(Restart)], C.c_int),
+    "soil_hip_box_write_restart": ([P, C.c_char_p, C.c_int, C.c_uint64], C.c_int),
+    "soil_hip_box_read_restart": ([P, C.c_char_p, C.c_int, U64P], C.c_int),
+    "soil_hip_box_write_dump": ([P, C.c_char_p, C.c_int, C.c_uint64, C.c_int], C.c_int),
     "soil_hip_get_bins": ([P, U32P, SZ, U32P, U32P, U32P, SZ], C.c_int),
     "soil_hip_get_swaps": ([P, DP, SZ, SZP], C.c_int),
     "soil_hip_get_grid": ([P, DP], C.c_int),

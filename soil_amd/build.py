@@ -22,7 +22,7 @@ ROCM_INCLUDE = "/opt/rocm/include"
 ARCH = os.environ.get("SOIL_HIP_ARCH", "gfx950")
 
 DEVICE_SOURCES = ["kernels.hip"]
-HOST_SOURCES = ["box.cpp", "batch.cpp", "geometry.cpp", "gen.cpp", "serve.cpp"]
+HOST_SOURCES = ["box.cpp", "batch.cpp", "geometry.cpp", "gen.cpp", "serve.cpp", "restart.cpp"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall",
           "-Wno-unused-function", "-I", os.path.join(ROOT, "include"), "-I", CSRC]

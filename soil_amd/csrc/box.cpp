@@ -971,6 +971,20 @@ int soil_hip_get_column(soil_hip_box* b, int col, double* out, size_t cap_atoms)
     return 0;
 }
 
+int soil_hip_get_types(soil_hip_box* b, uint32_t* type, size_t cap_atoms) {
+    int rc = check_box(b, true);
+    if (rc) return rc;
+    if (!type) return set_error(SOIL_HIP_EINVAL, "null out");
+    rc = pull_state(b);
+    if (rc) return rc;
+    rc = pull_desc(b);
+    if (rc) return rc;
+    const size_t n = (size_t)b->h_st->nlocal + b->h_st->nghost;
+    if (cap_atoms < n) return set_error(SOIL_HIP_ECAPACITY, "types needs %zu atoms", n);
+    if (n) HIPCHK(hipMemcpy(type, b->h_dev.type, n * 4, hipMemcpyDeviceToHost));
+    return 0;
+}
+
 int soil_hip_get_bins(soil_hip_box* b, uint32_t* bin_start, size_t cap_cells1, uint32_t* sorted_atoms,
                       uint32_t* sorted_idx, uint32_t* atom_cell, size_t cap_atoms) {
     int rc = check_box(b, true);

@@ -123,7 +123,7 @@ class RefBox:
         except Exception:
             pass
 
-    def add_atoms(self, pos, mass, radius, vel=None, force=None, tag=None, image=None):
+    def add_atoms(self, pos, mass, radius, vel=None, force=None, tag=None, image=None, type_=None):
         pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
         n = pos.shape[0]
         mass = np.ascontiguousarray(mass, dtype=np.float64)
@@ -132,8 +132,10 @@ class RefBox:
         force = None if force is None else np.ascontiguousarray(force, dtype=np.float64).reshape(-1, 3)
         tag = None if tag is None else np.ascontiguousarray(tag, dtype=np.uint32)
         image = None if image is None else np.ascontiguousarray(image, dtype=np.int32).reshape(-1, 3)
+        type_ = None if type_ is None else np.ascontiguousarray(type_, dtype=np.uint32)
         rc = lib().soil_ref_add_atoms(self.h, n, _dp(pos), _dp(vel), _dp(force), _dp(mass), _dp(radius),
-                                      tag.ctypes.data_as(U32P) if tag is not None else None, None,
+                                      tag.ctypes.data_as(U32P) if tag is not None else None,
+                                      type_.ctypes.data_as(U32P) if type_ is not None else None,
                                       image.ctypes.data_as(I32P) if image is not None else None)
         assert rc == 0
 
