@@ -201,19 +201,32 @@ int soil_hip_server_start(soil_hip_server* server);
 int soil_hip_server_submit(soil_hip_server* server, const soil_hip_box_desc* desc, size_t n, const double* pos,
                            const double* vel, const double* mass, const double* radius, int nsteps, double k,
                            int keep_result, int* job_id);
+/* Queues a job whose atoms come from a RestartData file (read_restart
+ * semantics); it runs `nsteps` more steps. */
+int soil_hip_server_submit_restart(soil_hip_server* server, const soil_hip_box_desc* desc, const char* path,
+                                   int format, int nsteps, double k, int keep_result, int* job_id);
+/* Finished jobs write their RestartData to `dir`/restart_{step}_job{id}.bin
+ * (or .json); NULL or "" stops it. */
+int soil_hip_server_set_output(soil_hip_server* server, const char* dir, int format);
+/* Preempts a job: a running one leaves at the next chunk boundary and its
+ * state is written to `path` (total_cycle = steps taken so far); a queued one
+ * is written as it was submitted. Resume it with soil_hip_server_submit_restart.
+ * No reference counterpart: SOIL runs one App per process. */
+int soil_hip_server_evict(soil_hip_server* server, int job_id, const char* path, int format);
 /* Waits for every submitted job, then stops the loop. */
 int soil_hip_server_drain(soil_hip_server* server);
-/* out20: jobs_done, jobs_failed, particle_steps, busy_s, wall_s, step_p50_ms, step_p99_ms,
+/* out21: jobs_done, jobs_failed, particle_steps, busy_s, wall_s, step_p50_ms, step_p99_ms,
  * completion_p50_s, completion_p99_s, wait_p50_s, wait_p99_s, mean_active_jobs, reconfigs,
- * reconfig_s, budget_gb, chunks, step_mean_ms, completion_max_s, wait_max_s, server_steps */
-int soil_hip_server_stats(soil_hip_server* server, double* out20);
+ * reconfig_s, budget_gb, chunks, step_mean_ms, completion_max_s, wait_max_s, server_steps,
+ * jobs_evicted */
+int soil_hip_server_stats(soil_hip_server* server, double* out21);
 /* Final local positions/velocities of a finished job submitted with keep_result. */
 int soil_hip_server_result(soil_hip_server* server, int job_id, double* pos, double* vel, size_t cap, size_t* n);
 int soil_hip_server_destroy(soil_hip_server* server);
 /* Benchmark driver: njobs jobs of the served-stream workload (config 5) arrive
  * as a Poisson process of `rate` jobs/s and are submitted by a client thread. */
 int soil_hip_server_bench_stream(soil_hip_ctx* ctx, int njobs, double rate, uint64_t seed, double budget_gb,
-                                 int max_slots, int chunk_steps, int use_graph, double* stats_out20);
+                                 int max_slots, int chunk_steps, int use_graph, double* stats_out21);
 
 /* ── restart and dump files (soil_print) ───────────────────────────────── */
 /* RestartData (soil_print/src/lib.rs:317-398): bincode 1 as written by

@@ -131,6 +131,10 @@ _SIGS = {
     "soil_hip_server_start": ([P], C.c_int),
     "soil_hip_server_submit": ([P, C.POINTER(BoxDesc), SZ, DP, DP, DP, DP, C.c_int, C.c_double, C.c_int,
                                 C.POINTER(C.c_int)], C.c_int),
+    "soil_hip_server_submit_restart": ([P, C.POINTER(BoxDesc), C.c_char_p, C.c_int, C.c_int, C.c_double, C.c_int,
+                                        C.POINTER(C.c_int)], C.c_int),
+    "soil_hip_server_set_output": ([P, C.c_char_p, C.c_int], C.c_int),
+    "soil_hip_server_evict": ([P, C.c_int, C.c_char_p, C.c_int], C.c_int),
     "soil_hip_server_drain": ([P], C.c_int),
     "soil_hip_server_stats": ([P, DP], C.c_int),
     "soil_hip_server_result": ([P, C.c_int, DP, DP, SZ, SZP], C.c_int),

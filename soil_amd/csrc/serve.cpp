@@ -14,6 +14,8 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <map>
+#include <string>
 #include <memory>
 #include <mutex>
 #include <thread>
@@ -51,6 +53,14 @@ struct Job {
     bool failed = false;
     std::vector<double> out_pos, out_vel;
     size_t n = 0;
+    // restart files: the job's input (when submitted from one), its first step
+    // number, and where an eviction request wants its state written
+    std::string restart_in;
+    int restart_in_format = 0;
+    uint64_t start_step = 0;
+    std::string evict_path;
+    int evict_format = 0;
+    bool evicted = false;
 };
 
 double pct(std::vector<double> v, double p) {
@@ -85,6 +95,11 @@ struct soil_hip_server {
     std::atomic<bool> draining{false};
     int error = 0;
     std::string error_msg;
+    // finished jobs' RestartData (empty dir: none written)
+    std::string out_dir;
+    int out_format = 0;
+    // eviction requests for jobs on the device: id -> (restart path, format)
+    std::map<int, std::pair<std::string, int>> evict_req;
     // statistics
     std::vector<double> step_ms;                // one sample per (active job, step)
     double particle_steps = 0.0;
@@ -103,8 +118,13 @@ int make_job_box(soil_hip_server* sv, Job* j) {
     soil_hip_box* b = nullptr;
     int rc = soil_hip_box_create(sv->ctx, &j->desc, &b);
     if (rc) return rc;
-    rc = soil_hip_upload(b, j->n, j->pos.data(), j->vel.data(), nullptr, j->mass.data(), j->rad.data(), nullptr,
-                         nullptr, nullptr);
+    if (!j->restart_in.empty()) {
+        rc = soil_hip_box_read_restart(b, j->restart_in.c_str(), j->restart_in_format, &j->start_step);
+        j->n = b->nlocal;
+    } else {
+        rc = soil_hip_upload(b, j->n, j->pos.data(), j->vel.data(), nullptr, j->mass.data(), j->rad.data(), nullptr,
+                             nullptr, nullptr);
+    }
     if (!rc) rc = soil_hip_set_consumer(b, 1, j->k);
     if (rc) {
         soil_hip_box_destroy(b);
@@ -118,9 +138,22 @@ int make_job_box(soil_hip_server* sv, Job* j) {
 // Finishes jobs whose step count is reached: copies out results, frees boxes.
 // The batch is released first (its membership changes and the boxes it points
 // at are about to be destroyed); the loop builds the next one.
+const char* restart_ext(int format) { return format == SOIL_HIP_RESTART_JSON ? "json" : "bin"; }
+
 int retire(soil_hip_server* sv) {
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        for (auto& j : sv->active) {
+            auto it = sv->evict_req.find(j->id);
+            if (it == sv->evict_req.end()) continue;
+            j->evict_path = it->second.first;
+            j->evict_format = it->second.second;
+            sv->evict_req.erase(it);
+            j->evicted = j->done_steps < j->steps;  // a job that just finished is simply done
+        }
+    }
     bool any = false;
-    for (auto& j : sv->active) any = any || j->failed || j->done_steps >= j->steps;
+    for (auto& j : sv->active) any = any || j->failed || j->evicted || j->done_steps >= j->steps;
     if (!any) return 0;
     if (sv->batch) {
         batch_free(sv->batch, false);
@@ -129,12 +162,21 @@ int retire(soil_hip_server* sv) {
     int err = 0;
     std::vector<std::unique_ptr<Job>> keep;
     for (auto& j : sv->active) {
-        if (!j->failed && j->done_steps < j->steps) {
+        if (!j->failed && !j->evicted && j->done_steps < j->steps) {
             keep.push_back(std::move(j));
             continue;
         }
         j->t_done = sv->now();
-        if (j->keep_result && !j->failed) {
+        const uint64_t at = j->start_step + (uint64_t)j->done_steps;
+        if (!j->failed && j->evicted) {
+            // preempted: its state goes to the requested restart file (write_restart semantics)
+            if (soil_hip_box_write_restart(j->box, j->evict_path.c_str(), j->evict_format, at)) j->failed = true;
+        } else if (!j->failed && !sv->out_dir.empty()) {
+            const std::string path = sv->out_dir + "/restart_" + std::to_string(at) + "_job" + std::to_string(j->id) +
+                                     "." + restart_ext(sv->out_format);
+            if (soil_hip_box_write_restart(j->box, path.c_str(), sv->out_format, at)) j->failed = true;
+        }
+        if (j->keep_result && !j->failed && !j->evicted) {
             uint64_t c[8];
             int rc = soil_hip_get_counts(j->box, c);
             const size_t na = rc ? 0 : c[0] + c[1];
@@ -368,6 +410,78 @@ int soil_hip_server_submit(soil_hip_server* sv, const soil_hip_box_desc* desc, s
     return 0;
 }
 
+int soil_hip_server_submit_restart(soil_hip_server* sv, const soil_hip_box_desc* desc, const char* path, int format,
+                                   int nsteps, double k, int keep_result, int* job_id) {
+    if (!sv || !desc || !path || nsteps < 1) return set_error(SOIL_HIP_EINVAL, "bad arguments");
+    if (format != SOIL_HIP_RESTART_BINCODE && format != SOIL_HIP_RESTART_JSON)
+        return set_error(SOIL_HIP_EINVAL, "unknown restart format %d", format);
+    auto j = std::make_unique<Job>();
+    j->desc = *desc;
+    j->restart_in = path;
+    j->restart_in_format = format;
+    j->steps = nsteps;
+    j->k = k;
+    j->keep_result = keep_result != 0;
+    std::lock_guard<std::mutex> lk(sv->mu);
+    j->t_arrive = sv->now();
+    j->id = sv->next_id++;
+    if (job_id) *job_id = j->id;
+    sv->pending.push_back(std::move(j));
+    sv->cv.notify_one();
+    return 0;
+}
+
+int soil_hip_server_set_output(soil_hip_server* sv, const char* dir, int format) {
+    if (!sv) return set_error(SOIL_HIP_EINVAL, "null server");
+    if (format != SOIL_HIP_RESTART_BINCODE && format != SOIL_HIP_RESTART_JSON)
+        return set_error(SOIL_HIP_EINVAL, "unknown restart format %d", format);
+    std::lock_guard<std::mutex> lk(sv->mu);
+    sv->out_dir = dir ? dir : "";
+    sv->out_format = format;
+    return 0;
+}
+
+int soil_hip_server_evict(soil_hip_server* sv, int job_id, const char* path, int format) {
+    if (!sv || !path) return set_error(SOIL_HIP_EINVAL, "null argument");
+    if (format != SOIL_HIP_RESTART_BINCODE && format != SOIL_HIP_RESTART_JSON)
+        return set_error(SOIL_HIP_EINVAL, "unknown restart format %d", format);
+    std::lock_guard<std::mutex> lk(sv->mu);
+    if (job_id < 0 || job_id >= sv->next_id) return set_error(SOIL_HIP_EINVAL, "no job %d", job_id);
+    for (auto& j : sv->done)
+        if (j->id == job_id) return set_error(SOIL_HIP_ESTATE, "job %d has finished", job_id);
+    for (auto it = sv->pending.begin(); it != sv->pending.end(); ++it) {
+        Job* j = it->get();
+        if (j->id != job_id) continue;
+        if (j->box || !j->restart_in.empty()) break;  // being admitted, or its input is already a file
+
+        // not started: its input state, as a restart at step 0
+        std::vector<uint32_t> tag(j->n), type(j->n, 0u);
+        for (size_t i = 0; i < j->n; ++i) tag[i] = (uint32_t)i;
+        std::vector<double> force(3 * j->n, 0.0);
+        soil_hip_restart r{};
+        r.natoms = j->n;
+        r.dt = j->desc.dt;
+        r.n = j->n;
+        r.tag = tag.data();
+        r.type = type.data();
+        r.pos = j->pos.data();
+        r.vel = j->vel.data();
+        r.force = force.data();
+        r.mass = j->mass.data();
+        r.radius = j->rad.data();
+        int rc = soil_hip_restart_write(path, format, &r);
+        if (rc) return rc;
+        j->evicted = true;
+        j->t_done = sv->now();
+        sv->done.push_back(std::move(*it));
+        sv->pending.erase(it);
+        return 0;
+    }
+    // on the device (or about to be): it leaves at the next chunk boundary
+    sv->evict_req[job_id] = {path, format};
+    return 0;
+}
+
 // Waits until every submitted job is finished, then stops the loop.
 int soil_hip_server_drain(soil_hip_server* sv) {
     if (!sv) return set_error(SOIL_HIP_EINVAL, "null server");
@@ -388,9 +502,14 @@ int soil_hip_server_stats(soil_hip_server* sv, double* out) {
     std::vector<double> comp, wait;
     int failed = 0;
     double tmax = 0.0, tmin = 1e300;
+    int evicted = 0;
     for (auto& j : sv->done) {
         if (j->failed) {
             ++failed;
+            continue;
+        }
+        if (j->evicted) {
+            ++evicted;
             continue;
         }
         comp.push_back(j->t_done - j->t_arrive);
@@ -398,7 +517,7 @@ int soil_hip_server_stats(soil_hip_server* sv, double* out) {
         tmax = std::max(tmax, j->t_done);
         tmin = std::min(tmin, j->t_arrive);
     }
-    std::memset(out, 0, 20 * sizeof(double));
+    std::memset(out, 0, 21 * sizeof(double));
     out[0] = (double)comp.size();
     out[1] = failed;
     out[2] = sv->particle_steps;
@@ -421,6 +540,7 @@ int soil_hip_server_stats(soil_hip_server* sv, double* out) {
     out[17] = comp.empty() ? 0.0 : *std::max_element(comp.begin(), comp.end());
     out[18] = wait.empty() ? 0.0 : *std::max_element(wait.begin(), wait.end());
     out[19] = (double)sv->server_steps;
+    out[20] = evicted;
     return 0;
 }
 
@@ -431,6 +551,7 @@ int soil_hip_server_result(soil_hip_server* sv, int job_id, double* pos, double*
     for (auto& j : sv->done) {
         if (j->id != job_id) continue;
         if (j->failed) return set_error(SOIL_HIP_EDEVICE, "job %d failed", job_id);
+        if (j->evicted) return set_error(SOIL_HIP_ESTATE, "job %d was evicted", job_id);
         if (!j->keep_result) return set_error(SOIL_HIP_ESTATE, "job %d was submitted without keep_result", job_id);
         if (n) *n = j->n;
         if (cap < j->n) return set_error(SOIL_HIP_ECAPACITY, "result needs %zu atoms", j->n);
@@ -462,7 +583,7 @@ int soil_hip_server_destroy(soil_hip_server* sv) {
 // arrival time while the server runs. Inputs are generated before the clock
 // starts. Returns the server statistics (see soil_hip_server_stats).
 int soil_hip_server_bench_stream(soil_hip_ctx* ctx, int njobs, double rate, uint64_t seed, double budget_gb,
-                                 int max_slots, int chunk_steps, int use_graph, double* stats_out) {
+                                 int max_slots, int chunk_steps, int use_graph, double* stats_out) {  // 21 values
     if (!ctx || njobs < 1 || rate <= 0 || !stats_out) return set_error(SOIL_HIP_EINVAL, "bad arguments");
     std::vector<double> arrival(njobs);
     std::vector<int32_t> steps(njobs);

@@ -15,7 +15,8 @@ from ._ffi import check, lib
 
 STAT_KEYS = ["jobs_done", "jobs_failed", "particle_steps", "busy_s", "wall_s", "step_p50_ms", "step_p99_ms",
              "completion_p50_s", "completion_p99_s", "wait_p50_s", "wait_p99_s", "mean_active_jobs", "reconfigs",
-             "reconfig_s", "budget_gb", "chunks", "step_mean_ms", "completion_max_s", "wait_max_s", "server_steps"]
+             "reconfig_s", "budget_gb", "chunks", "step_mean_ms", "completion_max_s", "wait_max_s", "server_steps",
+             "jobs_evicted"]
 
 
 def _dp(a):
@@ -43,11 +44,26 @@ class Server:
                                            C.byref(jid)))
         return jid.value
 
+    def submit_restart(self, desc, path, fmt, nsteps, k=1000.0, keep_result=False):
+        """Queues a job whose atoms come from a RestartData file; it runs `nsteps` more steps."""
+        jid = C.c_int(-1)
+        check(lib().soil_hip_server_submit_restart(self.h, C.byref(desc), str(path).encode(), fmt, int(nsteps),
+                                                   float(k), 1 if keep_result else 0, C.byref(jid)))
+        return jid.value
+
+    def set_output(self, directory, fmt):
+        """Finished jobs write restart_{step}_job{id}.bin/.json into `directory` (None: off)."""
+        check(lib().soil_hip_server_set_output(self.h, None if directory is None else str(directory).encode(), fmt))
+
+    def evict(self, job_id, path, fmt):
+        """Preempts a job; its state is written to `path` at the next chunk boundary."""
+        check(lib().soil_hip_server_evict(self.h, job_id, str(path).encode(), fmt))
+
     def drain(self):
         check(lib().soil_hip_server_drain(self.h))
 
     def stats(self):
-        out = np.zeros(20)
+        out = np.zeros(21)
         check(lib().soil_hip_server_stats(self.h, _dp(out)))
         return dict(zip(STAT_KEYS, out.tolist()))
 
@@ -71,7 +87,7 @@ class Server:
 
 
 def bench_stream(ctx, njobs, rate, seed=0x50494C, budget_gb=200.0, max_slots=256, chunk_steps=16, graph=True):
-    out = np.zeros(20)
+    out = np.zeros(21)
     check(lib().soil_hip_server_bench_stream(ctx.h, njobs, rate, seed, budget_gb, max_slots, chunk_steps,
                                              1 if graph else 0, _dp(out)))
     return dict(zip(STAT_KEYS, out.tolist()))
