@@ -107,6 +107,24 @@ int soil_hip_upload_column(soil_hip_box* box, int col, const double* data, size_
  * Sizes device buffers for the uploaded atoms. */
 int soil_hip_setup(soil_hip_box* box);
 
+/* [deform] x / y / z (soil_deform/src/lib.rs:98-205, applied by apply_deform
+ * :435-545): style 0 "erate" (value = rate, L = L0 (1 + rate dt step)),
+ * 1 "vel" (value = velocity, L = L0 + velocity dt step), 2 "final" (linear
+ * ramp to [target_lo, target_hi] over the stage's steps), -1 none. Before
+ * setup; bounds at setup are the start. */
+#define SOIL_HIP_DEFORM_NONE (-1)
+#define SOIL_HIP_DEFORM_ERATE 0
+#define SOIL_HIP_DEFORM_VEL 1
+#define SOIL_HIP_DEFORM_FINAL 2
+int soil_hip_set_axis_deform(soil_hip_box* box, int dim, int style, double value, double target_lo, double target_hi);
+/* [deform] remap (default 1): affine remap of the locals when the box changes. */
+int soil_hip_set_deform_remap(soil_hip_box* box, int remap);
+/* Steps of the current stage (RunState cycle_count + cycle_remaining): the
+ * horizon of the "final" style; default 1. */
+int soil_hip_set_stage_steps(soil_hip_box* box, uint64_t nsteps);
+/* Current domain bounds and sizes: lo[3], hi[3], size[3]. */
+int soil_hip_get_domain(soil_hip_box* box, double* out9);
+
 /* Enables the harmonic soft-sphere test consumer F = k(ri+rj-d) n̂ in the
  * Force phase of soil_hip_step (0 disables; the reference has no built-in
  * force, physics tiers provide it). */

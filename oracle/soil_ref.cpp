@@ -144,12 +144,27 @@ struct CommTopology {
 };
 
 // soil_deform/src/lib.rs:195-211 (xy Lees–Edwards part)
+// soil_deform/src/lib.rs:168-205: per-axis style (-1 none, 0 erate, 1 vel,
+// 2 final) with its parameter(s) and the bounds captured at setup
+struct AxisDeform {
+    int style = -1;
+    double value = 0.0;  // erate: rate, vel: velocity
+    double target_lo = 0.0, target_hi = 0.0;
+    double lo_0 = 0.0, hi_0 = 0.0;
+};
+
 struct DeformState {
+    AxisDeform axes[3];
     bool has_shear = false;
     double shear_xy_rate = 0.0;
     bool shear_initialized = false;
+    bool remap = true;
     size_t step = 0;
     bool initialized = false;
+    size_t stage_steps = 1;  // cycle_count + cycle_remaining of the stage (the "final" style's horizon)
+    bool has_any() const {
+        return has_shear || axes[0].style >= 0 || axes[1].style >= 0 || axes[2].style >= 0;
+    }
 };
 
 }  // namespace
@@ -183,6 +198,7 @@ struct soil_ref_box {
     std::vector<Column> cols;
     double harmonic_k = 0.0;
     bool use_consumer = false;
+    bool collapsed = false;  // a deformed axis reached zero size (the reference panics)
     uint64_t rebuilds = 0;
     uint64_t steps = 0;
 };
@@ -980,30 +996,84 @@ void final_integration(soil_ref_box* b) {
 }
 
 // soil_deform/src/lib.rs:435-604, Lees–Edwards xy branch (:559-603)
-void apply_deform(soil_ref_box* b) {
+// soil_deform/src/lib.rs:435-604: axis styles first (new bounds as functions of
+// the step, affine remap of the locals, domain update), then the xy shear.
+// Returns false when an axis would collapse (the reference panics).
+bool apply_deform(soil_ref_box* b) {
     DeformState& st = b->deform;
-    if (!st.initialized || !st.has_shear) return;
+    if (!st.initialized || !st.has_any()) return true;
     st.step += 1;
     Atom& a = b->atoms;
     Domain& d = b->domain;
     double dt = a.dt;
     size_t step = st.step;
     size_t nlocal = a.nlocal;
-    double rate = st.shear_xy_rate;
-    double lx = d.size[0], ly = d.size[1];
-    double dv = rate * ly;
-    if (!st.shear_initialized) {
-        double yc = 0.5 * (d.boundaries_low[1] + d.boundaries_high[1]);
-        for (size_t i = 0; i < nlocal; ++i) a.vel[3 * i] += rate * (a.pos[3 * i + 1] - yc);
-        st.shear_initialized = true;
+    size_t total_steps = st.stage_steps;
+    bool box_changed = false;
+    for (int dim = 0; dim < 3; ++dim) {
+        const AxisDeform& ax = st.axes[dim];
+        if (ax.style < 0) continue;
+        double old_lo = d.boundaries_low[dim], old_hi = d.boundaries_high[dim];
+        double old_size = old_hi - old_lo;
+        double lo_0 = ax.lo_0, hi_0 = ax.hi_0;
+        double l0 = hi_0 - lo_0;
+        double new_lo, new_hi;
+        if (ax.style == 0) {
+            double scale = 1.0 + ax.value * dt * (double)step;
+            double new_l = l0 * scale;
+            double center = (lo_0 + hi_0) * 0.5;
+            new_lo = center - new_l * 0.5;
+            new_hi = center + new_l * 0.5;
+        } else if (ax.style == 1) {
+            double delta = ax.value * dt * (double)step;
+            double new_l = l0 + delta;
+            double center = (lo_0 + hi_0) * 0.5;
+            new_lo = center - new_l * 0.5;
+            new_hi = center + new_l * 0.5;
+        } else {
+            double frac = total_steps > 0 ? (double)step / (double)total_steps : 1.0;
+            frac = std::fmin(frac, 1.0);
+            new_lo = lo_0 + (ax.target_lo - lo_0) * frac;
+            new_hi = hi_0 + (ax.target_hi - hi_0) * frac;
+        }
+        double new_size = new_hi - new_lo;
+        if (new_size <= 0.0) return false;
+        if (st.remap && old_size > 0.0) {
+            double scale = new_size / old_size;
+            double new_center = (new_lo + new_hi) * 0.5;
+            double old_center = (old_lo + old_hi) * 0.5;
+            for (size_t i = 0; i < nlocal; ++i) a.pos[3 * i + dim] = new_center + (a.pos[3 * i + dim] - old_center) * scale;
+        }
+        d.boundaries_low[dim] = new_lo;
+        d.boundaries_high[dim] = new_hi;
+        d.size[dim] = new_size;
+        d.sub_domain_low[dim] = new_lo;
+        d.sub_domain_high[dim] = new_hi;
+        d.sub_length[dim] = new_size;
+        box_changed = true;
     }
-    double raw = rate * dt * (double)step * ly;
-    double xy = raw - lx * std::round(raw / lx);
-    d.tilt[0] = xy;
-    d.boundary_vel[0] = dv;
-    d.triclinic = true;
-    d.bounds_changed = true;
-    b->neighbor.last_build_pos.clear();
+    if (st.has_shear) {
+        double rate = st.shear_xy_rate;
+        double lx = d.size[0], ly = d.size[1];
+        double dv = rate * ly;
+        if (!st.shear_initialized) {
+            double yc = 0.5 * (d.boundaries_low[1] + d.boundaries_high[1]);
+            for (size_t i = 0; i < nlocal; ++i) a.vel[3 * i] += rate * (a.pos[3 * i + 1] - yc);
+            st.shear_initialized = true;
+        }
+        double raw = rate * dt * (double)step * ly;
+        double xy = raw - lx * std::round(raw / lx);
+        d.tilt[0] = xy;
+        d.boundary_vel[0] = dv;
+        d.triclinic = true;
+        d.bounds_changed = true;
+        box_changed = true;
+    }
+    if (box_changed) {
+        d.bounds_changed = true;
+        b->neighbor.last_build_pos.clear();
+    }
+    return true;
 }
 
 // Harmonic soft-sphere test consumer (not reference code). For each CSR pair
@@ -1053,7 +1123,7 @@ void harmonic_force(soil_ref_box* b, double k) {
 // registrations of neighbor.rs:541-566, comm.rs:177-199, domain.rs:441-449,
 // atom.rs:613-619, soil_verlet/src/lib.rs:103-118, soil_deform/src/lib.rs:249-269.
 void step_once(soil_ref_box* b) {
-    apply_deform(b);                                   // PreInitialIntegration
+    if (!apply_deform(b)) b->collapsed = true;         // PreInitialIntegration
     initial_integration(b);                            // InitialIntegration
     if (b->state == CommunicateOnly) decide_rebuild(b);  // PostInitialIntegration
     if (b->state == FullRebuild) remove_ghost_atoms(b);
@@ -1161,21 +1231,47 @@ void soil_ref_set_consumer(soil_ref_box* b, int enabled, double k) {
 int soil_ref_setup(soil_ref_box* b) {
     comm_setup(b);
     neighbor_setup(b);
+    // setup_deform (soil_deform/src/lib.rs:346-420): bounds of the deformed axes at the start
+    for (int dim = 0; dim < 3; ++dim)
+        if (b->deform.axes[dim].style >= 0) {
+            b->deform.axes[dim].lo_0 = b->domain.boundaries_low[dim];
+            b->deform.axes[dim].hi_0 = b->domain.boundaries_high[dim];
+        }
     b->deform.initialized = true;
     b->deform.step = 0;
     b->deform.shear_initialized = false;
     return 0;
 }
 
-int soil_ref_step(soil_ref_box* b, int nsteps) {
-    for (int s = 0; s < nsteps; ++s) step_once(b);
+// [deform] x/y/z = { style = "erate" (0) | "vel" (1) | "final" (2), ... } (soil_deform/src/lib.rs:98-126)
+int soil_ref_set_axis_deform(soil_ref_box* b, int dim, int style, double value, double target_lo, double target_hi) {
+    if (dim < 0 || dim > 2 || style < -1 || style > 2) return -1;
+    AxisDeform& ax = b->deform.axes[dim];
+    ax.style = style;
+    ax.value = value;
+    ax.target_lo = target_lo;
+    ax.target_hi = target_hi;
     return 0;
+}
+void soil_ref_set_deform_remap(soil_ref_box* b, int remap) { b->deform.remap = remap != 0; }
+void soil_ref_set_stage_steps(soil_ref_box* b, uint64_t n) { b->deform.stage_steps = (size_t)n; }
+void soil_ref_get_domain(soil_ref_box* b, double* out9) {
+    for (int k = 0; k < 3; ++k) {
+        out9[k] = b->domain.boundaries_low[k];
+        out9[3 + k] = b->domain.boundaries_high[k];
+        out9[6 + k] = b->domain.size[k];
+    }
+}
+
+int soil_ref_step(soil_ref_box* b, int nsteps) {
+    for (int s = 0; s < nsteps && !b->collapsed; ++s) step_once(b);
+    return b->collapsed ? -1 : 0;
 }
 
 // Phase-by-phase access for unit tests. Ids follow the step order.
 int soil_ref_phase(soil_ref_box* b, int phase) {
     switch (phase) {
-        case 0: apply_deform(b); break;
+        case 0: if (!apply_deform(b)) b->collapsed = true; break;
         case 1: initial_integration(b); break;
         case 2: decide_rebuild(b); break;
         case 3: remove_ghost_atoms(b); break;

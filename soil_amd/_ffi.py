@@ -82,6 +82,10 @@ _SIGS = {
     "soil_hip_upload_column": ([P, C.c_int, DP, SZ], C.c_int),
     "soil_hip_setup": ([P], C.c_int),
     "soil_hip_set_consumer": ([P, C.c_int, C.c_double], C.c_int),
+    "soil_hip_set_axis_deform": ([P, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double], C.c_int),
+    "soil_hip_set_deform_remap": ([P, C.c_int], C.c_int),
+    "soil_hip_set_stage_steps": ([P, C.c_uint64], C.c_int),
+    "soil_hip_get_domain": ([P, DP], C.c_int),
     "soil_hip_apply_deform": ([P], C.c_int),
     "soil_hip_initial_integration": ([P], C.c_int),
     "soil_hip_decide_rebuild": ([P, C.POINTER(C.c_int)], C.c_int),

@@ -113,17 +113,32 @@ static int batch_run(soil_hip_batch* bt, int nsteps, bool use_graph) {
             bt->half_pending[h] = false;
         }
         // host geometry for the run: staging rows [slot][box]
+        bool regrid = false;
         for (uint32_t j = 0; j < run; ++j) {
             Geom* row = bt->h_ring + (size_t)(slot0 + j) * nb;
             for (uint32_t k = 0; k < nb; ++k) {
                 HostModel& m = bt->boxes[k]->model;
-                m.apply_deform();
+                if (!m.apply_deform())
+                    return set_error(SOIL_HIP_EINVAL, "box %u: [deform] an axis collapsed to zero size", k);
                 m.sort_prepare();
                 if (m.fill(row[k])) return set_error(SOIL_HIP_EUNSUPPORTED, "stencil exceeds %d offsets", SOIL_MAX_STENCIL);
-                if (row[k].ncells > bt->boxes[k]->caps.cells)
-                    return set_error(SOIL_HIP_EUNSUPPORTED, "box %u: grid of %u cells exceeds its capacity", k,
-                                     row[k].ncells);
+                if (row[k].ncells > bt->boxes[k]->caps.cells) {
+                    // a deformed box outgrew its cell arrays: drain the stream,
+                    // grow them, then re-tile the batch and recapture its graph
+                    HIPCHK(hipStreamSynchronize(bt->stream));
+                    bt->half_pending[0] = bt->half_pending[1] = false;
+                    int rc = ensure_cells(bt->boxes[k], row[k].ncells);
+                    if (rc) return rc;
+                    regrid = true;
+                }
                 bt->boxes[k]->last_geom = row[k];
+            }
+        }
+        if (regrid) {
+            batch_tiles(bt);
+            if (bt->graph_exec) {
+                int rc = capture_graph(bt);
+                if (rc) return rc;
             }
         }
         // reverse rounds: split a dim's swaps once any box can select an atom for both

@@ -200,6 +200,14 @@ int alloc_atoms(soil_hip_box* b, const Caps& nc, bool keep) {
         cp(nd.tstart, od.tstart, (n + 1) * 4);
         for (size_t i = 0; i < b->cols.size(); ++i)
             cp(nd.cols[i].data, od.cols[i].data, n * b->cols[i].width * 8);
+        // the spatial sort's per-cell and per-local arrays (scell, perm, sstart,
+        // counts) are not carried over: a build later in this step must bin
+        // every atom itself instead of reusing them (same order either way)
+        if (b->h_st->local_cells_valid) {
+            b->h_st->local_cells_valid = 0;
+            HIPCHK(hipMemcpyAsync(&b->d_st->local_cells_valid, &b->h_st->local_cells_valid, 4,
+                                  hipMemcpyHostToDevice, b->stream));
+        }
         HIPCHK(hipStreamSynchronize(b->stream));
     }
     nd.cap_local = nc.local;
@@ -408,7 +416,8 @@ int step_plain(soil_hip_box* b, int nsteps) {
 
 int step_plain_loop(soil_hip_box* b, int nsteps, bool& pending_final) {
     for (int s = 0; s < nsteps; ++s) {
-        b->model.apply_deform();
+        if (!b->model.apply_deform())
+            return set_error(SOIL_HIP_EINVAL, "[deform] an axis collapsed to zero size (soil_deform/src/lib.rs:506-515)");
         b->model.sort_prepare();
         Geom& g = b->h_ring[b->ring_pos];
         if (b->model.fill(g)) return set_error(SOIL_HIP_EUNSUPPORTED, "stencil exceeds %d offsets", SOIL_MAX_STENCIL);
@@ -422,7 +431,7 @@ int step_plain_loop(soil_hip_box* b, int nsteps, bool& pending_final) {
             phase_final(b->L);
             pending_final = false;
         }
-        if (g.init_shear) phase_deform(b->L);  // the kernel acts only then
+        if (g.init_shear || g.remap_any) phase_deform(b->L);  // the kernel acts only then
         if (pending_final) phase_final_initial_decide(b->L);
         else phase_initial_decide(b->L);
         pending_final = false;
@@ -696,6 +705,39 @@ int soil_hip_setup(soil_hip_box* b) {
     return 0;
 }
 
+int soil_hip_set_axis_deform(soil_hip_box* b, int dim, int style, double value, double target_lo, double target_hi) {
+    if (!b || dim < 0 || dim > 2 || style < -1 || style > 2) return set_error(SOIL_HIP_EINVAL, "bad arguments");
+    if (b->setup_done) return set_error(SOIL_HIP_ESTATE, "define deformation before setup (setup_deform captures bounds)");
+    if (style == 2 && !(target_hi > target_lo)) return set_error(SOIL_HIP_EINVAL, "final style needs lo < hi");
+    b->model.axis_style[dim] = style;
+    b->model.axis_value[dim] = value;
+    b->model.axis_target_lo[dim] = target_lo;
+    b->model.axis_target_hi[dim] = target_hi;
+    return 0;
+}
+
+int soil_hip_set_deform_remap(soil_hip_box* b, int remap) {
+    if (!b) return set_error(SOIL_HIP_EINVAL, "null box");
+    b->model.remap = remap != 0;
+    return 0;
+}
+
+int soil_hip_set_stage_steps(soil_hip_box* b, uint64_t nsteps) {
+    if (!b) return set_error(SOIL_HIP_EINVAL, "null box");
+    b->model.stage_steps = nsteps;
+    return 0;
+}
+
+int soil_hip_get_domain(soil_hip_box* b, double* out9) {
+    if (!b || !out9) return set_error(SOIL_HIP_EINVAL, "null argument");
+    for (int k = 0; k < 3; ++k) {
+        out9[k] = b->model.lo[k];
+        out9[3 + k] = b->model.hi[k];
+        out9[6 + k] = b->model.size[k];
+    }
+    return 0;
+}
+
 int soil_hip_set_consumer(soil_hip_box* b, int enabled, double k) {
     if (!b) return set_error(SOIL_HIP_EINVAL, "null box");
     if (b->setup_done) {
@@ -726,7 +768,8 @@ int soil_hip_set_consumer(soil_hip_box* b, int enabled, double k) {
 
 int soil_hip_apply_deform(soil_hip_box* b) {
     PHASE_PROLOGUE(b);
-    b->model.apply_deform();
+    if (!b->model.apply_deform())
+        return set_error(SOIL_HIP_EINVAL, "[deform] an axis collapsed to zero size (soil_deform/src/lib.rs:506-515)");
     int rc = upload_geom_now(b);
     if (rc) return rc;
     phase_deform(b->L);

@@ -43,6 +43,7 @@ int push_desc(soil_hip_box* b);
 int pull_desc(soil_hip_box* b);
 int pull_state(soil_hip_box* b);
 int rebuild_tiles(soil_hip_box* b);
+int ensure_cells(soil_hip_box* b, uint64_t ncells);
 int check_err_bits(uint32_t err);
 int batch_make(soil_hip_ctx* ctx, soil_hip_box** boxes, int nboxes, soil_hip_batch** out);
 int batch_free(soil_hip_batch* bt, bool restore);

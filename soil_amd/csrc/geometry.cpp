@@ -144,6 +144,14 @@ void HostModel::init(const soil_hip_box_desc& d) {
     shear_initialized = false;
     deform_step = 0;
     deform_initialized = false;
+    for (int k = 0; k < 3; ++k) {
+        axis_style[k] = -1;
+        axis_value[k] = axis_target_lo[k] = axis_target_hi[k] = axis_lo0[k] = axis_hi0[k] = 0.0;
+        ev_remap[k] = false;
+        remap_old_c[k] = remap_new_c[k] = remap_scale[k] = 0.0;
+    }
+    remap = true;
+    stage_steps = 1;
     ev_init_shear = ev_clear_last = ev_bounds_changed = ev_periodic_sort = false;
 }
 
@@ -177,30 +185,92 @@ void HostModel::setup(const double* radius, size_t n) {
     } else {
         ucut_sq = 0.0;
     }
+    // setup_deform (soil_deform/src/lib.rs:346-420): the deformed axes' bounds at the start
+    for (int k = 0; k < 3; ++k)
+        if (axis_style[k] >= 0) {
+            axis_lo0[k] = lo[k];
+            axis_hi0[k] = hi[k];
+        }
     deform_initialized = true;
     deform_step = 0;
     shear_initialized = false;
 }
 
-// apply_deform, xy Lees–Edwards branch (soil_deform/src/lib.rs:435-604)
-void HostModel::apply_deform() {
+bool HostModel::has_deform() const {
+    return has_shear || axis_style[0] >= 0 || axis_style[1] >= 0 || axis_style[2] >= 0;
+}
+
+// apply_deform (soil_deform/src/lib.rs:435-604): axis styles (new bounds as a
+// function of the step, the locals' affine remap recorded for the device, the
+// domain updated), then the xy Lees–Edwards tilt.
+bool HostModel::apply_deform() {
     ev_init_shear = false;
     ev_clear_last = false;
-    if (!deform_initialized || !has_shear) return;
+    for (int k = 0; k < 3; ++k) ev_remap[k] = false;
+    if (!deform_initialized || !has_deform()) return true;
     deform_step += 1;
-    const double lx = size[0], ly = size[1];
-    const double dv = shear_rate * ly;
-    if (!shear_initialized) {
-        ev_init_shear = true;
-        shear_initialized = true;
+    const uint64_t step = deform_step;
+    bool box_changed = false;
+    for (int dim = 0; dim < 3; ++dim) {
+        if (axis_style[dim] < 0) continue;
+        const double old_lo = lo[dim], old_hi = hi[dim];
+        const double old_size = old_hi - old_lo;
+        const double lo_0 = axis_lo0[dim], hi_0 = axis_hi0[dim];
+        const double l0 = hi_0 - lo_0;
+        double new_lo, new_hi;
+        if (axis_style[dim] == 0) {
+            const double scale = 1.0 + axis_value[dim] * dt * (double)step;
+            const double new_l = l0 * scale;
+            const double center = (lo_0 + hi_0) * 0.5;
+            new_lo = center - new_l * 0.5;
+            new_hi = center + new_l * 0.5;
+        } else if (axis_style[dim] == 1) {
+            const double delta = axis_value[dim] * dt * (double)step;
+            const double new_l = l0 + delta;
+            const double center = (lo_0 + hi_0) * 0.5;
+            new_lo = center - new_l * 0.5;
+            new_hi = center + new_l * 0.5;
+        } else {
+            double frac = stage_steps > 0 ? (double)step / (double)stage_steps : 1.0;
+            frac = std::fmin(frac, 1.0);
+            new_lo = lo_0 + (axis_target_lo[dim] - lo_0) * frac;
+            new_hi = hi_0 + (axis_target_hi[dim] - hi_0) * frac;
+        }
+        const double new_size = new_hi - new_lo;
+        if (new_size <= 0.0) return false;
+        if (remap && old_size > 0.0) {
+            ev_remap[dim] = true;
+            remap_scale[dim] = new_size / old_size;
+            remap_new_c[dim] = (new_lo + new_hi) * 0.5;
+            remap_old_c[dim] = (old_lo + old_hi) * 0.5;
+        }
+        lo[dim] = new_lo;
+        hi[dim] = new_hi;
+        size[dim] = new_size;
+        sub_lo[dim] = new_lo;
+        sub_hi[dim] = new_hi;
+        sub_len[dim] = new_size;
+        box_changed = true;
     }
-    const double raw = shear_rate * dt * (double)deform_step * ly;
-    const double xy = raw - lx * std::round(raw / lx);
-    tilt[0] = xy;
-    bvel[0] = dv;
-    triclinic = true;
-    bounds_changed = true;
-    ev_clear_last = true;
+    if (has_shear) {
+        const double lx = size[0], ly = size[1];
+        const double dv = shear_rate * ly;
+        if (!shear_initialized) {
+            ev_init_shear = true;
+            shear_initialized = true;
+        }
+        const double raw = shear_rate * dt * (double)step * ly;
+        const double xy = raw - lx * std::round(raw / lx);
+        tilt[0] = xy;
+        bvel[0] = dv;
+        triclinic = true;
+        box_changed = true;
+    }
+    if (box_changed) {
+        bounds_changed = true;
+        ev_clear_last = true;
+    }
+    return true;
 }
 
 void HostModel::set_tilt(double xy, double bvel_x) {
@@ -270,6 +340,13 @@ int HostModel::fill(Geom& g) const {
     g.dt = dt;
     g.shear_rate = shear_rate;
     g.yc = 0.5 * (lo[1] + hi[1]);
+    for (int k = 0; k < 3; ++k) {
+        g.remap[k] = ev_remap[k] ? 1 : 0;
+        g.remap_any |= g.remap[k];
+        g.remap_old_c[k] = remap_old_c[k];
+        g.remap_new_c[k] = remap_new_c[k];
+        g.remap_scale[k] = remap_scale[k];
+    }
     g.skin_fraction = skin_fraction;
     g.sf_sq = skin_fraction * skin_fraction;
     const double pas = (skin_fraction - 1.0) * cached_min_skin;

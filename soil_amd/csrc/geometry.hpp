@@ -28,18 +28,27 @@ struct HostModel {
     uint64_t ncells = 0;
     std::vector<int> stencil_fwd, stencil_full;
     bool has_self = false;
-    // Atom::dt and DeformState (soil_deform/src/lib.rs:195-211)
+    // Atom::dt and DeformState (soil_deform/src/lib.rs:168-211): per-axis style
+    // (-1 none, 0 erate, 1 vel, 2 final), parameters and bounds at setup
     double dt;
     bool has_shear, shear_initialized, deform_initialized;
     double shear_rate;
     uint64_t deform_step;
+    int axis_style[3];
+    double axis_value[3], axis_target_lo[3], axis_target_hi[3], axis_lo0[3], axis_hi0[3];
+    bool remap;
+    uint64_t stage_steps;
+    // this step's affine remap of the locals, per axis (consumed by fill())
+    bool ev_remap[3];
+    double remap_old_c[3], remap_new_c[3], remap_scale[3];
     // events of the current step (consumed by fill())
     bool ev_init_shear, ev_clear_last, ev_bounds_changed, ev_periodic_sort;
 
     void init(const soil_hip_box_desc& d);
     void setup(const double* radius, size_t n);
     void compute_grid();
-    void apply_deform();
+    bool has_deform() const;
+    bool apply_deform();  // false when an axis collapses
     void set_tilt(double xy, double bvel_x);
     void sort_prepare();
     int fill(Geom& g) const;

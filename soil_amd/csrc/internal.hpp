@@ -55,6 +55,9 @@ struct Geom {
     int32_t uniform, newton, has_self;
     int32_t nstencil;
     int32_t init_shear, clear_last, bounds_changed, periodic_sort;
+    // axis deformation this step: p = new_c + (p - old_c) * scale on the remapped axes
+    int32_t remap[3], remap_any;
+    double remap_old_c[3], remap_new_c[3], remap_scale[3];
     int32_t disjoint[3];
     int32_t stencil[SOIL_MAX_STENCIL];
     // stencil entries in ascending offset order; entry nstencil is the self cell (offset 0)

@@ -137,17 +137,29 @@ __global__ void __launch_bounds__(BLK) k_final(KARGS) {
         b.vz[i] = b.vz[i] + h * b.fz[i];
     }
 }
-// soil_deform/src/lib.rs:568-574: one-time streaming profile v_x += γ̇ (y - y_c)
+// apply_deform's per-atom work (soil_deform/src/lib.rs:522-534, 568-574): the
+// affine remap of the deformed axes, p = new_c + (p - old_c) * scale, then the
+// one-time streaming profile v_x += γ̇ (y - y_c) on the remapped positions.
 __global__ void __launch_bounds__(BLK) k_deform_init(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
         const BoxDev& b = *boxes[tl.box];
         if (!live(b)) continue;
         const Geom& g = geom_of(b);
-        if (!g.init_shear) continue;
+        if (!g.init_shear && !g.remap_any) continue;
         const uint32_t i = tl.start + threadIdx.x;
         if (i >= b.st->nlocal) continue;
-        b.vx[i] += g.shear_rate * (b.y[i] - g.yc);
+        if (g.remap_any) {
+            double p[3] = {b.x[i], b.y[i], b.z[i]};
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+                if (g.remap[k]) p[k] = g.remap_new_c[k] + (p[k] - g.remap_old_c[k]) * g.remap_scale[k];
+            b.x[i] = p[0];
+            b.y[i] = p[1];
+            b.z[i] = p[2];
+            put_xr(b, i, p[0], p[1], p[2]);
+        }
+        if (g.init_shear) b.vx[i] += g.shear_rate * (b.y[i] - g.yc);
     }
 }
 

@@ -81,6 +81,9 @@ class Context:
             pass
 
 
+DEFORM_STYLES = {None: -1, "none": -1, "erate": 0, "vel": 1, "final": 2}
+
+
 class Box:
     """One simulation box on the device."""
 
@@ -132,6 +135,26 @@ class Box:
 
     def set_consumer(self, k, enabled=True):
         check(lib().soil_hip_set_consumer(self.h, 1 if enabled else 0, float(k)))
+
+    def set_axis_deform(self, dim, style, value=0.0, target=(0.0, 0.0)):
+        """[deform] x/y/z (soil_deform/src/lib.rs:98-126): style "erate" (value =
+        strain rate), "vel" (value = boundary velocity) or "final" (target =
+        (lo, hi) reached at the stage's last step); None removes it. Before setup."""
+        check(lib().soil_hip_set_axis_deform(self.h, int(dim), DEFORM_STYLES[style], float(value),
+                                             float(target[0]), float(target[1])))
+
+    def set_deform_remap(self, remap):
+        check(lib().soil_hip_set_deform_remap(self.h, 1 if remap else 0))
+
+    def set_stage_steps(self, n):
+        """Steps in the current stage: the horizon of the "final" style."""
+        check(lib().soil_hip_set_stage_steps(self.h, int(n)))
+
+    def domain(self):
+        """(lo, hi, size) of the current domain."""
+        out = np.zeros(9)
+        check(lib().soil_hip_get_domain(self.h, _dp(out)))
+        return out[0:3].copy(), out[3:6].copy(), out[6:9].copy()
 
     # ── systems ──
     def apply_deform(self):

@@ -56,6 +56,10 @@ def lib():
             "soil_ref_set_column": ([P, C.c_int, DP, C.c_size_t], C.c_int),
             "soil_ref_get_column": ([P, C.c_int, DP, C.c_size_t], C.c_size_t),
             "soil_ref_set_consumer": ([P, C.c_int, C.c_double], None),
+            "soil_ref_set_axis_deform": ([P, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double], C.c_int),
+            "soil_ref_set_deform_remap": ([P, C.c_int], None),
+            "soil_ref_set_stage_steps": ([P, C.c_uint64], None),
+            "soil_ref_get_domain": ([P, DP], None),
             "soil_ref_setup": ([P], C.c_int),
             "soil_ref_step": ([P, C.c_int], C.c_int),
             "soil_ref_phase": ([P, C.c_int], C.c_int),
@@ -112,6 +116,9 @@ def desc_from(d):
     return r
 
 
+DEFORM_STYLES = {None: -1, "none": -1, "erate": 0, "vel": 1, "final": 2}
+
+
 class RefBox:
     def __init__(self, desc):
         self.desc = desc_from(desc)
@@ -155,11 +162,29 @@ class RefBox:
     def set_consumer(self, k, enabled=True):
         lib().soil_ref_set_consumer(self.h, 1 if enabled else 0, float(k))
 
+    def set_axis_deform(self, dim, style, value=0.0, target=(0.0, 0.0)):
+        """[deform] x/y/z: style "erate" | "vel" | "final" (soil_deform/src/lib.rs:98-126)."""
+        assert lib().soil_ref_set_axis_deform(self.h, int(dim), DEFORM_STYLES[style], float(value),
+                                              float(target[0]), float(target[1])) == 0
+
+    def set_deform_remap(self, remap):
+        lib().soil_ref_set_deform_remap(self.h, 1 if remap else 0)
+
+    def set_stage_steps(self, n):
+        lib().soil_ref_set_stage_steps(self.h, int(n))
+
+    def domain(self):
+        out = np.zeros(9)
+        lib().soil_ref_get_domain(self.h, _dp(out))
+        return out[0:3].copy(), out[3:6].copy(), out[6:9].copy()
+
     def setup(self):
         assert lib().soil_ref_setup(self.h) == 0
 
     def step(self, n=1):
-        assert lib().soil_ref_step(self.h, int(n)) == 0
+        """Steps n times; raises when an axis collapses (soil_deform/src/lib.rs:506-515)."""
+        if lib().soil_ref_step(self.h, int(n)) != 0:
+            raise RuntimeError("[deform] an axis collapsed to zero size")
 
     def phase(self, name):
         assert lib().soil_ref_phase(self.h, PH[name]) == 0

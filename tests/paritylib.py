@@ -5,7 +5,7 @@ import soil_amd
 from oracle_ref import RefBox
 
 
-def make_pair(ctx, desc, pos, vel, mass, rad, k=None, columns=(), col_data=()):
+def make_pair(ctx, desc, pos, vel, mass, rad, k=None, columns=(), col_data=(), pre_setup=None):
     g = soil_amd.Box(ctx, desc)
     r = RefBox(desc)
     for (w, f) in columns:
@@ -19,6 +19,9 @@ def make_pair(ctx, desc, pos, vel, mass, rad, k=None, columns=(), col_data=()):
     if k is not None:
         g.set_consumer(k)
         r.set_consumer(k)
+    if pre_setup is not None:
+        pre_setup(g)
+        pre_setup(r)
     g.setup()
     r.setup()
     return g, r

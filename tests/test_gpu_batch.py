@@ -68,3 +68,37 @@ def test_batch_autotune_does_not_change_state(ctx):
     for g, r in pairs:
         r.step(10)
         assert_same(g, r, what="autotune")
+
+
+def test_batch_axis_deform_grows_grid(ctx):
+    # boxes that deform (one expands past its cell capacity mid-run) stay in
+    # parity inside a batch; the batch re-tiles and recaptures its graph
+    print("case: batch axis deform", flush=True)
+    gens = ensemble(4, 3000, 3000)
+    pairs = []
+    for k, gen in enumerate(gens):
+        d = gen[0]
+        L = d.hi[k % 3] - d.lo[k % 3]
+        if k == 0:
+            spec = (0, "vel", 0.6 * L / (60 * d.dt))
+        elif k == 1:
+            spec = (2, "erate", -0.1 / (60 * d.dt))
+        else:
+            spec = None
+
+        def f(b, spec=spec):
+            if spec:
+                b.set_axis_deform(spec[0], spec[1], spec[2])
+                b.set_stage_steps(60)
+        pairs.append(make_pair(ctx, *gen, k=1000.0, pre_setup=f))
+    cells0 = pairs[0][0].grid()[:3].prod()
+    batch = soil_amd.Batch(ctx, [p[0] for p in pairs])
+    batch.step(30, graph=True)
+    batch.step(30, graph=True)
+    batch.sync()
+    batch.close()
+    for g, r in pairs:
+        r.step(60)
+    assert pairs[0][0].grid()[:3].prod() > cells0
+    for i, (g, r) in enumerate(pairs):
+        assert_same(g, r, what=f"deform box {i}")

@@ -28,11 +28,11 @@ def small(seed, n, rmin, rmax, phi, aspect=(1, 1, 1), nblobs=0, blob_frac=0.0, s
     return soil_amd.gen_custom(seed, n, rmin, rmax, phi, aspect, nblobs, blob_frac, sigma, 0.1, shear)
 
 
-def run_compare(ctx, gen, nsteps, every, k=None, exact=True, desc_mod=None, columns=(), col_data=()):
+def run_compare(ctx, gen, nsteps, every, k=None, exact=True, desc_mod=None, columns=(), col_data=(), pre_setup=None):
     d, pos, vel, mass, rad = gen
     if desc_mod:
         desc_mod(d)
-    g, r = make_pair(ctx, d, pos, vel, mass, rad, k=k, columns=columns, col_data=col_data)
+    g, r = make_pair(ctx, d, pos, vel, mass, rad, k=k, columns=columns, col_data=col_data, pre_setup=pre_setup)
     done = 0
     while done < nsteps:
         s = min(every, nsteps - done)
@@ -170,3 +170,64 @@ def test_sqrt_and_division_match_host(ctx):
     print("max rel force err", float(err.max()), "exact fraction", float(np.mean(fg == fr)), flush=True)
     assert float(err.max()) <= 1e-12
     assert np.array_equal(fg, fr)
+
+
+def axis_deform(spec, remap=True, stage=None):
+    # spec: [(dim, style, value, target)]; applied to both boxes before setup
+    def f(b):
+        for dim, style, value, target in spec:
+            b.set_axis_deform(dim, style, value, target)
+        b.set_deform_remap(remap)
+        if stage is not None:
+            b.set_stage_steps(stage)
+    return f
+
+
+@pytest.mark.parametrize("case", ["shrink_z_grow_x", "final_y_with_shear", "no_remap"])
+def test_axis_deform_matches_oracle(ctx, case):
+    # [deform] x/y/z styles with the affine remap (soil_deform/src/lib.rs:435-545):
+    # the box and the remapped atoms follow the oracle bit for bit, the grid is
+    # recomputed when the bounds move, and a growing box grows its cell arrays
+    print(f"case: axis deform {case}", flush=True)
+    nsteps = 120
+    shear = 0.1 if case == "final_y_with_shear" else 0.0
+    gen = small(4400, 5000, 0.4, 0.6, 0.45, shear=shear)
+    d = gen[0]
+    rate = 0.15 / (nsteps * d.dt)
+    if case == "shrink_z_grow_x":
+        L = d.hi[0] - d.lo[0]
+        spec = [(2, "erate", -rate, (0, 0)), (0, "vel", 0.3 * L / (nsteps * d.dt), (0, 0))]
+        f = axis_deform(spec, stage=nsteps)
+    elif case == "final_y_with_shear":
+        lo, hi = d.lo[1], d.hi[1]
+        f = axis_deform([(1, "final", 0.0, (lo - 0.1 * (hi - lo), hi + 0.1 * (hi - lo)))], stage=nsteps)
+    else:
+        f = axis_deform([(2, "erate", -0.5 * rate, (0, 0))], remap=False, stage=nsteps)
+    g, r = run_compare(ctx, gen, nsteps, 20, k=1000.0, pre_setup=f)
+    for a, b in zip(g.domain(), r.domain()):
+        assert np.array_equal(a, b)
+    lo0 = np.array(d.lo)
+    assert not np.array_equal(g.domain()[0], lo0)
+
+
+def test_axis_collapse_grows_capacity_then_errors(ctx):
+    # x shrinks by a tenth of its length per step: by step 9 the box is thinner
+    # than the ghost cutoff, so every local is ghosted both ways and the ghost
+    # and pair arenas both outgrow their set-up capacity inside one rebuild
+    # (the regrown atom arena then bins every atom itself). Step 10 collapses
+    # the axis, an error on both sides (soil_deform/src/lib.rs:506-515).
+    print("case: axis collapse through capacity growth", flush=True)
+    d, pos, vel, mass, rad = small(4401, 500, 0.4, 0.6, 0.3)
+    L = d.hi[0] - d.lo[0]
+    g, r = make_pair(ctx, d, pos, vel, mass, rad, k=1000.0,
+                     pre_setup=axis_deform([(0, "vel", -L / (10 * d.dt), (0, 0))], stage=100))
+    for s in range(1, 10):
+        g.step(1)
+        r.step(1)
+        print(f"compare step {s}", flush=True)
+        assert_same(g, r, what=f"collapse step {s}")
+    assert g.counts()["nghost"] > 3 * len(mass)
+    with pytest.raises(RuntimeError, match="collapsed"):
+        g.step(1)
+    with pytest.raises(RuntimeError, match="collapsed"):
+        r.step(1)

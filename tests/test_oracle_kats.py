@@ -207,3 +207,109 @@ def test_fixture_formula_boxes_match_reference_all_pairs(tilt):
     ref = set(_reference_all_pairs(pos, rad, 1.0))
     assert ref <= got  # the reference's assertion
     assert got == ref  # and nothing extra (stricter than the reference test)
+
+
+def unit_box(dt, hi=(1.0, 1.0, 1.0), atoms=()):
+    # soil_deform/src/lib.rs:626-637 make_domain([0,0,0], hi): the deform tests' box
+    b = RefBox(ref_desc([0, 0, 0], list(hi), dt=dt, skin_fraction=1.1, bin_min_size=0.2))
+    if atoms:
+        b.add_atoms(list(atoms), [1.0] * len(atoms), [0.1] * len(atoms))
+    return b
+
+
+def test_erate_shrinks_box_symmetrically():
+    # soil_deform/src/lib.rs:650-698: z erate -1, dt 0.001 -> [0.0005, 0.9995], size 0.999
+    b = unit_box(0.001)
+    b.set_axis_deform(2, "erate", -1.0)
+    b.set_stage_steps(1000)
+    b.setup()
+    b.phase("apply_deform")
+    lo, hi, size = b.domain()
+    assert abs(lo[2] - 0.0005) < 1e-12 and abs(hi[2] - 0.9995) < 1e-12 and abs(size[2] - 0.999) < 1e-12
+    assert abs(lo[0]) < 1e-12 and abs(hi[0] - 1.0) < 1e-12
+
+
+def test_vel_expands_box():
+    # soil_deform/src/lib.rs:700-738: x vel 1, dt 0.01 -> size 1.01 about the fixed center
+    b = unit_box(0.01)
+    b.set_axis_deform(0, "vel", 1.0)
+    b.set_stage_steps(100)
+    b.setup()
+    b.phase("apply_deform")
+    lo, hi, size = b.domain()
+    assert abs(size[0] - 1.01) < 1e-12
+    assert abs(lo[0] + 0.005) < 1e-12 and abs(hi[0] - 1.005) < 1e-12
+
+
+def test_final_ramps_to_target():
+    # soil_deform/src/lib.rs:740-789: y final [0.1, 0.9] reached after 10 of 10 steps,
+    # linear in between: lo = lo0 + (tlo - lo0) step / total
+    b = unit_box(0.001)
+    b.set_axis_deform(1, "final", target=(0.1, 0.9))
+    b.set_stage_steps(10)
+    b.setup()
+    for step in range(1, 11):
+        b.phase("apply_deform")
+        lo, hi, _ = b.domain()
+        assert abs(lo[1] - 0.1 * step / 10) < 1e-12 and abs(hi[1] - (1.0 - 0.1 * step / 10)) < 1e-12
+    assert abs(lo[1] - 0.1) < 1e-10 and abs(hi[1] - 0.9) < 1e-10
+
+
+def test_remap_scales_atom_positions():
+    # soil_deform/src/lib.rs:791-846: the center atom stays at the center, x untouched;
+    # the 3/4 atom maps affinely: 0.5 + 0.25 * 0.999
+    b = unit_box(0.001, atoms=([0.5, 0.5, 0.5], [0.75, 0.5, 0.75]))
+    b.set_axis_deform(2, "erate", -1.0)
+    b.set_stage_steps(1000)
+    b.setup()
+    b.phase("apply_deform")
+    a = b.atoms()
+    lo, hi, _ = b.domain()
+    assert abs(a["pos"][0, 2] - 0.5 * (lo[2] + hi[2])) < 1e-12
+    assert abs(a["pos"][0, 0] - 0.5) < 1e-12 and abs(a["pos"][1, 0] - 0.75) < 1e-12
+    assert abs(a["pos"][1, 2] - (0.5 + 0.25 * 0.999)) < 1e-12
+
+
+def test_no_remap_leaves_atoms_in_place():
+    # soil_deform/src/lib.rs:848-885
+    b = unit_box(0.001, atoms=([0.5, 0.5, 0.5],))
+    b.set_axis_deform(2, "erate", -1.0)
+    b.set_deform_remap(False)
+    b.set_stage_steps(1000)
+    b.setup()
+    b.phase("apply_deform")
+    assert abs(b.atoms()["pos"][0, 2] - 0.5) < 1e-12
+    assert abs(b.domain()[2][2] - 0.999) < 1e-12
+
+
+def test_multiple_axes_deform_independently():
+    # soil_deform/src/lib.rs:887-937: x vel 1 -> 1.01, y unchanged, z erate -0.5 -> 0.995
+    b = unit_box(0.01)
+    b.set_axis_deform(0, "vel", 1.0)
+    b.set_axis_deform(2, "erate", -0.5)
+    b.set_stage_steps(100)
+    b.setup()
+    b.phase("apply_deform")
+    size = b.domain()[2]
+    assert abs(size[0] - 1.01) < 1e-12 and abs(size[1] - 1.0) < 1e-12 and abs(size[2] - 0.995) < 1e-12
+
+
+def test_volume_follows_the_sizes():
+    # soil_deform/src/lib.rs:939-976: box 2x3x4, z erate -1 -> volume = product of sizes
+    b = unit_box(0.001, hi=(2.0, 3.0, 4.0))
+    b.set_axis_deform(2, "erate", -1.0)
+    b.set_stage_steps(1000)
+    b.setup()
+    b.phase("apply_deform")
+    size = b.domain()[2]
+    assert abs(size[2] - 4.0 * 0.999) < 1e-12 and abs(size[0] * size[1] * size[2] - 2 * 3 * 4 * 0.999) < 1e-10
+
+
+def test_collapsed_axis_is_an_error():
+    # soil_deform/src/lib.rs:506-515: a size <= 0 ends the run
+    b = unit_box(0.1, atoms=([0.5, 0.5, 0.5],))
+    b.set_axis_deform(0, "vel", -1.0)
+    b.set_stage_steps(100)
+    b.setup()
+    with pytest.raises(RuntimeError, match="collapsed"):
+        b.step(20)
