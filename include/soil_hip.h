@@ -107,6 +107,17 @@ int soil_hip_upload_column(soil_hip_box* box, int col, const double* data, size_
  * Sizes device buffers for the uploaded atoms. */
 int soil_hip_setup(soil_hip_box* box);
 
+/* [domain] boundary of one axis (domain.rs:27-34): 0 periodic, 1 fixed,
+ * 2 shrink-wrap (the bounds follow the locals' extent plus a padding every
+ * step, shrink_wrap at domain.rs:509-589). Before setup; overrides the
+ * descriptor's periodic flag. Shrink-wrap boxes step alone (not in a batch). */
+#define SOIL_HIP_BOUNDARY_PERIODIC 0
+#define SOIL_HIP_BOUNDARY_FIXED 1
+#define SOIL_HIP_BOUNDARY_SHRINK_WRAP 2
+int soil_hip_set_boundary(soil_hip_box* box, int dim, int type);
+/* shrink_wrap_padding; 0 = the ghost cutoff (domain.rs:518-528). */
+int soil_hip_set_shrink_wrap_padding(soil_hip_box* box, double padding);
+
 /* [deform] x / y / z (soil_deform/src/lib.rs:98-205, applied by apply_deform
  * :435-545): style 0 "erate" (value = rate, L = L0 (1 + rate dt step)),
  * 1 "vel" (value = velocity, L = L0 + velocity dt step), 2 "final" (linear
@@ -132,7 +143,7 @@ int soil_hip_set_consumer(soil_hip_box* box, int enabled, double k);
 
 /* ── per-phase systems (one call = one reference system on this box) ───────
  * Each enqueues on the box stream; results are read back with the getters. */
-int soil_hip_apply_deform(soil_hip_box* box);          /* soil_deform/src/lib.rs:435-604 (xy LE) */
+int soil_hip_apply_deform(soil_hip_box* box);          /* soil_deform/src/lib.rs:435-604 (axes, xy LE) */
 int soil_hip_initial_integration(soil_hip_box* box);   /* soil_verlet/src/lib.rs:131-162 */
 /* decide_rebuild (neighbor.rs:796-822, needs_rebuild :762-784,
  * displacement_exceeded :693-758). Runs only in CommunicateOnly, like its
@@ -140,6 +151,7 @@ int soil_hip_initial_integration(soil_hip_box* box);   /* soil_verlet/src/lib.rs
 int soil_hip_decide_rebuild(soil_hip_box* box, int* full_rebuild);
 int soil_hip_remove_ghost_atoms(soil_hip_box* box);    /* atom.rs:625-629 (FullRebuild only) */
 int soil_hip_zero_all_forces(soil_hip_box* box);       /* atom.rs:631-635 */
+int soil_hip_shrink_wrap(soil_hip_box* box);           /* domain.rs:509-589 (shrink-wrap axes; waits for the device) */
 int soil_hip_pbc(soil_hip_box* box);                   /* domain.rs:611-729 (FullRebuild only) */
 int soil_hip_sort_atoms_by_bin(soil_hip_box* box);     /* neighbor.rs:834-997 */
 int soil_hip_borders(soil_hip_box* box);               /* comm.rs:682-860 (FullRebuild only) */

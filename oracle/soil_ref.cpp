@@ -67,6 +67,8 @@ struct Domain {
     double sub_domain_low[3] = {0, 0, 0}, sub_domain_high[3] = {1, 1, 1};
     double sub_length[3] = {1, 1, 1}, size[3] = {1, 1, 1};
     int periodic[3] = {1, 1, 1};
+    bool shrink[3] = {false, false, false};  // BoundaryType::ShrinkWrap (domain.rs:27-34)
+    double shrink_wrap_padding = 0.0;
     bool bounds_changed = false;
     double ghost_cutoff = 0.0;
     double tilt[3] = {0, 0, 0};
@@ -1122,13 +1124,60 @@ void harmonic_force(soil_ref_box* b, double k) {
 // One timestep in the phase order of schedule.rs:22-52 with the plugin
 // registrations of neighbor.rs:541-566, comm.rs:177-199, domain.rs:441-449,
 // atom.rs:613-619, soil_verlet/src/lib.rs:103-118, soil_deform/src/lib.rs:249-269.
+// domain.rs:509-569 shrink_wrap_update: each shrink-wrap axis takes the
+// extent of the locals plus a padding (explicit, else the ghost cutoff, else
+// 1% of the largest size); a move beyond 1e-12 on either side changes the
+// bounds and marks them changed.
+bool shrink_wrap(soil_ref_box* b) {
+    Domain& d = b->domain;
+    const Atom& a = b->atoms;
+    const size_t nlocal = a.nlocal;
+    if (!(d.shrink[0] || d.shrink[1] || d.shrink[2]) || nlocal == 0) return false;
+    double padding;
+    if (d.shrink_wrap_padding > 0.0) {
+        padding = d.shrink_wrap_padding;
+    } else if (d.ghost_cutoff > 0.0) {
+        padding = d.ghost_cutoff;
+    } else {
+        const double m = std::fmax(std::fmax(d.size[0], d.size[1]), d.size[2]);
+        padding = m > 0.0 ? m * 0.01 : 1.0;
+    }
+    bool changed = false;
+    for (int k = 0; k < 3; ++k) {
+        if (!d.shrink[k]) continue;
+        double pmin = DBL_MAX, pmax = -DBL_MAX;
+        for (size_t i = 0; i < nlocal; ++i) {
+            const double p = a.pos[3 * i + k];
+            if (p < pmin) pmin = p;
+            if (p > pmax) pmax = p;
+        }
+        const double nl = pmin - padding, nh = pmax + padding;
+        if (std::fabs(nl - d.boundaries_low[k]) > 1e-12 || std::fabs(nh - d.boundaries_high[k]) > 1e-12) {
+            d.boundaries_low[k] = nl;
+            d.boundaries_high[k] = nh;
+            d.sub_domain_low[k] = nl;
+            d.sub_domain_high[k] = nh;
+            changed = true;
+        }
+    }
+    if (changed) {
+        for (int k = 0; k < 3; ++k) {  // update_derived (domain.rs:173-179)
+            d.size[k] = d.boundaries_high[k] - d.boundaries_low[k];
+            d.sub_length[k] = d.sub_domain_high[k] - d.sub_domain_low[k];
+        }
+        d.bounds_changed = true;
+    }
+    return changed;
+}
+
 void step_once(soil_ref_box* b) {
     if (!apply_deform(b)) b->collapsed = true;         // PreInitialIntegration
     initial_integration(b);                            // InitialIntegration
     if (b->state == CommunicateOnly) decide_rebuild(b);  // PostInitialIntegration
     if (b->state == FullRebuild) remove_ghost_atoms(b);
     zero_all_forces(b);
-    if (b->state == FullRebuild) pbc(b);               // PreExchange
+    shrink_wrap(b);                                    // PreExchange, before pbc
+    if (b->state == FullRebuild) pbc(b);
     sort_atoms_by_bin(b);                              // PreNeighbor
     if (b->state == FullRebuild) borders(b);
     else forward_comm_borders(b);
@@ -1255,6 +1304,14 @@ int soil_ref_set_axis_deform(soil_ref_box* b, int dim, int style, double value, 
 }
 void soil_ref_set_deform_remap(soil_ref_box* b, int remap) { b->deform.remap = remap != 0; }
 void soil_ref_set_stage_steps(soil_ref_box* b, uint64_t n) { b->deform.stage_steps = (size_t)n; }
+// [domain] boundary per axis: 0 periodic, 1 fixed, 2 shrink-wrap (domain.rs:27-34)
+int soil_ref_set_boundary(soil_ref_box* b, int dim, int type) {
+    if (dim < 0 || dim > 2 || type < 0 || type > 2) return -1;
+    b->domain.periodic[dim] = type == 0 ? 1 : 0;
+    b->domain.shrink[dim] = type == 2;
+    return 0;
+}
+void soil_ref_set_shrink_wrap_padding(soil_ref_box* b, double padding) { b->domain.shrink_wrap_padding = padding; }
 void soil_ref_get_domain(soil_ref_box* b, double* out9) {
     for (int k = 0; k < 3; ++k) {
         out9[k] = b->domain.boundaries_low[k];
@@ -1284,6 +1341,7 @@ int soil_ref_phase(soil_ref_box* b, int phase) {
         case 10: harmonic_force(b, b->harmonic_k); break;
         case 11: reverse_send_force(b); break;
         case 12: final_integration(b); break;
+        case 13: shrink_wrap(b); break;
         default: return -1;
     }
     return 0;

@@ -90,6 +90,9 @@ _SIGS = {
     "soil_hip_initial_integration": ([P], C.c_int),
     "soil_hip_decide_rebuild": ([P, C.POINTER(C.c_int)], C.c_int),
     "soil_hip_remove_ghost_atoms": ([P], C.c_int),
+    "soil_hip_shrink_wrap": ([P], C.c_int),
+    "soil_hip_set_boundary": ([P, C.c_int, C.c_int], C.c_int),
+    "soil_hip_set_shrink_wrap_padding": ([P, C.c_double], C.c_int),
     "soil_hip_zero_all_forces": ([P], C.c_int),
     "soil_hip_pbc": ([P], C.c_int),
     "soil_hip_sort_atoms_by_bin": ([P], C.c_int),

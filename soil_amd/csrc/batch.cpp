@@ -178,6 +178,9 @@ int batch_make(soil_hip_ctx* ctx, soil_hip_box** boxes, int nboxes, soil_hip_bat
     for (int k = 0; k < nboxes; ++k) {
         if (!boxes[k] || !boxes[k]->setup_done) return set_error(SOIL_HIP_ESTATE, "box %d not set up", k);
         if (boxes[k]->in_batch) return set_error(SOIL_HIP_ESTATE, "box %d already in a batch", k);
+        if (boxes[k]->model.has_shrink())
+            return set_error(SOIL_HIP_EUNSUPPORTED,
+                             "box %d has shrink-wrap boundaries, whose bounds need the host every step: step it alone", k);
         for (int j = 0; j < k; ++j)
             if (boxes[j] == boxes[k]) return set_error(SOIL_HIP_EINVAL, "box %d listed twice", k);
     }

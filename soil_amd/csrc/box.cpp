@@ -414,28 +414,73 @@ int step_plain(soil_hip_box* b, int nsteps) {
     return 0;
 }
 
+// Stages the host model's geometry in this step's ring slot (the stream is
+// ordered, so a slot rewritten later in the step reaches the later kernels).
+static int stage_geom(soil_hip_box* b, Geom& g) {
+    if (b->model.fill(g)) return set_error(SOIL_HIP_EUNSUPPORTED, "stencil exceeds %d offsets", SOIL_MAX_STENCIL);
+    if (g.ncells > b->caps.cells) {
+        int rcc = ensure_cells(b, g.ncells);
+        if (rcc) return rcc;
+    }
+    b->last_geom = g;
+    HIPCHK(hipMemcpyAsync(b->d_ring + b->ring_pos, &g, sizeof(Geom), hipMemcpyHostToDevice, b->stream));
+    return 0;
+}
+
+static double ext_value(uint64_t key) {
+    const uint64_t u = (key >> 63) ? (key & 0x7FFFFFFFFFFFFFFFull) : ~key;
+    double v;
+    std::memcpy(&v, &u, 8);
+    return v;
+}
+
+// shrink_wrap (domain.rs:509-569) for this step: the locals' extents after the
+// kick-drift come back to the host, which moves the bounds; sort_atoms_by_bin's
+// bookkeeping then sees the change (neighbor.rs:836-841).
+static int shrink_wrap_now(soil_hip_box* b) {
+    phase_extent(b->L);
+    int rc = pull_state(b);
+    if (rc) return rc;
+    double pmin[3], pmax[3];
+    for (int k = 0; k < 3; ++k) {
+        pmin[k] = ext_value(b->h_st->ext_key[k]);
+        pmax[k] = ext_value(b->h_st->ext_key[3 + k]);
+    }
+    b->model.shrink_wrap_update(pmin, pmax, b->h_st->nlocal);
+    return 0;
+}
+
 int step_plain_loop(soil_hip_box* b, int nsteps, bool& pending_final) {
+    const bool shrink = b->model.has_shrink();
     for (int s = 0; s < nsteps; ++s) {
         if (!b->model.apply_deform())
             return set_error(SOIL_HIP_EINVAL, "[deform] an axis collapsed to zero size (soil_deform/src/lib.rs:506-515)");
-        b->model.sort_prepare();
+        // a shrink-wrap box learns its bounds after the kick-drift: its sort
+        // bookkeeping waits until then, and the kick reads the bounds so far
+        if (!shrink) b->model.sort_prepare();
         Geom& g = b->h_ring[b->ring_pos];
-        if (b->model.fill(g)) return set_error(SOIL_HIP_EUNSUPPORTED, "stencil exceeds %d offsets", SOIL_MAX_STENCIL);
-        if (g.ncells > b->caps.cells) {
-            int rcc = ensure_cells(b, g.ncells);
-            if (rcc) return rcc;
-        }
-        b->last_geom = g;
-        HIPCHK(hipMemcpyAsync(b->d_ring + b->ring_pos, &g, sizeof(Geom), hipMemcpyHostToDevice, b->stream));
+        int rc = stage_geom(b, g);
+        if (rc) return rc;
         if (pending_final && g.init_shear) {  // the one-time profile adds to the kicked velocities
             phase_final(b->L);
             pending_final = false;
         }
         if (g.init_shear || g.remap_any) phase_deform(b->L);  // the kernel acts only then
-        if (pending_final) phase_final_initial_decide(b->L);
-        else phase_initial_decide(b->L);
+        if (shrink) {
+            phase_kick(b->L, pending_final);
+            rc = shrink_wrap_now(b);
+            if (rc) return rc;
+            b->model.sort_prepare();
+            rc = stage_geom(b, g);
+            if (rc) return rc;
+            phase_decide_fused(b->L);
+        } else if (pending_final) {
+            phase_final_initial_decide(b->L);
+        } else {
+            phase_initial_decide(b->L);
+        }
         pending_final = false;
-        int rc = pull_state(b);
+        rc = pull_state(b);
         if (rc) return rc;
         if (b->h_st->err) return check_err_bits(b->h_st->err);
         const bool rebuild = b->h_st->state == 0 || g.periodic_sort || g.bounds_changed;
@@ -705,6 +750,22 @@ int soil_hip_setup(soil_hip_box* b) {
     return 0;
 }
 
+int soil_hip_set_boundary(soil_hip_box* b, int dim, int type) {
+    if (!b || dim < 0 || dim > 2 || type < 0 || type > 2) return set_error(SOIL_HIP_EINVAL, "bad arguments");
+    if (b->setup_done) return set_error(SOIL_HIP_ESTATE, "set boundaries before setup");
+    if (type != 0 && dim < 2 && b->model.has_shear)
+        return set_error(SOIL_HIP_EINVAL, "[deform] xy shear requires periodic x and y boundaries");
+    b->model.periodic[dim] = type == 0;
+    b->model.shrink[dim] = type == 2;
+    return 0;
+}
+
+int soil_hip_set_shrink_wrap_padding(soil_hip_box* b, double padding) {
+    if (!b) return set_error(SOIL_HIP_EINVAL, "null box");
+    b->model.shrink_padding = padding;
+    return 0;
+}
+
 int soil_hip_set_axis_deform(soil_hip_box* b, int dim, int style, double value, double target_lo, double target_hi) {
     if (!b || dim < 0 || dim > 2 || style < -1 || style > 2) return set_error(SOIL_HIP_EINVAL, "bad arguments");
     if (b->setup_done) return set_error(SOIL_HIP_ESTATE, "define deformation before setup (setup_deform captures bounds)");
@@ -794,6 +855,12 @@ int soil_hip_decide_rebuild(soil_hip_box* b, int* full_rebuild) {
     if (rc) return rc;
     if (full_rebuild) *full_rebuild = b->h_st->state == 0 ? 1 : 0;
     return 0;
+}
+
+int soil_hip_shrink_wrap(soil_hip_box* b) {
+    PHASE_PROLOGUE(b);
+    if (!b->model.has_shrink()) return 0;
+    return shrink_wrap_now(b);
 }
 
 int soil_hip_remove_ghost_atoms(soil_hip_box* b) {
@@ -1064,7 +1131,7 @@ int soil_hip_get_swaps(soil_hip_box* b, double* out, size_t cap_rows, size_t* nr
         const int d = s / 2;
         o[0] = b->h_st->swap_start[s];
         o[1] = b->h_st->swap_count[s];
-        o[2] = b->h_st->swap_count[s];
+        o[2] = b->h_st->swap_send[s];
         o[3] = g.periodic[d] ? 0.0 : -1.0;
         o[4] = g.periodic[d] ? 0.0 : -1.0;
         for (int k = 0; k < 3; ++k) {

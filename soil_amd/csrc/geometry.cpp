@@ -125,11 +125,13 @@ void HostModel::init(const soil_hip_box_desc& d) {
         sub_hi[k] = d.hi[k];
         sub_len[k] = size[k];
         periodic[k] = d.periodic[k] != 0;
+        shrink[k] = false;
         tilt[k] = 0.0;
         bvel[k] = 0.0;
     }
     triclinic = false;
     bounds_changed = false;
+    shrink_padding = 0.0;
     ghost_cutoff = 0.0;
     skin_fraction = d.skin_fraction;
     bin_min_size = d.bin_min_size;
@@ -194,6 +196,41 @@ void HostModel::setup(const double* radius, size_t n) {
     deform_initialized = true;
     deform_step = 0;
     shear_initialized = false;
+}
+
+// shrink_wrap_update (domain.rs:509-569), with the extents of the locals
+// reduced on the device (min and max are exact in any order)
+bool HostModel::shrink_wrap_update(const double* pmin, const double* pmax, size_t nlocal) {
+    if (!has_shrink() || nlocal == 0) return false;
+    double padding;
+    if (shrink_padding > 0.0) {
+        padding = shrink_padding;
+    } else if (ghost_cutoff > 0.0) {
+        padding = ghost_cutoff;
+    } else {
+        const double m = std::fmax(std::fmax(size[0], size[1]), size[2]);
+        padding = m > 0.0 ? m * 0.01 : 1.0;
+    }
+    bool changed = false;
+    for (int k = 0; k < 3; ++k) {
+        if (!shrink[k]) continue;
+        const double nl = pmin[k] - padding, nh = pmax[k] + padding;
+        if (std::fabs(nl - lo[k]) > 1e-12 || std::fabs(nh - hi[k]) > 1e-12) {
+            lo[k] = nl;
+            hi[k] = nh;
+            sub_lo[k] = nl;
+            sub_hi[k] = nh;
+            changed = true;
+        }
+    }
+    if (changed) {
+        for (int k = 0; k < 3; ++k) {  // update_derived (domain.rs:173-179)
+            size[k] = hi[k] - lo[k];
+            sub_len[k] = sub_hi[k] - sub_lo[k];
+        }
+        bounds_changed = true;
+    }
+    return changed;
 }
 
 bool HostModel::has_deform() const {

@@ -13,6 +13,8 @@ struct HostModel {
     // Domain (domain.rs:105-142), one rank: sub-domain = box
     double lo[3], hi[3], size[3], sub_lo[3], sub_hi[3], sub_len[3];
     bool periodic[3];
+    bool shrink[3];  // shrink-wrap axes (non-periodic), domain.rs:27-34
+    double shrink_padding;
     double ghost_cutoff;
     double tilt[3], bvel[3];
     bool triclinic, bounds_changed;
@@ -48,6 +50,9 @@ struct HostModel {
     void setup(const double* radius, size_t n);
     void compute_grid();
     bool has_deform() const;
+    bool has_shrink() const { return shrink[0] || shrink[1] || shrink[2]; }
+    // shrink_wrap from the locals' extents (min[3], max[3]); true when the bounds moved
+    bool shrink_wrap_update(const double* pmin, const double* pmax, size_t nlocal);
     bool apply_deform();  // false when an axis collapses
     void set_tilt(double xy, double bvel_x);
     void sort_prepare();

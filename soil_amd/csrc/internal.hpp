@@ -78,6 +78,7 @@ struct BoxState {
     uint64_t steps_since_build;
     uint64_t rebuilds, steps;
     uint32_t swap_start[SOIL_NSWAP], swap_count[SOIL_NSWAP];
+    uint32_t swap_send[SOIL_NSWAP];             // selected (send_indices); a non-periodic swap receives none
     uint32_t rebuild_now;                       // this step takes the FullRebuild path
     uint32_t err;
     uint32_t ring_pos;                          // geometry slot of the current step
@@ -86,6 +87,7 @@ struct BoxState {
     uint32_t pad2;
     uint64_t csr_bytes, csr_builds;             // algorithmic bytes of CSR builds (timing)
     uint64_t step_limit;                        // the box stops (every kernel skips it) at this step count
+    uint64_t ext_key[6];                        // locals' min x,y,z then max x,y,z as order-preserving keys
 };
 
 struct ColDev {

@@ -291,6 +291,60 @@ __global__ void k_decide(KARGS) {
     }
 }
 
+// shrink_wrap's extents (domain.rs:531-548): min and max of each coordinate
+// over the locals, reduced as order-preserving u64 keys (negative values
+// bit-inverted, positive ones with the sign bit set) so that atomic min/max
+// give the exact extremes in any order
+__device__ __forceinline__ unsigned long long ext_key(double v) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+
+__global__ void k_extent_init(KARGS) {
+    FOR_TILES(t) {
+        if (threadIdx.x != 0) continue;
+        const BoxDev& b = *boxes[tiles[t].box];
+        if (!live(b)) continue;
+        for (int k = 0; k < 3; ++k) {
+            b.st->ext_key[k] = ~0ull;
+            b.st->ext_key[3 + k] = 0ull;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(BLK) k_extent(KARGS) {
+    FOR_TILES(t) {
+        const Tile tl = tiles[t];
+        const BoxDev& b = *boxes[tl.box];
+        if (!live(b)) continue;  // block-uniform
+        const uint32_t i = tl.start + threadIdx.x;
+        const bool act = i < b.st->nlocal;
+        unsigned long long mn[3], mx[3];
+        const double p[3] = {act ? b.x[i] : 0.0, act ? b.y[i] : 0.0, act ? b.z[i] : 0.0};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            mn[k] = act ? ext_key(p[k]) : ~0ull;
+            mx[k] = act ? ext_key(p[k]) : 0ull;
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const unsigned long long a = __shfl_xor(mn[k], o, 64), c = __shfl_xor(mx[k], o, 64);
+                mn[k] = a < mn[k] ? a : mn[k];
+                mx[k] = c > mx[k] ? c : mx[k];
+            }
+        const bool any = __ballot(act) != 0ull;
+        if ((threadIdx.x & 63u) == 0u && any) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                atomicMin(&b.st->ext_key[k], mn[k]);
+                atomicMax(&b.st->ext_key[3 + k], mx[k]);
+            }
+        }
+    }
+}
+
 // remove_ghost_atoms (atom.rs:625-629), FullRebuild only
 __global__ void k_remove_ghosts(KARGS) {
     FOR_TILES(t) {
@@ -868,7 +922,7 @@ __global__ void __launch_bounds__(BLK) k_border_count(KARGS) {
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
                 bool p0 = false, p1 = false;
-                if (in && g.periodic[d]) {
+                if (in) {  // non-periodic swaps select too (send_indices) but receive nothing
                     const double v = border_coord(g, x, y, z, d);
                     p0 = v < g.thr_lo[d];
                     p1 = v >= g.thr_hi[d];
@@ -986,16 +1040,7 @@ __global__ void __launch_bounds__(BLK) k_border_write(KARGS, int d) {
         const uint32_t scan_end = d == 0 ? nl : st->swap_start[2 * d - 1] + st->swap_count[2 * d - 1];
         const uint32_t tix = tl.start / BORDER_TILE;
         if (st->err & ERR_GHOST_CAPACITY) continue;
-        if (!g.periodic[d]) {
-            if (tix == 0 && threadIdx.x == 0) {
-                st->swap_start[2 * d] = scan_end;
-                st->swap_count[2 * d] = 0u;
-                st->swap_start[2 * d + 1] = scan_end;
-                st->swap_count[2 * d + 1] = 0u;
-                st->nghost = scan_end - nl;
-            }
-            continue;
-        }
+        if (!g.periodic[d] && tix != 0) continue;  // only the totals are needed there (block-uniform)
         // totals of the dim and this tile's exclusive prefixes
         const uint32_t nt = (scan_end + BORDER_TILE - 1) / BORDER_TILE;
         uint32_t pb0 = 0, pb1 = 0, tot0 = 0, tot1 = 0;
@@ -1032,6 +1077,19 @@ __global__ void __launch_bounds__(BLK) k_border_write(KARGS, int d) {
             tot0 += red[2][w];
             tot1 += red[3][w];
         }
+        if (!g.periodic[d]) {  // no image (to_proc = -1 on one rank): the selection is only recorded
+            if (tix == 0 && threadIdx.x == 0) {
+                st->swap_start[2 * d] = scan_end;
+                st->swap_count[2 * d] = 0u;
+                st->swap_send[2 * d] = tot0;
+                st->swap_start[2 * d + 1] = scan_end;
+                st->swap_count[2 * d + 1] = 0u;
+                st->swap_send[2 * d + 1] = tot1;
+                st->nghost = scan_end - nl;
+            }
+            __syncthreads();  // red[] is reused by the next tile
+            continue;
+        }
         const bool over = (uint64_t)scan_end + tot0 + tot1 > (uint64_t)b.cap_atoms;
         if (tix == 0 && threadIdx.x == 0) {
             if (over) {
@@ -1039,8 +1097,10 @@ __global__ void __launch_bounds__(BLK) k_border_write(KARGS, int d) {
             } else {
                 st->swap_start[2 * d] = scan_end;
                 st->swap_count[2 * d] = tot0;
+                st->swap_send[2 * d] = tot0;
                 st->swap_start[2 * d + 1] = scan_end + tot0;
                 st->swap_count[2 * d + 1] = tot1;
+                st->swap_send[2 * d + 1] = tot1;
                 st->nghost = scan_end + tot0 + tot1 - nl;
             }
         }
@@ -1093,7 +1153,6 @@ __global__ void __launch_bounds__(BLK) k_border_write(KARGS, int d) {
             double gx, gy, gz;
             place_ghost(b, g, dst, i, lo ? 2 * d : 2 * d + 1, gx, gy, gz);
             for (int e = d + 1; e < 3; ++e) {
-                if (!g.periodic[e]) continue;
                 const double v = border_coord(g, gx, gy, gz, e);
                 const uint32_t which = dst / BORDER_TILE - (lo ? ft0 : ft1);
                 const uint32_t slot = (lo ? 0u : 8u) + which * 4u + (uint32_t)(e - d - 1) * 2u;
@@ -1753,6 +1812,17 @@ void phase_final_initial_decide(const Launch& L) {
 void phase_decide(const Launch& L) {
     launch(k_disp, L.t_local, L.boxes, L.stream, L.maxgrid);
     launch(k_decide<true>, L.t_box, L.boxes, L.stream, L.maxgrid);
+}
+// the kick-drift (and the displacement test) alone, for steps whose decision
+// waits on host work in between (shrink-wrap bounds)
+void phase_kick(const Launch& L, bool with_final) {
+    if (with_final) launch(k_initial<true, true>, L.t_local, L.boxes, L.stream, L.maxgrid);
+    else launch(k_initial<true, false>, L.t_local, L.boxes, L.stream, L.maxgrid);
+}
+void phase_decide_fused(const Launch& L) { launch(k_decide<true>, L.t_box, L.boxes, L.stream, L.maxgrid); }
+void phase_extent(const Launch& L) {
+    launch(k_extent_init, L.t_box, L.boxes, L.stream, L.maxgrid);
+    launch(k_extent, L.t_local, L.boxes, L.stream, L.maxgrid);
 }
 void phase_decide_only(const Launch& L) {
     launch(k_disp, L.t_local, L.boxes, L.stream, L.maxgrid);

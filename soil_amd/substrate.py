@@ -82,6 +82,7 @@ class Context:
 
 
 DEFORM_STYLES = {None: -1, "none": -1, "erate": 0, "vel": 1, "final": 2}
+BOUNDARY = {"p": 0, "f": 1, "s": 2}
 
 
 class Box:
@@ -136,6 +137,14 @@ class Box:
     def set_consumer(self, k, enabled=True):
         check(lib().soil_hip_set_consumer(self.h, 1 if enabled else 0, float(k)))
 
+    def set_boundary(self, dim, kind):
+        """[domain] boundary of one axis (domain.rs:27-34): "p" periodic, "f"
+        fixed, "s" shrink-wrap. Before setup."""
+        check(lib().soil_hip_set_boundary(self.h, int(dim), BOUNDARY[kind]))
+
+    def set_shrink_wrap_padding(self, padding):
+        check(lib().soil_hip_set_shrink_wrap_padding(self.h, float(padding)))
+
     def set_axis_deform(self, dim, style, value=0.0, target=(0.0, 0.0)):
         """[deform] x/y/z (soil_deform/src/lib.rs:98-126): style "erate" (value =
         strain rate), "vel" (value = boundary velocity) or "final" (target =
@@ -170,6 +179,9 @@ class Box:
 
     def remove_ghost_atoms(self):
         check(lib().soil_hip_remove_ghost_atoms(self.h))
+
+    def shrink_wrap(self):
+        check(lib().soil_hip_shrink_wrap(self.h))
 
     def zero_all_forces(self):
         check(lib().soil_hip_zero_all_forces(self.h))

@@ -60,6 +60,8 @@ def lib():
             "soil_ref_set_deform_remap": ([P, C.c_int], None),
             "soil_ref_set_stage_steps": ([P, C.c_uint64], None),
             "soil_ref_get_domain": ([P, DP], None),
+            "soil_ref_set_boundary": ([P, C.c_int, C.c_int], C.c_int),
+            "soil_ref_set_shrink_wrap_padding": ([P, C.c_double], None),
             "soil_ref_setup": ([P], C.c_int),
             "soil_ref_step": ([P, C.c_int], C.c_int),
             "soil_ref_phase": ([P, C.c_int], C.c_int),
@@ -95,7 +97,8 @@ def lib():
 # phase ids of soil_ref_phase
 PH = dict(apply_deform=0, initial_integration=1, decide_rebuild=2, remove_ghost_atoms=3, zero_all_forces=4,
           pbc=5, sort_atoms_by_bin=6, borders=7, forward_comm_borders=8, bin_neighbor_list=9, harmonic_force=10,
-          reverse_send_force=11, final_integration=12)
+          reverse_send_force=11, final_integration=12, shrink_wrap=13)
+BOUNDARY = {"p": 0, "f": 1, "s": 2}
 
 
 def _dp(a):
@@ -172,6 +175,13 @@ class RefBox:
 
     def set_stage_steps(self, n):
         lib().soil_ref_set_stage_steps(self.h, int(n))
+
+    def set_boundary(self, dim, kind):
+        """[domain] boundary of one axis: "p" periodic, "f" fixed, "s" shrink-wrap."""
+        assert lib().soil_ref_set_boundary(self.h, int(dim), BOUNDARY[kind]) == 0
+
+    def set_shrink_wrap_padding(self, padding):
+        lib().soil_ref_set_shrink_wrap_padding(self.h, float(padding))
 
     def domain(self):
         out = np.zeros(9)

@@ -231,3 +231,70 @@ def test_axis_collapse_grows_capacity_then_errors(ctx):
         g.step(1)
     with pytest.raises(RuntimeError, match="collapsed"):
         r.step(1)
+
+
+def boundaries(kinds, padding=None):
+    def f(b):
+        for dim, kind in enumerate(kinds):
+            b.set_boundary(dim, kind)
+        if padding is not None:
+            b.set_shrink_wrap_padding(padding)
+    return f
+
+
+@pytest.mark.parametrize("kinds,k,padding", [("pps", 1000.0, None), ("sss", 1000.0, 0.75), ("psf", None, None)])
+def test_shrink_wrap_matches_oracle(ctx, kinds, k, padding):
+    # shrink-wrap axes follow the locals' extent every step (domain.rs:509-589):
+    # the bounds move, the grid is recomputed and every step rebuilds; the
+    # device reduces the extents exactly and the host moves the bounds
+    print(f"case: shrink-wrap {kinds}", flush=True)
+    g, r = run_compare(ctx, small(5100, 3000, 0.4, 0.6, 0.35), 100, 10, k=k, pre_setup=boundaries(kinds, padding))
+    for a, b in zip(g.domain(), r.domain()):
+        assert np.array_equal(a, b)
+
+
+def test_shrink_wrap_phase_by_phase(ctx):
+    print("case: shrink-wrap per phase", flush=True)
+    d, pos, vel, mass, rad = small(5101, 2000, 0.4, 0.6, 0.35)
+    g, r = make_pair(ctx, d, pos, vel, mass, rad, k=1000.0, pre_setup=boundaries("spp"))
+    for step in range(3):
+        g.apply_deform(); r.phase("apply_deform")
+        g.initial_integration(); r.phase("initial_integration")
+        if step > 0:
+            g.decide_rebuild(); r.phase("decide_rebuild")
+        if r.state == 0:
+            g.remove_ghost_atoms(); r.phase("remove_ghost_atoms")
+        g.zero_all_forces(); r.phase("zero_all_forces")
+        g.shrink_wrap(); r.phase("shrink_wrap")
+        if r.state == 0:
+            g.pbc(); r.phase("pbc")
+        g.sort_atoms_by_bin(); r.phase("sort_atoms_by_bin")
+        if r.state == 0:
+            g.borders(); r.phase("borders")
+            g.bin_neighbor_list(); r.phase("bin_neighbor_list")
+        else:
+            g.forward_comm_borders(); r.phase("forward_comm_borders")
+        g.harmonic_force(); r.phase("harmonic_force")
+        g.reverse_send_force(); r.phase("reverse_send_force")
+        g.final_integration(); r.phase("final_integration")
+        for a, b in zip(g.domain(), r.domain()):
+            assert np.array_equal(a, b), f"step {step}: domain differs"
+        ag, ar = g.atoms(), r.atoms()
+        for key in ("pos", "vel", "force"):
+            assert np.array_equal(ag[key], ar[key]), f"step {step}: {key} differs"
+        og, ig = g.csr()
+        orr, ir = r.csr()
+        assert np.array_equal(og, orr) and np.array_equal(ig, ir)
+
+
+def test_shrink_wrap_box_is_refused_by_a_batch(ctx):
+    print("case: shrink-wrap in a batch", flush=True)
+    d, pos, vel, mass, rad = small(5102, 500, 0.4, 0.6, 0.3)
+    b = soil_amd.Box(ctx, d)
+    b.upload(pos, mass, rad, vel=vel)
+    b.set_boundary(2, "s")
+    b.setup()
+    with pytest.raises(RuntimeError, match="shrink-wrap"):
+        soil_amd.Batch(ctx, [b])
+    b.step(3)
+    b.close()

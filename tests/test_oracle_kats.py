@@ -313,3 +313,65 @@ def test_collapsed_axis_is_an_error():
     b.setup()
     with pytest.raises(RuntimeError, match="collapsed"):
         b.step(20)
+
+
+def shrink_box(lo, hi, kinds, padding, atoms):
+    # domain.rs:880-996: a domain with the given boundary kinds and padding
+    b = RefBox(ref_desc(list(lo), list(hi)))
+    for d, k in enumerate(kinds):
+        b.set_boundary(d, k)
+    b.set_shrink_wrap_padding(padding)
+    if atoms:
+        b.add_atoms(list(atoms), [1.0] * len(atoms), [0.1] * len(atoms))
+    return b
+
+
+def test_shrink_wrap_expands_to_atom_positions():
+    # domain.rs:880-907: z -> [1 - 0.5, 8 + 0.5], x and y untouched
+    b = shrink_box([0, 0, 0], [10, 10, 10], "pps", 0.5, ([1.0, 2.0, 3.0], [5.0, 5.0, 8.0], [9.0, 1.0, 1.0]))
+    b.phase("shrink_wrap")
+    lo, hi, size = b.domain()
+    assert abs(lo[2] - 0.5) < 1e-10 and abs(hi[2] - 8.5) < 1e-10 and abs(size[2] - 8.0) < 1e-10
+    assert abs(lo[0]) < 1e-10 and abs(hi[0] - 10.0) < 1e-10
+
+
+def test_shrink_wrap_no_change_within_tolerance():
+    # domain.rs:909-933: bounds already at the wrap leave the domain as it is
+    b = shrink_box([0, 0, 0.5], [10, 10, 8.5], "pps", 0.5, ([1.0, 2.0, 3.0], [5.0, 5.0, 8.0], [9.0, 1.0, 1.0]))
+    b.phase("shrink_wrap")
+    lo, hi, _ = b.domain()
+    assert lo[2] == 0.5 and hi[2] == 8.5
+
+
+def test_shrink_wrap_no_atoms_is_noop():
+    # domain.rs:935-944
+    b = shrink_box([0, 0, 0], [1, 1, 1], "sss", 0.0, ())
+    b.phase("shrink_wrap")
+    lo, hi, _ = b.domain()
+    assert list(lo) == [0, 0, 0] and list(hi) == [1, 1, 1]
+
+
+def test_shrink_wrap_all_axes():
+    # domain.rs:946-975: padding 1 -> [9,51] [19,61] [29,71], sizes 42
+    b = shrink_box([0, 0, 0], [100, 100, 100], "sss", 1.0, ([10.0, 20.0, 30.0], [50.0, 60.0, 70.0]))
+    b.phase("shrink_wrap")
+    lo, hi, size = b.domain()
+    assert np.allclose(lo, [9, 19, 29], atol=1e-10) and np.allclose(hi, [51, 61, 71], atol=1e-10)
+    assert np.allclose(size, [42, 42, 42], atol=1e-10)
+
+
+def test_shrink_wrap_uses_ghost_cutoff_fallback():
+    # domain.rs:977-994: no padding -> the ghost cutoff (2.0): z -> [3, 7]
+    b = shrink_box([0, 0, 0], [10, 10, 10], "pps", 0.0, ([5.0, 5.0, 5.0],))
+    b.set_ghost_cutoff(2.0)
+    b.phase("shrink_wrap")
+    lo, hi, _ = b.domain()
+    assert abs(lo[2] - 3.0) < 1e-10 and abs(hi[2] - 7.0) < 1e-10
+
+
+def test_shrink_wrap_size_fallback():
+    # domain.rs:518-528: no padding and no ghost cutoff -> 1% of the largest size
+    b = shrink_box([0, 0, 0], [10, 20, 5], "spp", 0.0, ([4.0, 5.0, 1.0], [6.0, 5.0, 1.0]))
+    b.phase("shrink_wrap")
+    lo, hi, _ = b.domain()
+    assert lo[0] == 4.0 - 0.2 and hi[0] == 6.0 + 0.2
