@@ -36,6 +36,7 @@ extern "C" {
 #define SOIL_HIP_ESTATE (-4)     /* call out of order (e.g. step before setup) */
 #define SOIL_HIP_EUNSUPPORTED (-5) /* case outside the device path (see message) */
 #define SOIL_HIP_EBUDGET (-6)    /* serving: job does not fit the HBM budget */
+#define SOIL_HIP_EHOOK (-7)      /* a force hook returned non-zero */
 
 /* AtomData column flags (soil_derive/src/lib.rs:200-500,
  * docs/SOIL_ATOMDATA_CONTRACT.md:87-143): FORWARD = overwritten on ghosts by
@@ -135,6 +136,49 @@ int soil_hip_set_deform_remap(soil_hip_box* box, int remap);
 int soil_hip_set_stage_steps(soil_hip_box* box, uint64_t nsteps);
 /* Current domain bounds and sizes: lo[3], hi[3], size[3]. */
 int soil_hip_get_domain(soil_hip_box* box, double* out9);
+
+/* ── force hook (the GpuForce role, soil_gpu/src/gpu_state.rs:45-55) ────────
+ * A constitutive force registered by a consumer runs in the Force phase of
+ * every step, after the built-in consumer and before reverse_send_force, on
+ * the box's stream, and accumulates into fx/fy/fz (zeroed at
+ * zero_all_forces). With newton on, forces on ghosts reach their owners
+ * through the reverse comm, so a half-list hook may scatter to j. The view
+ * holds device pointers; locals are 0..nlocal, ghosts follow. `offsets` and
+ * `indices` are the half CSR over the locals (row i = indices[offsets[i] ..
+ * offsets[i+1])); `tstart`/`tsrc` the incoming pairs of every atom (column a
+ * = tsrc[tstart[a] .. tstart[a+1]], any order), present when the hook asked
+ * for SOIL_HIP_HOOK_TRANSPOSE or the consumer is on. */
+#define SOIL_HIP_MAX_COLS 8
+#define SOIL_HIP_HOOK_TRANSPOSE 1
+typedef struct soil_hip_device_view {
+    uint32_t nlocal, nghost, npairs, rebuilt; /* rebuilt: the CSR was rebuilt this step */
+    uint32_t cap_local, cap_atoms, cap_pairs, ncols;
+    double *x, *y, *z, *vx, *vy, *vz, *fx, *fy, *fz;
+    const double* xr; /* {x, y, z, radius} per atom, 32 B */
+    const double *mass, *inv_mass, *radius;
+    const uint32_t *tag, *type;
+    const int32_t* origin; /* a ghost's source atom */
+    const uint32_t *offsets, *indices, *tstart, *tsrc;
+    double* cols[SOIL_HIP_MAX_COLS]; /* registered AtomData columns, width f64 per atom */
+    int32_t col_width[SOIL_HIP_MAX_COLS];
+    uint64_t step;
+} soil_hip_device_view;
+/* `view` (host memory) is exact in soil_hip_step and soil_hip_run_force_hook.
+ * In a batch the hook is also recorded into the captured step graph: there
+ * only its cap_* fields are exact (counts are the capacities), and kernels
+ * must read counts and pointers at run time from `d_view`, a device copy of
+ * the view refreshed on the stream just before the hook runs (the spatial
+ * sort swaps column buffers). Launch on `stream` (a hipStream_t); return 0
+ * or an error code. */
+typedef int (*soil_hip_force_hook)(void* user, const soil_hip_device_view* view,
+                                   const soil_hip_device_view* d_view, void* stream);
+/* Registers (fn != NULL) or removes the box's hook; flags: SOIL_HIP_HOOK_*.
+ * Asking for the transpose must precede the first neighbor build. */
+int soil_hip_set_force_hook(soil_hip_box* box, soil_hip_force_hook fn, void* user, int flags);
+/* The view as it stands now (waits for the box stream). */
+int soil_hip_get_device_view(soil_hip_box* box, soil_hip_device_view* out);
+/* The hook alone, as a per-phase system (Force phase). */
+int soil_hip_run_force_hook(soil_hip_box* box);
 
 /* Enables the harmonic soft-sphere test consumer F = k(ri+rj-d) n̂ in the
  * Force phase of soil_hip_step (0 disables; the reference has no built-in

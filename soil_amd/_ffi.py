@@ -67,6 +67,20 @@ class Restart(C.Structure):
                 ("radius", DP), ("nbuf", C.c_size_t), ("buf_len", SZP), ("buf", C.POINTER(DP))]
 
 
+MAX_COLS = 8
+
+
+class DeviceView(C.Structure):
+    """soil_hip_device_view (a force hook's view of a box; device pointers)."""
+    _fields_ = ([(n, C.c_uint32) for n in ("nlocal", "nghost", "npairs", "rebuilt", "cap_local", "cap_atoms",
+                                           "cap_pairs", "ncols")]
+                + [(n, C.c_void_p) for n in ("x", "y", "z", "vx", "vy", "vz", "fx", "fy", "fz", "xr", "mass",
+                                             "inv_mass", "radius", "tag", "type", "origin", "offsets", "indices",
+                                             "tstart", "tsrc")]
+                + [("cols", C.c_void_p * MAX_COLS), ("col_width", C.c_int32 * MAX_COLS), ("step", C.c_uint64)])
+
+
+HOOK_TRANSPOSE = 1
 RESTART_BINCODE, RESTART_JSON = 0, 1
 DUMP_TEXT, DUMP_BINARY, DUMP_LAMMPS = 0, 1, 2
 
@@ -82,6 +96,9 @@ _SIGS = {
     "soil_hip_upload_column": ([P, C.c_int, DP, SZ], C.c_int),
     "soil_hip_setup": ([P], C.c_int),
     "soil_hip_set_consumer": ([P, C.c_int, C.c_double], C.c_int),
+    "soil_hip_set_force_hook": ([P, C.c_void_p, C.c_void_p, C.c_int], C.c_int),
+    "soil_hip_get_device_view": ([P, C.POINTER(DeviceView)], C.c_int),
+    "soil_hip_run_force_hook": ([P], C.c_int),
     "soil_hip_set_axis_deform": ([P, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double], C.c_int),
     "soil_hip_set_deform_remap": ([P, C.c_int], C.c_int),
     "soil_hip_set_stage_steps": ([P, C.c_uint64], C.c_int),

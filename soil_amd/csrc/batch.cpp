@@ -37,7 +37,11 @@ static int batch_tiles(soil_hip_batch* bt) {
     return 0;
 }
 
-static void launch_step(const Launch& L) {
+// One step of every box. Force hooks are called here too, so under capture
+// their launches are recorded into the graph; they read counts and buffers
+// from their device views, refreshed by k_sync_view just before.
+static int launch_step(soil_hip_batch* bt) {
+    const Launch& L = bt->L;
     phase_deform(L);
     phase_initial_decide(L);
     phase_zero_fused(L);
@@ -47,9 +51,23 @@ static void launch_step(const Launch& L) {
     phase_forward(L);
     phase_build_fused(L, nullptr, nullptr, nullptr, nullptr);
     phase_force_fused(L);
+    int rc = 0;
+    bool hooks = false;
+    for (auto* b : bt->boxes) hooks |= b->hook != nullptr;
+    if (hooks) {
+        phase_sync_view(L);
+        for (auto* b : bt->boxes) {
+            if (!b->hook) continue;
+            soil_hip_device_view v;
+            fill_view(b, v, false, 0);
+            const int r = b->hook(b->hook_user, &v, b->h_dev.view, (void*)bt->stream);
+            if (r && !rc) rc = set_error(SOIL_HIP_EHOOK, "force hook returned %d", r);
+        }
+    }
     phase_reverse(L);
     phase_final(L);
     phase_step_end(L);
+    return rc;
 }
 
 static void drop_graph(soil_hip_batch* bt) {
@@ -62,8 +80,12 @@ static void drop_graph(soil_hip_batch* bt) {
 static int capture_graph(soil_hip_batch* bt) {
     drop_graph(bt);
     HIPCHK(hipStreamBeginCapture(bt->stream, hipStreamCaptureModeThreadLocal));
-    launch_step(bt->L);
+    const int rc = launch_step(bt);
     HIPCHK(hipStreamEndCapture(bt->stream, &bt->graph));
+    if (rc) {
+        drop_graph(bt);
+        return rc;
+    }
     HIPCHK(hipGraphInstantiate(&bt->graph_exec, bt->graph, nullptr, nullptr, 0));
     return 0;
 }
@@ -161,7 +183,10 @@ static int batch_run(soil_hip_batch* bt, int nsteps, bool use_graph) {
                               sizeof(Geom) * nb * run, hipMemcpyHostToDevice, bt->stream));
         for (uint32_t j = 0; j < run; ++j) {
             if (use_graph) HIPCHK(hipGraphLaunch(bt->graph_exec, bt->stream));
-            else launch_step(bt->L);
+            else {
+                int rc = launch_step(bt);
+                if (rc) return rc;
+            }
         }
         HIPCHK(hipEventRecord(bt->half_done[h], bt->stream));
         bt->half_pending[h] = true;

@@ -496,6 +496,13 @@ int step_plain_loop(soil_hip_box* b, int nsteps, bool& pending_final) {
             phase_forward(b->L);  // k_decide<true> left rebuild_now = 0 and the sort flags cleared
         }
         if (b->consumer) phase_force_fused(b->L);
+        if (b->hook) {
+            // the host counts are current (read after the decision, the
+            // borders and the build); the column buffers swap at each sort
+            rc = pull_desc(b);
+            if (!rc) rc = call_hook(b, true, b->h_st->steps);
+            if (rc) return rc;
+        }
         if (b->model.newton) {
             for (int d = 0; d < 3; ++d) b->L.rev_split[d] = g.disjoint[d] ? 0 : 1;
             phase_reverse(b->L);
@@ -595,6 +602,7 @@ int soil_hip_box_destroy(soil_hip_box* b) {
     if (b->d_dev) hipFree(b->d_dev);
     if (b->d_ptrs) hipFree(b->d_ptrs);
     if (b->d_st) hipFree(b->d_st);
+    if (b->d_view) hipFree(b->d_view);
     if (b->d_ring) hipFree(b->d_ring);
     if (b->h_st) hipHostFree(b->h_st);
     if (b->h_ring) hipHostFree(b->h_ring);
@@ -682,6 +690,7 @@ int soil_hip_setup(soil_hip_box* b) {
     HIPCHK(hipMemcpy(b->d_ptrs, &b->d_dev, sizeof(BoxDev*), hipMemcpyHostToDevice));
     HIPCHK(hipMalloc(&b->d_st, sizeof(BoxState)));
     HIPCHK(hipHostMalloc(&b->h_st, sizeof(BoxState), hipHostMallocDefault));
+    HIPCHK(hipMalloc(&b->d_view, sizeof(soil_hip_device_view)));
     HIPCHK(hipMalloc(&b->d_ring, sizeof(Geom) * b->ring_len));
     HIPCHK(hipHostMalloc(&b->h_ring, sizeof(Geom) * b->ring_len, hipHostMallocDefault));
     int rc = alloc_atoms(b, c, false);
@@ -717,12 +726,14 @@ int soil_hip_setup(soil_hip_box* b) {
         HIPCHK(up(d.xr, xr.data(), n * 32));
     }
     d.st = b->d_st;
+    d.view = b->d_view;
     d.geom_ring = b->d_ring;
     d.ring_len = b->ring_len;
     d.ring_stride = 1;
     d.harmonic_k = b->harmonic_k;
     d.dt = b->model.dt;
     d.consumer = b->consumer ? 1 : 0;
+    d.transpose = (b->consumer || (b->hook && (b->hook_flags & SOIL_HIP_HOOK_TRANSPOSE))) ? 1 : 0;
     {
         const char* m = getenv("SOIL_HIP_CSR_MODE");
         d.csr_mode = m ? atoi(m) : 2;
@@ -812,13 +823,100 @@ int soil_hip_set_consumer(soil_hip_box* b, int enabled, double k) {
     if (b->setup_done) {
         b->h_dev.consumer = b->consumer ? 1 : 0;
         b->h_dev.harmonic_k = k;
-        // patch the two fields in place (the rest of the device descriptor may hold swapped pointers)
+        b->h_dev.transpose = (b->consumer || (b->hook && (b->hook_flags & SOIL_HIP_HOOK_TRANSPOSE))) ? 1 : 0;
+        // patch the fields in place (the rest of the device descriptor may hold swapped pointers)
         HIPCHK(hipStreamSynchronize(b->stream));
         HIPCHK(hipMemcpy(&b->d_dev->harmonic_k, &b->h_dev.harmonic_k, sizeof(double), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(&b->d_dev->consumer, &b->h_dev.consumer, sizeof(int32_t), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(&b->d_dev->transpose, &b->h_dev.transpose, sizeof(int32_t), hipMemcpyHostToDevice));
     }
     return 0;
 }
+
+int soil_hip_set_force_hook(soil_hip_box* b, soil_hip_force_hook fn, void* user, int flags) {
+    if (!b) return set_error(SOIL_HIP_EINVAL, "null box");
+    if (b->in_batch) return set_error(SOIL_HIP_ESTATE, "set hooks before batching the box");
+    const bool want_t = fn && (flags & SOIL_HIP_HOOK_TRANSPOSE);
+    if (b->setup_done && want_t && !b->h_dev.transpose) {
+        int rc = pull_state(b);
+        if (rc) return rc;
+        if (b->h_st->rebuilds != 0)
+            return set_error(SOIL_HIP_ESTATE, "ask for the transpose before the first neighbor build");
+    }
+    b->hook = fn;
+    b->hook_user = user;
+    b->hook_flags = fn ? flags : 0;
+    if (b->setup_done) {
+        b->h_dev.transpose = (b->consumer || want_t) ? 1 : 0;
+        HIPCHK(hipStreamSynchronize(b->stream));
+        HIPCHK(hipMemcpy(&b->d_dev->transpose, &b->h_dev.transpose, sizeof(int32_t), hipMemcpyHostToDevice));
+    }
+    return 0;
+}
+
+}  // extern "C"
+
+namespace soil_hip {
+void fill_view(soil_hip_box* b, soil_hip_device_view& v, bool exact, uint64_t step) {
+    std::memset(&v, 0, sizeof(v));
+    const BoxDev& d = b->h_dev;
+    v.cap_local = d.cap_local;
+    v.cap_atoms = d.cap_atoms;
+    v.cap_pairs = d.cap_pairs;
+    v.step = step;
+    if (!exact) {  // inside a batch: counts are the capacities, pointers come from d_view
+        v.nlocal = d.cap_local;
+        v.nghost = d.cap_atoms - d.cap_local;
+        v.npairs = d.cap_pairs;
+        return;
+    }
+    v.nlocal = b->h_st->nlocal;
+    v.nghost = b->h_st->nghost;
+    v.npairs = b->h_st->npairs;
+    v.rebuilt = b->h_st->rebuild_now;
+    v.ncols = (uint32_t)d.ncols;
+    v.x = d.x; v.y = d.y; v.z = d.z;
+    v.vx = d.vx; v.vy = d.vy; v.vz = d.vz;
+    v.fx = d.fx; v.fy = d.fy; v.fz = d.fz;
+    v.xr = d.xr;
+    v.mass = d.mass; v.inv_mass = d.inv_mass; v.radius = d.rad;
+    v.tag = d.tag; v.type = d.type; v.origin = d.origin;
+    v.offsets = d.offsets;
+    v.indices = d.indices;
+    v.tstart = d.transpose ? d.tstart : nullptr;
+    v.tsrc = d.transpose ? d.tsrc : nullptr;
+    for (int c = 0; c < d.ncols && c < SOIL_HIP_MAX_COLS; ++c) {
+        v.cols[c] = d.cols[c].data;
+        v.col_width[c] = d.cols[c].width;
+    }
+}
+
+// Refreshes the device view on the stream, then calls the hook (which
+// enqueues its kernels behind the refresh).
+int call_hook(soil_hip_box* b, bool exact, uint64_t step) {
+    phase_sync_view(b->L);
+    soil_hip_device_view v;
+    fill_view(b, v, exact, step);
+    const int rc = b->hook(b->hook_user, &v, b->h_dev.view, (void*)b->stream);
+    if (rc) return set_error(SOIL_HIP_EHOOK, "force hook returned %d", rc);
+    return 0;
+}
+}  // namespace soil_hip
+
+extern "C" {
+
+int soil_hip_get_device_view(soil_hip_box* b, soil_hip_device_view* out) {
+    int rc = check_box(b, true);
+    if (rc) return rc;
+    if (!out) return set_error(SOIL_HIP_EINVAL, "null view");
+    rc = pull_state(b);
+    if (rc) return rc;
+    rc = pull_desc(b);  // column buffers swap with every spatial sort
+    if (rc) return rc;
+    fill_view(b, *out, true, b->h_st->steps);
+    return 0;
+}
+
 
 // ── per-phase systems ──────────────────────────────────────────────────────
 #define PHASE_PROLOGUE(b)              \
@@ -941,6 +1039,16 @@ int soil_hip_harmonic_force(soil_hip_box* b) {
     if (!b->consumer) return set_error(SOIL_HIP_ESTATE, "consumer disabled (soil_hip_set_consumer)");
     phase_force(b->L);
     return 0;
+}
+
+int soil_hip_run_force_hook(soil_hip_box* b) {
+    PHASE_PROLOGUE(b);
+    if (!b->hook) return set_error(SOIL_HIP_ESTATE, "no force hook (soil_hip_set_force_hook)");
+    int rc = pull_state(b);
+    if (rc) return rc;
+    rc = pull_desc(b);
+    if (rc) return rc;
+    return call_hook(b, true, b->h_st->steps);
 }
 
 int soil_hip_reverse_send_force(soil_hip_box* b) {

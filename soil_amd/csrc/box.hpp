@@ -44,6 +44,9 @@ int pull_desc(soil_hip_box* b);
 int pull_state(soil_hip_box* b);
 int rebuild_tiles(soil_hip_box* b);
 int ensure_cells(soil_hip_box* b, uint64_t ncells);
+// device view for a force hook; exact = host counts are current (plain step)
+void fill_view(soil_hip_box* b, soil_hip_device_view& v, bool exact, uint64_t step);
+int call_hook(soil_hip_box* b, bool exact, uint64_t step);
 int check_err_bits(uint32_t err);
 int batch_make(soil_hip_ctx* ctx, soil_hip_box** boxes, int nboxes, soil_hip_batch** out);
 int batch_free(soil_hip_batch* bt, bool restore);
@@ -73,6 +76,7 @@ struct soil_hip_box {
     soil_hip::BoxDev** d_ptrs = nullptr;  // one-entry pointer array for single-box launches
     soil_hip::BoxState* d_st = nullptr;
     soil_hip::BoxState* h_st = nullptr;   // pinned
+    soil_hip_device_view* d_view = nullptr;  // device copy of the force hook's view
     soil_hip::Geom* d_ring = nullptr;
     soil_hip::Geom* h_ring = nullptr;     // pinned
     uint32_t ring_len = 64;
@@ -81,6 +85,10 @@ struct soil_hip_box {
     soil_hip::Launch L;
     bool consumer = false;
     double harmonic_k = 0.0;
+    // force hook (Force phase, after the consumer)
+    soil_hip_force_hook hook = nullptr;
+    void* hook_user = nullptr;
+    int hook_flags = 0;
     // timing
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;

@@ -16,8 +16,10 @@
 #include <cstddef>
 #include <cstdint>
 
+#include "soil_hip.h"
+
 #define SOIL_MAX_STENCIL 256
-#define SOIL_MAX_COLS 8
+#define SOIL_MAX_COLS SOIL_HIP_MAX_COLS
 #define SOIL_NSWAP 6
 
 namespace soil_hip {
@@ -145,6 +147,8 @@ struct BoxDev {
     double harmonic_k;
     double dt;            // Atom::dt (constant per box)
     int32_t consumer;
+    int32_t transpose;    // build the incoming-pair transpose with the CSR (consumer or a hook that asks)
+    soil_hip_device_view* view;  // device copy of the hook's view (refreshed by k_sync_view), or null
     int32_t csr_mode;     // CSR strategy (bit-neutral): 0 LDS tiles, 1 two global passes, 2 one pass + row slots
 };
 

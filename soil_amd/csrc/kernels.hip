@@ -724,10 +724,11 @@ __global__ void k_post_sort(KARGS) {
     }
 }
 
-// consumer transpose is rebuilt with the CSR, for boxes that have a consumer
+// the incoming-pair transpose is rebuilt with the CSR, for boxes whose
+// consumer or force hook reads it
 __device__ __forceinline__ bool transpose_active(const BoxDev& b) {
     const BoxState* st = b.st;
-    return st->rebuild_now != 0 && b.consumer && (st->err & (ERR_PAIR_CAPACITY | ERR_GHOST_CAPACITY)) == 0;
+    return st->rebuild_now != 0 && b.transpose && (st->err & (ERR_PAIR_CAPACITY | ERR_GHOST_CAPACITY)) == 0;
 }
 
 // ── exclusive scans (reduce tiles -> scan tile sums per box -> down-sweep) ──
@@ -1283,7 +1284,7 @@ __global__ void __launch_bounds__(BLK) k_save_build(KARGS) {
         if (!live(b)) continue;
         if (!b.st->rebuild_now) continue;
         const uint32_t i = tl.start + threadIdx.x;
-        if (b.consumer && i <= b.st->nlocal + b.st->nghost) b.tcount[i] = 0u;
+        if (b.transpose && i <= b.st->nlocal + b.st->nghost) b.tcount[i] = 0u;
         if (i >= b.st->nlocal || b.st->lb_fresh) continue;
         b.lbx[i] = b.x[i];
         b.lby[i] = b.y[i];
@@ -1738,6 +1739,45 @@ __global__ void __launch_bounds__(BLK) k_harmonic(KARGS) {
     }
 }
 
+// The force hook's device view: this step's counts and the current buffers
+// (the spatial sort swaps the permuted columns), one thread per box.
+__global__ void k_sync_view(KARGS) {
+    FOR_TILES(t) {
+        if (threadIdx.x != 0) continue;
+        const BoxDev& b = *boxes[tiles[t].box];
+        if (!b.view) continue;
+        soil_hip_device_view& v = *b.view;
+        if (!live(b)) {  // a stopped box: the hook sees no atoms
+            v.nlocal = v.nghost = v.npairs = v.rebuilt = 0;
+            continue;
+        }
+        const BoxState* st = b.st;
+        v.nlocal = st->nlocal;
+        v.nghost = st->nghost;
+        v.npairs = st->npairs;
+        v.rebuilt = st->rebuild_now;
+        v.cap_local = b.cap_local;
+        v.cap_atoms = b.cap_atoms;
+        v.cap_pairs = b.cap_pairs;
+        v.ncols = (uint32_t)b.ncols;
+        v.x = b.x; v.y = b.y; v.z = b.z;
+        v.vx = b.vx; v.vy = b.vy; v.vz = b.vz;
+        v.fx = b.fx; v.fy = b.fy; v.fz = b.fz;
+        v.xr = b.xr;
+        v.mass = b.mass; v.inv_mass = b.inv_mass; v.radius = b.rad;
+        v.tag = b.tag; v.type = b.type; v.origin = b.origin;
+        v.offsets = b.offsets;
+        v.indices = b.indices;
+        v.tstart = b.transpose ? b.tstart : nullptr;
+        v.tsrc = b.transpose ? b.tsrc : nullptr;
+        for (int c = 0; c < SOIL_HIP_MAX_COLS; ++c) {
+            v.cols[c] = c < b.ncols ? b.cols[c].data : nullptr;
+            v.col_width[c] = c < b.ncols ? b.cols[c].width : 0;
+        }
+        v.step = st->steps;
+    }
+}
+
 __global__ void k_step_end(KARGS) {
     FOR_TILES(t) {
         if (threadIdx.x != 0) continue;
@@ -1884,6 +1924,7 @@ void phase_reverse(const Launch& L) {
     }
 }
 void phase_final(const Launch& L) { launch(k_final, L.t_local, L.boxes, L.stream, L.maxgrid); }
+void phase_sync_view(const Launch& L) { launch(k_sync_view, L.t_box, L.boxes, L.stream, L.maxgrid); }
 void phase_step_end(const Launch& L) { launch(k_step_end, L.t_box, L.boxes, L.stream, L.maxgrid); }
 void phase_collect(const Launch& L, uint32_t* out) { launch(k_collect, L.t_box, L.boxes, L.stream, L.maxgrid, out); }
 

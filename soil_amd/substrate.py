@@ -137,6 +137,22 @@ class Box:
     def set_consumer(self, k, enabled=True):
         check(lib().soil_hip_set_consumer(self.h, 1 if enabled else 0, float(k)))
 
+    def set_force_hook(self, fn, user=None, transpose=False):
+        """Registers a force hook (soil_hip_force_hook; `fn` and `user` are raw
+        addresses, e.g. from a ctypes-loaded library) run in the Force phase;
+        fn=None removes it. transpose: also build the incoming-pair transpose."""
+        flags = _ffi.HOOK_TRANSPOSE if transpose else 0
+        check(lib().soil_hip_set_force_hook(self.h, fn, user, flags))
+
+    def device_view(self):
+        """The hook view as it stands (counts, capacities, device addresses)."""
+        v = _ffi.DeviceView()
+        check(lib().soil_hip_get_device_view(self.h, C.byref(v)))
+        return v
+
+    def run_force_hook(self):
+        check(lib().soil_hip_run_force_hook(self.h))
+
     def set_boundary(self, dim, kind):
         """[domain] boundary of one axis (domain.rs:27-34): "p" periodic, "f"
         fixed, "s" shrink-wrap. Before setup."""

@@ -1,7 +1,9 @@
 """CPU checks of the product library: it loads, exports every header symbol,
 and its host-side generator is deterministic. No GPU call is made here."""
+import ctypes as C
 import os
 import re
+import subprocess
 
 import numpy as np
 
@@ -95,3 +97,19 @@ def test_host_geometry_model_matches_oracle_grids():
                 g, s = _grid_for(d, rad, xy)
                 assert np.array_equal(g[:12], g_ref[:12]), (cfg, box_id, xy)
                 assert np.array_equal(s, s_ref), (cfg, box_id, xy)
+
+
+def test_device_view_mirror_matches_the_header(tmp_path):
+    # the ctypes mirror of soil_hip_device_view has the C struct's size and
+    # field offsets (a C program built against the header prints them)
+    fields = [f[0] for f in _ffi.DeviceView._fields_]
+    body = "".join(f'printf(" %zu", offsetof(soil_hip_device_view, {f}));' for f in fields)
+    src = tmp_path / "view.c"
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "soil_hip.h"\n'
+                   'int main(void) { printf("%zu", sizeof(soil_hip_device_view)); ' + body + " return 0; }\n")
+    exe = tmp_path / "view"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = [int(t) for t in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    assert out[0] == C.sizeof(_ffi.DeviceView)
+    for name, off in zip(fields, out[1:]):
+        assert getattr(_ffi.DeviceView, name).offset == off, name
