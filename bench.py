@@ -42,6 +42,10 @@ def cpu_model():
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_plain_1M.json")
 
 
+# the count-pass kernel of each CSR strategy (SOIL_HIP_CSR_MODE; 3 is the default)
+COUNT_KERNEL = {0: "k_csr_count", 1: "k_csr_count_g", 2: "k_csr_count_g", 3: "k_csr_count_w"}
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary of this workload (copy-probe calibrated), or None."""
@@ -243,9 +247,10 @@ def main():
             per_launch_bytes = tm["csr_bytes"] / tm["builds"]
             avg_count_ms = tm["count_ms"] / tm["builds"]
             ach = per_launch_bytes / (avg_count_ms * 1e-3) / 1e9
-            traffic, traffic_src = measured_traffic("k_csr_count_g")
+            kname = COUNT_KERNEL[int(os.environ.get("SOIL_HIP_CSR_MODE", "3") or 3)]
+            traffic, traffic_src = measured_traffic(kname)
             out["roofline"] = {
-                "kernel": "k_csr_count_g (CSR count pass: candidate tests, row counts and row entries)",
+                "kernel": f"{kname} (CSR count pass: candidate tests, row counts and row entries)",
                 "bound": "hbm",
                 "achieved": ach,
                 "peak": HBM_PEAK_GBS,

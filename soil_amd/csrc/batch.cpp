@@ -33,7 +33,7 @@ static int batch_tiles(soil_hip_batch* bt) {
     bt->L.boxes = bt->d_ptrs;
     bt->L.stream = bt->stream;
     bt->L.maxgrid = maxgrid;
-    bt->L.csr_mode = bt->boxes.empty() ? 2 : bt->boxes[0]->h_dev.csr_mode;
+    bt->L.csr_mode = bt->boxes.empty() ? 3 : bt->boxes[0]->h_dev.csr_mode;
     return 0;
 }
 

@@ -92,64 +92,73 @@ struct BoxState {
     uint64_t ext_key[6];                        // locals' min x,y,z then max x,y,z as order-preserving keys
 };
 
+// Device builds see the arena pointers as global-memory pointers (address
+// space 1), so kernels issue global loads, stores and atomics instead of flat
+// ones; the layout is the same 64-bit pointer either way.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define GMEM __attribute__((address_space(1)))
+#else
+#define GMEM
+#endif
+
 struct ColDev {
-    double* data;
-    double* alt;
+    GMEM double* data;
+    GMEM double* alt;
     int32_t width;
     uint32_t flags;
 };
 
 struct BoxDev {
-    double *x, *y, *z, *vx, *vy, *vz, *fx, *fy, *fz, *mass, *inv_mass, *rad;
-    uint32_t *tag, *type;
-    int32_t *origin, *imx, *imy, *imz;
+    GMEM double *x, *y, *z, *vx, *vy, *vz, *fx, *fy, *fz, *mass, *inv_mass, *rad;
+    GMEM uint32_t *tag, *type;
+    GMEM int32_t *origin, *imx, *imy, *imz;
     // spatial-sort alternates (same capacity)
-    double *ax, *ay, *az, *avx, *avy, *avz, *amass, *ainv_mass, *arad;
-    uint32_t *atag, *atype;
-    int32_t *aimx, *aimy, *aimz;
-    uint8_t* gswap;
-    double* xr;           // AoS mirror (x, y, z, rad) per atom for neighbor gathers
-    double *lbx, *lby, *lbz;
+    GMEM double *ax, *ay, *az, *avx, *avy, *avz, *amass, *ainv_mass, *arad;
+    GMEM uint32_t *atag, *atype;
+    GMEM int32_t *aimx, *aimy, *aimz;
+    GMEM uint8_t* gswap;
+    GMEM double* xr;           // AoS mirror (x, y, z, rad) per atom for neighbor gathers
+    GMEM double *lbx, *lby, *lbz;
     // sort / bin scratch
-    uint32_t* cell;       // per atom cell (locals: spatial sort result reused by the build)
-    uint32_t* slot;       // atomic arrival slot within a cell
-    uint32_t* tmp;        // unordered scatter
-    uint32_t* sorted;     // bin_sorted_atoms (or spatial-sort perm)
-    uint32_t* sidx;       // bin_atom_sorted_idx
-    double* spos;         // 4 f64 per sorted slot: x y z r
-    uint32_t* bin_start;  // cap_cells + 1
-    uint32_t* counts;     // cap_cells + 1
-    uint32_t* scan_part;  // per-tile partial sums (scan scratch)
+    GMEM uint32_t* cell;       // per atom cell (locals: spatial sort result reused by the build)
+    GMEM uint32_t* slot;       // atomic arrival slot within a cell
+    GMEM uint32_t* tmp;        // unordered scatter
+    GMEM uint32_t* sorted;     // bin_sorted_atoms (or spatial-sort perm)
+    GMEM uint32_t* sidx;       // bin_atom_sorted_idx
+    GMEM double* spos;         // 4 f64 per sorted slot: x y z r
+    GMEM uint32_t* bin_start;  // cap_cells + 1
+    GMEM uint32_t* counts;     // cap_cells + 1
+    GMEM uint32_t* scan_part;  // per-tile partial sums (scan scratch)
     // spatial sort (sort_atoms_by_bin) keeps its own arrays, like the reference's
     // sort_cell / sort_counts / sort_perm, so the last build's bins stay readable
-    uint32_t* scell;      // cap_local
-    uint32_t* sstart;     // cap_cells + 1
-    uint32_t* perm;       // cap_local
+    GMEM uint32_t* scell;      // cap_local
+    GMEM uint32_t* sstart;     // cap_cells + 1
+    GMEM uint32_t* perm;       // cap_local
     // CSR
-    uint32_t* offsets;    // cap_local + 1
-    uint32_t* indices;    // cap_pairs
-    uint32_t* ttmp;       // cap_pairs: CSR temp region (rows before placement)
+    GMEM uint32_t* offsets;    // cap_local + 1
+    GMEM uint32_t* indices;    // cap_pairs
+    GMEM uint32_t* ttmp;       // cap_pairs: CSR temp region (rows before placement)
     // consumer: incoming pairs per atom (transpose of the half list, any order)
-    uint32_t* tcount;     // cap_atoms + 1
-    uint32_t* tstart;     // cap_atoms + 1
-    uint32_t* tslot;      // cap_pairs: arrival slot of each CSR entry in its column
-    uint32_t* tsrc;       // cap_pairs: row atom of each incoming pair, grouped by column
+    GMEM uint32_t* tcount;     // cap_atoms + 1
+    GMEM uint32_t* tstart;     // cap_atoms + 1
+    GMEM uint32_t* tslot;      // cap_pairs: arrival slot of each CSR entry in its column
+    GMEM uint32_t* tsrc;       // cap_pairs: row atom of each incoming pair, grouped by column
     // border compaction scratch: per tile counts of the two swaps
-    uint32_t* ctile;      // CSR: per local tile, entries staged in the temp region (or overflow mark)
-    uint32_t* bcnt;       // 6 per border tile: selections of the 6 swaps
+    GMEM uint32_t* ctile;      // CSR: per local tile, entries staged in the temp region (or overflow mark)
+    GMEM uint32_t* bcnt;       // 6 per border tile: selections of the 6 swaps
     ColDev cols[SOIL_MAX_COLS];
     int32_t ncols;
     uint32_t cap_local, cap_atoms, cap_cells, cap_pairs, cap_tiles;
-    BoxState* st;
-    const Geom* geom_ring;  // per-step geometry ring (host-written)
+    GMEM BoxState* st;
+    const GMEM Geom* geom_ring;  // per-step geometry ring (host-written)
     uint32_t ring_len;
     uint32_t ring_stride;   // Geom records between consecutive slots of this box
     double harmonic_k;
     double dt;            // Atom::dt (constant per box)
     int32_t consumer;
     int32_t transpose;    // build the incoming-pair transpose with the CSR (consumer or a hook that asks)
-    soil_hip_device_view* view;  // device copy of the hook's view (refreshed by k_sync_view), or null
-    int32_t csr_mode;     // CSR strategy (bit-neutral): 0 LDS tiles, 1 two global passes, 2 one pass + row slots
+    GMEM soil_hip_device_view* view;  // device copy of the hook's view (refreshed by k_sync_view), or null
+    int32_t csr_mode;     // CSR strategy (bit-neutral): 0 LDS tiles, 1 two global passes, 2 one pass + row slots, 3 (default) as 2 with wave-staged candidates
 };
 
 int set_error(int code, const char* fmt, ...);

@@ -34,12 +34,34 @@ struct TileIter {
         if (const uint32_t t = it_.t; true)
 #define KARGS BoxDev* const* __restrict__ boxes, const Tile* __restrict__ tiles, uint32_t ntiles
 
-// A box whose step count reached its limit is frozen: every phase skips it.
-__device__ __forceinline__ bool live(const BoxDev& b) { return b.st->steps < b.st->step_limit; }
+// Kernels read box descriptors and the step geometry through the constant
+// address space: both are fixed while a kernel runs (the host writes the
+// geometry between launches; only k_swap_ptrs and k_post_sort rewrite a
+// descriptor, through BoxDev&). Stores to the arenas cannot alias them, so
+// their fields load once, into scalar registers, instead of being re-read
+// after every store.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define CMEM __attribute__((address_space(4)))
+#else
+#define CMEM
+#endif
+typedef const CMEM BoxDev CBox;
+typedef const CMEM Geom CGeom;
+__device__ __forceinline__ CBox& box_at(BoxDev* const* boxes, uint32_t k) { return *(CBox*)boxes[k]; }
 
-__device__ __forceinline__ const Geom& geom_of(const BoxDev& b) {
-    return b.geom_ring[(size_t)b.st->ring_pos * b.ring_stride];
+// A box whose step count reached its limit is frozen: every phase skips it.
+__device__ __forceinline__ bool live(CBox& b) { return b.st->steps < b.st->step_limit; }
+
+__device__ __forceinline__ CGeom& geom_of(CBox& b) {
+    return *(CGeom*)(b.geom_ring + (size_t)b.st->ring_pos * b.ring_stride);
 }
+#if defined(__HIP_DEVICE_COMPILE__)
+// the two kernels that rewrite a descriptor hold it as BoxDev&
+__device__ __forceinline__ bool live(const BoxDev& b) { return b.st->steps < b.st->step_limit; }
+__device__ __forceinline__ CGeom& geom_of(const BoxDev& b) {
+    return *(CGeom*)(b.geom_ring + (size_t)b.st->ring_pos * b.ring_stride);
+}
+#endif
 
 // neighbor.rs:915-922 / 1065-1076: floor, saturating `as i32`, clamp.
 __device__ __forceinline__ int cell_coord(double v, int n) {
@@ -50,7 +72,7 @@ __device__ __forceinline__ int cell_coord(double v, int n) {
 }
 
 // domain.rs:211-217
-__device__ __forceinline__ void x2lamda(const Geom& g, double x, double y, double z, double& lx, double& ly,
+__device__ __forceinline__ void x2lamda(CGeom& g, double x, double y, double z, double& lx, double& ly,
                                         double& lz) {
     double xy = g.tilt[0], xz = g.tilt[1], yz = g.tilt[2];
     lz = (z - g.lo[2]) / g.size[2];
@@ -59,7 +81,7 @@ __device__ __forceinline__ void x2lamda(const Geom& g, double x, double y, doubl
 }
 
 // domain.rs:222-229
-__device__ __forceinline__ void lamda2x(const Geom& g, double lx, double ly, double lz, double& x, double& y,
+__device__ __forceinline__ void lamda2x(CGeom& g, double lx, double ly, double lz, double& x, double& y,
                                         double& z) {
     double xy = g.tilt[0], xz = g.tilt[1], yz = g.tilt[2];
     x = g.lo[0] + g.size[0] * lx + xy * ly + xz * lz;
@@ -67,7 +89,7 @@ __device__ __forceinline__ void lamda2x(const Geom& g, double lx, double ly, dou
     z = g.lo[2] + g.size[2] * lz;
 }
 
-__device__ __forceinline__ uint32_t cell_of(const Geom& g, double x, double y, double z) {
+__device__ __forceinline__ uint32_t cell_of(CGeom& g, double x, double y, double z) {
     double px = x, py = y, pz = z;
     if (g.triclinic) x2lamda(g, x, y, z, px, py, pz);
     int cx = cell_coord((px - g.bin_origin[0]) * g.inv_bin[0], g.nbin[0]);
@@ -87,7 +109,7 @@ __device__ __forceinline__ void wave_flag(uint32_t* flag, bool pred) {
     }
 }
 
-__device__ __forceinline__ void put_xr(const BoxDev& b, uint32_t i, double x, double y, double z) {
+__device__ __forceinline__ void put_xr(CBox& b, uint32_t i, double x, double y, double z) {
     double* p = b.xr + 4 * (size_t)i;
     p[0] = x;
     p[1] = y;
@@ -125,7 +147,7 @@ __device__ uint32_t block_excl_scan(uint32_t v, uint32_t* lds, uint32_t& total) 
 __global__ void __launch_bounds__(BLK) k_final(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
-        const BoxDev& b = *boxes[tl.box];
+        CBox& b = box_at(boxes, tl.box);
         if (!live(b)) continue;
         const uint32_t i = tl.start + threadIdx.x;
         if (i >= b.st->nlocal) continue;
@@ -143,9 +165,9 @@ __global__ void __launch_bounds__(BLK) k_final(KARGS) {
 __global__ void __launch_bounds__(BLK) k_deform_init(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
-        const BoxDev& b = *boxes[tl.box];
+        CBox& b = box_at(boxes, tl.box);
         if (!live(b)) continue;
-        const Geom& g = geom_of(b);
+        CGeom& g = geom_of(b);
         if (!g.init_shear && !g.remap_any) continue;
         const uint32_t i = tl.start + threadIdx.x;
         if (i >= b.st->nlocal) continue;
@@ -165,11 +187,11 @@ __global__ void __launch_bounds__(BLK) k_deform_init(KARGS) {
 
 // displacement_exceeded (neighbor.rs:693-758) applies in CommunicateOnly with
 // a valid last build, unless `every` is set without `check`
-__device__ __forceinline__ bool disp_active(const Geom& g, const BoxState* st) {
+__device__ __forceinline__ bool disp_active(CGeom& g, const BoxState* st) {
     return st->state == 1u && st->last_valid && !g.clear_last && (g.every == 0 || g.check);
 }
 
-__device__ __forceinline__ bool disp_exceeds(const Geom& g, const BoxDev& b, uint32_t i, double x, double y,
+__device__ __forceinline__ bool disp_exceeds(CGeom& g, CBox& b, uint32_t i, double x, double y,
                                              double z) {
     double dx = x - b.lbx[i];
     double dy = y - b.lby[i];
@@ -207,10 +229,10 @@ template <bool DISP, bool FINAL>
 __global__ void __launch_bounds__(BLK) k_initial(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
-        const BoxDev& b = *boxes[tl.box];
+        CBox& b = box_at(boxes, tl.box);
         if (!live(b)) continue;
         BoxState* st = b.st;
-        const Geom& g = geom_of(b);
+        CGeom& g = geom_of(b);
         const uint32_t i = tl.start + threadIdx.x;
         const bool act = i < st->nlocal;
         bool ex = false;
@@ -245,10 +267,10 @@ __global__ void __launch_bounds__(BLK) k_initial(KARGS) {
 __global__ void __launch_bounds__(BLK) k_disp(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
-        const BoxDev& b = *boxes[tl.box];
+        CBox& b = box_at(boxes, tl.box);
         if (!live(b)) continue;
         BoxState* st = b.st;
-        const Geom& g = geom_of(b);
+        CGeom& g = geom_of(b);
         if (!disp_active(g, st)) continue;
         const uint32_t i = tl.start + threadIdx.x;
         if (i >= st->nlocal) continue;
@@ -259,10 +281,10 @@ template <bool REMOVE>
 __global__ void k_decide(KARGS) {
     FOR_TILES(t) {
         if (threadIdx.x != 0) continue;
-        const BoxDev& b = *boxes[tiles[t].box];
+        CBox& b = box_at(boxes, tiles[t].box);
         if (!live(b)) continue;
         BoxState* st = b.st;
-        const Geom& g = geom_of(b);
+        CGeom& g = geom_of(b);
         if (g.clear_last) st->last_valid = 0;
         if (st->state == 1u) {
             bool need;
@@ -303,7 +325,7 @@ __device__ __forceinline__ unsigned long long ext_key(double v) {
 __global__ void k_extent_init(KARGS) {
     FOR_TILES(t) {
         if (threadIdx.x != 0) continue;
-        const BoxDev& b = *boxes[tiles[t].box];
+        CBox& b = box_at(boxes, tiles[t].box);
         if (!live(b)) continue;
         for (int k = 0; k < 3; ++k) {
             b.st->ext_key[k] = ~0ull;
@@ -315,7 +337,7 @@ __global__ void k_extent_init(KARGS) {
 __global__ void __launch_bounds__(BLK) k_extent(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
-        const BoxDev& b = *boxes[tl.box];
+        CBox& b = box_at(boxes, tl.box);
         if (!live(b)) continue;  // block-uniform
         const uint32_t i = tl.start + threadIdx.x;
         const bool act = i < b.st->nlocal;
@@ -349,7 +371,7 @@ __global__ void __launch_bounds__(BLK) k_extent(KARGS) {
 __global__ void k_remove_ghosts(KARGS) {
     FOR_TILES(t) {
         if (threadIdx.x != 0) continue;
-        const BoxDev& bb = *boxes[tiles[t].box];
+        CBox& bb = box_at(boxes, tiles[t].box);
         if (!live(bb)) continue;
         BoxState* st = bb.st;
         if (st->state == 0u) st->nghost = 0;
@@ -360,7 +382,7 @@ __global__ void k_remove_ghosts(KARGS) {
 __global__ void k_set_rebuild_now(KARGS) {
     FOR_TILES(t) {
         if (threadIdx.x != 0) continue;
-        const BoxDev& bb = *boxes[tiles[t].box];
+        CBox& bb = box_at(boxes, tiles[t].box);
         if (!live(bb)) continue;
         BoxState* st = bb.st;
         st->rebuild_now = st->state == 0u ? 1u : 0u;
@@ -374,7 +396,7 @@ template <bool CONSUMER_WRITES>
 __global__ void __launch_bounds__(BLK) k_zero(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
-        const BoxDev& b = *boxes[tl.box];
+        CBox& b = box_at(boxes, tl.box);
         if (!live(b)) continue;
         const uint32_t i = tl.start + threadIdx.x;
         if (i >= b.st->nlocal + b.st->nghost) continue;
@@ -393,13 +415,13 @@ __global__ void __launch_bounds__(BLK) k_zero(KARGS) {
 __global__ void __launch_bounds__(BLK) k_pbc(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
-        const BoxDev& b = *boxes[tl.box];
+        CBox& b = box_at(boxes, tl.box);
         if (!live(b)) continue;
         BoxState* st = b.st;
         if (st->state != 0u) continue;
         const uint32_t i = tl.start + threadIdx.x;
         if (i >= st->nlocal) continue;
-        const Geom& g = geom_of(b);
+        CGeom& g = geom_of(b);
         if (g.triclinic) {
             double l[3];
             x2lamda(g, b.x[i], b.y[i], b.z[i], l[0], l[1], l[2]);
@@ -456,10 +478,10 @@ __global__ void __launch_bounds__(BLK) k_pbc(KARGS) {
 __global__ void k_pre_sort(KARGS) {
     FOR_TILES(t) {
         if (threadIdx.x != 0) continue;
-        const BoxDev& b = *boxes[tiles[t].box];
+        CBox& b = box_at(boxes, tiles[t].box);
         if (!live(b)) continue;
         BoxState* st = b.st;
-        const Geom& g = geom_of(b);
+        CGeom& g = geom_of(b);
         if (g.bounds_changed) st->state = 0u;
         st->do_sort = (st->nlocal > 0 && g.ncells > 0 && (g.periodic_sort || st->state == 0u)) ? 1u : 0u;
         st->not_identity = 0;
@@ -490,10 +512,10 @@ struct SortView {
 };
 
 template <int W>
-__device__ __forceinline__ SortView<W> sort_view(const BoxDev& b) {
+__device__ __forceinline__ SortView<W> sort_view(CBox& b) {
     SortView<W> v;
     const BoxState* st = b.st;
-    const Geom& g = geom_of(b);
+    CGeom& g = geom_of(b);
     v.fast = false;
     if (W == SORT_SPATIAL) {
         v.n = st->nlocal; v.nkeys = g.ncells; v.active = st->do_sort != 0;
@@ -511,7 +533,7 @@ template <int W>
 __global__ void __launch_bounds__(BLK) k_sort_zero(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
-        const BoxDev& b = *boxes[tl.box];
+        CBox& b = box_at(boxes, tl.box);
         if (!live(b)) continue;
         SortView<W> v = sort_view<W>(b);
         if (!v.active || v.fast) continue;
@@ -524,7 +546,7 @@ template <int W>
 __global__ void __launch_bounds__(BLK) k_sort_count(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
-        const BoxDev& b = *boxes[tl.box];
+        CBox& b = box_at(boxes, tl.box);
         if (!live(b)) continue;
         SortView<W> v = sort_view<W>(b);
         if (!v.active) continue;
@@ -563,7 +585,7 @@ template <int W>
 __global__ void __launch_bounds__(BLK) k_sort_scatter(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
-        const BoxDev& b = *boxes[tl.box];
+        CBox& b = box_at(boxes, tl.box);
         if (!live(b)) continue;
         SortView<W> v = sort_view<W>(b);
         if (!v.active) continue;
@@ -578,7 +600,7 @@ template <int W>
 __global__ void __launch_bounds__(BLK) k_sort_rank(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
-        const BoxDev& b = *boxes[tl.box];
+        CBox& b = box_at(boxes, tl.box);
         if (!live(b)) continue;
         SortView<W> v = sort_view<W>(b);
         if (!v.active) continue;
@@ -625,7 +647,7 @@ template <bool FUSED>
 __global__ void __launch_bounds__(BLK) k_permute(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
-        const BoxDev& b = *boxes[tl.box];
+        CBox& b = box_at(boxes, tl.box);
         if (!live(b)) continue;
         const BoxState* st = b.st;
         if (!st->do_sort || !st->not_identity) continue;
@@ -697,7 +719,7 @@ __global__ void k_post_sort(KARGS) {
         BoxDev& b = *boxes[tiles[t].box];
         if (!live(b)) continue;
         BoxState* st = b.st;
-        const Geom& g = geom_of(b);
+        CGeom& g = geom_of(b);
         if (g.bounds_changed) st->state = 0u;
         if (FUSED && st->do_sort && st->not_identity) {
             swap_ptr(b.x, b.ax); swap_ptr(b.y, b.ay); swap_ptr(b.z, b.az);
@@ -726,14 +748,14 @@ __global__ void k_post_sort(KARGS) {
 
 // the incoming-pair transpose is rebuilt with the CSR, for boxes whose
 // consumer or force hook reads it
-__device__ __forceinline__ bool transpose_active(const BoxDev& b) {
+__device__ __forceinline__ bool transpose_active(CBox& b) {
     const BoxState* st = b.st;
     return st->rebuild_now != 0 && b.transpose && (st->err & (ERR_PAIR_CAPACITY | ERR_GHOST_CAPACITY)) == 0;
 }
 
 // ── exclusive scans (reduce tiles -> scan tile sums per box -> down-sweep) ──
 template <int W>
-__device__ __forceinline__ void scan_view(const BoxDev& b, uint32_t& n, bool& active, uint32_t*& in, uint32_t*& out) {
+__device__ __forceinline__ void scan_view(CBox& b, uint32_t& n, bool& active, uint32_t*& in, uint32_t*& out) {
     const BoxState* st = b.st;
     if (W == SCAN_CELLS_SPATIAL || W == SCAN_CELLS_BUILD) {
         n = geom_of(b).ncells + 1;
@@ -759,7 +781,7 @@ __global__ void __launch_bounds__(BLK) k_scan_reduce(KARGS) {
     __shared__ uint32_t lds[BLK / 64];
     FOR_TILES(t) {
         const Tile tl = tiles[t];
-        const BoxDev& b = *boxes[tl.box];
+        CBox& b = box_at(boxes, tl.box);
         if (!live(b)) continue;
         uint32_t n;
         bool active;
@@ -781,7 +803,7 @@ template <int W>
 __global__ void __launch_bounds__(BLK) k_scan_parts(KARGS) {
     __shared__ uint32_t lds[BLK / 64];
     FOR_TILES(t) {
-        const BoxDev& b = *boxes[tiles[t].box];
+        CBox& b = box_at(boxes, tiles[t].box);
         if (!live(b)) continue;
         uint32_t n;
         bool active;
@@ -816,7 +838,7 @@ __global__ void __launch_bounds__(BLK) k_scan_parts(KARGS) {
                 if (carry > b.cap_pairs) {
                     atomicOr(&st->err, ERR_PAIR_CAPACITY);
                 } else {
-                    const Geom& g = geom_of(b);
+                    CGeom& g = geom_of(b);
                     const uint64_t na = (uint64_t)st->nlocal + st->nghost;
                     const uint64_t per = 24u + (g.uniform ? 0u : 8u) + 4u;
                     st->csr_bytes += per * na + 4u * ((uint64_t)g.ncells + 1u) + 4u * ((uint64_t)st->nlocal + 1u) +
@@ -833,7 +855,7 @@ __global__ void __launch_bounds__(BLK) k_scan_down(KARGS) {
     __shared__ uint32_t lds[BLK / 64];
     FOR_TILES(t) {
         const Tile tl = tiles[t];
-        const BoxDev& b = *boxes[tl.box];
+        CBox& b = box_at(boxes, tl.box);
         if (!live(b)) continue;
         uint32_t n;
         bool active;
@@ -860,7 +882,7 @@ __global__ void __launch_bounds__(BLK) k_scan_down(KARGS) {
 
 // ── borders: comm.rs:682-860 + pack_border_atoms :288-358 ──────────────────
 // selection value of a position along dim d: the coordinate, or lamda when triclinic
-__device__ __forceinline__ double border_coord(const Geom& g, double x, double y, double z, int d) {
+__device__ __forceinline__ double border_coord(CGeom& g, double x, double y, double z, int d) {
     if (g.triclinic) {
         double lx, ly, lz;
         x2lamda(g, x, y, z, lx, ly, lz);
@@ -869,7 +891,7 @@ __device__ __forceinline__ double border_coord(const Geom& g, double x, double y
     return d == 0 ? x : (d == 1 ? y : z);
 }
 
-__device__ __forceinline__ void border_preds(const Geom& g, const BoxDev& b, uint32_t i, int d, bool& p0, bool& p1) {
+__device__ __forceinline__ void border_preds(CGeom& g, CBox& b, uint32_t i, int d, bool& p0, bool& p1) {
     double v;
     if (g.triclinic) {
         double lx, ly, lz;
@@ -897,10 +919,10 @@ __global__ void __launch_bounds__(BLK) k_border_count(KARGS) {
     __shared__ uint32_t wc[6][BLK / 64];
     FOR_TILES(t) {
         const Tile tl = tiles[t];
-        const BoxDev& b = *boxes[tl.box];
+        CBox& b = box_at(boxes, tl.box);
         if (!live(b)) continue;
         const BoxState* st = b.st;
-        const Geom& g = geom_of(b);
+        CGeom& g = geom_of(b);
         if (!st->rebuild_now) continue;
         const uint32_t nl = st->nlocal;
         const uint32_t tix = tl.start / BORDER_TILE;
@@ -950,7 +972,7 @@ __global__ void __launch_bounds__(BLK) k_border_count(KARGS) {
 // writes what later dims and the chain need (position = source + shift, the
 // source index and the swap); k_ghost_fill copies the rest once all six swaps
 // are placed.
-__device__ __forceinline__ void place_ghost(const BoxDev& b, const Geom& g, uint32_t dst, uint32_t i, int s, double& gx,
+__device__ __forceinline__ void place_ghost(CBox& b, CGeom& g, uint32_t dst, uint32_t i, int s, double& gx,
                                             double& gy, double& gz) {
     // the source's {x, y, z, rad} record: for a ghost source that is the one
     // placed by an earlier dim (its other fields are filled later)
@@ -973,14 +995,14 @@ __device__ __forceinline__ void place_ghost(const BoxDev& b, const Geom& g, uint
 __global__ void __launch_bounds__(BLK) k_ghost_fill(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
-        const BoxDev& b = *boxes[tl.box];
+        CBox& b = box_at(boxes, tl.box);
         if (!live(b)) continue;
         const BoxState* st = b.st;
         if (!st->rebuild_now || (st->err & ERR_GHOST_CAPACITY)) continue;
         const uint32_t nl = st->nlocal;
         const uint32_t gi = nl + tl.start + threadIdx.x;
         if (gi >= nl + st->nghost) continue;
-        const Geom& g = geom_of(b);
+        CGeom& g = geom_of(b);
         int sw[3];
         int depth = 0;
         uint32_t r = gi;
@@ -1031,10 +1053,10 @@ __global__ void __launch_bounds__(BLK) k_border_write(KARGS, int d) {
     __shared__ uint32_t lc[16];
     FOR_TILES(t) {
         const Tile tl = tiles[t];
-        const BoxDev& b = *boxes[tl.box];
+        CBox& b = box_at(boxes, tl.box);
         if (!live(b)) continue;
         BoxState* st = b.st;
-        const Geom& g = geom_of(b);
+        CGeom& g = geom_of(b);
         if (!st->rebuild_now) continue;
         const uint32_t nl = st->nlocal;
         // candidates: locals and the ghosts of the earlier dims (borders truncates old ghosts first)
@@ -1178,14 +1200,14 @@ __global__ void __launch_bounds__(BLK) k_border_write(KARGS, int d) {
 __global__ void __launch_bounds__(BLK) k_forward(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
-        const BoxDev& b = *boxes[tl.box];
+        CBox& b = box_at(boxes, tl.box);
         if (!live(b)) continue;
         const BoxState* st = b.st;
         if (st->rebuild_now) continue;
         const uint32_t nl = st->nlocal;
         const uint32_t gi = nl + tl.start + threadIdx.x;
         if (gi >= nl + st->nghost) continue;
-        const Geom& g = geom_of(b);
+        CGeom& g = geom_of(b);
         int sw[3];
         int depth = 0;
         uint32_t cur = gi;
@@ -1229,10 +1251,10 @@ __global__ void __launch_bounds__(BLK) k_forward(KARGS) {
 __global__ void __launch_bounds__(BLK) k_reverse(KARGS, int d, int part) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
-        const BoxDev& b = *boxes[tl.box];
+        CBox& b = box_at(boxes, tl.box);
         if (!live(b)) continue;
         const BoxState* st = b.st;
-        const Geom& g = geom_of(b);
+        CGeom& g = geom_of(b);
         if (!g.newton || (st->err & ERR_GHOST_CAPACITY)) continue;
         const bool both = g.disjoint[d] != 0;
         uint32_t k = tl.start + threadIdx.x;
@@ -1265,7 +1287,7 @@ __global__ void __launch_bounds__(BLK) k_reverse(KARGS, int d, int part) {
 __global__ void k_build_prep(KARGS) {
     FOR_TILES(t) {
         if (threadIdx.x != 0) continue;
-        const BoxDev& b = *boxes[tiles[t].box];
+        CBox& b = box_at(boxes, tiles[t].box);
         if (!live(b)) continue;
         BoxState* st = b.st;
         if (!st->rebuild_now) continue;
@@ -1280,7 +1302,7 @@ __global__ void k_build_prep(KARGS) {
 __global__ void __launch_bounds__(BLK) k_save_build(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
-        const BoxDev& b = *boxes[tl.box];
+        CBox& b = box_at(boxes, tl.box);
         if (!live(b)) continue;
         if (!b.st->rebuild_now) continue;
         const uint32_t i = tl.start + threadIdx.x;
@@ -1312,7 +1334,7 @@ __device__ __forceinline__ bool accept_pair(double xi, double yi, double zi, dou
 
 // Global-memory row walk, used by tiles whose neighborhood does not fit LDS.
 template <bool FILL, bool UNIFORM>
-__device__ uint32_t csr_row_g(const BoxDev& b, const Geom& g, uint32_t i, uint32_t* out, uint32_t out_cap = 0xffffffffu) {
+__device__ uint32_t csr_row_g(CBox& b, CGeom& g, uint32_t i, GMEM uint32_t* out, uint32_t out_cap = 0xffffffffu) {
     const uint32_t c = b.cell[i];
     const double4 pi = ((const double4*)b.xr)[i];
     const double4* __restrict__ sp4 = (const double4*)b.spos;
@@ -1398,7 +1420,7 @@ struct CsrLds {
 };
 
 template <bool FILL, bool UNIFORM>
-__device__ __forceinline__ uint32_t csr_row_lds(const BoxDev& b, const Geom& g, const CsrLds& L, uint32_t i,
+__device__ __forceinline__ uint32_t csr_row_lds(CBox& b, CGeom& g, const CsrLds& L, uint32_t i,
                                                 uint32_t c, const double4& pi, uint32_t* out) {
     const uint32_t skip = g.newton ? 0xffffffffu : i;
     const int ns = g.nstencil;
@@ -1429,7 +1451,7 @@ __device__ __forceinline__ uint32_t csr_row_lds(const BoxDev& b, const Geom& g, 
 // Pass 1: per tile, stage the neighborhood, count every row (offsets[i]) and
 // emit the rows into the tile's slot of the temp region when they fit.
 template <bool UNIFORM>
-__device__ void csr_tile(const BoxDev& b, const Geom& g, CsrLds& L, uint32_t i0) {
+__device__ void csr_tile(CBox& b, CGeom& g, CsrLds& L, uint32_t i0) {
     const BoxState* st = b.st;
     const uint32_t nl = st->nlocal;
     const uint32_t tid = threadIdx.x;
@@ -1541,11 +1563,11 @@ __global__ void __launch_bounds__(BLK) k_csr_count(KARGS) {
     __shared__ CsrLds L;
     FOR_TILES(t) {
         const Tile tl = tiles[t];
-        const BoxDev& b = *boxes[tl.box];
+        CBox& b = box_at(boxes, tl.box);
         if (!live(b)) continue;
         const BoxState* st = b.st;
         if (!st->rebuild_now || (st->err & ERR_GHOST_CAPACITY)) continue;
-        const Geom& g = geom_of(b);
+        CGeom& g = geom_of(b);
         if (g.uniform) csr_tile<true>(b, g, L, tl.start);
         else csr_tile<false>(b, g, L, tl.start);
     }
@@ -1556,11 +1578,11 @@ __global__ void __launch_bounds__(BLK) k_csr_count(KARGS) {
 __global__ void __launch_bounds__(BLK) k_csr_count_g(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
-        const BoxDev& b = *boxes[tl.box];
+        CBox& b = box_at(boxes, tl.box);
         if (!live(b)) continue;
         const BoxState* st = b.st;
         if (!st->rebuild_now || (st->err & ERR_GHOST_CAPACITY)) continue;
-        const Geom& g = geom_of(b);
+        CGeom& g = geom_of(b);
         const uint32_t i = tl.start + threadIdx.x;
         const uint32_t nl = st->nlocal;
         if (i > nl) continue;
@@ -1571,7 +1593,7 @@ __global__ void __launch_bounds__(BLK) k_csr_count_g(KARGS) {
         const uint32_t slot = b.csr_mode == 2 ? b.cap_pairs / (b.cap_local ? b.cap_local : 1u) : 0u;
         uint32_t n;
         if (slot) {
-            uint32_t* out = b.ttmp + (size_t)i * slot;
+            GMEM uint32_t* out = b.ttmp + (size_t)i * slot;
             n = g.uniform ? csr_row_g<true, true>(b, g, i, out, slot) : csr_row_g<true, false>(b, g, i, out, slot);
         } else {
             n = g.uniform ? csr_row_g<false, true>(b, g, i, nullptr) : csr_row_g<false, false>(b, g, i, nullptr);
@@ -1580,12 +1602,137 @@ __global__ void __launch_bounds__(BLK) k_csr_count_g(KARGS) {
     }
 }
 
+// Wave-staged count (mode 3). A wave's 64 consecutive locals lie in
+// nondecreasing cells (the step's spatial sort), so for each stencil run the
+// union of their candidate ranges is one contiguous span of the bin-sorted
+// records. The wave stages that span through LDS in coalesced chunks and each
+// lane tests its own sub-range there; rows come out exactly as csr_row_g
+// writes them (self cell, then the runs in order, each range ascending), into
+// the same per-row slots. A wave whose cells are not sorted, or a run whose
+// union is very wide, walks global memory per lane instead.
+constexpr uint32_t CSRW_CH = 128;         // staged records per chunk and wave
+constexpr uint32_t CSRW_MAX_SPAN = 1024;  // wider unions are walked per lane
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <bool UNIFORM, bool NEWTON>
+__device__ uint32_t csr_row_w(CBox& b, CGeom& g, uint32_t i, bool act, uint32_t c, uint32_t c_first,
+                              uint32_t c_last, bool staged, uint32_t lane, double4* __restrict__ lpos,
+                              uint32_t* __restrict__ lj, GMEM uint32_t* out, uint32_t slot) {
+    const double4* __restrict__ sp4 = (const double4*)b.spos;
+    const double4 pi = act ? ((const double4*)b.xr)[i] : make_double4(0.0, 0.0, 0.0, 0.0);
+    const int32_t ncell = (int32_t)g.ncells;
+    uint32_t n = 0;
+    auto take = [&](uint32_t j) {
+        if (NEWTON || j != i) {  // the full list skips the row atom itself
+            if (n < slot) out[n] = j;
+            ++n;
+        }
+    };
+    // this lane's range [r0, r1) inside the wave's union [u0, u1) (wave-uniform)
+    auto span = [&](uint32_t r0, uint32_t r1, uint32_t u0, uint32_t u1) {
+        if (!staged || u1 - u0 > CSRW_MAX_SPAN) {
+            for (uint32_t m = r0; m < r1; ++m)
+                if (accept_pair<UNIFORM>(pi.x, pi.y, pi.z, pi.w, sp4[m], g.ucut_sq, g.sf_sq)) take(b.sorted[m]);
+            return;
+        }
+        for (uint32_t base = u0; base < u1; base += CSRW_CH) {  // empty when u1 <= u0
+#pragma unroll
+            for (uint32_t q = 0; q < CSRW_CH; q += 64u) {
+                const uint32_t m = base + q + lane;
+                if (m < u1) {
+                    lpos[q + lane] = sp4[m];
+                    lj[q + lane] = b.sorted[m];
+                }
+            }
+            wave_sync();
+            const uint32_t lo = r0 > base ? r0 : base;
+            const uint32_t hi = r1 < base + CSRW_CH ? r1 : base + CSRW_CH;
+            for (uint32_t m = lo; m < hi; ++m)
+                if (accept_pair<UNIFORM>(pi.x, pi.y, pi.z, pi.w, lpos[m - base], g.ucut_sq, g.sf_sq)) take(lj[m - base]);
+            wave_sync();  // the next chunk overwrites the staging
+        }
+    };
+    auto clampc = [&](int32_t x) { return x < 0 ? 0 : (x > ncell ? ncell : x); };
+    int r0 = 0;
+    if (g.has_self) {
+        if (NEWTON) {
+            int32_t extra = 1;
+            if (g.nruns > 0 && g.run_off[0] == 1) {
+                extra = 1 + g.run_len[0];
+                r0 = 1;
+            }
+            const uint32_t ce = (uint32_t)clampc((int32_t)c + extra);
+            const uint32_t ue = (uint32_t)clampc((int32_t)c_last + extra);
+            span(act ? b.sidx[i] + 1u : 0u, act ? b.bin_start[ce] : 0u, b.bin_start[c_first], b.bin_start[ue]);
+        } else {
+            span(act ? b.bin_start[c] : 0u, act ? b.bin_start[c + 1] : 0u, b.bin_start[c_first],
+                 b.bin_start[c_last + 1]);
+        }
+    }
+    for (int k = r0; k < g.nruns; ++k) {
+        const int32_t off = g.run_off[k], len = g.run_len[k];
+        const int32_t uc0 = clampc((int32_t)c_first + off), uc1 = clampc((int32_t)c_last + off + len);
+        if (staged && uc0 >= uc1) continue;  // wave-uniform: no lane has candidates
+        int32_t c0 = (int32_t)c + off, c1 = c0 + len;
+        if (c0 < 0) c0 = 0;
+        if (c1 > ncell) c1 = ncell;
+        uint32_t rs = 0u, re = 0u;
+        if (act && c0 < c1) {
+            rs = b.bin_start[c0];
+            re = b.bin_start[c1];
+        }
+        span(rs, re, b.bin_start[uc0], b.bin_start[uc1]);
+    }
+    return n;
+}
+
+__global__ void __launch_bounds__(BLK) k_csr_count_w(KARGS) {
+    __shared__ double4 lpos[BLK / 64][CSRW_CH];
+    __shared__ uint32_t lj[BLK / 64][CSRW_CH];
+    FOR_TILES(t) {
+        const Tile tl = tiles[t];
+        CBox& b = box_at(boxes, tl.box);
+        if (!live(b)) continue;
+        const BoxState* st = b.st;
+        if (!st->rebuild_now || (st->err & ERR_GHOST_CAPACITY)) continue;
+        CGeom& g = geom_of(b);
+        const uint32_t nl = st->nlocal;
+        const uint32_t wid = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+        const uint32_t i0w = tl.start + wid * 64u;
+        const uint32_t i = i0w + lane;
+        if (i == nl) b.offsets[nl] = 0u;
+        if (i0w >= nl) continue;  // wave-uniform; no block barrier in this kernel
+        const bool act = i < nl;
+        const uint32_t c = act ? b.cell[i] : 0u;
+        const uint32_t last = (nl - i0w < 64u ? nl - i0w : 64u) - 1u;
+        const uint32_t c_first = __shfl(c, 0, 64);
+        const uint32_t c_last = __shfl(c, (int)last, 64);
+        const uint32_t c_next = __shfl_down(c, 1, 64);
+        const bool sorted = __ballot(act && lane < last && c > c_next) == 0ull;
+        const uint32_t slot = b.cap_pairs / (b.cap_local ? b.cap_local : 1u);
+        GMEM uint32_t* out = b.ttmp + (size_t)(act ? i : 0u) * slot;
+        uint32_t n;
+        if (g.newton)
+            n = g.uniform ? csr_row_w<true, true>(b, g, i, act, c, c_first, c_last, sorted, lane, lpos[wid], lj[wid], out, slot)
+                          : csr_row_w<false, true>(b, g, i, act, c, c_first, c_last, sorted, lane, lpos[wid], lj[wid], out, slot);
+        else
+            n = g.uniform ? csr_row_w<true, false>(b, g, i, act, c, c_first, c_last, sorted, lane, lpos[wid], lj[wid], out, slot)
+                          : csr_row_w<false, false>(b, g, i, act, c, c_first, c_last, sorted, lane, lpos[wid], lj[wid], out, slot);
+        if (act) b.offsets[i] = n;
+    }
+}
+
 // Pass 2 (after the offset scan): move each tile's rows from the temp region
 // to their place (one contiguous copy), or walk overflowed tiles' rows again.
 __global__ void __launch_bounds__(BLK) k_csr_fill(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
-        const BoxDev& b = *boxes[tl.box];
+        CBox& b = box_at(boxes, tl.box);
         if (!live(b)) continue;
         const BoxState* st = b.st;
         if (!st->rebuild_now || (st->err & (ERR_GHOST_CAPACITY | ERR_PAIR_CAPACITY))) continue;
@@ -1595,13 +1742,13 @@ __global__ void __launch_bounds__(BLK) k_csr_fill(KARGS) {
         if (b.csr_mode != 0) {
             const uint32_t i = i0 + threadIdx.x;
             if (i >= nl) continue;
-            const Geom& g = geom_of(b);
+            CGeom& g = geom_of(b);
             const uint32_t off = b.offsets[i], n = b.offsets[i + 1] - off;
-            const uint32_t slot = b.csr_mode == 2 ? b.cap_pairs / (b.cap_local ? b.cap_local : 1u) : 0u;
-            uint32_t* out = b.indices + off;
+            const uint32_t slot = b.csr_mode >= 2 ? b.cap_pairs / (b.cap_local ? b.cap_local : 1u) : 0u;
+            GMEM uint32_t* out = b.indices + off;
             const bool tr = transpose_active(b);
             if (n <= slot) {
-                const uint32_t* src = b.ttmp + (size_t)i * slot;
+                const GMEM uint32_t* src = b.ttmp + (size_t)i * slot;
                 for (uint32_t k = 0; k < n; ++k) {
                     const uint32_t j = src[k];
                     out[k] = j;
@@ -1620,8 +1767,8 @@ __global__ void __launch_bounds__(BLK) k_csr_fill(KARGS) {
         if (ct == CSR_OVERFLOW) {
             const uint32_t i = i0 + threadIdx.x;
             if (i >= nl) continue;
-            const Geom& g = geom_of(b);
-            uint32_t* out = b.indices + b.offsets[i];
+            CGeom& g = geom_of(b);
+            GMEM uint32_t* out = b.indices + b.offsets[i];
             if (g.uniform) csr_row_g<true, true>(b, g, i, out);
             else csr_row_g<true, false>(b, g, i, out);
             continue;
@@ -1640,7 +1787,7 @@ __global__ void __launch_bounds__(BLK) k_csr_fill(KARGS) {
 __global__ void __launch_bounds__(BLK) k_tcount(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
-        const BoxDev& b = *boxes[tl.box];
+        CBox& b = box_at(boxes, tl.box);
         if (!live(b) || !transpose_active(b)) continue;
         const uint32_t i = tl.start + threadIdx.x;
         if (i >= b.st->nlocal) continue;
@@ -1651,7 +1798,7 @@ __global__ void __launch_bounds__(BLK) k_tcount(KARGS) {
 __global__ void __launch_bounds__(BLK) k_tplace(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
-        const BoxDev& b = *boxes[tl.box];
+        CBox& b = box_at(boxes, tl.box);
         if (!live(b) || !transpose_active(b)) continue;
         const uint32_t i = tl.start + threadIdx.x;
         if (i >= b.st->nlocal) continue;
@@ -1693,7 +1840,7 @@ template <bool ASSIGN>
 __global__ void __launch_bounds__(BLK) k_harmonic(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
-        const BoxDev& b = *boxes[tl.box];
+        CBox& b = box_at(boxes, tl.box);
         if (!live(b) || !b.consumer) continue;
         const BoxState* st = b.st;
         const uint32_t a = tl.start + threadIdx.x;
@@ -1744,7 +1891,7 @@ __global__ void __launch_bounds__(BLK) k_harmonic(KARGS) {
 __global__ void k_sync_view(KARGS) {
     FOR_TILES(t) {
         if (threadIdx.x != 0) continue;
-        const BoxDev& b = *boxes[tiles[t].box];
+        CBox& b = box_at(boxes, tiles[t].box);
         if (!b.view) continue;
         soil_hip_device_view& v = *b.view;
         if (!live(b)) {  // a stopped box: the hook sees no atoms
@@ -1781,7 +1928,7 @@ __global__ void k_sync_view(KARGS) {
 __global__ void k_step_end(KARGS) {
     FOR_TILES(t) {
         if (threadIdx.x != 0) continue;
-        const BoxDev& b = *boxes[tiles[t].box];
+        CBox& b = box_at(boxes, tiles[t].box);
         if (!live(b)) continue;
         BoxState* st = b.st;
         st->steps += 1;
@@ -1902,6 +2049,7 @@ void phase_build_fused(const Launch& L, hipEvent_t ev0, hipEvent_t evc, hipEvent
     run_sort<SORT_BUILD, SCAN_CELLS_BUILD>(L, L.t_cell, L.t_atom, L.t_scan_cell);
     if (ev0) (void)hipEventRecord(ev0, L.stream);
     if (L.csr_mode == 0) launch(k_csr_count, L.t_local1, L.boxes, L.stream, L.maxgrid);  // tiles + end marker
+    else if (L.csr_mode == 3) launch(k_csr_count_w, L.t_local1, L.boxes, L.stream, L.maxgrid);
     else launch(k_csr_count_g, L.t_local1, L.boxes, L.stream, L.maxgrid);
     if (evc) (void)hipEventRecord(evc, L.stream);
     run_scan<SCAN_CSR>(L, L.t_scan_local);

@@ -32,7 +32,7 @@ struct Launch {
     // reverse comm: 1 = some box may have overlapping lo/hi selections in that
     // dim, so its two swaps need separate ordered launches
     uint8_t rev_split[3] = {1, 1, 1};
-    int csr_mode = 2;
+    int csr_mode = 3;
     TileSet t_box, t_local, t_local1, t_atom, t_atom1, t_ghost, t_cell, t_border;
     TileSet t_scan_cell, t_scan_local, t_scan_atom;
 };
