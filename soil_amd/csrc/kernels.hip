@@ -755,7 +755,7 @@ __device__ __forceinline__ bool transpose_active(CBox& b) {
 
 // ── exclusive scans (reduce tiles -> scan tile sums per box -> down-sweep) ──
 template <int W>
-__device__ __forceinline__ void scan_view(CBox& b, uint32_t& n, bool& active, uint32_t*& in, uint32_t*& out) {
+__device__ __forceinline__ void scan_view(CBox& b, uint32_t& n, bool& active, GMEM uint32_t*& in, GMEM uint32_t*& out) {
     const BoxState* st = b.st;
     if (W == SCAN_CELLS_SPATIAL || W == SCAN_CELLS_BUILD) {
         n = geom_of(b).ncells + 1;
@@ -785,7 +785,7 @@ __global__ void __launch_bounds__(BLK) k_scan_reduce(KARGS) {
         if (!live(b)) continue;
         uint32_t n;
         bool active;
-        uint32_t *in, *out;
+        GMEM uint32_t *in, *out;
         scan_view<W>(b, n, active, in, out);
         if (!active || tl.start >= n) continue;  // block-uniform
         uint32_t s = 0;
@@ -799,79 +799,61 @@ __global__ void __launch_bounds__(BLK) k_scan_reduce(KARGS) {
     }
 }
 
-template <int W>
-__global__ void __launch_bounds__(BLK) k_scan_parts(KARGS) {
-    __shared__ uint32_t lds[BLK / 64];
-    FOR_TILES(t) {
-        CBox& b = box_at(boxes, tiles[t].box);
-        if (!live(b)) continue;
-        uint32_t n;
-        bool active;
-        uint32_t *in, *out;
-        scan_view<W>(b, n, active, in, out);
-        if (!active) continue;
-        const uint32_t nparts = (n + SCAN_TILE - 1) / SCAN_TILE;
-        uint32_t carry = 0;
-        for (uint32_t base = 0; base < nparts; base += BLK * SCAN_PER_THREAD) {
-            uint32_t v[SCAN_PER_THREAD];
-            uint32_t s = 0;
-            const uint32_t i0 = base + threadIdx.x * SCAN_PER_THREAD;
-#pragma unroll
-            for (int k = 0; k < SCAN_PER_THREAD; ++k) {
-                v[k] = i0 + k < nparts ? b.scan_part[i0 + k] : 0u;
-                s += v[k];
-            }
-            uint32_t tot;
-            uint32_t pre = block_excl_scan(s, lds, tot) + carry;
-#pragma unroll
-            for (int k = 0; k < SCAN_PER_THREAD; ++k) {
-                if (i0 + k < nparts) b.scan_part[i0 + k] = pre;
-                pre += v[k];
-            }
-            carry += tot;
-        }
-        if (threadIdx.x == 0) {
-            BoxState* st = b.st;
-            if (W == SCAN_CSR) {
-                st->local_cells_valid = 0;  // consumed by this build (neighbor.rs:596)
-                st->npairs = carry;
-                if (carry > b.cap_pairs) {
-                    atomicOr(&st->err, ERR_PAIR_CAPACITY);
-                } else {
-                    CGeom& g = geom_of(b);
-                    const uint64_t na = (uint64_t)st->nlocal + st->nghost;
-                    const uint64_t per = 24u + (g.uniform ? 0u : 8u) + 4u;
-                    st->csr_bytes += per * na + 4u * ((uint64_t)g.ncells + 1u) + 4u * ((uint64_t)st->nlocal + 1u) +
-                                     4u * (uint64_t)carry;
-                    st->csr_builds += 1;
-                }
-            }
-        }
-    }
-}
-
+// Down-sweep: each tile adds the totals of the box's earlier tiles (summed
+// here from k_scan_reduce's per-tile totals, a few loads per thread) to its
+// own exclusive scan; the first tile also sums them all for the CSR count.
 template <int W>
 __global__ void __launch_bounds__(BLK) k_scan_down(KARGS) {
     __shared__ uint32_t lds[BLK / 64];
+    __shared__ uint32_t red[BLK / 64];
     FOR_TILES(t) {
         const Tile tl = tiles[t];
         CBox& b = box_at(boxes, tl.box);
         if (!live(b)) continue;
         uint32_t n;
         bool active;
-        uint32_t *in, *out;
+        GMEM uint32_t *in, *out;
         scan_view<W>(b, n, active, in, out);
-        if (!active || tl.start >= n) continue;
+        if (!active || tl.start >= n) continue;  // block-uniform
+        const uint32_t tix = tl.start / SCAN_TILE;
+        const bool first = tix == 0u;
+        const uint32_t lim = first ? (n + SCAN_TILE - 1) / SCAN_TILE : tix;
+        uint32_t acc = 0;
+        for (uint32_t k = threadIdx.x; k < lim; k += BLK) acc += b.scan_part[k];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+        if ((threadIdx.x & 63u) == 0u) red[threadIdx.x >> 6] = acc;
+        __syncthreads();
+        acc = 0;
+#pragma unroll
+        for (uint32_t w = 0; w < BLK / 64; ++w) acc += red[w];
+        __syncthreads();  // red[] is reused by the next tile
+        if (W == SCAN_CSR && first && threadIdx.x == 0) {
+            BoxState* st = b.st;
+            st->local_cells_valid = 0;  // consumed by this build (neighbor.rs:596)
+            st->npairs = acc;
+            if (acc > b.cap_pairs) {
+                atomicOr(&st->err, ERR_PAIR_CAPACITY);
+            } else {
+                CGeom& g = geom_of(b);
+                const uint64_t na = (uint64_t)st->nlocal + st->nghost;
+                const uint64_t per = 24u + (g.uniform ? 0u : 8u) + 4u;
+                st->csr_bytes += per * na + 4u * ((uint64_t)g.ncells + 1u) + 4u * ((uint64_t)st->nlocal + 1u) +
+                                 4u * (uint64_t)acc;
+                st->csr_builds += 1;
+            }
+        }
+        const uint32_t prefix = first ? 0u : acc;
         uint32_t v[SCAN_PER_THREAD];
-        uint32_t s = 0;
+        uint32_t sum = 0;
         const uint32_t base = tl.start + threadIdx.x * SCAN_PER_THREAD;
 #pragma unroll
         for (int k = 0; k < SCAN_PER_THREAD; ++k) {
             v[k] = base + k < n ? in[base + k] : 0u;
-            s += v[k];
+            sum += v[k];
         }
         uint32_t tot;
-        uint32_t pre = block_excl_scan(s, lds, tot) + b.scan_part[tl.start / SCAN_TILE];
+        uint32_t pre = block_excl_scan(sum, lds, tot) + prefix;
 #pragma unroll
         for (int k = 0; k < SCAN_PER_THREAD; ++k) {
             if (base + k < n) out[base + k] = pre;
@@ -1973,7 +1955,6 @@ static void launch(void (*k)(BoxDev* const*, const Tile*, uint32_t, A...), const
 template <int W>
 static void run_scan(const Launch& L, const TileSet& scan_tiles) {
     launch(k_scan_reduce<W>, scan_tiles, L.boxes, L.stream, L.maxgrid);
-    launch(k_scan_parts<W>, L.t_box, L.boxes, L.stream, L.maxgrid);
     launch(k_scan_down<W>, scan_tiles, L.boxes, L.stream, L.maxgrid);
 }
 
