@@ -334,13 +334,25 @@ hipEvent_t take_event(soil_hip_box* b) {
 
 // Rebuild tail after borders: build + capacity handling. Synchronizes twice
 // (after borders, after the CSR count) so that capacity growth is exact.
+// borders, forward comm and the neighbor build of a rebuild step, with one
+// state read at the end: a ghost overflow makes every build kernel skip, so
+// the host grows the atoms arena and redoes the three; a pair overflow skips
+// the fill, and the host grows the pairs and redoes the build.
 int rebuild_tail(soil_hip_box* b) {
-    for (int attempt = 0; attempt < 4; ++attempt) {
-        phase_borders(b->L);
-        int rc = pull_state(b);
-        if (rc) return rc;
-        if (!(b->h_st->err & ERR_GHOST_CAPACITY)) break;
-        // grow the atoms arena (locals kept) and redo borders
+    hipEvent_t e0 = nullptr, ec = nullptr, e1 = nullptr, e2 = nullptr;
+    if (b->timing) {
+        e0 = take_event(b);
+        ec = take_event(b);
+        e1 = take_event(b);
+        e2 = take_event(b);
+    }
+    phase_borders(b->L);
+    phase_forward(b->L);
+    phase_build_fused(b->L, e0, ec, e1, e2);  // count, scan (flags the pair capacity), fill
+    int rc = pull_state(b);
+    if (rc) return rc;
+    bool redone = false;
+    for (int attempt = 0; (b->h_st->err & ERR_GHOST_CAPACITY) && attempt < 4; ++attempt) {
         Caps nc = b->caps;
         uint64_t ghost_cap = (uint64_t)(b->caps.atoms - b->caps.local);
         ghost_cap = ghost_cap * 2 + 4096;
@@ -353,26 +365,16 @@ int rebuild_tail(soil_hip_box* b) {
         rc = push_desc(b);
         if (rc) return rc;
         rebuild_tiles(b);
+        phase_borders(b->L);
+        phase_forward(b->L);
+        phase_build_fused(b->L, nullptr, nullptr, nullptr, nullptr);
+        rc = pull_state(b);
+        if (rc) return rc;
+        redone = true;
     }
     if (b->h_st->err & ERR_GHOST_CAPACITY) return set_error(SOIL_HIP_ECAPACITY, "ghost capacity growth failed");
-    phase_forward(b->L);
-    hipEvent_t e0 = nullptr, ec = nullptr, e1 = nullptr, e2 = nullptr;
-    if (b->timing) {
-        e0 = take_event(b);
-        ec = take_event(b);
-        e1 = take_event(b);
-        e2 = take_event(b);
-    }
-    // build, split at the CSR scan so the pair capacity is known before the fill
-    {
-        Launch L = b->L;
-        // count, scan (flags the capacity), fill (skipped on overflow)
-        phase_build_fused(L, e0, ec, e1, e2);
-    }
-    int rc = pull_state(b);
-    if (rc) return rc;
     if (b->h_st->err & ERR_PAIR_CAPACITY) {
-        // grow pairs and redo the fill + transpose (state is intact up to the fill)
+        // grow pairs and redo the build (the state is intact up to the fill)
         uint32_t np = b->h_st->npairs;
         rc = alloc_pairs(b, (uint32_t)std::min<uint64_t>((uint64_t)np + np / 2 + 4096, 0xFFFFFFF0u), false);
         if (rc) return rc;
@@ -390,14 +392,15 @@ int rebuild_tail(soil_hip_box* b) {
         rc = pull_state(b);
         if (rc) return rc;
         if (b->h_st->err & ERR_PAIR_CAPACITY) return set_error(SOIL_HIP_ECAPACITY, "pair capacity growth failed");
-        if (e0) {
-            b->ev_pool.push_back(e0);
-            b->ev_pool.push_back(e1);
-            b->ev_pool.push_back(e2);
-            e0 = nullptr;
+        redone = true;
+    }
+    if (e0) {
+        if (redone) {  // the timed build was skipped: no sample for this step
+            for (hipEvent_t e : {e0, ec, e1, e2}) b->ev_pool.push_back(e);
+        } else {
+            b->pending.push_back({e0, ec, e1, e2});
         }
     }
-    if (e0) b->pending.push_back({e0, ec, e1, e2});
     return 0;
 }
 
