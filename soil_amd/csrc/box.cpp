@@ -152,7 +152,38 @@ int pull_desc(soil_hip_box* b) {
 int pull_state(soil_hip_box* b) {
     HIPCHK(hipMemcpyAsync(b->h_st, b->d_st, sizeof(BoxState), hipMemcpyDeviceToHost, b->stream));
     HIPCHK(hipStreamSynchronize(b->stream));
+    b->pub_expect = b->h_st->pub_seq;  // the stream is drained: the publish count is exact
     return 0;
+}
+
+// Waits for the publish the host enqueued last (pub_expect) and copies its
+// fields (state, error bits, counts) into h_st. Falls back to a full state
+// read when the stream drains without it (a stopped box publishes nothing)
+// or the count is out of step.
+int wait_mirror(soil_hip_box* b) {
+    const uint64_t want = b->pub_expect;
+    const HostMirror* m = b->h_mirror;
+    for (uint32_t it = 1;; ++it) {
+        const uint64_t seq = __atomic_load_n(&m->seq, __ATOMIC_ACQUIRE);
+        if (seq == want) {
+            b->h_st->state = m->state;
+            b->h_st->err = m->err;
+            b->h_st->nlocal = m->nlocal;
+            b->h_st->nghost = m->nghost;
+            b->h_st->npairs = m->npairs;
+            return 0;
+        }
+        if (seq > want) return pull_state(b);
+        if ((it & 1023u) == 0u) {
+            const hipError_t e = hipStreamQuery(b->stream);
+            if (e == hipSuccess) {
+                if (__atomic_load_n(&m->seq, __ATOMIC_ACQUIRE) == want) continue;
+                return pull_state(b);
+            }
+            if (e != hipErrorNotReady) return set_error(SOIL_HIP_EDEVICE, "hipStreamQuery: %s", hipGetErrorString(e));
+        }
+        __builtin_ia32_pause();
+    }
 }
 
 int rebuild_tiles(soil_hip_box* b) {
@@ -349,7 +380,9 @@ int rebuild_tail(soil_hip_box* b) {
     phase_borders(b->L);
     phase_forward(b->L);
     phase_build_fused(b->L, e0, ec, e1, e2);  // count, scan (flags the pair capacity), fill
-    int rc = pull_state(b);
+    phase_publish(b->L);
+    b->pub_expect += 1;
+    int rc = wait_mirror(b);
     if (rc) return rc;
     bool redone = false;
     for (int attempt = 0; (b->h_st->err & ERR_GHOST_CAPACITY) && attempt < 4; ++attempt) {
@@ -407,6 +440,8 @@ int rebuild_tail(soil_hip_box* b) {
 int step_plain_loop(soil_hip_box* b, int nsteps, bool& pending_final);
 
 int step_plain(soil_hip_box* b, int nsteps) {
+    int rc0 = pull_state(b);  // exact publish count (batches and other calls publish too)
+    if (rc0) return rc0;
     // final_integration of the previous step, when it is carried into this
     // step's initial_integration kernel; always launched before returning
     bool pending_final = false;
@@ -482,8 +517,9 @@ int step_plain_loop(soil_hip_box* b, int nsteps, bool& pending_final) {
         } else {
             phase_initial_decide(b->L);
         }
+        b->pub_expect += 1;  // k_decide publishes the decision
         pending_final = false;
-        rc = pull_state(b);
+        rc = wait_mirror(b);
         if (rc) return rc;
         if (b->h_st->err) return check_err_bits(b->h_st->err);
         const bool rebuild = b->h_st->state == 0 || g.periodic_sort || g.bounds_changed;
@@ -503,6 +539,7 @@ int step_plain_loop(soil_hip_box* b, int nsteps, bool& pending_final) {
             // the host counts are current (read after the decision, the
             // borders and the build); the column buffers swap at each sort
             rc = pull_desc(b);
+            if (!rc) rc = pull_state(b);
             if (!rc) rc = call_hook(b, true, b->h_st->steps);
             if (rc) return rc;
         }
@@ -608,6 +645,7 @@ int soil_hip_box_destroy(soil_hip_box* b) {
     if (b->d_view) hipFree(b->d_view);
     if (b->d_ring) hipFree(b->d_ring);
     if (b->h_st) hipHostFree(b->h_st);
+    if (b->h_mirror) hipHostFree(b->h_mirror);
     if (b->h_ring) hipHostFree(b->h_ring);
     if (b->stream) hipStreamDestroy(b->stream);
     delete b;
@@ -693,6 +731,10 @@ int soil_hip_setup(soil_hip_box* b) {
     HIPCHK(hipMemcpy(b->d_ptrs, &b->d_dev, sizeof(BoxDev*), hipMemcpyHostToDevice));
     HIPCHK(hipMalloc(&b->d_st, sizeof(BoxState)));
     HIPCHK(hipHostMalloc(&b->h_st, sizeof(BoxState), hipHostMallocDefault));
+    HIPCHK(hipHostMalloc(&b->h_mirror, sizeof(HostMirror), hipHostMallocMapped | hipHostMallocCoherent));
+    std::memset(b->h_mirror, 0, sizeof(HostMirror));
+    void* mirror_dev = nullptr;
+    HIPCHK(hipHostGetDevicePointer(&mirror_dev, b->h_mirror, 0));
     HIPCHK(hipMalloc(&b->d_view, sizeof(soil_hip_device_view)));
     HIPCHK(hipMalloc(&b->d_ring, sizeof(Geom) * b->ring_len));
     HIPCHK(hipHostMalloc(&b->h_ring, sizeof(Geom) * b->ring_len, hipHostMallocDefault));
@@ -730,6 +772,7 @@ int soil_hip_setup(soil_hip_box* b) {
     }
     d.st = b->d_st;
     d.view = b->d_view;
+    d.mirror = (HostMirror*)mirror_dev;
     d.geom_ring = b->d_ring;
     d.ring_len = b->ring_len;
     d.ring_stride = 1;

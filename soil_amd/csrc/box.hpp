@@ -77,6 +77,8 @@ struct soil_hip_box {
     soil_hip::BoxState* d_st = nullptr;
     soil_hip::BoxState* h_st = nullptr;   // pinned
     soil_hip_device_view* d_view = nullptr;  // device copy of the force hook's view
+    soil_hip::HostMirror* h_mirror = nullptr;  // pinned, device-mapped (fields published by kernels)
+    uint64_t pub_expect = 0;                   // publishes enqueued so far
     soil_hip::Geom* d_ring = nullptr;
     soil_hip::Geom* h_ring = nullptr;     // pinned
     uint32_t ring_len = 64;

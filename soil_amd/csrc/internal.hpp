@@ -90,6 +90,15 @@ struct BoxState {
     uint64_t csr_bytes, csr_builds;             // algorithmic bytes of CSR builds (timing)
     uint64_t step_limit;                        // the box stops (every kernel skips it) at this step count
     uint64_t ext_key[6];                        // locals' min x,y,z then max x,y,z as order-preserving keys
+    uint64_t pub_seq;                           // publishes to the host mirror so far
+};
+
+// The fields the plain step waits on, published by kernels into pinned host
+// memory that the device maps (system-scope stores, the sequence word last),
+// so the host polls one word instead of copying the state back.
+struct HostMirror {
+    uint64_t seq;
+    uint32_t state, err, nlocal, nghost, npairs, pad;
 };
 
 // Device builds see the arena pointers as global-memory pointers (address
@@ -158,6 +167,7 @@ struct BoxDev {
     int32_t consumer;
     int32_t transpose;    // build the incoming-pair transpose with the CSR (consumer or a hook that asks)
     GMEM soil_hip_device_view* view;  // device copy of the hook's view (refreshed by k_sync_view), or null
+    GMEM HostMirror* mirror;          // device address of the box's host mirror, or null
     int32_t csr_mode;     // CSR strategy (bit-neutral): 0 LDS tiles, 1 two global passes, 2 one pass + row slots, 3 (default) as 2 with wave-staged candidates
 };
 

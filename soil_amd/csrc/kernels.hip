@@ -277,6 +277,22 @@ __global__ void __launch_bounds__(BLK) k_disp(KARGS) {
         wave_flag(&st->exceed, disp_exceeds(g, b, i, b.x[i], b.y[i], b.z[i]));
     }
 }
+// One thread of the box: the state fields the plain step's host waits on,
+// into the host mirror; the sequence word goes last, at system scope.
+__device__ __forceinline__ void publish(CBox& b) {
+    GMEM HostMirror* m = b.mirror;
+    if (!m) return;
+    GMEM BoxState* st = b.st;
+    const uint64_t seq = st->pub_seq + 1u;
+    st->pub_seq = seq;
+    m->state = st->state;
+    m->err = st->err;
+    m->nlocal = st->nlocal;
+    m->nghost = st->nghost;
+    m->npairs = st->npairs;
+    __hip_atomic_store(&m->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 template <bool REMOVE>
 __global__ void k_decide(KARGS) {
     FOR_TILES(t) {
@@ -309,7 +325,17 @@ __global__ void k_decide(KARGS) {
             st->local_cells_valid = 0;
             st->lb_fresh = 0;
             st->rebuild_now = st->state == 0u ? 1u : 0u;  // final value set by k_post_sort when it runs
+            publish(b);
         }
+    }
+}
+
+__global__ void k_publish(KARGS) {
+    FOR_TILES(t) {
+        if (threadIdx.x != 0) continue;
+        CBox& b = box_at(boxes, tiles[t].box);
+        if (!live(b)) continue;
+        publish(b);
     }
 }
 
@@ -2053,6 +2079,7 @@ void phase_reverse(const Launch& L) {
     }
 }
 void phase_final(const Launch& L) { launch(k_final, L.t_local, L.boxes, L.stream, L.maxgrid); }
+void phase_publish(const Launch& L) { launch(k_publish, L.t_box, L.boxes, L.stream, L.maxgrid); }
 void phase_sync_view(const Launch& L) { launch(k_sync_view, L.t_box, L.boxes, L.stream, L.maxgrid); }
 void phase_step_end(const Launch& L) { launch(k_step_end, L.t_box, L.boxes, L.stream, L.maxgrid); }
 void phase_collect(const Launch& L, uint32_t* out) { launch(k_collect, L.t_box, L.boxes, L.stream, L.maxgrid, out); }
