@@ -358,13 +358,13 @@ hipEvent_t take_event(soil_hip_box* b) {
         b->ev_pool.pop_back();
         return e;
     }
+    // timing only: no system-scope fence at record (it would write back caches
+    // in the middle of the step being timed)
     hipEvent_t e;
-    hipEventCreate(&e);
+    hipEventCreateWithFlags(&e, hipEventDisableSystemFence);
     return e;
 }
 
-// Rebuild tail after borders: build + capacity handling. Synchronizes twice
-// (after borders, after the CSR count) so that capacity growth is exact.
 // borders, forward comm and the neighbor build of a rebuild step, with one
 // state read at the end: a ghost overflow makes every build kernel skip, so
 // the host grows the atoms arena and redoes the three; a pair overflow skips
