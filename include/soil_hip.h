@@ -217,6 +217,11 @@ int soil_hip_set_tilt(soil_hip_box* box, double xy, double boundary_vel_x);
 int soil_hip_get_counts(soil_hip_box* box, uint64_t* counts8);
 /* Neighbor::neighbor_offsets (nlocal+1) and neighbor_indices
  * (neighbor.rs:91-95). ECAPACITY when cap_idx < npairs (*n_idx = npairs). */
+/* classify_boundary_atoms (neighbor.rs:1321-1336): mask[i] = 1 when local row
+ * i of the current list has a ghost neighbour (index >= nlocal), else 0 — the
+ * boundary / interior split that lets interior forces overlap a halo exchange.
+ * *n (may be NULL) receives nlocal; mask may be NULL to query it. */
+int soil_hip_classify_boundary_atoms(soil_hip_box* box, uint8_t* mask, size_t cap, size_t* n);
 int soil_hip_get_csr(soil_hip_box* box, uint32_t* offsets, size_t cap_off, uint32_t* indices, size_t cap_idx,
                      size_t* n_idx);
 /* All atoms, locals then ghosts (cap = nlocal+nghost); any pointer may be NULL. */

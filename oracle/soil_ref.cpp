@@ -1388,6 +1388,25 @@ size_t soil_ref_stencil(soil_ref_box* b, int forward, int32_t* out, size_t cap) 
     return s.size();
 }
 
+// neighbor.rs:1321-1336 classify_boundary_atoms: a local row is "boundary"
+// when any of its neighbours is a ghost (index >= nlocal)
+void soil_ref_classify_csr(const uint32_t* offsets, const uint32_t* indices, size_t nlocal, uint8_t* out) {
+    for (size_t i = 0; i < nlocal; ++i) {
+        bool any = false;
+        for (uint32_t e = offsets[i]; e < offsets[i + 1]; ++e) any = any || indices[e] >= nlocal;
+        out[i] = any ? 1 : 0;
+    }
+}
+
+void soil_ref_classify_boundary_atoms(soil_ref_box* b, uint8_t* out) {
+    if (b->neighbor.neighbor_offsets.size() < (size_t)b->atoms.nlocal + 1) {  // no list built yet
+        std::memset(out, 0, b->atoms.nlocal);
+        return;
+    }
+    soil_ref_classify_csr(b->neighbor.neighbor_offsets.data(), b->neighbor.neighbor_indices.data(), b->atoms.nlocal,
+                          out);
+}
+
 size_t soil_ref_csr(soil_ref_box* b, uint32_t* offsets, uint32_t* indices, size_t cap) {
     const Neighbor& nb = b->neighbor;
     if (offsets) std::memcpy(offsets, nb.neighbor_offsets.data(), nb.neighbor_offsets.size() * 4);

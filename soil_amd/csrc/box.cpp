@@ -1175,6 +1175,28 @@ int soil_hip_get_csr(soil_hip_box* b, uint32_t* offsets, size_t cap_off, uint32_
     return 0;
 }
 
+int soil_hip_classify_boundary_atoms(soil_hip_box* b, uint8_t* mask, size_t cap, size_t* n) {
+    int rc = check_box(b, true);
+    if (rc) return rc;
+    rc = pull_state(b);
+    if (rc) return rc;
+    const size_t nl = b->h_st->nlocal;
+    if (n) *n = nl;
+    if (!mask) return 0;
+    if (cap < nl) return set_error(SOIL_HIP_ECAPACITY, "mask needs %zu", nl);
+    if (b->h_st->rebuilds == 0) {  // no list built yet: every row is empty
+        std::memset(mask, 0, nl);
+        return 0;
+    }
+    phase_classify(b->L);
+    rc = pull_desc(b);
+    if (rc) return rc;
+    std::vector<uint32_t> flags(nl);
+    if (nl) HIPCHK(hipMemcpy(flags.data(), b->h_dev.slot, nl * 4, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < nl; ++i) mask[i] = (uint8_t)flags[i];
+    return 0;
+}
+
 int soil_hip_get_atoms(soil_hip_box* b, size_t cap, double* pos, double* vel, double* force, double* radius,
                        double* mass, double* inv_mass, uint32_t* tag, int32_t* origin, int32_t* image) {
     int rc = check_box(b, true);

@@ -330,6 +330,22 @@ __global__ void k_decide(KARGS) {
     }
 }
 
+// classify_boundary_atoms (neighbor.rs:1321-1336): a local row with a ghost
+// neighbour (index >= nlocal) is boundary; the flag goes to the slot scratch
+__global__ void __launch_bounds__(BLK) k_classify(KARGS) {
+    FOR_TILES(t) {
+        const Tile tl = tiles[t];
+        CBox& b = box_at(boxes, tl.box);
+        if (!live(b)) continue;
+        const uint32_t nl = b.st->nlocal;
+        const uint32_t i = tl.start + threadIdx.x;
+        if (i >= nl) continue;
+        uint32_t any = 0u;
+        for (uint32_t e = b.offsets[i]; e < b.offsets[i + 1]; ++e) any |= b.indices[e] >= nl ? 1u : 0u;
+        b.slot[i] = any;
+    }
+}
+
 __global__ void k_publish(KARGS) {
     FOR_TILES(t) {
         if (threadIdx.x != 0) continue;
@@ -2079,6 +2095,7 @@ void phase_reverse(const Launch& L) {
     }
 }
 void phase_final(const Launch& L) { launch(k_final, L.t_local, L.boxes, L.stream, L.maxgrid); }
+void phase_classify(const Launch& L) { launch(k_classify, L.t_local, L.boxes, L.stream, L.maxgrid); }
 void phase_publish(const Launch& L) { launch(k_publish, L.t_box, L.boxes, L.stream, L.maxgrid); }
 void phase_sync_view(const Launch& L) { launch(k_sync_view, L.t_box, L.boxes, L.stream, L.maxgrid); }
 void phase_step_end(const Launch& L) { launch(k_step_end, L.t_box, L.boxes, L.stream, L.maxgrid); }

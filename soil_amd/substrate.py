@@ -150,6 +150,15 @@ class Box:
         check(lib().soil_hip_get_device_view(self.h, C.byref(v)))
         return v
 
+    def classify_boundary_atoms(self):
+        """neighbor.rs:1321-1336: True for local rows with a ghost neighbour."""
+        n = C.c_size_t(0)
+        check(lib().soil_hip_classify_boundary_atoms(self.h, None, 0, C.byref(n)))
+        out = np.zeros(n.value, dtype=np.uint8)
+        check(lib().soil_hip_classify_boundary_atoms(self.h, out.ctypes.data_as(C.POINTER(C.c_uint8)), out.size,
+                                                     C.byref(n)))
+        return out.astype(bool)
+
     def run_force_hook(self):
         check(lib().soil_hip_run_force_hook(self.h))
 

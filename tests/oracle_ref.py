@@ -60,6 +60,8 @@ def lib():
             "soil_ref_set_deform_remap": ([P, C.c_int], None),
             "soil_ref_set_stage_steps": ([P, C.c_uint64], None),
             "soil_ref_get_domain": ([P, DP], None),
+            "soil_ref_classify_csr": ([U32P, U32P, C.c_size_t, C.POINTER(C.c_uint8)], None),
+            "soil_ref_classify_boundary_atoms": ([P, C.POINTER(C.c_uint8)], None),
             "soil_ref_set_boundary": ([P, C.c_int, C.c_int], C.c_int),
             "soil_ref_set_shrink_wrap_padding": ([P, C.c_double], None),
             "soil_ref_setup": ([P], C.c_int),
@@ -175,6 +177,12 @@ class RefBox:
 
     def set_stage_steps(self, n):
         lib().soil_ref_set_stage_steps(self.h, int(n))
+
+    def classify_boundary_atoms(self):
+        """neighbor.rs:1321-1336 on the current list."""
+        out = np.zeros(self.counts()["nlocal"], dtype=np.uint8)
+        lib().soil_ref_classify_boundary_atoms(self.h, out.ctypes.data_as(C.POINTER(C.c_uint8)))
+        return out.astype(bool)
 
     def set_boundary(self, dim, kind):
         """[domain] boundary of one axis: "p" periodic, "f" fixed, "s" shrink-wrap."""
@@ -315,3 +323,13 @@ def ref_desc(lo, hi, periodic=(1, 1, 1), skin_fraction=1.0, bin_min_size=1.0, ev
     return dict(lo=list(lo), hi=list(hi), periodic=list(periodic), newton=newton, skin_fraction=skin_fraction,
                 bin_min_size=bin_min_size, every=every, check=check, sort_every=sort_every, dt=dt,
                 shear_rate=shear_rate, has_shear=1 if shear_rate != 0.0 else 0)
+
+
+def classify_csr(offsets, indices, nlocal):
+    """neighbor.rs:1321-1336 on a given CSR."""
+    off = np.ascontiguousarray(offsets, dtype=np.uint32)
+    idx = np.ascontiguousarray(indices, dtype=np.uint32)
+    out = np.zeros(nlocal, dtype=np.uint8)
+    lib().soil_ref_classify_csr(off.ctypes.data_as(U32P), idx.ctypes.data_as(U32P), nlocal,
+                                out.ctypes.data_as(C.POINTER(C.c_uint8)))
+    return out.astype(bool)

@@ -298,3 +298,13 @@ def test_shrink_wrap_box_is_refused_by_a_batch(ctx):
         soil_amd.Batch(ctx, [b])
     b.step(3)
     b.close()
+
+
+@pytest.mark.parametrize("shear", [0.0, 0.15])
+def test_classify_boundary_atoms_matches_oracle(ctx, shear):
+    # neighbor.rs:1321-1336 on the device list equals the oracle's mask
+    print(f"case: classify boundary shear={shear}", flush=True)
+    g, r = run_compare(ctx, small(4501, 4000, 0.4, 0.6, 0.45, shear=shear), 30, 15, k=1000.0)
+    mg, mr = g.classify_boundary_atoms(), r.classify_boundary_atoms()
+    assert mg.dtype == bool and mg.size == mr.size and np.array_equal(mg, mr)
+    assert 0 < mg.sum() < mg.size

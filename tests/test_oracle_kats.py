@@ -375,3 +375,33 @@ def test_shrink_wrap_size_fallback():
     b.phase("shrink_wrap")
     lo, hi, _ = b.domain()
     assert lo[0] == 4.0 - 0.2 and hi[0] == 6.0 + 0.2
+
+
+def test_classify_boundary_atoms_fixture():
+    # neighbor.rs:1344-1354: offsets [0,1,3,3], indices [1,0,3], 3 locals -> [F, T, F]
+    from oracle_ref import classify_csr
+    assert list(classify_csr([0, 1, 3, 3], [1, 0, 3], 3)) == [False, True, False]
+
+
+def test_boundary_atoms_lie_near_a_periodic_face():
+    # a local with a ghost neighbour is within the pair cutoff of a face whose
+    # images it meets; rows without ghosts are interior
+    import soil_amd
+    d, pos, vel, mass, rad = soil_amd.gen_custom(4500, 3000, 0.4, 0.6, 0.4, (1, 1, 1), 0, 0.0, 0.0, 0.1, 0.0)
+    b = RefBox(d)
+    b.add_atoms(pos, mass, rad, vel=vel)
+    b.setup()
+    b.step(3)
+    mask = b.classify_boundary_atoms()
+    a = b.atoms()
+    n = len(mass)
+    off, idx = b.csr()
+    lo, hi = np.array(d.lo), np.array(d.hi)
+    cut = 2 * 0.6 * d.skin_fraction
+    p = a["pos"][:n]
+    near = np.any((p - lo < cut) | (hi - p <= cut), axis=1)
+    assert mask.sum() > 0 and np.all(near[mask])
+    for i in np.nonzero(mask)[0][:50]:
+        assert np.any(idx[off[i]:off[i + 1]] >= n)
+    for i in np.nonzero(~mask)[0][:50]:
+        assert np.all(idx[off[i]:off[i + 1]] < n)
