@@ -217,6 +217,12 @@ int soil_hip_set_tilt(soil_hip_box* box, double xy, double boundary_vel_x);
 int soil_hip_get_counts(soil_hip_box* box, uint64_t* counts8);
 /* Neighbor::neighbor_offsets (nlocal+1) and neighbor_indices
  * (neighbor.rs:91-95). ECAPACITY when cap_idx < npairs (*n_idx = npairs). */
+/* compute_ke (atom.rs:589-604): 0.5 * sum of m (vx^2 + vy^2 + vz^2) over the
+ * locals (mask: nlocal bytes, non-zero = in the group; NULL = all). Each term
+ * is evaluated as the reference does; the sum is a fixed-order tree (per-tile
+ * on the device, tiles in order on the host), deterministic and within a few
+ * ulps of the reference's sequential sum. */
+int soil_hip_compute_ke(soil_hip_box* box, const uint8_t* mask, double* out);
 /* classify_boundary_atoms (neighbor.rs:1321-1336): mask[i] = 1 when local row
  * i of the current list has a ghost neighbour (index >= nlocal), else 0 — the
  * boundary / interior split that lets interior forces overlap a halo exchange.

@@ -1388,6 +1388,18 @@ size_t soil_ref_stencil(soil_ref_box* b, int forward, int32_t* out, size_t cap) 
     return s.size();
 }
 
+// atom.rs:589-604 compute_ke: sequential sum over the (masked) locals
+double soil_ref_compute_ke(soil_ref_box* b, const uint8_t* mask) {
+    const Atom& a = b->atoms;
+    double ke = 0.0;
+    for (size_t i = 0; i < a.nlocal; ++i) {
+        if (mask && !mask[i]) continue;
+        const double vx = a.vel[3 * i], vy = a.vel[3 * i + 1], vz = a.vel[3 * i + 2];
+        ke += a.mass[i] * (vx * vx + vy * vy + vz * vz);
+    }
+    return 0.5 * ke;
+}
+
 // neighbor.rs:1321-1336 classify_boundary_atoms: a local row is "boundary"
 // when any of its neighbours is a ghost (index >= nlocal)
 void soil_ref_classify_csr(const uint32_t* offsets, const uint32_t* indices, size_t nlocal, uint8_t* out) {

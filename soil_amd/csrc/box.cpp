@@ -113,6 +113,7 @@ void carve_atoms(Carver& c, BoxDev& d, const Caps& k, const std::vector<HostCol>
     c.take(d.spos, 4 * A);
     c.take(d.bin_start, C1); c.take(d.counts, C1);
     c.take(d.scan_part, std::max(C1, A + 1) / SCAN_TILE + 2);
+    c.take(d.red, (size_t)k.local / BLK + 2);
     c.take(d.scell, k.local); c.take(d.sstart, C1); c.take(d.perm, k.local);
     c.take(d.offsets, L1);
     c.take(d.tcount, A + 1); c.take(d.tstart, A + 1);
@@ -1194,6 +1195,31 @@ int soil_hip_classify_boundary_atoms(soil_hip_box* b, uint8_t* mask, size_t cap,
     std::vector<uint32_t> flags(nl);
     if (nl) HIPCHK(hipMemcpy(flags.data(), b->h_dev.slot, nl * 4, hipMemcpyDeviceToHost));
     for (size_t i = 0; i < nl; ++i) mask[i] = (uint8_t)flags[i];
+    return 0;
+}
+
+int soil_hip_compute_ke(soil_hip_box* b, const uint8_t* mask, double* out) {
+    int rc = check_box(b, true);
+    if (rc) return rc;
+    if (!out) return set_error(SOIL_HIP_EINVAL, "null output");
+    rc = pull_state(b);
+    if (rc) return rc;
+    rc = pull_desc(b);
+    if (rc) return rc;
+    const size_t nl = b->h_st->nlocal;
+    if (mask && nl) {
+        std::vector<uint32_t> m(nl);
+        for (size_t i = 0; i < nl; ++i) m[i] = mask[i] ? 1u : 0u;
+        HIPCHK(hipMemcpy(b->h_dev.slot, m.data(), nl * 4, hipMemcpyHostToDevice));
+    }
+    phase_ke(b->L, mask != nullptr);
+    const size_t ntile = (nl + BLK - 1) / BLK;
+    std::vector<double> part(ntile);
+    HIPCHK(hipStreamSynchronize(b->stream));
+    if (ntile) HIPCHK(hipMemcpy(part.data(), b->h_dev.red, ntile * 8, hipMemcpyDeviceToHost));
+    double ke = 0.0;
+    for (size_t k = 0; k < ntile; ++k) ke += part[k];
+    *out = 0.5 * ke;
     return 0;
 }
 

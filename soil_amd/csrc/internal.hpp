@@ -138,6 +138,7 @@ struct BoxDev {
     GMEM uint32_t* bin_start;  // cap_cells + 1
     GMEM uint32_t* counts;     // cap_cells + 1
     GMEM uint32_t* scan_part;  // per-tile partial sums (scan scratch)
+    GMEM double* red;          // per-tile partial sums of f64 reductions (compute_ke)
     // spatial sort (sort_atoms_by_bin) keeps its own arrays, like the reference's
     // sort_cell / sort_counts / sort_perm, so the last build's bins stay readable
     GMEM uint32_t* scell;      // cap_local

@@ -346,6 +346,36 @@ __global__ void __launch_bounds__(BLK) k_classify(KARGS) {
     }
 }
 
+// compute_ke (atom.rs:589-604): each local's m (vx² + vy² + vz²) in the
+// reference's order, summed per 256-atom tile in a fixed tree order; the host
+// adds the tile sums in tile order. `use_mask`: only locals whose slot-scratch
+// word is non-zero (the caller's group mask).
+__global__ void __launch_bounds__(BLK) k_ke(KARGS, int use_mask) {
+    __shared__ double red[BLK / 64];
+    FOR_TILES(t) {
+        const Tile tl = tiles[t];
+        CBox& b = box_at(boxes, tl.box);
+        if (!live(b)) continue;
+        const uint32_t i = tl.start + threadIdx.x;
+        double v = 0.0;
+        if (i < b.st->nlocal && (!use_mask || b.slot[i] != 0u)) {
+            const double vx = b.vx[i], vy = b.vy[i], vz = b.vz[i];
+            v = b.mass[i] * (vx * vx + vy * vy + vz * vz);
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        if ((threadIdx.x & 63u) == 0u) red[threadIdx.x >> 6] = v;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double s = 0.0;
+#pragma unroll
+            for (int w = 0; w < BLK / 64; ++w) s += red[w];
+            b.red[tl.start / BLK] = s;
+        }
+        __syncthreads();
+    }
+}
+
 __global__ void k_publish(KARGS) {
     FOR_TILES(t) {
         if (threadIdx.x != 0) continue;
@@ -2095,6 +2125,7 @@ void phase_reverse(const Launch& L) {
     }
 }
 void phase_final(const Launch& L) { launch(k_final, L.t_local, L.boxes, L.stream, L.maxgrid); }
+void phase_ke(const Launch& L, bool use_mask) { launch(k_ke, L.t_local, L.boxes, L.stream, L.maxgrid, use_mask ? 1 : 0); }
 void phase_classify(const Launch& L) { launch(k_classify, L.t_local, L.boxes, L.stream, L.maxgrid); }
 void phase_publish(const Launch& L) { launch(k_publish, L.t_box, L.boxes, L.stream, L.maxgrid); }
 void phase_sync_view(const Launch& L) { launch(k_sync_view, L.t_box, L.boxes, L.stream, L.maxgrid); }

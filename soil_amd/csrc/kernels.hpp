@@ -47,6 +47,7 @@ void phase_kick(const Launch& L, bool with_final);  // k_initial with the displa
 void phase_decide_fused(const Launch& L);           // k_decide with the sort bookkeeping
 void phase_extent(const Launch& L);                 // locals' extents into BoxState::ext_key
 void phase_sync_view(const Launch& L);              // force hooks' device views
+void phase_ke(const Launch& L, bool use_mask);      // per-tile kinetic-energy partials into BoxDev::red
 void phase_classify(const Launch& L);               // boundary flags of the local rows into the slot scratch
 void phase_publish(const Launch& L);                // state fields into the host mirror (k_decide<true> does it too)
 void phase_remove_ghosts(const Launch& L);

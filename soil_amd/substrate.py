@@ -150,6 +150,14 @@ class Box:
         check(lib().soil_hip_get_device_view(self.h, C.byref(v)))
         return v
 
+    def compute_ke(self, mask=None):
+        """atom.rs:589-604: kinetic energy of the locals (mask: bool per local)."""
+        out = C.c_double(0.0)
+        m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
+        check(lib().soil_hip_compute_ke(self.h, None if m is None else m.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                        C.byref(out)))
+        return out.value
+
     def classify_boundary_atoms(self):
         """neighbor.rs:1321-1336: True for local rows with a ghost neighbour."""
         n = C.c_size_t(0)

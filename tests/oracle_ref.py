@@ -62,6 +62,7 @@ def lib():
             "soil_ref_get_domain": ([P, DP], None),
             "soil_ref_classify_csr": ([U32P, U32P, C.c_size_t, C.POINTER(C.c_uint8)], None),
             "soil_ref_classify_boundary_atoms": ([P, C.POINTER(C.c_uint8)], None),
+            "soil_ref_compute_ke": ([P, C.POINTER(C.c_uint8)], C.c_double),
             "soil_ref_set_boundary": ([P, C.c_int, C.c_int], C.c_int),
             "soil_ref_set_shrink_wrap_padding": ([P, C.c_double], None),
             "soil_ref_setup": ([P], C.c_int),
@@ -177,6 +178,10 @@ class RefBox:
 
     def set_stage_steps(self, n):
         lib().soil_ref_set_stage_steps(self.h, int(n))
+
+    def compute_ke(self, mask=None):
+        m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
+        return lib().soil_ref_compute_ke(self.h, None if m is None else m.ctypes.data_as(C.POINTER(C.c_uint8)))
 
     def classify_boundary_atoms(self):
         """neighbor.rs:1321-1336 on the current list."""

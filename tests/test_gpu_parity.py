@@ -308,3 +308,17 @@ def test_classify_boundary_atoms_matches_oracle(ctx, shear):
     mg, mr = g.classify_boundary_atoms(), r.classify_boundary_atoms()
     assert mg.dtype == bool and mg.size == mr.size and np.array_equal(mg, mr)
     assert 0 < mg.sum() < mg.size
+
+
+def test_compute_ke_matches_oracle(ctx):
+    # atom.rs:589-604: same per-atom terms; the device sums them in a fixed
+    # tree order, the reference sequentially (stated bound: 1e-12 relative)
+    print("case: compute_ke", flush=True)
+    g, r = run_compare(ctx, small(4502, 5000, 0.4, 0.6, 0.45), 20, 20, k=1000.0)
+    n = g.counts()["nlocal"]
+    mask = (np.arange(n) % 3 == 0)
+    for m in (None, mask):
+        kg, kr = g.compute_ke(m), r.compute_ke(m)
+        assert kr > 0 and abs(kg - kr) <= 1e-12 * kr, (kg, kr)
+    assert g.compute_ke(np.zeros(n, dtype=bool)) == 0.0
+    assert g.compute_ke() == g.compute_ke()  # deterministic

@@ -405,3 +405,13 @@ def test_boundary_atoms_lie_near_a_periodic_face():
         assert np.any(idx[off[i]:off[i + 1]] >= n)
     for i in np.nonzero(~mask)[0][:50]:
         assert np.all(idx[off[i]:off[i + 1]] < n)
+
+
+def test_compute_ke_closed_form():
+    # atom.rs:589-604: 0.5 * sum m |v|^2 over the (masked) locals; exact here
+    b = RefBox(ref_desc([0, 0, 0], [10, 10, 10]))
+    b.add_atoms([[1, 1, 1], [5, 5, 5], [8, 2, 3]], [2.0, 1.0, 4.0], [0.1] * 3,
+                vel=[[1.0, 2.0, 2.0], [0.5, 0.0, -0.5], [0.0, 0.0, 3.0]])
+    assert b.compute_ke() == 0.5 * (2.0 * 9.0 + 1.0 * 0.5 + 4.0 * 9.0)
+    assert b.compute_ke([1, 0, 1]) == 0.5 * (2.0 * 9.0 + 4.0 * 9.0)
+    assert b.compute_ke([0, 0, 0]) == 0.0
