@@ -33,6 +33,12 @@ int set_error(int code, const char* fmt, ...) {
         if (e_ != hipSuccess) return set_error(SOIL_HIP_EDEVICE, "%s: %s", #call, hipGetErrorString(e_)); \
     } while (0)
 
+// The harmonic consumer's pair force is at most |k| (ri + rj) per component;
+// below 2^19 (with a 1 % margin) every q = F 2^32 is below 2^51 in magnitude
+static int32_t fix_fast_ok(double k, double rmax) {
+    return std::fabs(k) * 2.0 * rmax * 1.01 < 524288.0 ? 1 : 0;
+}
+
 static uint32_t ceil_div(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
 static void push_tiles(std::vector<Tile>& v, uint32_t box, uint64_t n, uint32_t step) {
@@ -778,6 +784,9 @@ int soil_hip_setup(soil_hip_box* b) {
     d.ring_len = b->ring_len;
     d.ring_stride = 1;
     d.harmonic_k = b->harmonic_k;
+    b->max_radius = 0.0;
+    for (size_t i = 0; i < n; ++i) b->max_radius = std::fmax(b->max_radius, b->hrad[i]);
+    d.fix_fast = fix_fast_ok(b->harmonic_k, b->max_radius);
     d.dt = b->model.dt;
     d.consumer = b->consumer ? 1 : 0;
     d.transpose = (b->consumer || (b->hook && (b->hook_flags & SOIL_HIP_HOOK_TRANSPOSE))) ? 1 : 0;
@@ -870,11 +879,13 @@ int soil_hip_set_consumer(soil_hip_box* b, int enabled, double k) {
     if (b->setup_done) {
         b->h_dev.consumer = b->consumer ? 1 : 0;
         b->h_dev.harmonic_k = k;
+        b->h_dev.fix_fast = fix_fast_ok(k, b->max_radius);
         b->h_dev.transpose = (b->consumer || (b->hook && (b->hook_flags & SOIL_HIP_HOOK_TRANSPOSE))) ? 1 : 0;
         // patch the fields in place (the rest of the device descriptor may hold swapped pointers)
         HIPCHK(hipStreamSynchronize(b->stream));
         HIPCHK(hipMemcpy(&b->d_dev->harmonic_k, &b->h_dev.harmonic_k, sizeof(double), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(&b->d_dev->consumer, &b->h_dev.consumer, sizeof(int32_t), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(&b->d_dev->fix_fast, &b->h_dev.fix_fast, sizeof(int32_t), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(&b->d_dev->transpose, &b->h_dev.transpose, sizeof(int32_t), hipMemcpyHostToDevice));
     }
     return 0;

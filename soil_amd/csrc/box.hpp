@@ -87,6 +87,7 @@ struct soil_hip_box {
     soil_hip::Launch L;
     bool consumer = false;
     double harmonic_k = 0.0;
+    double max_radius = 0.0;  // of the uploaded atoms (bounds the consumer's forces)
     // force hook (Force phase, after the consumer)
     soil_hip_force_hook hook = nullptr;
     void* hook_user = nullptr;

@@ -166,6 +166,7 @@ struct BoxDev {
     double harmonic_k;
     double dt;            // Atom::dt (constant per box)
     int32_t consumer;
+    int32_t fix_fast;     // consumer forces stay below 2^19 in magnitude (fast fixed-point rounding)
     int32_t transpose;    // build the incoming-pair transpose with the CSR (consumer or a hook that asks)
     GMEM soil_hip_device_view* view;  // device copy of the hook's view (refreshed by k_sync_view), or null
     GMEM HostMirror* mirror;          // device address of the box's host mirror, or null

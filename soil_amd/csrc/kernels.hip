@@ -1888,6 +1888,46 @@ __device__ __forceinline__ void pair_force(const double4& pi, const double4& pj,
 
 constexpr double kFixScale = 4294967296.0;  // 2^32
 
+// q = round_half_even(f * 2^32). FAST: the box's forces are below 2^19 in
+// magnitude (host check: |k| (ri + rj) with margin), so |f 2^32| < 2^51 and
+// adding 1.5 * 2^52 rounds it to the nearest integer, ties to even, which then
+// sits in the low bits of the sum's representation: one add and one integer
+// subtraction instead of the general conversion.
+template <bool FAST>
+__device__ __forceinline__ long long fix_q(double f) {
+    if (FAST) {
+        const double t = f * kFixScale + 6755399441055744.0;  // 1.5 * 2^52
+        return (long long)__double_as_longlong(t) - 0x4338000000000000ll;
+    }
+    return __double2ll_rn(f * kFixScale);
+}
+
+// An atom's exact fixed-point force sum: its row (as the row atom) and its
+// incoming pairs (as the neighbour).
+template <bool FAST>
+__device__ __forceinline__ void harmonic_sums(CBox& b, const BoxState* st, uint32_t a, long long& ax,
+                                              long long& ay, long long& az) {
+    const double k = b.harmonic_k;
+    const GMEM double4* xr = (const GMEM double4*)b.xr;
+    const double4 pa = xr[a];
+    if (a < st->nlocal) {
+        for (uint32_t e = b.offsets[a]; e < b.offsets[a + 1]; ++e) {
+            double fx, fy, fz;
+            pair_force(pa, xr[b.indices[e]], k, fx, fy, fz);
+            ax -= fix_q<FAST>(fx);
+            ay -= fix_q<FAST>(fy);
+            az -= fix_q<FAST>(fz);
+        }
+    }
+    for (uint32_t e = b.tstart[a]; e < b.tstart[a + 1]; ++e) {
+        double fx, fy, fz;
+        pair_force(xr[b.tsrc[e]], pa, k, fx, fy, fz);
+        ax += fix_q<FAST>(fx);
+        ay += fix_q<FAST>(fy);
+        az += fix_q<FAST>(fz);
+    }
+}
+
 // ASSIGN (fused step, forces were not zeroed): f = sum * 2^-32, which equals
 // 0 + sum * 2^-32 bit for bit; otherwise f += sum * 2^-32.
 template <bool ASSIGN>
@@ -1907,26 +1947,9 @@ __global__ void __launch_bounds__(BLK) k_harmonic(KARGS) {
             }
             continue;
         }
-        const double k = b.harmonic_k;
-        const double4* xr = (const double4*)b.xr;
-        const double4 pa = xr[a];
         long long ax = 0, ay = 0, az = 0;
-        if (a < st->nlocal) {
-            for (uint32_t e = b.offsets[a]; e < b.offsets[a + 1]; ++e) {
-                double fx, fy, fz;
-                pair_force(pa, xr[b.indices[e]], k, fx, fy, fz);
-                ax -= __double2ll_rn(fx * kFixScale);
-                ay -= __double2ll_rn(fy * kFixScale);
-                az -= __double2ll_rn(fz * kFixScale);
-            }
-        }
-        for (uint32_t e = b.tstart[a]; e < b.tstart[a + 1]; ++e) {
-            double fx, fy, fz;
-            pair_force(xr[b.tsrc[e]], pa, k, fx, fy, fz);
-            ax += __double2ll_rn(fx * kFixScale);
-            ay += __double2ll_rn(fy * kFixScale);
-            az += __double2ll_rn(fz * kFixScale);
-        }
+        if (b.fix_fast) harmonic_sums<true>(b, st, a, ax, ay, az);  // box-uniform
+        else harmonic_sums<false>(b, st, a, ax, ay, az);
         const double inv = 1.0 / kFixScale;
         if (ASSIGN) {
             b.fx[a] = (double)ax * inv;
