@@ -516,8 +516,9 @@ int step_plain_loop(soil_hip_box* b, int nsteps, bool& pending_final) {
             rc = shrink_wrap_now(b);
             if (rc) return rc;
             b->model.sort_prepare();
-            rc = stage_geom(b, g);
+            rc = stage_geom(b, g);  // may regrid and regrow the cell arrays
             if (rc) return rc;
+            phase_zero_counts(b->L);  // k_initial zeroed the counts of the provisional grid only
             phase_decide_fused(b->L);
         } else if (pending_final) {
             phase_final_initial_decide(b->L);

@@ -261,7 +261,14 @@ __global__ void __launch_bounds__(BLK) k_initial(KARGS) {
             put_xr(b, i, x, y, z);
             if (DISP && disp_active(g, st)) ex = disp_exceeds(g, b, i, x, y, z);
         }
-        if (DISP) wave_flag(&st->exceed, ex);
+        if (DISP) {
+            // the fused step's spatial sort counts from zero (k_sort_zero's work,
+            // done here for every step; the previous step's build no longer
+            // needs the counts)
+            const uint32_t stride = ((b.cap_local + BLK - 1u) / BLK) * BLK;
+            for (uint32_t c = i; c <= g.ncells; c += stride) b.counts[c] = 0u;
+            wave_flag(&st->exceed, ex);
+        }
     }
 }
 __global__ void __launch_bounds__(BLK) k_disp(KARGS) {
@@ -373,6 +380,18 @@ __global__ void __launch_bounds__(BLK) k_ke(KARGS, int use_mask) {
             b.red[tl.start / BLK] = s;
         }
         __syncthreads();
+    }
+}
+
+// counts[0..ncells] = 0 for the step's geometry: the shrink-wrap step stages
+// its grid after the kick-drift, whose zeroing used the provisional grid
+__global__ void __launch_bounds__(BLK) k_zero_counts(KARGS) {
+    FOR_TILES(t) {
+        const Tile tl = tiles[t];
+        CBox& b = box_at(boxes, tl.box);
+        if (!live(b)) continue;
+        const uint32_t c = tl.start + threadIdx.x;
+        if (c <= geom_of(b).ncells) b.counts[c] = 0u;
     }
 }
 
@@ -1361,6 +1380,13 @@ __global__ void __launch_bounds__(BLK) k_save_build(KARGS) {
         if (!b.st->rebuild_now) continue;
         const uint32_t i = tl.start + threadIdx.x;
         if (b.transpose && i <= b.st->nlocal + b.st->nghost) b.tcount[i] = 0u;
+        // the build sort counts from zero unless it reuses the spatial sort's
+        // local counts (k_sort_zero's work)
+        if (!b.st->local_cells_valid && !(b.st->err & ERR_GHOST_CAPACITY)) {
+            const uint32_t stride = ((b.cap_atoms + 1u + BLK - 1u) / BLK) * BLK;
+            const uint32_t nc = geom_of(b).ncells;
+            for (uint32_t c = i; c <= nc; c += stride) b.counts[c] = 0u;
+        }
         if (i >= b.st->nlocal || b.st->lb_fresh) continue;
         b.lbx[i] = b.x[i];
         b.lby[i] = b.y[i];
@@ -2053,9 +2079,10 @@ static void run_scan(const Launch& L, const TileSet& scan_tiles) {
     launch(k_scan_down<W>, scan_tiles, L.boxes, L.stream, L.maxgrid);
 }
 
-template <int W, int SW>
+// ZERO: launch k_sort_zero (the fused step zeroes the counts in an earlier kernel)
+template <int W, int SW, bool ZERO = true>
 static void run_sort(const Launch& L, const TileSet& zero_tiles, const TileSet& item_tiles, const TileSet& scan_tiles) {
-    launch(k_sort_zero<W>, zero_tiles, L.boxes, L.stream, L.maxgrid);
+    if (ZERO) launch(k_sort_zero<W>, zero_tiles, L.boxes, L.stream, L.maxgrid);
     launch(k_sort_count<W>, item_tiles, L.boxes, L.stream, L.maxgrid);
     run_scan<SW>(L, scan_tiles);
     launch(k_sort_scatter<W>, item_tiles, L.boxes, L.stream, L.maxgrid);
@@ -2105,7 +2132,7 @@ void phase_sort(const Launch& L) {
 }
 // sort + permute + (when rebuilding) the build's bookkeeping in one per-box kernel
 void phase_sort_fused(const Launch& L) {  // k_decide<true> did the pre-sort bookkeeping
-    run_sort<SORT_SPATIAL, SCAN_CELLS_SPATIAL>(L, L.t_cell, L.t_local, L.t_scan_cell);
+    run_sort<SORT_SPATIAL, SCAN_CELLS_SPATIAL, false>(L, L.t_cell, L.t_local, L.t_scan_cell);  // k_initial zeroed
     launch(k_permute<true>, L.t_local, L.boxes, L.stream, L.maxgrid);
     launch(k_post_sort<true>, L.t_box, L.boxes, L.stream, L.maxgrid);
 }
@@ -2122,7 +2149,7 @@ void phase_build(const Launch& L, hipEvent_t ev0, hipEvent_t evc, hipEvent_t ev1
 // the build after phase_sort_fused (which already did the bookkeeping)
 void phase_build_fused(const Launch& L, hipEvent_t ev0, hipEvent_t evc, hipEvent_t ev1, hipEvent_t ev2) {
     launch(k_save_build, L.t_atom1, L.boxes, L.stream, L.maxgrid);
-    run_sort<SORT_BUILD, SCAN_CELLS_BUILD>(L, L.t_cell, L.t_atom, L.t_scan_cell);
+    run_sort<SORT_BUILD, SCAN_CELLS_BUILD, false>(L, L.t_cell, L.t_atom, L.t_scan_cell);  // k_save_build zeroed
     if (ev0) (void)hipEventRecord(ev0, L.stream);
     if (L.csr_mode == 0) launch(k_csr_count, L.t_local1, L.boxes, L.stream, L.maxgrid);  // tiles + end marker
     else if (L.csr_mode == 3) launch(k_csr_count_w, L.t_local1, L.boxes, L.stream, L.maxgrid);
@@ -2150,6 +2177,7 @@ void phase_reverse(const Launch& L) {
 void phase_final(const Launch& L) { launch(k_final, L.t_local, L.boxes, L.stream, L.maxgrid); }
 void phase_ke(const Launch& L, bool use_mask) { launch(k_ke, L.t_local, L.boxes, L.stream, L.maxgrid, use_mask ? 1 : 0); }
 void phase_classify(const Launch& L) { launch(k_classify, L.t_local, L.boxes, L.stream, L.maxgrid); }
+void phase_zero_counts(const Launch& L) { launch(k_zero_counts, L.t_cell, L.boxes, L.stream, L.maxgrid); }
 void phase_publish(const Launch& L) { launch(k_publish, L.t_box, L.boxes, L.stream, L.maxgrid); }
 void phase_sync_view(const Launch& L) { launch(k_sync_view, L.t_box, L.boxes, L.stream, L.maxgrid); }
 void phase_step_end(const Launch& L) { launch(k_step_end, L.t_box, L.boxes, L.stream, L.maxgrid); }

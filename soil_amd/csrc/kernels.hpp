@@ -49,6 +49,7 @@ void phase_extent(const Launch& L);                 // locals' extents into BoxS
 void phase_sync_view(const Launch& L);              // force hooks' device views
 void phase_ke(const Launch& L, bool use_mask);      // per-tile kinetic-energy partials into BoxDev::red
 void phase_classify(const Launch& L);               // boundary flags of the local rows into the slot scratch
+void phase_zero_counts(const Launch& L);            // the spatial sort's counts for a regridded step
 void phase_publish(const Launch& L);                // state fields into the host mirror (k_decide<true> does it too)
 void phase_remove_ghosts(const Launch& L);
 void phase_set_rebuild_now(const Launch& L);
