@@ -265,10 +265,13 @@ int soil_hip_batch_autotune(soil_hip_batch* batch);
 int soil_hip_batch_step(soil_hip_batch* batch, int nsteps, int use_graph);
 int soil_hip_batch_sync(soil_hip_batch* batch);
 
-/* ── kernel timing (HIP events around the CSR build of every rebuild) ──── */
-int soil_hip_set_timing(soil_hip_box* box, int enabled);
+/* ── kernel timing (HIP events around the CSR build of a rebuild) ──────── */
+/* period 0: off; n > 0: every n-th rebuild's build is timed (events cost a
+ * few microseconds each, so a period above 1 keeps them out of most steps). */
+int soil_hip_set_timing(soil_hip_box* box, int period);
 /* out[0] = builds timed, out[1] = total ms of the CSR build (count, offset
- * scan, fill), out[2] = total algorithmic bytes of those builds, out[3] = total
+ * scan, fill), out[2] = algorithmic bytes of those builds (every build's
+ * average since timing was set, times out[0]), out[3] = total
  * ms of the fill kernel, out[4] = total ms of the count kernel */
 int soil_hip_get_timing(soil_hip_box* box, double* out5);
 int soil_hip_batch_set_timing(soil_hip_batch* batch, int enabled);

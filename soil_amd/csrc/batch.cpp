@@ -45,11 +45,8 @@ static int launch_step(soil_hip_batch* bt) {
     phase_deform(L);
     phase_initial_decide(L);
     phase_zero_fused(L);
-    phase_pbc(L);
-    phase_sort_fused(L);
-    phase_borders(L);
-    phase_forward(L);
-    phase_build_fused(L, nullptr, nullptr, nullptr, nullptr);
+    phase_pbc_sort_fused(L);
+    phase_rebuild_fused(L, nullptr, nullptr, nullptr, nullptr, false);
     phase_force_fused(L);
     int rc = 0;
     bool hooks = false;
@@ -267,6 +264,7 @@ int batch_free(soil_hip_batch* bt, bool restore) {
             b->h_st->ring_pos = 0;
             hipMemcpy(&b->d_st->ring_pos, &b->h_st->ring_pos, 4, hipMemcpyHostToDevice);
             hipMemcpy(b->d_ring, &b->last_geom, sizeof(Geom), hipMemcpyHostToDevice);
+            b->ring_ok.assign(b->ring_ok.size(), 0);  // slot 0 no longer holds h_ring[0]
             rebuild_tiles(b);
         }
         b->in_batch = false;

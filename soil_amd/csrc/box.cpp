@@ -178,6 +178,7 @@ int wait_mirror(soil_hip_box* b) {
             b->h_st->nlocal = m->nlocal;
             b->h_st->nghost = m->nghost;
             b->h_st->npairs = m->npairs;
+            b->h_st->rebuild_now = m->rebuild_now;
             return 0;
         }
         if (seq > want) return pull_state(b);
@@ -333,6 +334,7 @@ int upload_geom_now(soil_hip_box* b) {
     b->last_geom = g;
     HIPCHK(hipStreamSynchronize(b->stream));
     HIPCHK(hipMemcpy(b->d_ring + b->ring_pos, &g, sizeof(Geom), hipMemcpyHostToDevice));
+    b->ring_ok[b->ring_pos] = 1;
     return 0;
 }
 
@@ -376,22 +378,28 @@ hipEvent_t take_event(soil_hip_box* b) {
 // state read at the end: a ghost overflow makes every build kernel skip, so
 // the host grows the atoms arena and redoes the three; a pair overflow skips
 // the fill, and the host grows the pairs and redoes the build.
-int rebuild_tail(soil_hip_box* b) {
-    hipEvent_t e0 = nullptr, ec = nullptr, e1 = nullptr, e2 = nullptr;
-    if (b->timing) {
-        e0 = take_event(b);
-        ec = take_event(b);
-        e1 = take_event(b);
-        e2 = take_event(b);
-    }
-    phase_borders(b->L);
-    phase_forward(b->L);
-    phase_build_fused(b->L, e0, ec, e1, e2);  // count, scan (flags the pair capacity), fill
-    phase_publish(b->L);
+// The rebuild's borders, build and publish, enqueued without waiting; the
+// timing events (when timing) go to `ev`.
+static void rebuild_launch(soil_hip_box* b, hipEvent_t ev[4]) {
+    const bool timed = b->timing && (b->timing_ctr++ % b->timing) == 0;
+    for (int k = 0; k < 4; ++k) ev[k] = timed ? take_event(b) : nullptr;
+    // borders, ghosts, build (count, scan flags the pair capacity, fill), publish
+    phase_rebuild_fused(b->L, ev[0], ev[1], ev[2], ev[3], true);
     b->pub_expect += 1;
+}
+
+// Waits for rebuild_launch's publish. A build that ran out of ghost or pair
+// capacity grows the arena and is redone (`redone`); the kernels enqueued
+// after the publish skip their work while the error bits are set.
+static int rebuild_check(soil_hip_box* b, hipEvent_t ev[4], bool& redone) {
+    redone = false;
     int rc = wait_mirror(b);
     if (rc) return rc;
-    bool redone = false;
+    const bool grow = (b->h_st->err & (ERR_GHOST_CAPACITY | ERR_PAIR_CAPACITY)) != 0;
+    if (grow) {
+        rc = pull_state(b);  // drains the skipped kernels; every field current
+        if (rc) return rc;
+    }
     for (int attempt = 0; (b->h_st->err & ERR_GHOST_CAPACITY) && attempt < 4; ++attempt) {
         Caps nc = b->caps;
         uint64_t ghost_cap = (uint64_t)(b->caps.atoms - b->caps.local);
@@ -419,7 +427,6 @@ int rebuild_tail(soil_hip_box* b) {
         rc = alloc_pairs(b, (uint32_t)std::min<uint64_t>((uint64_t)np + np / 2 + 4096, 0xFFFFFFF0u), false);
         if (rc) return rc;
         b->h_st->err &= ~ERR_PAIR_CAPACITY;
-        HIPCHK(hipMemcpy(&b->d_st->err, &b->h_st->err, 4, hipMemcpyHostToDevice));
         rc = push_desc(b);
         if (rc) return rc;
         rebuild_tiles(b);
@@ -434,11 +441,11 @@ int rebuild_tail(soil_hip_box* b) {
         if (b->h_st->err & ERR_PAIR_CAPACITY) return set_error(SOIL_HIP_ECAPACITY, "pair capacity growth failed");
         redone = true;
     }
-    if (e0) {
-        if (redone) {  // the timed build was skipped: no sample for this step
-            for (hipEvent_t e : {e0, ec, e1, e2}) b->ev_pool.push_back(e);
+    if (ev[0]) {
+        if (redone || !b->h_st->rebuild_now) {  // the timed build was skipped or redone: no sample
+            for (int k = 0; k < 4; ++k) b->ev_pool.push_back(ev[k]);
         } else {
-            b->pending.push_back({e0, ec, e1, e2});
+            b->pending.push_back({ev[0], ev[1], ev[2], ev[3]});
         }
     }
     return 0;
@@ -461,14 +468,21 @@ int step_plain(soil_hip_box* b, int nsteps) {
 
 // Stages the host model's geometry in this step's ring slot (the stream is
 // ordered, so a slot rewritten later in the step reaches the later kernels).
+// A slot that already holds the same bytes is not uploaded again (a box whose
+// geometry does not change reuses it every ring_len steps).
 static int stage_geom(soil_hip_box* b, Geom& g) {
-    if (b->model.fill(g)) return set_error(SOIL_HIP_EUNSUPPORTED, "stencil exceeds %d offsets", SOIL_MAX_STENCIL);
-    if (g.ncells > b->caps.cells) {
-        int rcc = ensure_cells(b, g.ncells);
+    Geom t;
+    if (b->model.fill(t)) return set_error(SOIL_HIP_EUNSUPPORTED, "stencil exceeds %d offsets", SOIL_MAX_STENCIL);
+    if (t.ncells > b->caps.cells) {
+        int rcc = ensure_cells(b, t.ncells);
         if (rcc) return rcc;
     }
-    b->last_geom = g;
+    b->last_geom = t;
+    uint8_t& ok = b->ring_ok[b->ring_pos];
+    if (ok && std::memcmp(&g, &t, sizeof(Geom)) == 0) return 0;
+    std::memcpy(&g, &t, sizeof(Geom));  // fill() cleared the padding, so equal geometries compare equal
     HIPCHK(hipMemcpyAsync(b->d_ring + b->ring_pos, &g, sizeof(Geom), hipMemcpyHostToDevice, b->stream));
+    ok = 1;
     return 0;
 }
 
@@ -527,36 +541,68 @@ int step_plain_loop(soil_hip_box* b, int nsteps, bool& pending_final) {
         }
         b->pub_expect += 1;  // k_decide publishes the decision
         pending_final = false;
-        rc = wait_mirror(b);
-        if (rc) return rc;
-        if (b->h_st->err) return check_err_bits(b->h_st->err);
-        const bool rebuild = b->h_st->state == 0 || g.periodic_sort || g.bounds_changed;
+        // A box that rebuilt on its last two steps enqueues the rebuild path
+        // without waiting for this step's decision: each kernel on it acts on
+        // the device's decision (pbc on FullRebuild; the sorts, borders and
+        // build on rebuild_now; the forward refresh otherwise), so a step that
+        // does not rebuild runs it as the forward step.
+        const bool ahead = !shrink && !b->hook && b->rebuild_streak >= 2;
+        bool rebuild = true;
+        if (!ahead) {
+            rc = wait_mirror(b);
+            if (rc) return rc;
+            if (b->h_st->err) return check_err_bits(b->h_st->err);
+            rebuild = b->h_st->state == 0 || g.periodic_sort || g.bounds_changed;
+        }
         bool zero_cols = false;
         for (const auto& c : b->cols) zero_cols |= (c.flags & SOIL_HIP_COL_ZERO) != 0;
         if (!b->consumer || zero_cols) phase_zero_fused(b->L);
+        // consumer, hook, reverse comm and the step's end
+        auto tail = [&]() -> int {
+            if (b->consumer) phase_force_fused(b->L);
+            if (b->hook) {
+                // the host counts are current (read after the decision, the
+                // borders and the build); the column buffers swap at each sort
+                int r = pull_desc(b);
+                if (!r) r = pull_state(b);
+                if (!r) r = call_hook(b, true, b->h_st->steps);
+                if (r) return r;
+            }
+            if (b->model.newton) {
+                for (int d = 0; d < 3; ++d) b->L.rev_split[d] = g.disjoint[d] ? 0 : 1;
+                phase_reverse(b->L);
+            }
+            phase_step_end(b->L);
+            return 0;
+        };
         if (rebuild) {
-            phase_pbc(b->L);
-            phase_sort_fused(b->L);
-            rc = rebuild_tail(b);
+            phase_pbc_sort_fused(b->L);
+            hipEvent_t ev[4];
+            rebuild_launch(b, ev);
+            // the tail goes out before the host reads the build's counts
+            // (a hook reads them first); after a capacity error it was a no-op
+            const bool early = !b->hook;
+            if (early) {
+                rc = tail();
+                if (rc) return rc;
+            }
+            bool redone = false;
+            rc = rebuild_check(b, ev, redone);
             if (rc) return rc;
+            if (!early || redone) {
+                rc = tail();
+                if (rc) return rc;
+            }
+            b->rebuild_streak = b->h_st->rebuild_now ? b->rebuild_streak + 1 : 0;
+            pending_final = true;  // launched with the next step's initial integration, or on return
+            if (b->h_st->err) return check_err_bits(b->h_st->err);
         } else {
             phase_forward(b->L);  // k_decide<true> left rebuild_now = 0 and the sort flags cleared
-        }
-        if (b->consumer) phase_force_fused(b->L);
-        if (b->hook) {
-            // the host counts are current (read after the decision, the
-            // borders and the build); the column buffers swap at each sort
-            rc = pull_desc(b);
-            if (!rc) rc = pull_state(b);
-            if (!rc) rc = call_hook(b, true, b->h_st->steps);
+            rc = tail();
             if (rc) return rc;
+            b->rebuild_streak = 0;
+            pending_final = true;
         }
-        if (b->model.newton) {
-            for (int d = 0; d < 3; ++d) b->L.rev_split[d] = g.disjoint[d] ? 0 : 1;
-            phase_reverse(b->L);
-        }
-        pending_final = true;  // launched with the next step's initial integration, or on return
-        phase_step_end(b->L);
         b->ring_pos = (b->ring_pos + 1) % b->ring_len;
     }
     return 0;
@@ -746,6 +792,7 @@ int soil_hip_setup(soil_hip_box* b) {
     HIPCHK(hipMalloc(&b->d_view, sizeof(soil_hip_device_view)));
     HIPCHK(hipMalloc(&b->d_ring, sizeof(Geom) * b->ring_len));
     HIPCHK(hipHostMalloc(&b->h_ring, sizeof(Geom) * b->ring_len, hipHostMallocDefault));
+    b->ring_ok.assign(b->ring_len, 0);
     int rc = alloc_atoms(b, c, false);
     if (rc) return rc;
     rc = alloc_pairs(b, (uint32_t)std::min<uint64_t>(std::max<uint64_t>(8 * (uint64_t)n, 4096), 0xFFFFFFF0u), false);
@@ -811,6 +858,11 @@ int soil_hip_setup(soil_hip_box* b) {
     b->setup_done = true;
     rc = upload_geom_now(b);
     if (rc) return rc;
+    // every slot starts with the set-up geometry: a box whose geometry stays
+    // the same then uploads none while it steps
+    for (uint32_t k = 1; k < b->ring_len; ++k) std::memcpy(&b->h_ring[k], &b->h_ring[0], sizeof(Geom));
+    HIPCHK(hipMemcpy(b->d_ring, b->h_ring, sizeof(Geom) * b->ring_len, hipMemcpyHostToDevice));
+    b->ring_ok.assign(b->ring_len, 1);
     // host copies no longer needed
     b->hx.clear(); b->hy.clear(); b->hz.clear(); b->hvx.clear(); b->hvy.clear(); b->hvz.clear();
     b->hfx.clear(); b->hfy.clear(); b->hfz.clear(); b->hmass.clear(); b->htag.clear(); b->htype.clear();
@@ -1453,9 +1505,12 @@ int soil_hip_set_timing(soil_hip_box* b, int enabled) {
     rc = pull_state(b);
     if (rc) return rc;
     resolve_timing(b);
-    b->timing = enabled != 0;
+    if (enabled < 0) return set_error(SOIL_HIP_EINVAL, "timing period %d < 0", enabled);
+    b->timing = (uint32_t)enabled;
+    b->timing_ctr = 0;
     b->t_builds = b->t_csr_ms = b->t_fill_ms = b->t_count_ms = 0.0;
     b->bytes_at_enable = b->h_st->csr_bytes;
+    b->builds_at_enable = b->h_st->csr_builds;
     return 0;
 }
 
@@ -1468,7 +1523,10 @@ int soil_hip_get_timing(soil_hip_box* b, double* out5) {
     resolve_timing(b);
     out5[0] = b->t_builds;
     out5[1] = b->t_csr_ms;
-    out5[2] = (double)(b->h_st->csr_bytes - b->bytes_at_enable);
+    // the timed builds' share of the algorithmic bytes of every build since
+    // timing was set (at the average per build)
+    const uint64_t nb = b->h_st->csr_builds - b->builds_at_enable;
+    out5[2] = nb ? (double)(b->h_st->csr_bytes - b->bytes_at_enable) * (b->t_builds / (double)nb) : 0.0;
     out5[3] = b->t_fill_ms;
     out5[4] = b->t_count_ms;
     return 0;

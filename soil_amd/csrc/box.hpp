@@ -79,6 +79,8 @@ struct soil_hip_box {
     soil_hip_device_view* d_view = nullptr;  // device copy of the force hook's view
     soil_hip::HostMirror* h_mirror = nullptr;  // pinned, device-mapped (fields published by kernels)
     uint64_t pub_expect = 0;                   // publishes enqueued so far
+    uint32_t rebuild_streak = 0;               // consecutive steps that rebuilt (plain stepping)
+    std::vector<uint8_t> ring_ok;              // device ring slot k holds h_ring[k]
     soil_hip::Geom* d_ring = nullptr;
     soil_hip::Geom* h_ring = nullptr;     // pinned
     uint32_t ring_len = 64;
@@ -93,7 +95,9 @@ struct soil_hip_box {
     void* hook_user = nullptr;
     int hook_flags = 0;
     // timing
-    bool timing = false;
+    uint32_t timing = 0;       // time every timing-th rebuild's CSR build (0: off)
+    uint64_t timing_ctr = 0;   // rebuilds launched since timing was set
+    uint64_t builds_at_enable = 0;
     std::vector<hipEvent_t> ev_pool;
     struct Pending { hipEvent_t e0, ec, e1, e2; };
     std::vector<Pending> pending;

@@ -98,7 +98,7 @@ struct BoxState {
 // so the host polls one word instead of copying the state back.
 struct HostMirror {
     uint64_t seq;
-    uint32_t state, err, nlocal, nghost, npairs, pad;
+    uint32_t state, err, nlocal, nghost, npairs, rebuild_now;
 };
 
 // Device builds see the arena pointers as global-memory pointers (address

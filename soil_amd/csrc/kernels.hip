@@ -297,7 +297,41 @@ __device__ __forceinline__ void publish(CBox& b) {
     m->nlocal = st->nlocal;
     m->nghost = st->nghost;
     m->npairs = st->npairs;
+    m->rebuild_now = st->rebuild_now;
     __hip_atomic_store(&m->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// exceed: some local moved past the threshold since the last build
+template <bool REMOVE>
+__device__ __forceinline__ void decide_box(CBox& b, bool exceed) {
+    BoxState* st = b.st;
+    CGeom& g = geom_of(b);
+    if (g.clear_last) st->last_valid = 0;
+    if (st->state == 1u) {
+        bool need;
+        if (!st->last_valid || st->nlocal == 0) need = true;
+        else if (g.every == 0) need = exceed;
+        else if (g.check) need = st->steps_since_build >= g.every || exceed;
+        else need = st->steps_since_build >= g.every;
+        if (need) st->state = 0u;
+        else st->steps_since_build += 1;
+    }
+    st->exceed = 0;
+    if (REMOVE) {
+        if (st->state == 0u) st->nghost = 0;
+        // The fused step also does sort_atoms_by_bin's bookkeeping here
+        // (k_pre_sort): nothing before the sort changes its inputs. A
+        // bounds change sets FullRebuild only in k_post_sort, after pbc's
+        // run_if has seen the state decided here (neighbor.rs:837-841).
+        st->do_sort = (st->nlocal > 0 && g.ncells > 0 && (g.periodic_sort || st->state == 0u || g.bounds_changed))
+                          ? 1u
+                          : 0u;
+        st->not_identity = 0;
+        st->local_cells_valid = 0;
+        st->lb_fresh = 0;
+        st->rebuild_now = st->state == 0u ? 1u : 0u;  // final value set by k_post_sort when it runs
+        publish(b);
+    }
 }
 
 template <bool REMOVE>
@@ -306,34 +340,7 @@ __global__ void k_decide(KARGS) {
         if (threadIdx.x != 0) continue;
         CBox& b = box_at(boxes, tiles[t].box);
         if (!live(b)) continue;
-        BoxState* st = b.st;
-        CGeom& g = geom_of(b);
-        if (g.clear_last) st->last_valid = 0;
-        if (st->state == 1u) {
-            bool need;
-            if (!st->last_valid || st->nlocal == 0) need = true;
-            else if (g.every == 0) need = st->exceed != 0;
-            else if (g.check) need = st->steps_since_build >= g.every || st->exceed != 0;
-            else need = st->steps_since_build >= g.every;
-            if (need) st->state = 0u;
-            else st->steps_since_build += 1;
-        }
-        st->exceed = 0;
-        if (REMOVE) {
-            if (st->state == 0u) st->nghost = 0;
-            // The fused step also does sort_atoms_by_bin's bookkeeping here
-            // (k_pre_sort): nothing before the sort changes its inputs. A
-            // bounds change sets FullRebuild only in k_post_sort, after pbc's
-            // run_if has seen the state decided here (neighbor.rs:837-841).
-            st->do_sort = (st->nlocal > 0 && g.ncells > 0 && (g.periodic_sort || st->state == 0u || g.bounds_changed))
-                              ? 1u
-                              : 0u;
-            st->not_identity = 0;
-            st->local_cells_valid = 0;
-            st->lb_fresh = 0;
-            st->rebuild_now = st->state == 0u ? 1u : 0u;  // final value set by k_post_sort when it runs
-            publish(b);
-        }
+        decide_box<REMOVE>(b, b.st->exceed != 0u);
     }
 }
 
@@ -502,7 +509,61 @@ __global__ void __launch_bounds__(BLK) k_zero(KARGS) {
     }
 }
 
-// pbc / wrap_periodic / pbc_triclinic: domain.rs:594-729 (FullRebuild only)
+// pbc / wrap_periodic / pbc_triclinic: domain.rs:594-729, one local's wrap
+// (FullRebuild only; the caller checks the state and the index)
+__device__ __forceinline__ void pbc_atom(CBox& b, BoxState* st, uint32_t i) {
+    CGeom& g = geom_of(b);
+    if (g.triclinic) {
+        double l[3];
+        x2lamda(g, b.x[i], b.y[i], b.z[i], l[0], l[1], l[2]);
+        int delta[3] = {0, 0, 0};
+        bool gone = false;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            if (g.periodic[k]) {
+                if (l[k] < 0.0) { l[k] += 1.0; delta[k] = -1; }
+                else if (l[k] >= 1.0) { l[k] -= 1.0; delta[k] = 1; }
+            } else if (l[k] < 0.0 || l[k] >= 1.0) {
+                gone = true;
+            }
+        }
+        if (gone) { atomicOr(&st->err, ERR_ATOM_LEFT_BOX); return; }
+        double x, y, z;
+        lamda2x(g, l[0], l[1], l[2], x, y, z);
+        b.x[i] = x;
+        b.y[i] = y;
+        b.z[i] = z;
+        put_xr(b, i, x, y, z);
+        b.imx[i] += delta[0];
+        b.imy[i] += delta[1];
+        b.imz[i] += delta[2];
+        if (delta[1] != 0) {
+            const double s = (double)delta[1];
+            b.vx[i] -= s * g.bvel[0];
+            b.vy[i] -= s * g.bvel[1];
+            b.vz[i] -= s * g.bvel[2];
+        }
+        return;
+    }
+    double* p[3] = {b.x, b.y, b.z};
+    int32_t* im[3] = {b.imx, b.imy, b.imz};
+    double q[3];
+    bool moved = false;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        double v = p[k][i];
+        if (g.periodic[k]) {
+            const double high = g.lo[k] + g.size[k];
+            if (v < g.lo[k]) { v = v + g.size[k]; p[k][i] = v; im[k][i] -= 1; moved = true; }
+            else if (v >= high) { v = v - g.size[k]; p[k][i] = v; im[k][i] += 1; moved = true; }
+        } else if (v < g.lo[k] || v >= g.hi[k]) {
+            atomicOr(&st->err, ERR_ATOM_LEFT_BOX);
+        }
+        q[k] = v;
+    }
+    if (moved) put_xr(b, i, q[0], q[1], q[2]);  // xr already holds the unwrapped ones
+}
+
 __global__ void __launch_bounds__(BLK) k_pbc(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
@@ -512,56 +573,7 @@ __global__ void __launch_bounds__(BLK) k_pbc(KARGS) {
         if (st->state != 0u) continue;
         const uint32_t i = tl.start + threadIdx.x;
         if (i >= st->nlocal) continue;
-        CGeom& g = geom_of(b);
-        if (g.triclinic) {
-            double l[3];
-            x2lamda(g, b.x[i], b.y[i], b.z[i], l[0], l[1], l[2]);
-            int delta[3] = {0, 0, 0};
-            bool gone = false;
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                if (g.periodic[k]) {
-                    if (l[k] < 0.0) { l[k] += 1.0; delta[k] = -1; }
-                    else if (l[k] >= 1.0) { l[k] -= 1.0; delta[k] = 1; }
-                } else if (l[k] < 0.0 || l[k] >= 1.0) {
-                    gone = true;
-                }
-            }
-            if (gone) { atomicOr(&st->err, ERR_ATOM_LEFT_BOX); continue; }
-            double x, y, z;
-            lamda2x(g, l[0], l[1], l[2], x, y, z);
-            b.x[i] = x;
-            b.y[i] = y;
-            b.z[i] = z;
-            put_xr(b, i, x, y, z);
-            b.imx[i] += delta[0];
-            b.imy[i] += delta[1];
-            b.imz[i] += delta[2];
-            if (delta[1] != 0) {
-                const double s = (double)delta[1];
-                b.vx[i] -= s * g.bvel[0];
-                b.vy[i] -= s * g.bvel[1];
-                b.vz[i] -= s * g.bvel[2];
-            }
-            continue;
-        }
-        double* p[3] = {b.x, b.y, b.z};
-        int32_t* im[3] = {b.imx, b.imy, b.imz};
-        double q[3];
-        bool moved = false;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            double v = p[k][i];
-            if (g.periodic[k]) {
-                const double high = g.lo[k] + g.size[k];
-                if (v < g.lo[k]) { v = v + g.size[k]; p[k][i] = v; im[k][i] -= 1; moved = true; }
-                else if (v >= high) { v = v - g.size[k]; p[k][i] = v; im[k][i] += 1; moved = true; }
-            } else if (v < g.lo[k] || v >= g.hi[k]) {
-                atomicOr(&st->err, ERR_ATOM_LEFT_BOX);
-            }
-            q[k] = v;
-        }
-        if (moved) put_xr(b, i, q[0], q[1], q[2]);  // xr already holds the unwrapped ones
+        pbc_atom(b, st, i);
     }
 }
 
@@ -633,15 +645,18 @@ __global__ void __launch_bounds__(BLK) k_sort_zero(KARGS) {
     }
 }
 
-template <int W>
+// PBC (the fused step's spatial sort): each thread first wraps its local, as
+// k_pbc would; the count reads only the thread's own atom.
+template <int W, bool PBC = false>
 __global__ void __launch_bounds__(BLK) k_sort_count(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
         CBox& b = box_at(boxes, tl.box);
         if (!live(b)) continue;
         SortView<W> v = sort_view<W>(b);
-        if (!v.active) continue;
         const uint32_t i = tl.start + threadIdx.x;
+        if (PBC && b.st->state == 0u && i < b.st->nlocal) pbc_atom(b, b.st, i);
+        if (!v.active) continue;
         // Cell sorts: the input is nearly cell-ordered (locals were sorted at the
         // last rebuild, ghosts follow their sources), so lanes form runs of equal
         // keys; one atomic per run hands out consecutive slots.
@@ -804,36 +819,41 @@ __global__ void k_swap_ptrs(KARGS) {
 }
 
 template <bool FUSED>
+__device__ __forceinline__ void post_sort_box(BoxDev& b) {
+    BoxState* st = b.st;
+    CGeom& g = geom_of(b);
+    if (g.bounds_changed) st->state = 0u;
+    if (FUSED && st->do_sort && st->not_identity) {
+        swap_ptr(b.x, b.ax); swap_ptr(b.y, b.ay); swap_ptr(b.z, b.az);
+        swap_ptr(b.vx, b.avx); swap_ptr(b.vy, b.avy); swap_ptr(b.vz, b.avz);
+        swap_ptr(b.mass, b.amass); swap_ptr(b.inv_mass, b.ainv_mass); swap_ptr(b.rad, b.arad);
+        swap_ptr(b.tag, b.atag); swap_ptr(b.type, b.atype);
+        swap_ptr(b.imx, b.aimx); swap_ptr(b.imy, b.aimy); swap_ptr(b.imz, b.aimz);
+        for (int c = 0; c < b.ncols; ++c)
+            if (!(b.cols[c].flags & 4u)) swap_ptr(b.cols[c].data, b.cols[c].alt);
+    }
+    if (g.periodic_sort && st->do_sort && st->not_identity) {
+        st->last_valid = 0;
+        st->state = 0u;
+    }
+    if (st->do_sort) st->local_cells_valid = 1;
+    if (FUSED && st->do_sort && st->not_identity) st->lb_fresh = 1;
+    st->rebuild_now = st->state == 0u ? 1u : 0u;
+    if (FUSED && st->rebuild_now) {
+        st->state = 1u;
+        st->steps_since_build = 0;
+        st->last_valid = 1;
+        st->rebuilds += 1;
+    }
+}
+
+template <bool FUSED>
 __global__ void k_post_sort(KARGS) {
     FOR_TILES(t) {
         if (threadIdx.x != 0) continue;
         BoxDev& b = *boxes[tiles[t].box];
         if (!live(b)) continue;
-        BoxState* st = b.st;
-        CGeom& g = geom_of(b);
-        if (g.bounds_changed) st->state = 0u;
-        if (FUSED && st->do_sort && st->not_identity) {
-            swap_ptr(b.x, b.ax); swap_ptr(b.y, b.ay); swap_ptr(b.z, b.az);
-            swap_ptr(b.vx, b.avx); swap_ptr(b.vy, b.avy); swap_ptr(b.vz, b.avz);
-            swap_ptr(b.mass, b.amass); swap_ptr(b.inv_mass, b.ainv_mass); swap_ptr(b.rad, b.arad);
-            swap_ptr(b.tag, b.atag); swap_ptr(b.type, b.atype);
-            swap_ptr(b.imx, b.aimx); swap_ptr(b.imy, b.aimy); swap_ptr(b.imz, b.aimz);
-            for (int c = 0; c < b.ncols; ++c)
-                if (!(b.cols[c].flags & 4u)) swap_ptr(b.cols[c].data, b.cols[c].alt);
-        }
-        if (g.periodic_sort && st->do_sort && st->not_identity) {
-            st->last_valid = 0;
-            st->state = 0u;
-        }
-        if (st->do_sort) st->local_cells_valid = 1;
-        if (FUSED && st->do_sort && st->not_identity) st->lb_fresh = 1;
-        st->rebuild_now = st->state == 0u ? 1u : 0u;
-        if (FUSED && st->rebuild_now) {
-            st->state = 1u;
-            st->steps_since_build = 0;
-            st->last_valid = 1;
-            st->rebuilds += 1;
-        }
+        post_sort_box<FUSED>(b);
     }
 }
 
@@ -1065,6 +1085,44 @@ __device__ __forceinline__ void place_ghost(CBox& b, CGeom& g, uint32_t dst, uin
 // copied its own source, so every copied field equals the local at the root
 // of the chain; the velocity adds each swap's shift root-first, as the
 // swap-by-swap copies do; inv_mass = 1 / mass as in unpack_atom.
+__device__ __forceinline__ void fill_ghost(CBox& b, const BoxState* st, uint32_t gi) {
+    const uint32_t nl = st->nlocal;
+    CGeom& g = geom_of(b);
+    int sw[3];
+    int depth = 0;
+    uint32_t r = gi;
+    while (r >= nl && depth < 3) {
+        sw[depth++] = b.gswap[r];
+        r = (uint32_t)b.origin[r];
+    }
+    double qx = b.vx[r], qy = b.vy[r], qz = b.vz[r];
+    for (int k = depth - 1; k >= 0; --k) {
+        const int s = sw[k];
+        qx = qx + g.swap_vshift[s][0];
+        qy = qy + g.swap_vshift[s][1];
+        qz = qz + g.swap_vshift[s][2];
+    }
+    b.vx[gi] = qx;
+    b.vy[gi] = qy;
+    b.vz[gi] = qz;
+    b.fx[gi] = b.fx[r];
+    b.fy[gi] = b.fy[r];
+    b.fz[gi] = b.fz[r];
+    const double m = b.mass[r];
+    b.mass[gi] = m;
+    b.inv_mass[gi] = 1.0 / m;
+    b.rad[gi] = b.rad[r];
+    b.tag[gi] = b.tag[r];
+    b.type[gi] = b.type[r];
+    b.imx[gi] = b.imx[r];
+    b.imy[gi] = b.imy[r];
+    b.imz[gi] = b.imz[r];
+    for (int c = 0; c < b.ncols; ++c) {
+        const ColDev& cd = b.cols[c];
+        for (int w = 0; w < cd.width; ++w) cd.data[(size_t)gi * cd.width + w] = cd.data[(size_t)r * cd.width + w];
+    }
+}
+
 __global__ void __launch_bounds__(BLK) k_ghost_fill(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
@@ -1075,40 +1133,7 @@ __global__ void __launch_bounds__(BLK) k_ghost_fill(KARGS) {
         const uint32_t nl = st->nlocal;
         const uint32_t gi = nl + tl.start + threadIdx.x;
         if (gi >= nl + st->nghost) continue;
-        CGeom& g = geom_of(b);
-        int sw[3];
-        int depth = 0;
-        uint32_t r = gi;
-        while (r >= nl && depth < 3) {
-            sw[depth++] = b.gswap[r];
-            r = (uint32_t)b.origin[r];
-        }
-        double qx = b.vx[r], qy = b.vy[r], qz = b.vz[r];
-        for (int k = depth - 1; k >= 0; --k) {
-            const int s = sw[k];
-            qx = qx + g.swap_vshift[s][0];
-            qy = qy + g.swap_vshift[s][1];
-            qz = qz + g.swap_vshift[s][2];
-        }
-        b.vx[gi] = qx;
-        b.vy[gi] = qy;
-        b.vz[gi] = qz;
-        b.fx[gi] = b.fx[r];
-        b.fy[gi] = b.fy[r];
-        b.fz[gi] = b.fz[r];
-        const double m = b.mass[r];
-        b.mass[gi] = m;
-        b.inv_mass[gi] = 1.0 / m;
-        b.rad[gi] = b.rad[r];
-        b.tag[gi] = b.tag[r];
-        b.type[gi] = b.type[r];
-        b.imx[gi] = b.imx[r];
-        b.imy[gi] = b.imy[r];
-        b.imz[gi] = b.imz[r];
-        for (int c = 0; c < b.ncols; ++c) {
-            const ColDev& cd = b.cols[c];
-            for (int w = 0; w < cd.width; ++w) cd.data[(size_t)gi * cd.width + w] = cd.data[(size_t)r * cd.width + w];
-        }
+        fill_ghost(b, st, gi);
     }
 }
 
@@ -1270,6 +1295,41 @@ __global__ void __launch_bounds__(BLK) k_border_write(KARGS, int d) {
 // ── forward comm: comm.rs:400-432, 496-592 ─────────────────────────────────
 // A ghost refreshes from the chain ghost -> ... -> local root, adding the
 // fixed swap offsets root-first, which equals the reference's swap-by-swap order.
+__device__ __forceinline__ void forward_ghost(CBox& b, const BoxState* st, uint32_t gi) {
+    const uint32_t nl = st->nlocal;
+    CGeom& g = geom_of(b);
+    int sw[3];
+    int depth = 0;
+    uint32_t cur = gi;
+    while (cur >= nl && depth < 3) {
+        sw[depth++] = b.gswap[cur];
+        cur = (uint32_t)b.origin[cur];
+    }
+    double px = b.x[cur], py = b.y[cur], pz = b.z[cur];
+    double qx = b.vx[cur], qy = b.vy[cur], qz = b.vz[cur];
+    for (int k = depth - 1; k >= 0; --k) {
+        const int s = sw[k];
+        px = px + g.swap_shift[s][0];
+        py = py + g.swap_shift[s][1];
+        pz = pz + g.swap_shift[s][2];
+        qx = qx + g.swap_vshift[s][0];
+        qy = qy + g.swap_vshift[s][1];
+        qz = qz + g.swap_vshift[s][2];
+    }
+    b.x[gi] = px;
+    b.y[gi] = py;
+    b.z[gi] = pz;
+    put_xr(b, gi, px, py, pz);
+    b.vx[gi] = qx;
+    b.vy[gi] = qy;
+    b.vz[gi] = qz;
+    for (int c = 0; c < b.ncols; ++c) {
+        const ColDev& cd = b.cols[c];
+        if (!(cd.flags & 1u)) continue;
+        for (int w = 0; w < cd.width; ++w) cd.data[(size_t)gi * cd.width + w] = cd.data[(size_t)cur * cd.width + w];
+    }
+}
+
 __global__ void __launch_bounds__(BLK) k_forward(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
@@ -1280,37 +1340,7 @@ __global__ void __launch_bounds__(BLK) k_forward(KARGS) {
         const uint32_t nl = st->nlocal;
         const uint32_t gi = nl + tl.start + threadIdx.x;
         if (gi >= nl + st->nghost) continue;
-        CGeom& g = geom_of(b);
-        int sw[3];
-        int depth = 0;
-        uint32_t cur = gi;
-        while (cur >= nl && depth < 3) {
-            sw[depth++] = b.gswap[cur];
-            cur = (uint32_t)b.origin[cur];
-        }
-        double px = b.x[cur], py = b.y[cur], pz = b.z[cur];
-        double qx = b.vx[cur], qy = b.vy[cur], qz = b.vz[cur];
-        for (int k = depth - 1; k >= 0; --k) {
-            const int s = sw[k];
-            px = px + g.swap_shift[s][0];
-            py = py + g.swap_shift[s][1];
-            pz = pz + g.swap_shift[s][2];
-            qx = qx + g.swap_vshift[s][0];
-            qy = qy + g.swap_vshift[s][1];
-            qz = qz + g.swap_vshift[s][2];
-        }
-        b.x[gi] = px;
-        b.y[gi] = py;
-        b.z[gi] = pz;
-        put_xr(b, gi, px, py, pz);
-        b.vx[gi] = qx;
-        b.vy[gi] = qy;
-        b.vz[gi] = qz;
-        for (int c = 0; c < b.ncols; ++c) {
-            const ColDev& cd = b.cols[c];
-            if (!(cd.flags & 1u)) continue;
-            for (int w = 0; w < cd.width; ++w) cd.data[(size_t)gi * cd.width + w] = cd.data[(size_t)cur * cd.width + w];
-        }
+        forward_ghost(b, st, gi);
     }
 }
 
@@ -1328,7 +1358,8 @@ __global__ void __launch_bounds__(BLK) k_reverse(KARGS, int d, int part) {
         if (!live(b)) continue;
         const BoxState* st = b.st;
         CGeom& g = geom_of(b);
-        if (!g.newton || (st->err & ERR_GHOST_CAPACITY)) continue;
+        // a failed build is redone by the host, and this round with it
+        if (!g.newton || (st->err & (ERR_GHOST_CAPACITY | ERR_PAIR_CAPACITY))) continue;
         const bool both = g.disjoint[d] != 0;
         uint32_t k = tl.start + threadIdx.x;
         int s;
@@ -1372,6 +1403,21 @@ __global__ void k_build_prep(KARGS) {
 }
 
 // save_build_positions (neighbor.rs:669-690); radius extrema are fixed per box
+__device__ __forceinline__ void save_build_atom(CBox& b, uint32_t i) {
+    if (b.transpose && i <= b.st->nlocal + b.st->nghost) b.tcount[i] = 0u;
+    // the build sort counts from zero unless it reuses the spatial sort's
+    // local counts (k_sort_zero's work)
+    if (!b.st->local_cells_valid && !(b.st->err & ERR_GHOST_CAPACITY)) {
+        const uint32_t stride = ((b.cap_atoms + 1u + BLK - 1u) / BLK) * BLK;
+        const uint32_t nc = geom_of(b).ncells;
+        for (uint32_t c = i; c <= nc; c += stride) b.counts[c] = 0u;
+    }
+    if (i >= b.st->nlocal || b.st->lb_fresh) return;
+    b.lbx[i] = b.x[i];
+    b.lby[i] = b.y[i];
+    b.lbz[i] = b.z[i];
+}
+
 __global__ void __launch_bounds__(BLK) k_save_build(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
@@ -1379,18 +1425,27 @@ __global__ void __launch_bounds__(BLK) k_save_build(KARGS) {
         if (!live(b)) continue;
         if (!b.st->rebuild_now) continue;
         const uint32_t i = tl.start + threadIdx.x;
-        if (b.transpose && i <= b.st->nlocal + b.st->nghost) b.tcount[i] = 0u;
-        // the build sort counts from zero unless it reuses the spatial sort's
-        // local counts (k_sort_zero's work)
-        if (!b.st->local_cells_valid && !(b.st->err & ERR_GHOST_CAPACITY)) {
-            const uint32_t stride = ((b.cap_atoms + 1u + BLK - 1u) / BLK) * BLK;
-            const uint32_t nc = geom_of(b).ncells;
-            for (uint32_t c = i; c <= nc; c += stride) b.counts[c] = 0u;
+        save_build_atom(b, i);
+    }
+}
+
+// The rebuild's remaining ghost fields (k_ghost_fill) or, when the step does
+// not rebuild, the forward refresh (k_forward), with save_build_positions
+// (k_save_build), in one launch over every atom slot: the three touch
+// different fields, and none reads what another writes.
+__global__ void __launch_bounds__(BLK) k_ghosts_save(KARGS) {
+    FOR_TILES(t) {
+        const Tile tl = tiles[t];
+        CBox& b = box_at(boxes, tl.box);
+        if (!live(b)) continue;
+        const BoxState* st = b.st;
+        const uint32_t a = tl.start + threadIdx.x;
+        const uint32_t nl = st->nlocal;
+        if (a >= nl && a < nl + st->nghost) {
+            if (!st->rebuild_now) forward_ghost(b, st, a);
+            else if (!(st->err & ERR_GHOST_CAPACITY)) fill_ghost(b, st, a);
         }
-        if (i >= b.st->nlocal || b.st->lb_fresh) continue;
-        b.lbx[i] = b.x[i];
-        b.lby[i] = b.y[i];
-        b.lbz[i] = b.z[i];
+        if (st->rebuild_now) save_build_atom(b, a);
     }
 }
 
@@ -2028,14 +2083,19 @@ __global__ void k_sync_view(KARGS) {
     }
 }
 
+__device__ __forceinline__ void step_end_box(CBox& b) {
+    BoxState* st = b.st;
+    if (st->err & (ERR_GHOST_CAPACITY | ERR_PAIR_CAPACITY)) return;  // the host redoes the step's tail
+    st->steps += 1;
+    st->ring_pos = (st->ring_pos + 1u) % b.ring_len;
+}
+
 __global__ void k_step_end(KARGS) {
     FOR_TILES(t) {
         if (threadIdx.x != 0) continue;
         CBox& b = box_at(boxes, tiles[t].box);
         if (!live(b)) continue;
-        BoxState* st = b.st;
-        st->steps += 1;
-        st->ring_pos = (st->ring_pos + 1u) % b.ring_len;
+        step_end_box(b);
     }
 }
 
@@ -2136,6 +2196,14 @@ void phase_sort_fused(const Launch& L) {  // k_decide<true> did the pre-sort boo
     launch(k_permute<true>, L.t_local, L.boxes, L.stream, L.maxgrid);
     launch(k_post_sort<true>, L.t_box, L.boxes, L.stream, L.maxgrid);
 }
+void phase_pbc_sort_fused(const Launch& L) {
+    launch(k_sort_count<SORT_SPATIAL, true>, L.t_local, L.boxes, L.stream, L.maxgrid);  // k_initial zeroed
+    run_scan<SCAN_CELLS_SPATIAL>(L, L.t_scan_cell);
+    launch(k_sort_scatter<SORT_SPATIAL>, L.t_local, L.boxes, L.stream, L.maxgrid);
+    launch(k_sort_rank<SORT_SPATIAL>, L.t_local, L.boxes, L.stream, L.maxgrid);
+    launch(k_permute<true>, L.t_local, L.boxes, L.stream, L.maxgrid);
+    launch(k_post_sort<true>, L.t_box, L.boxes, L.stream, L.maxgrid);
+}
 void phase_borders(const Launch& L) {
     launch(k_border_count, L.t_border, L.boxes, L.stream, L.maxgrid);
     for (int d = 0; d < 3; ++d) launch(k_border_write, L.t_border, L.boxes, L.stream, L.maxgrid, d);
@@ -2146,10 +2214,10 @@ void phase_build(const Launch& L, hipEvent_t ev0, hipEvent_t evc, hipEvent_t ev1
     launch(k_build_prep, L.t_box, L.boxes, L.stream, L.maxgrid);
     phase_build_fused(L, ev0, evc, ev1, ev2);
 }
-// the build after phase_sort_fused (which already did the bookkeeping)
-void phase_build_fused(const Launch& L, hipEvent_t ev0, hipEvent_t evc, hipEvent_t ev1, hipEvent_t ev2) {
-    launch(k_save_build, L.t_atom1, L.boxes, L.stream, L.maxgrid);
-    run_sort<SORT_BUILD, SCAN_CELLS_BUILD, false>(L, L.t_cell, L.t_atom, L.t_scan_cell);  // k_save_build zeroed
+// the build after save_build: bins, CSR count, offsets, fill and transpose,
+// then the state into the host mirror when `publish`
+static void build_rest(const Launch& L, hipEvent_t ev0, hipEvent_t evc, hipEvent_t ev1, hipEvent_t ev2, bool publish) {
+    run_sort<SORT_BUILD, SCAN_CELLS_BUILD, false>(L, L.t_cell, L.t_atom, L.t_scan_cell);  // save_build zeroed
     if (ev0) (void)hipEventRecord(ev0, L.stream);
     if (L.csr_mode == 0) launch(k_csr_count, L.t_local1, L.boxes, L.stream, L.maxgrid);  // tiles + end marker
     else if (L.csr_mode == 3) launch(k_csr_count_w, L.t_local1, L.boxes, L.stream, L.maxgrid);
@@ -2159,11 +2227,23 @@ void phase_build_fused(const Launch& L, hipEvent_t ev0, hipEvent_t evc, hipEvent
     if (ev1) (void)hipEventRecord(ev1, L.stream);
     launch(k_csr_fill, L.t_local, L.boxes, L.stream, L.maxgrid);
     if (ev2) (void)hipEventRecord(ev2, L.stream);
-    // consumer transpose (tcount was cleared by k_save_build; modes 1 and 2
+    // consumer transpose (tcount was cleared by save_build; modes 1 and 2
     // take the arrival slots in k_csr_fill)
     if (L.csr_mode == 0) launch(k_tcount, L.t_local, L.boxes, L.stream, L.maxgrid);
     run_scan<SCAN_TRANSPOSE>(L, L.t_scan_atom);
     launch(k_tplace, L.t_local, L.boxes, L.stream, L.maxgrid);
+    if (publish) launch(k_publish, L.t_box, L.boxes, L.stream, L.maxgrid);
+}
+// the build after phase_sort_fused (which already did the bookkeeping)
+void phase_build_fused(const Launch& L, hipEvent_t ev0, hipEvent_t evc, hipEvent_t ev1, hipEvent_t ev2) {
+    launch(k_save_build, L.t_atom1, L.boxes, L.stream, L.maxgrid);
+    build_rest(L, ev0, evc, ev1, ev2, false);
+}
+void phase_rebuild_fused(const Launch& L, hipEvent_t ev0, hipEvent_t evc, hipEvent_t ev1, hipEvent_t ev2, bool publish) {
+    launch(k_border_count, L.t_border, L.boxes, L.stream, L.maxgrid);
+    for (int d = 0; d < 3; ++d) launch(k_border_write, L.t_border, L.boxes, L.stream, L.maxgrid, d);
+    launch(k_ghosts_save, L.t_atom1, L.boxes, L.stream, L.maxgrid);
+    build_rest(L, ev0, evc, ev1, ev2, publish);
 }
 void phase_force(const Launch& L) { launch(k_harmonic<false>, L.t_atom, L.boxes, L.stream, L.maxgrid); }
 void phase_force_fused(const Launch& L) { launch(k_harmonic<true>, L.t_atom, L.boxes, L.stream, L.maxgrid); }

@@ -58,12 +58,16 @@ void phase_zero_fused(const Launch& L);  // leaves forces of consumer boxes to p
 void phase_pbc(const Launch& L);
 void phase_sort(const Launch& L);
 void phase_sort_fused(const Launch& L);
+void phase_pbc_sort_fused(const Launch& L);  // phase_pbc + phase_sort_fused, pbc inside the sort's count
 void phase_borders(const Launch& L);
 void phase_forward(const Launch& L);
 // timing events (any may be null): ev0 before the CSR count, evc after it,
 // ev1 after the offset scan, ev2 after the fill
 void phase_build(const Launch& L, hipEvent_t ev0, hipEvent_t evc, hipEvent_t ev1, hipEvent_t ev2);
 void phase_build_fused(const Launch& L, hipEvent_t ev0, hipEvent_t evc, hipEvent_t ev1, hipEvent_t ev2);
+// phase_borders + phase_forward + phase_build_fused (+ phase_publish when
+// `publish`), with the ghost fill, forward refresh and save_build in one launch
+void phase_rebuild_fused(const Launch& L, hipEvent_t ev0, hipEvent_t evc, hipEvent_t ev1, hipEvent_t ev2, bool publish);
 void phase_force(const Launch& L);
 void phase_force_fused(const Launch& L);  // assigns forces of consumer boxes
 void phase_reverse(const Launch& L);

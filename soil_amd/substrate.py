@@ -319,8 +319,9 @@ class Box:
         check(lib().soil_hip_get_stencil(self.h, _i32p(out), out.size, C.byref(n)))
         return out[: n.value]
 
-    def set_timing(self, enabled):
-        check(lib().soil_hip_set_timing(self.h, 1 if enabled else 0))
+    def set_timing(self, period):
+        """period: False/0 off, True/1 every rebuild, n every n-th rebuild."""
+        check(lib().soil_hip_set_timing(self.h, int(period)))
 
     def timing(self):
         out = np.zeros(5)

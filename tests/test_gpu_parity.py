@@ -233,6 +233,21 @@ def test_axis_collapse_grows_capacity_then_errors(ctx):
         r.step(1)
 
 
+def test_axis_compress_grows_capacity_within_one_call(ctx):
+    # the collapse case's first nine steps in one call: the ghost and pair
+    # arenas regrow between steps of the same call, while the box is stepping
+    # ahead of its rebuild decisions (every step rebuilds)
+    print("case: capacity growth inside one multi-step call", flush=True)
+    d, pos, vel, mass, rad = small(4401, 500, 0.4, 0.6, 0.3)
+    L = d.hi[0] - d.lo[0]
+    g, r = make_pair(ctx, d, pos, vel, mass, rad, k=1000.0,
+                     pre_setup=axis_deform([(0, "vel", -L / (10 * d.dt), (0, 0))], stage=100))
+    g.step(9)
+    r.step(9)
+    assert_same(g, r, what="compressed 9 steps")
+    assert g.counts()["nghost"] > 3 * len(mass)
+
+
 def boundaries(kinds, padding=None):
     def f(b):
         for dim, kind in enumerate(kinds):
