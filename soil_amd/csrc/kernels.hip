@@ -1864,7 +1864,12 @@ __global__ void __launch_bounds__(BLK) k_csr_count_w(KARGS) {
 
 // Pass 2 (after the offset scan): move each tile's rows from the temp region
 // to their place (one contiguous copy), or walk overflowed tiles' rows again.
+// Per-row slots (modes 2 and 3): a wave's 64 rows own one contiguous output
+// range, copied one entry per lane (each lane finds its entry's row among the
+// wave's row starts in LDS); a row that overflowed its slots is walked again
+// by its own lane.
 __global__ void __launch_bounds__(BLK) k_csr_fill(KARGS) {
+    __shared__ uint32_t lstart[BLK / 64][64];
     FOR_TILES(t) {
         const Tile tl = tiles[t];
         CBox& b = box_at(boxes, tl.box);
@@ -1874,6 +1879,44 @@ __global__ void __launch_bounds__(BLK) k_csr_fill(KARGS) {
         const uint32_t nl = st->nlocal;
         const uint32_t i0 = tl.start;
         if (i0 >= nl) continue;
+        if (b.csr_mode >= 2) {
+            const uint32_t wid = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+            const uint32_t i0w = i0 + wid * 64u;
+            if (i0w >= nl) continue;  // wave-uniform; no block barrier in this path
+            const uint32_t i = i0w + lane;
+            const bool act = i < nl;
+            const uint32_t last = (nl - i0w < 64u ? nl - i0w : 64u) - 1u;
+            const uint32_t slot = b.cap_pairs / (b.cap_local ? b.cap_local : 1u);
+            const uint32_t off = act ? b.offsets[i] : 0u;
+            const uint32_t end = act ? b.offsets[i + 1] : 0u;
+            const bool tr = transpose_active(b);
+            if (act && end - off > slot) {  // overflowed: walk the row again
+                CGeom& g = geom_of(b);
+                GMEM uint32_t* out = b.indices + off;
+                if (g.uniform) csr_row_g<true, true>(b, g, i, out);
+                else csr_row_g<true, false>(b, g, i, out);
+                if (tr)
+                    for (uint32_t e = off; e < end; ++e) b.tslot[e] = atomicAdd(&b.tcount[b.indices[e]], 1u);
+            }
+            const uint32_t o0 = __shfl(off, 0, 64), o1 = __shfl(end, (int)last, 64);
+            uint32_t* ls = lstart[wid];
+            ls[lane] = act ? off : o1;
+            wave_sync();
+            for (uint32_t e = o0 + lane; e < o1; e += 64u) {
+                uint32_t r = 0;  // the last row starting at or before e holds it
+#pragma unroll
+                for (uint32_t step = 32; step > 0; step >>= 1)
+                    if (r + step <= last && ls[r + step] <= e) r += step;
+                const uint32_t rs = ls[r];
+                const uint32_t re = r < last ? ls[r + 1] : o1;
+                if (re - rs > slot) continue;  // its own lane walks it
+                const uint32_t j = b.ttmp[(size_t)(i0w + r) * slot + (e - rs)];
+                b.indices[e] = j;
+                if (tr) b.tslot[e] = atomicAdd(&b.tcount[j], 1u);  // consumer transpose slot
+            }
+            wave_sync();  // the next tile rewrites the row starts
+            continue;
+        }
         if (b.csr_mode != 0) {
             const uint32_t i = i0 + threadIdx.x;
             if (i >= nl) continue;
@@ -1930,14 +1973,33 @@ __global__ void __launch_bounds__(BLK) k_tcount(KARGS) {
     }
 }
 
+// One entry per lane over the wave's contiguous entries, as in k_csr_fill.
 __global__ void __launch_bounds__(BLK) k_tplace(KARGS) {
+    __shared__ uint32_t lstart[BLK / 64][64];
     FOR_TILES(t) {
         const Tile tl = tiles[t];
         CBox& b = box_at(boxes, tl.box);
         if (!live(b) || !transpose_active(b)) continue;
-        const uint32_t i = tl.start + threadIdx.x;
-        if (i >= b.st->nlocal) continue;
-        for (uint32_t e = b.offsets[i]; e < b.offsets[i + 1]; ++e) b.tsrc[b.tstart[b.indices[e]] + b.tslot[e]] = i;
+        const uint32_t nl = b.st->nlocal;
+        const uint32_t wid = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+        const uint32_t i0w = tl.start + wid * 64u;
+        if (i0w >= nl) continue;  // wave-uniform
+        const uint32_t i = i0w + lane;
+        const bool act = i < nl;
+        const uint32_t last = (nl - i0w < 64u ? nl - i0w : 64u) - 1u;
+        const uint32_t off = act ? b.offsets[i] : 0u;
+        const uint32_t o0 = __shfl(off, 0, 64), o1 = b.offsets[i0w + last + 1u];
+        uint32_t* ls = lstart[wid];
+        ls[lane] = act ? off : o1;
+        wave_sync();
+        for (uint32_t e = o0 + lane; e < o1; e += 64u) {
+            uint32_t r = 0;
+#pragma unroll
+            for (uint32_t step = 32; step > 0; step >>= 1)
+                if (r + step <= last && ls[r + step] <= e) r += step;
+            b.tsrc[b.tstart[b.indices[e]] + b.tslot[e]] = i0w + r;
+        }
+        wave_sync();
     }
 }
 
