@@ -21,17 +21,25 @@ namespace soil_hip {
 // capacity-sized tile table is active.
 struct TileIter {
     uint32_t t, hi, step;
-    __device__ __forceinline__ explicit TileIter(uint32_t n) {
+    // spread: plain block order (see FOR_TILES_SPREAD)
+    __device__ __forceinline__ explicit TileIter(uint32_t n, bool spread = false) {
         const uint32_t g = gridDim.x;
         hi = n;
         step = g;
-        if ((g & 7u) == 0u) t = (blockIdx.x & 7u) * (g >> 3) + (blockIdx.x >> 3);
+        if ((g & 7u) == 0u && !spread) t = (blockIdx.x & 7u) * (g >> 3) + (blockIdx.x >> 3);
         else t = blockIdx.x;
     }
 };
 #define FOR_TILES(t) \
     for (TileIter it_(ntiles); it_.t < it_.hi; it_.t += it_.step) \
         if (const uint32_t t = it_.t; true)
+// Tiles in block order, which deals consecutive tiles to different XCDs: for
+// work that only the last tiles of a box have (the ghosts), which the XCD
+// windows of FOR_TILES would leave to one XCD.
+#define FOR_TILES_SPREAD_IF(t, spread) \
+    for (TileIter it_(ntiles, spread); it_.t < it_.hi; it_.t += it_.step) \
+        if (const uint32_t t = it_.t; true)
+#define FOR_TILES_SPREAD(t) FOR_TILES_SPREAD_IF(t, true)
 #define KARGS BoxDev* const* __restrict__ boxes, const Tile* __restrict__ tiles, uint32_t ntiles
 
 // Kernels read box descriptors and the step geometry through the constant
@@ -649,7 +657,7 @@ __global__ void __launch_bounds__(BLK) k_sort_zero(KARGS) {
 // k_pbc would; the count reads only the thread's own atom.
 template <int W, bool PBC = false>
 __global__ void __launch_bounds__(BLK) k_sort_count(KARGS) {
-    FOR_TILES(t) {
+    FOR_TILES_SPREAD_IF(t, W == SORT_BUILD) {  // the build counts only ghosts when it reuses the sort
         const Tile tl = tiles[t];
         CBox& b = box_at(boxes, tl.box);
         if (!live(b)) continue;
@@ -689,7 +697,7 @@ __global__ void __launch_bounds__(BLK) k_sort_count(KARGS) {
 
 template <int W>
 __global__ void __launch_bounds__(BLK) k_sort_scatter(KARGS) {
-    FOR_TILES(t) {
+    FOR_TILES_SPREAD_IF(t, W == SORT_BUILD) {
         const Tile tl = tiles[t];
         CBox& b = box_at(boxes, tl.box);
         if (!live(b)) continue;
@@ -704,7 +712,7 @@ __global__ void __launch_bounds__(BLK) k_sort_scatter(KARGS) {
 
 template <int W>
 __global__ void __launch_bounds__(BLK) k_sort_rank(KARGS) {
-    FOR_TILES(t) {
+    FOR_TILES_SPREAD_IF(t, W == SORT_BUILD) {  // ghosts rank against their cells, locals only copy
         const Tile tl = tiles[t];
         CBox& b = box_at(boxes, tl.box);
         if (!live(b)) continue;
@@ -1434,18 +1442,22 @@ __global__ void __launch_bounds__(BLK) k_save_build(KARGS) {
 // (k_save_build), in one launch over every atom slot: the three touch
 // different fields, and none reads what another writes.
 __global__ void __launch_bounds__(BLK) k_ghosts_save(KARGS) {
-    FOR_TILES(t) {
+    FOR_TILES_SPREAD(t) {  // the ghosts are the last tiles of each box
         const Tile tl = tiles[t];
         CBox& b = box_at(boxes, tl.box);
         if (!live(b)) continue;
         const BoxState* st = b.st;
         const uint32_t a = tl.start + threadIdx.x;
         const uint32_t nl = st->nlocal;
-        if (a >= nl && a < nl + st->nghost) {
-            if (!st->rebuild_now) forward_ghost(b, st, a);
-            else if (!(st->err & ERR_GHOST_CAPACITY)) fill_ghost(b, st, a);
-        }
-        if (st->rebuild_now) save_build_atom(b, a);
+        if (a < nl || a >= nl + st->nghost) continue;
+        if (!st->rebuild_now) forward_ghost(b, st, a);
+        else if (!(st->err & ERR_GHOST_CAPACITY)) fill_ghost(b, st, a);
+    }
+    FOR_TILES(t) {
+        const Tile tl = tiles[t];
+        CBox& b = box_at(boxes, tl.box);
+        if (!live(b)) continue;
+        if (b.st->rebuild_now) save_build_atom(b, tl.start + threadIdx.x);
     }
 }
 
