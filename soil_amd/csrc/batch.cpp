@@ -42,9 +42,13 @@ static int batch_tiles(soil_hip_batch* bt) {
 // from their device views, refreshed by k_sync_view just before.
 static int launch_step(soil_hip_batch* bt) {
     const Launch& L = bt->L;
-    phase_deform(L);
-    phase_initial_decide(L);
-    phase_zero_fused(L);
+    phase_initial_decide_pending(L);  // the previous step's final, the step's deform work, the kick-drift
+    bool zero = false;        // forces of boxes without a consumer, #[zero] columns
+    for (auto* b : bt->boxes) {
+        zero |= !b->consumer;
+        for (const auto& c : b->cols) zero |= (c.flags & SOIL_HIP_COL_ZERO) != 0;
+    }
+    if (zero) phase_zero_fused(L);
     phase_pbc_sort_fused(L);
     phase_rebuild_fused(L, nullptr, nullptr, nullptr, nullptr, false);
     phase_force_fused(L);
@@ -62,8 +66,7 @@ static int launch_step(soil_hip_batch* bt) {
         }
     }
     phase_reverse(L);
-    phase_final(L);
-    phase_step_end(L);
+    phase_step_end_pending(L);  // final_integration: the next step's kick-drift or phase_final_pending
     return rc;
 }
 
@@ -185,6 +188,7 @@ static int batch_run(soil_hip_batch* bt, int nsteps, bool use_graph) {
                 if (rc) return rc;
             }
         }
+        if (done + (int)run >= nsteps) phase_final_pending(bt->L);  // the call's last step
         HIPCHK(hipEventRecord(bt->half_done[h], bt->stream));
         bt->half_pending[h] = true;
         bt->steps_done += run;

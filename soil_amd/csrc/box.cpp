@@ -520,11 +520,7 @@ int step_plain_loop(soil_hip_box* b, int nsteps, bool& pending_final) {
         Geom& g = b->h_ring[b->ring_pos];
         int rc = stage_geom(b, g);
         if (rc) return rc;
-        if (pending_final && g.init_shear) {  // the one-time profile adds to the kicked velocities
-            phase_final(b->L);
-            pending_final = false;
-        }
-        if (g.init_shear || g.remap_any) phase_deform(b->L);  // the kernel acts only then
+        // the step's deform work (remap, one-time shear profile) runs inside the kick-drift
         if (shrink) {
             phase_kick(b->L, pending_final);
             rc = shrink_wrap_now(b);

@@ -86,7 +86,7 @@ struct BoxState {
     uint32_t ring_pos;                          // geometry slot of the current step
     uint32_t local_cells_valid;                 // locals are in cell order from this step's sort (neighbor.rs:503,540)
     uint32_t lb_fresh;                          // the fused permute already wrote this build's last_build_pos
-    uint32_t pad2;
+    uint32_t final_pending;                     // batches: this step's final_integration is still to apply
     uint64_t csr_bytes, csr_builds;             // algorithmic bytes of CSR builds (timing)
     uint64_t step_limit;                        // the box stops (every kernel skips it) at this step count
     uint64_t ext_key[6];                        // locals' min x,y,z then max x,y,z as order-preserving keys

@@ -167,6 +167,27 @@ __global__ void __launch_bounds__(BLK) k_final(KARGS) {
         b.vz[i] = b.vz[i] + h * b.fz[i];
     }
 }
+// The pending finals at the end of a batch call, also of boxes that reached
+// their step limit during it (k_clear_pending then clears the flags)
+__global__ void __launch_bounds__(BLK) k_final_pending(KARGS) {
+    FOR_TILES(t) {
+        const Tile tl = tiles[t];
+        CBox& b = box_at(boxes, tl.box);
+        if (!b.st->final_pending) continue;
+        const uint32_t i = tl.start + threadIdx.x;
+        if (i >= b.st->nlocal) continue;
+        const double h = 0.5 * b.dt * b.inv_mass[i];
+        b.vx[i] = b.vx[i] + h * b.fx[i];
+        b.vy[i] = b.vy[i] + h * b.fy[i];
+        b.vz[i] = b.vz[i] + h * b.fz[i];
+    }
+}
+__global__ void k_clear_pending(KARGS) {
+    FOR_TILES(t) {
+        if (threadIdx.x == 0) box_at(boxes, tiles[t].box).st->final_pending = 0u;
+    }
+}
+
 // apply_deform's per-atom work (soil_deform/src/lib.rs:522-534, 568-574): the
 // affine remap of the deformed axes, p = new_c + (p - old_c) * scale, then the
 // one-time streaming profile v_x += γ̇ (y - y_c) on the remapped positions.
@@ -225,15 +246,18 @@ __device__ __forceinline__ bool disp_exceeds(CGeom& g, CBox& b, uint32_t i, doub
     return dx * dx + dy * dy + dz * dz > g.disp_thr_sq;
 }
 
-// initial_integration (soil_verlet/src/lib.rs:131-162). With DISP the
-// displacement check of the following decide_rebuild runs on the new
-// positions while they are in registers (nothing between the two systems
-// changes positions or the state it depends on).
+// initial_integration (soil_verlet/src/lib.rs:131-162). With DISP (the fused
+// step) the step's apply_deform work comes first (k_deform_init's remap and
+// one-time shear profile, in registers), and the displacement check of the
+// following decide_rebuild runs on the new positions while they are in
+// registers (nothing between the two systems changes positions or the state
+// it depends on).
 // With FINAL the previous step's final_integration (v += h f) is applied
-// first, in registers: within a multi-step call nothing between the two
-// systems reads or writes velocities or forces (the host runs the final
-// separately when the one-time shear profile is due in between).
-template <bool DISP, bool FINAL>
+// first of all, in registers: within a multi-step call nothing between the
+// two steps reads or writes velocities or forces, and the order final ->
+// deform -> initial is the reference's. FINAL: 0 none, 1 always, 2 when the
+// box's final is pending (batches; k_decide clears the flag).
+template <bool DISP, int FINAL>
 __global__ void __launch_bounds__(BLK) k_initial(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
@@ -249,20 +273,27 @@ __global__ void __launch_bounds__(BLK) k_initial(KARGS) {
             const double h = 0.5 * dt * b.inv_mass[i];
             const double fx = b.fx[i], fy = b.fy[i], fz = b.fz[i];
             double u0 = b.vx[i], u1 = b.vy[i], u2 = b.vz[i];
-            if (FINAL) {
+            double p0 = b.x[i], p1 = b.y[i], p2 = b.z[i];
+            if (FINAL == 1 || (FINAL == 2 && st->final_pending != 0u)) {
                 u0 = u0 + h * fx;
                 u1 = u1 + h * fy;
                 u2 = u2 + h * fz;
             }
+            if (DISP && g.remap_any) {  // as k_deform_init
+                if (g.remap[0]) p0 = g.remap_new_c[0] + (p0 - g.remap_old_c[0]) * g.remap_scale[0];
+                if (g.remap[1]) p1 = g.remap_new_c[1] + (p1 - g.remap_old_c[1]) * g.remap_scale[1];
+                if (g.remap[2]) p2 = g.remap_new_c[2] + (p2 - g.remap_old_c[2]) * g.remap_scale[2];
+            }
+            if (DISP && g.init_shear) u0 += g.shear_rate * (p1 - g.yc);
             const double v0 = u0 + h * fx;
             const double v1 = u1 + h * fy;
             const double v2 = u2 + h * fz;
             b.vx[i] = v0;
             b.vy[i] = v1;
             b.vz[i] = v2;
-            const double x = b.x[i] + v0 * dt;
-            const double y = b.y[i] + v1 * dt;
-            const double z = b.z[i] + v2 * dt;
+            const double x = p0 + v0 * dt;
+            const double y = p1 + v1 * dt;
+            const double z = p2 + v2 * dt;
             b.x[i] = x;
             b.y[i] = y;
             b.z[i] = z;
@@ -325,6 +356,7 @@ __device__ __forceinline__ void decide_box(CBox& b, bool exceed) {
         else st->steps_since_build += 1;
     }
     st->exceed = 0;
+    st->final_pending = 0;  // applied by the kick-drift before this
     if (REMOVE) {
         if (st->state == 0u) st->nghost = 0;
         // The fused step also does sort_atoms_by_bin's bookkeeping here
@@ -2157,19 +2189,19 @@ __global__ void k_sync_view(KARGS) {
     }
 }
 
-__device__ __forceinline__ void step_end_box(CBox& b) {
-    BoxState* st = b.st;
-    if (st->err & (ERR_GHOST_CAPACITY | ERR_PAIR_CAPACITY)) return;  // the host redoes the step's tail
-    st->steps += 1;
-    st->ring_pos = (st->ring_pos + 1u) % b.ring_len;
-}
-
+// PEND (batches): the step's final_integration is left to the next step's
+// kick-drift or the end of the call (k_final_pending)
+template <bool PEND>
 __global__ void k_step_end(KARGS) {
     FOR_TILES(t) {
         if (threadIdx.x != 0) continue;
         CBox& b = box_at(boxes, tiles[t].box);
         if (!live(b)) continue;
-        step_end_box(b);
+        BoxState* st = b.st;
+        if (st->err & (ERR_GHOST_CAPACITY | ERR_PAIR_CAPACITY)) continue;  // the host redoes the step's tail
+        if (PEND) st->final_pending = 1u;
+        st->steps += 1;
+        st->ring_pos = (st->ring_pos + 1u) % b.ring_len;
     }
 }
 
@@ -2224,13 +2256,13 @@ static void run_sort(const Launch& L, const TileSet& zero_tiles, const TileSet& 
 }
 
 void phase_deform(const Launch& L) { launch(k_deform_init, L.t_local, L.boxes, L.stream, L.maxgrid); }
-void phase_initial(const Launch& L) { launch(k_initial<false, false>, L.t_local, L.boxes, L.stream, L.maxgrid); }
+void phase_initial(const Launch& L) { launch(k_initial<false, 0>, L.t_local, L.boxes, L.stream, L.maxgrid); }
 void phase_initial_decide(const Launch& L) {
-    launch(k_initial<true, false>, L.t_local, L.boxes, L.stream, L.maxgrid);
+    launch(k_initial<true, 0>, L.t_local, L.boxes, L.stream, L.maxgrid);
     launch(k_decide<true>, L.t_box, L.boxes, L.stream, L.maxgrid);
 }
 void phase_final_initial_decide(const Launch& L) {
-    launch(k_initial<true, true>, L.t_local, L.boxes, L.stream, L.maxgrid);
+    launch(k_initial<true, 1>, L.t_local, L.boxes, L.stream, L.maxgrid);
     launch(k_decide<true>, L.t_box, L.boxes, L.stream, L.maxgrid);
 }
 void phase_decide(const Launch& L) {
@@ -2240,8 +2272,8 @@ void phase_decide(const Launch& L) {
 // the kick-drift (and the displacement test) alone, for steps whose decision
 // waits on host work in between (shrink-wrap bounds)
 void phase_kick(const Launch& L, bool with_final) {
-    if (with_final) launch(k_initial<true, true>, L.t_local, L.boxes, L.stream, L.maxgrid);
-    else launch(k_initial<true, false>, L.t_local, L.boxes, L.stream, L.maxgrid);
+    if (with_final) launch(k_initial<true, 1>, L.t_local, L.boxes, L.stream, L.maxgrid);
+    else launch(k_initial<true, 0>, L.t_local, L.boxes, L.stream, L.maxgrid);
 }
 void phase_decide_fused(const Launch& L) { launch(k_decide<true>, L.t_box, L.boxes, L.stream, L.maxgrid); }
 void phase_extent(const Launch& L) {
@@ -2334,7 +2366,16 @@ void phase_classify(const Launch& L) { launch(k_classify, L.t_local, L.boxes, L.
 void phase_zero_counts(const Launch& L) { launch(k_zero_counts, L.t_cell, L.boxes, L.stream, L.maxgrid); }
 void phase_publish(const Launch& L) { launch(k_publish, L.t_box, L.boxes, L.stream, L.maxgrid); }
 void phase_sync_view(const Launch& L) { launch(k_sync_view, L.t_box, L.boxes, L.stream, L.maxgrid); }
-void phase_step_end(const Launch& L) { launch(k_step_end, L.t_box, L.boxes, L.stream, L.maxgrid); }
+void phase_step_end(const Launch& L) { launch(k_step_end<false>, L.t_box, L.boxes, L.stream, L.maxgrid); }
+void phase_initial_decide_pending(const Launch& L) {
+    launch(k_initial<true, 2>, L.t_local, L.boxes, L.stream, L.maxgrid);
+    launch(k_decide<true>, L.t_box, L.boxes, L.stream, L.maxgrid);
+}
+void phase_step_end_pending(const Launch& L) { launch(k_step_end<true>, L.t_box, L.boxes, L.stream, L.maxgrid); }
+void phase_final_pending(const Launch& L) {
+    launch(k_final_pending, L.t_local, L.boxes, L.stream, L.maxgrid);
+    launch(k_clear_pending, L.t_box, L.boxes, L.stream, L.maxgrid);
+}
 void phase_collect(const Launch& L, uint32_t* out) { launch(k_collect, L.t_box, L.boxes, L.stream, L.maxgrid, out); }
 
 }  // namespace soil_hip

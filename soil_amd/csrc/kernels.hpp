@@ -73,6 +73,11 @@ void phase_force_fused(const Launch& L);  // assigns forces of consumer boxes
 void phase_reverse(const Launch& L);
 void phase_final(const Launch& L);
 void phase_step_end(const Launch& L);
+// batches carry each step's final into the next step's kick-drift (a
+// per-box flag) and apply what is left at the end of the call
+void phase_initial_decide_pending(const Launch& L);
+void phase_step_end_pending(const Launch& L);
+void phase_final_pending(const Launch& L);
 void phase_collect(const Launch& L, uint32_t* out);
 void copy_probe(const double* src, double* dst, size_t n, hipStream_t s);
 
