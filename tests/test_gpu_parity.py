@@ -122,6 +122,31 @@ def test_tiny_and_single_atom_boxes(ctx):
     run_compare(ctx, small(708, 20, 0.4, 0.6, 0.05), 10, 2)
 
 
+@pytest.mark.parametrize("shear", [0.0, 0.1])
+def test_boundary_and_coincident_atoms(ctx, shear):
+    # placements random boxes never hit: exactly on the lower faces, exactly
+    # on the upper faces (outside [lo, hi) until the first pbc wraps them), a
+    # corner (imaged through every swap), one ulp inside the upper face, on
+    # bin faces (multiples of L/8), and a coincident pair (r2 = 0: a CSR pair
+    # that the consumer skips)
+    print(f"case: boundary placements shear={shear}", flush=True)
+    d, pos, vel, mass, rad = small(909, 800, 0.4, 0.6, 0.3, shear=shear)
+    lo, hi = np.array(d.lo), np.array(d.hi)
+    L = hi - lo
+    pos = pos.copy()
+    pos[0] = [lo[0], pos[0][1], pos[0][2]]
+    pos[1] = [pos[1][0], hi[1], pos[1][2]]
+    pos[2] = lo
+    pos[3] = hi
+    pos[4] = np.nextafter(hi, lo)
+    for k in range(5, 13):
+        pos[k] = lo + L * ((k - 5) / 8.0)
+    pos[14] = pos[13]
+    vel = vel.copy()
+    vel[14] = vel[13]
+    run_compare(ctx, (d, pos, vel, mass, rad), 30, 3, k=1000.0)
+
+
 def test_phase_by_phase_matches_oracle(ctx):
     # the per-system entry points follow the reference systems one at a time
     print("case: per-phase", flush=True)
