@@ -325,20 +325,22 @@ __global__ void __launch_bounds__(BLK) k_disp(KARGS) {
 }
 // One thread of the box: the state fields the plain step's host waits on,
 // into the host mirror; the sequence word goes last, at system scope.
-__device__ __forceinline__ void publish(CBox& b) {
+// err and npairs come from the caller when it has just changed them itself.
+__device__ __forceinline__ void publish_vals(CBox& b, uint32_t err, uint32_t npairs) {
     GMEM HostMirror* m = b.mirror;
     if (!m) return;
     GMEM BoxState* st = b.st;
     const uint64_t seq = st->pub_seq + 1u;
     st->pub_seq = seq;
     m->state = st->state;
-    m->err = st->err;
+    m->err = err;
     m->nlocal = st->nlocal;
     m->nghost = st->nghost;
-    m->npairs = st->npairs;
+    m->npairs = npairs;
     m->rebuild_now = st->rebuild_now;
     __hip_atomic_store(&m->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
+__device__ __forceinline__ void publish(CBox& b) { publish_vals(b, b.st->err, b.st->npairs); }
 
 // exceed: some local moved past the threshold since the last build
 template <bool REMOVE>
@@ -953,7 +955,9 @@ __global__ void __launch_bounds__(BLK) k_scan_reduce(KARGS) {
 // Down-sweep: each tile adds the totals of the box's earlier tiles (summed
 // here from k_scan_reduce's per-tile totals, a few loads per thread) to its
 // own exclusive scan; the first tile also sums them all for the CSR count.
-template <int W>
+// PUB (the plain step's CSR offsets): the box's first tile also publishes the
+// build's state to the host mirror, active or not (k_publish's work).
+template <int W, bool PUB = false>
 __global__ void __launch_bounds__(BLK) k_scan_down(KARGS) {
     __shared__ uint32_t lds[BLK / 64];
     __shared__ uint32_t red[BLK / 64];
@@ -965,7 +969,12 @@ __global__ void __launch_bounds__(BLK) k_scan_down(KARGS) {
         bool active;
         GMEM uint32_t *in, *out;
         scan_view<W>(b, n, active, in, out);
-        if (!active || tl.start >= n) continue;  // block-uniform
+        if (!active || tl.start >= n) {  // block-uniform
+            if (PUB && tl.start == 0u && threadIdx.x == 0)
+                publish_vals(b, __hip_atomic_load(&b.st->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                             b.st->npairs);
+            continue;
+        }
         const uint32_t tix = tl.start / SCAN_TILE;
         const bool first = tix == 0u;
         const uint32_t lim = first ? (n + SCAN_TILE - 1) / SCAN_TILE : tix;
@@ -983,9 +992,11 @@ __global__ void __launch_bounds__(BLK) k_scan_down(KARGS) {
             BoxState* st = b.st;
             st->local_cells_valid = 0;  // consumed by this build (neighbor.rs:596)
             st->npairs = acc;
+            uint32_t err;  // the bits as they stand after this kernel's own update
             if (acc > b.cap_pairs) {
-                atomicOr(&st->err, ERR_PAIR_CAPACITY);
+                err = atomicOr(&st->err, ERR_PAIR_CAPACITY) | ERR_PAIR_CAPACITY;
             } else {
+                err = __hip_atomic_load(&st->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 CGeom& g = geom_of(b);
                 const uint64_t na = (uint64_t)st->nlocal + st->nghost;
                 const uint64_t per = 24u + (g.uniform ? 0u : 8u) + 4u;
@@ -993,6 +1004,7 @@ __global__ void __launch_bounds__(BLK) k_scan_down(KARGS) {
                                  4u * (uint64_t)acc;
                 st->csr_builds += 1;
             }
+            if (PUB) publish_vals(b, err, acc);
         }
         const uint32_t prefix = first ? 0u : acc;
         uint32_t v[SCAN_PER_THREAD];
@@ -2239,10 +2251,10 @@ static void launch(void (*k)(BoxDev* const*, const Tile*, uint32_t, A...), const
     hipLaunchKernelGGL(k, dim3(grid), dim3(BLK), 0, s, boxes, (const Tile*)ts.d, ts.n, args...);
 }
 
-template <int W>
+template <int W, bool PUB = false>
 static void run_scan(const Launch& L, const TileSet& scan_tiles) {
     launch(k_scan_reduce<W>, scan_tiles, L.boxes, L.stream, L.maxgrid);
-    launch(k_scan_down<W>, scan_tiles, L.boxes, L.stream, L.maxgrid);
+    launch(k_scan_down<W, PUB>, scan_tiles, L.boxes, L.stream, L.maxgrid);
 }
 
 // ZERO: launch k_sort_zero (the fused step zeroes the counts in an earlier kernel)
@@ -2320,8 +2332,8 @@ void phase_build(const Launch& L, hipEvent_t ev0, hipEvent_t evc, hipEvent_t ev1
     launch(k_build_prep, L.t_box, L.boxes, L.stream, L.maxgrid);
     phase_build_fused(L, ev0, evc, ev1, ev2);
 }
-// the build after save_build: bins, CSR count, offsets, fill and transpose,
-// then the state into the host mirror when `publish`
+// the build after save_build: bins, CSR count, offsets, fill and transpose;
+// with `publish` the offset scan writes the state into the host mirror
 static void build_rest(const Launch& L, hipEvent_t ev0, hipEvent_t evc, hipEvent_t ev1, hipEvent_t ev2, bool publish) {
     run_sort<SORT_BUILD, SCAN_CELLS_BUILD, false>(L, L.t_cell, L.t_atom, L.t_scan_cell);  // save_build zeroed
     if (ev0) (void)hipEventRecord(ev0, L.stream);
@@ -2329,7 +2341,8 @@ static void build_rest(const Launch& L, hipEvent_t ev0, hipEvent_t evc, hipEvent
     else if (L.csr_mode == 3) launch(k_csr_count_w, L.t_local1, L.boxes, L.stream, L.maxgrid);
     else launch(k_csr_count_g, L.t_local1, L.boxes, L.stream, L.maxgrid);
     if (evc) (void)hipEventRecord(evc, L.stream);
-    run_scan<SCAN_CSR>(L, L.t_scan_local);
+    if (publish) run_scan<SCAN_CSR, true>(L, L.t_scan_local);  // publishes the build's counts
+    else run_scan<SCAN_CSR>(L, L.t_scan_local);
     if (ev1) (void)hipEventRecord(ev1, L.stream);
     launch(k_csr_fill, L.t_local, L.boxes, L.stream, L.maxgrid);
     if (ev2) (void)hipEventRecord(ev2, L.stream);
@@ -2338,7 +2351,6 @@ static void build_rest(const Launch& L, hipEvent_t ev0, hipEvent_t evc, hipEvent
     if (L.csr_mode == 0) launch(k_tcount, L.t_local, L.boxes, L.stream, L.maxgrid);
     run_scan<SCAN_TRANSPOSE>(L, L.t_scan_atom);
     launch(k_tplace, L.t_local, L.boxes, L.stream, L.maxgrid);
-    if (publish) launch(k_publish, L.t_box, L.boxes, L.stream, L.maxgrid);
 }
 // the build after phase_sort_fused (which already did the bookkeeping)
 void phase_build_fused(const Launch& L, hipEvent_t ev0, hipEvent_t evc, hipEvent_t ev1, hipEvent_t ev2) {
