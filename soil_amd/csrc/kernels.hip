@@ -2101,64 +2101,106 @@ __device__ __forceinline__ long long fix_q(double f) {
     return __double2ll_rn(f * kFixScale);
 }
 
-// An atom's exact fixed-point force sum: its row (as the row atom) and its
-// incoming pairs (as the neighbour).
+// The consumer. Each atom's force is its row's pairs (as the row atom) and its
+// incoming pairs (as the neighbour): a wave's 64 atoms own one contiguous
+// range of row entries and one of incoming entries; each lane evaluates every
+// 64th pair of them (its atom found by a binary search over the wave's
+// starts in LDS) and adds the fixed-point force into the atom's LDS
+// accumulator. The sums are integers, so any evaluation order gives the same
+// bits, and the lanes stay busy whatever the rows' lengths.
 template <bool FAST>
-__device__ __forceinline__ void harmonic_sums(CBox& b, const BoxState* st, uint32_t a, long long& ax,
-                                              long long& ay, long long& az) {
+__device__ __forceinline__ void harmonic_wave(CBox& b, const BoxState* st, uint32_t a0, uint32_t lane, uint32_t last,
+                                              uint32_t* ls, const double4* pw, unsigned long long (*acc)[3]) {
     const double k = b.harmonic_k;
     const GMEM double4* xr = (const GMEM double4*)b.xr;
-    const double4 pa = xr[a];
-    if (a < st->nlocal) {
-        for (uint32_t e = b.offsets[a]; e < b.offsets[a + 1]; ++e) {
+    const uint32_t nl = st->nlocal, nall = nl + st->nghost;
+    const uint32_t a = a0 + lane;
+    auto owner = [&](uint32_t e) {
+        uint32_t r = 0;
+#pragma unroll
+        for (uint32_t step = 32; step > 0; step >>= 1)
+            if (r + step <= last && ls[r + step] <= e) r += step;
+        return r;
+    };
+    // rows (locals): -q to the row atom
+    if (a0 < nl) {
+        const uint32_t rend = b.offsets[a0 + last + 1u < nl ? a0 + last + 1u : nl];
+        ls[lane] = a < nl ? b.offsets[a] : rend;
+        wave_sync();
+        for (uint32_t e = ls[0] + lane; e < rend; e += 64u) {
+            const uint32_t r = owner(e);
             double fx, fy, fz;
-            pair_force(pa, xr[b.indices[e]], k, fx, fy, fz);
-            ax -= fix_q<FAST>(fx);
-            ay -= fix_q<FAST>(fy);
-            az -= fix_q<FAST>(fz);
+            pair_force(pw[r], xr[b.indices[e]], k, fx, fy, fz);
+            atomicAdd(&acc[r][0], (unsigned long long)(-fix_q<FAST>(fx)));
+            atomicAdd(&acc[r][1], (unsigned long long)(-fix_q<FAST>(fy)));
+            atomicAdd(&acc[r][2], (unsigned long long)(-fix_q<FAST>(fz)));
         }
+        wave_sync();
     }
-    for (uint32_t e = b.tstart[a]; e < b.tstart[a + 1]; ++e) {
+    // incoming pairs: +q to the neighbour, the row atom evaluated first
+    const uint32_t tend = b.tstart[a0 + last + 1u];
+    ls[lane] = a < nall ? b.tstart[a] : tend;
+    wave_sync();
+    for (uint32_t e = ls[0] + lane; e < tend; e += 64u) {
+        const uint32_t r = owner(e);
         double fx, fy, fz;
-        pair_force(xr[b.tsrc[e]], pa, k, fx, fy, fz);
-        ax += fix_q<FAST>(fx);
-        ay += fix_q<FAST>(fy);
-        az += fix_q<FAST>(fz);
+        pair_force(xr[b.tsrc[e]], pw[r], k, fx, fy, fz);
+        atomicAdd(&acc[r][0], (unsigned long long)fix_q<FAST>(fx));
+        atomicAdd(&acc[r][1], (unsigned long long)fix_q<FAST>(fy));
+        atomicAdd(&acc[r][2], (unsigned long long)fix_q<FAST>(fz));
     }
+    wave_sync();
 }
 
 // ASSIGN (fused step, forces were not zeroed): f = sum * 2^-32, which equals
 // 0 + sum * 2^-32 bit for bit; otherwise f += sum * 2^-32.
 template <bool ASSIGN>
 __global__ void __launch_bounds__(BLK) k_harmonic(KARGS) {
+    __shared__ uint32_t lstart[BLK / 64][64];
+    __shared__ double4 lpos[BLK / 64][64];
+    __shared__ unsigned long long lacc[BLK / 64][64][3];
     FOR_TILES(t) {
         const Tile tl = tiles[t];
         CBox& b = box_at(boxes, tl.box);
         if (!live(b) || !b.consumer) continue;
         const BoxState* st = b.st;
-        const uint32_t a = tl.start + threadIdx.x;
-        if (a >= st->nlocal + st->nghost) continue;
+        const uint32_t nall = st->nlocal + st->nghost;
+        const uint32_t wid = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+        const uint32_t a0 = tl.start + wid * 64u;
+        if (a0 >= nall) continue;  // wave-uniform; no block barrier in this kernel
+        const uint32_t a = a0 + lane;
+        const bool act = a < nall;
         if (st->rebuilds == 0 || (st->err & (ERR_GHOST_CAPACITY | ERR_PAIR_CAPACITY))) {
-            if (ASSIGN) {
+            if (ASSIGN && act) {
                 b.fx[a] = 0.0;
                 b.fy[a] = 0.0;
                 b.fz[a] = 0.0;
             }
             continue;
         }
-        long long ax = 0, ay = 0, az = 0;
-        if (b.fix_fast) harmonic_sums<true>(b, st, a, ax, ay, az);  // box-uniform
-        else harmonic_sums<false>(b, st, a, ax, ay, az);
-        const double inv = 1.0 / kFixScale;
-        if (ASSIGN) {
-            b.fx[a] = (double)ax * inv;
-            b.fy[a] = (double)ay * inv;
-            b.fz[a] = (double)az * inv;
-        } else {
-            b.fx[a] = b.fx[a] + (double)ax * inv;
-            b.fy[a] = b.fy[a] + (double)ay * inv;
-            b.fz[a] = b.fz[a] + (double)az * inv;
+        const uint32_t last = (nall - a0 < 64u ? nall - a0 : 64u) - 1u;
+        unsigned long long (*acc)[3] = lacc[wid];
+        acc[lane][0] = acc[lane][1] = acc[lane][2] = 0ull;
+        if (act) lpos[wid][lane] = ((const GMEM double4*)b.xr)[a];
+        wave_sync();
+        if (b.fix_fast) harmonic_wave<true>(b, st, a0, lane, last, lstart[wid], lpos[wid], acc);  // box-uniform
+        else harmonic_wave<false>(b, st, a0, lane, last, lstart[wid], lpos[wid], acc);
+        if (act) {
+            const double inv = 1.0 / kFixScale;
+            const double gx = (double)(long long)acc[lane][0] * inv;
+            const double gy = (double)(long long)acc[lane][1] * inv;
+            const double gz = (double)(long long)acc[lane][2] * inv;
+            if (ASSIGN) {
+                b.fx[a] = gx;
+                b.fy[a] = gy;
+                b.fz[a] = gz;
+            } else {
+                b.fx[a] = b.fx[a] + gx;
+                b.fy[a] = b.fy[a] + gy;
+                b.fz[a] = b.fz[a] + gz;
+            }
         }
+        wave_sync();  // the next tile rewrites the wave's LDS
     }
 }
 
