@@ -444,15 +444,6 @@ __global__ void __launch_bounds__(BLK) k_zero_counts(KARGS) {
     }
 }
 
-__global__ void k_publish(KARGS) {
-    FOR_TILES(t) {
-        if (threadIdx.x != 0) continue;
-        CBox& b = box_at(boxes, tiles[t].box);
-        if (!live(b)) continue;
-        publish(b);
-    }
-}
-
 // shrink_wrap's extents (domain.rs:531-548): min and max of each coordinate
 // over the locals, reduced as order-preserving u64 keys (negative values
 // bit-inverted, positive ones with the sign bit set) so that atomic min/max
@@ -956,7 +947,7 @@ __global__ void __launch_bounds__(BLK) k_scan_reduce(KARGS) {
 // here from k_scan_reduce's per-tile totals, a few loads per thread) to its
 // own exclusive scan; the first tile also sums them all for the CSR count.
 // PUB (the plain step's CSR offsets): the box's first tile also publishes the
-// build's state to the host mirror, active or not (k_publish's work).
+// build's state to the host mirror, active or not.
 template <int W, bool PUB = false>
 __global__ void __launch_bounds__(BLK) k_scan_down(KARGS) {
     __shared__ uint32_t lds[BLK / 64];
@@ -2351,11 +2342,6 @@ void phase_sort(const Launch& L) {
     launch(k_post_sort<false>, L.t_box, L.boxes, L.stream, L.maxgrid);
 }
 // sort + permute + (when rebuilding) the build's bookkeeping in one per-box kernel
-void phase_sort_fused(const Launch& L) {  // k_decide<true> did the pre-sort bookkeeping
-    run_sort<SORT_SPATIAL, SCAN_CELLS_SPATIAL, false>(L, L.t_cell, L.t_local, L.t_scan_cell);  // k_initial zeroed
-    launch(k_permute<true>, L.t_local, L.boxes, L.stream, L.maxgrid);
-    launch(k_post_sort<true>, L.t_box, L.boxes, L.stream, L.maxgrid);
-}
 void phase_pbc_sort_fused(const Launch& L) {
     launch(k_sort_count<SORT_SPATIAL, true>, L.t_local, L.boxes, L.stream, L.maxgrid);  // k_initial zeroed
     run_scan<SCAN_CELLS_SPATIAL>(L, L.t_scan_cell);
@@ -2394,7 +2380,7 @@ static void build_rest(const Launch& L, hipEvent_t ev0, hipEvent_t evc, hipEvent
     run_scan<SCAN_TRANSPOSE>(L, L.t_scan_atom);
     launch(k_tplace, L.t_local, L.boxes, L.stream, L.maxgrid);
 }
-// the build after phase_sort_fused (which already did the bookkeeping)
+// the build after phase_pbc_sort_fused (which already did the bookkeeping)
 void phase_build_fused(const Launch& L, hipEvent_t ev0, hipEvent_t evc, hipEvent_t ev1, hipEvent_t ev2) {
     launch(k_save_build, L.t_atom1, L.boxes, L.stream, L.maxgrid);
     build_rest(L, ev0, evc, ev1, ev2, false);
@@ -2418,7 +2404,6 @@ void phase_final(const Launch& L) { launch(k_final, L.t_local, L.boxes, L.stream
 void phase_ke(const Launch& L, bool use_mask) { launch(k_ke, L.t_local, L.boxes, L.stream, L.maxgrid, use_mask ? 1 : 0); }
 void phase_classify(const Launch& L) { launch(k_classify, L.t_local, L.boxes, L.stream, L.maxgrid); }
 void phase_zero_counts(const Launch& L) { launch(k_zero_counts, L.t_cell, L.boxes, L.stream, L.maxgrid); }
-void phase_publish(const Launch& L) { launch(k_publish, L.t_box, L.boxes, L.stream, L.maxgrid); }
 void phase_sync_view(const Launch& L) { launch(k_sync_view, L.t_box, L.boxes, L.stream, L.maxgrid); }
 void phase_step_end(const Launch& L) { launch(k_step_end<false>, L.t_box, L.boxes, L.stream, L.maxgrid); }
 void phase_initial_decide_pending(const Launch& L) {

@@ -50,23 +50,24 @@ void phase_sync_view(const Launch& L);              // force hooks' device views
 void phase_ke(const Launch& L, bool use_mask);      // per-tile kinetic-energy partials into BoxDev::red
 void phase_classify(const Launch& L);               // boundary flags of the local rows into the slot scratch
 void phase_zero_counts(const Launch& L);            // the spatial sort's counts for a regridded step
-void phase_publish(const Launch& L);                // state fields into the host mirror (k_decide<true> does it too)
 void phase_remove_ghosts(const Launch& L);
 void phase_set_rebuild_now(const Launch& L);
 void phase_zero(const Launch& L);
 void phase_zero_fused(const Launch& L);  // leaves forces of consumer boxes to phase_force_fused
 void phase_pbc(const Launch& L);
 void phase_sort(const Launch& L);
-void phase_sort_fused(const Launch& L);
-void phase_pbc_sort_fused(const Launch& L);  // phase_pbc + phase_sort_fused, pbc inside the sort's count
+// the fused step's pbc and spatial sort (pbc inside the sort's count; k_decide<true>
+// did the pre-sort bookkeeping), then the permute and the post-sort bookkeeping
+void phase_pbc_sort_fused(const Launch& L);
 void phase_borders(const Launch& L);
 void phase_forward(const Launch& L);
 // timing events (any may be null): ev0 before the CSR count, evc after it,
 // ev1 after the offset scan, ev2 after the fill
 void phase_build(const Launch& L, hipEvent_t ev0, hipEvent_t evc, hipEvent_t ev1, hipEvent_t ev2);
 void phase_build_fused(const Launch& L, hipEvent_t ev0, hipEvent_t evc, hipEvent_t ev1, hipEvent_t ev2);
-// phase_borders + phase_forward + phase_build_fused (+ phase_publish when
-// `publish`), with the ghost fill, forward refresh and save_build in one launch
+// phase_borders + phase_forward + phase_build_fused, with the ghost fill,
+// forward refresh and save_build in one launch; with `publish` the CSR offset
+// scan also writes the build's state to the host mirror
 void phase_rebuild_fused(const Launch& L, hipEvent_t ev0, hipEvent_t evc, hipEvent_t ev1, hipEvent_t ev2, bool publish);
 void phase_force(const Launch& L);
 void phase_force_fused(const Launch& L);  // assigns forces of consumer boxes
