@@ -537,12 +537,13 @@ int step_plain_loop(soil_hip_box* b, int nsteps, bool& pending_final) {
         }
         b->pub_expect += 1;  // k_decide publishes the decision
         pending_final = false;
-        // A box that rebuilt on its last two steps enqueues the rebuild path
-        // without waiting for this step's decision: each kernel on it acts on
-        // the device's decision (pbc on FullRebuild; the sorts, borders and
-        // build on rebuild_now; the forward refresh otherwise), so a step that
-        // does not rebuild runs it as the forward step.
-        const bool ahead = !shrink && !b->hook && b->rebuild_streak >= 2;
+        // A box whose step before last rebuilt enqueues the rebuild path
+        // without waiting for this step's decision (rebuilds come in runs, or
+        // every other step): each kernel on it acts on the device's decision
+        // (pbc on FullRebuild; the sorts, borders and build on rebuild_now;
+        // the forward refresh otherwise), so a step that does not rebuild runs
+        // it as the forward step.
+        const bool ahead = !shrink && !b->hook && (b->rebuild_hist & 2u) != 0;
         bool rebuild = true;
         if (!ahead) {
             rc = wait_mirror(b);
@@ -589,14 +590,14 @@ int step_plain_loop(soil_hip_box* b, int nsteps, bool& pending_final) {
                 rc = tail();
                 if (rc) return rc;
             }
-            b->rebuild_streak = b->h_st->rebuild_now ? b->rebuild_streak + 1 : 0;
+            b->rebuild_hist = ((b->rebuild_hist << 1) | (b->h_st->rebuild_now ? 1u : 0u)) & 3u;
             pending_final = true;  // launched with the next step's initial integration, or on return
             if (b->h_st->err) return check_err_bits(b->h_st->err);
         } else {
             phase_forward(b->L);  // k_decide<true> left rebuild_now = 0 and the sort flags cleared
             rc = tail();
             if (rc) return rc;
-            b->rebuild_streak = 0;
+            b->rebuild_hist = (b->rebuild_hist << 1) & 3u;
             pending_final = true;
         }
         b->ring_pos = (b->ring_pos + 1) % b->ring_len;

@@ -79,7 +79,7 @@ struct soil_hip_box {
     soil_hip_device_view* d_view = nullptr;  // device copy of the force hook's view
     soil_hip::HostMirror* h_mirror = nullptr;  // pinned, device-mapped (fields published by kernels)
     uint64_t pub_expect = 0;                   // publishes enqueued so far
-    uint32_t rebuild_streak = 0;               // consecutive steps that rebuilt (plain stepping)
+    uint32_t rebuild_hist = 0;                 // plain stepping: bit 0 the last step rebuilt, bit 1 the one before
     std::vector<uint8_t> ring_ok;              // device ring slot k holds h_ring[k]
     soil_hip::Geom* d_ring = nullptr;
     soil_hip::Geom* h_ring = nullptr;     // pinned
