@@ -193,7 +193,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--parts", default="plain,batched,cpu")
+    ap.add_argument("--parts", default="plain,batched,served,cpu")
     ap.add_argument("--batched-steps", type=int, default=200)
     ap.add_argument("--plain-ensemble-steps", type=int, default=10)
     ap.add_argument("--probe-copy-mb", type=int, default=0,
