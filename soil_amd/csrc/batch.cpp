@@ -29,7 +29,8 @@ static int batch_tiles(soil_hip_batch* bt) {
     TileTables tt;
     for (size_t k = 0; k < bt->boxes.size(); ++k) build_tiles(tt, (uint32_t)k, bt->boxes[k]->caps);
     uint32_t maxgrid = bt->L.maxgrid ? bt->L.maxgrid : 4096;
-    upload_tiles(tt, bt->L, &bt->arena_tiles);
+    const int rc = upload_tiles(tt, bt->L, &bt->arena_tiles);
+    if (rc) return rc;
     bt->L.boxes = bt->d_ptrs;
     bt->L.stream = bt->stream;
     bt->L.maxgrid = maxgrid;
@@ -71,8 +72,8 @@ static int launch_step(soil_hip_batch* bt) {
 }
 
 static void drop_graph(soil_hip_batch* bt) {
-    if (bt->graph_exec) hipGraphExecDestroy(bt->graph_exec);
-    if (bt->graph) hipGraphDestroy(bt->graph);
+    if (bt->graph_exec) (void)hipGraphExecDestroy(bt->graph_exec);
+    if (bt->graph) (void)hipGraphDestroy(bt->graph);
     bt->graph_exec = nullptr;
     bt->graph = nullptr;
 }
@@ -157,7 +158,8 @@ static int batch_run(soil_hip_batch* bt, int nsteps, bool use_graph) {
             }
         }
         if (regrid) {
-            batch_tiles(bt);
+            int rt = batch_tiles(bt);
+            if (rt) return rt;
             if (bt->graph_exec) {
                 int rc = capture_graph(bt);
                 if (rc) return rc;
@@ -246,7 +248,11 @@ int batch_make(soil_hip_ctx* ctx, soil_hip_box** boxes, int nboxes, soil_hip_bat
     HIPCHK(hipMemcpy(bt->d_ptrs, ptrs.data(), sizeof(BoxDev*) * nb, hipMemcpyHostToDevice));
     bt->L.maxgrid = 4096;
     for (int d = 0; d < 3; ++d) bt->L.rev_split[d] = 0;  // raised by batch_run when a box needs it
-    batch_tiles(bt);
+    const int rt = batch_tiles(bt);
+    if (rt) {
+        batch_free(bt, true);
+        return rt;
+    }
     *out = bt;
     return 0;
 }
@@ -255,7 +261,7 @@ int batch_make(soil_hip_ctx* ctx, soil_hip_box** boxes, int nboxes, soil_hip_bat
 // can be stepped alone again.
 int batch_free(soil_hip_batch* bt, bool restore) {
     if (!bt) return 0;
-    hipStreamSynchronize(bt->stream);
+    (void)hipStreamSynchronize(bt->stream);  // best effort: the batch goes away either way
     drop_graph(bt);
     for (size_t k = 0; k < bt->boxes.size(); ++k) {
         soil_hip_box* b = bt->boxes[k];
@@ -266,22 +272,22 @@ int batch_free(soil_hip_batch* bt, bool restore) {
             push_desc(b);
             b->ring_pos = 0;
             b->h_st->ring_pos = 0;
-            hipMemcpy(&b->d_st->ring_pos, &b->h_st->ring_pos, 4, hipMemcpyHostToDevice);
-            hipMemcpy(b->d_ring, &b->last_geom, sizeof(Geom), hipMemcpyHostToDevice);
+            (void)hipMemcpy(&b->d_st->ring_pos, &b->h_st->ring_pos, 4, hipMemcpyHostToDevice);
+            (void)hipMemcpy(b->d_ring, &b->last_geom, sizeof(Geom), hipMemcpyHostToDevice);
             b->ring_ok.assign(b->ring_ok.size(), 0);  // slot 0 no longer holds h_ring[0]
-            rebuild_tiles(b);
+            (void)rebuild_tiles(b);
         }
         b->in_batch = false;
     }
-    if (bt->arena_tiles) hipFree(bt->arena_tiles);
-    if (bt->d_ptrs) hipFree(bt->d_ptrs);
-    if (bt->d_ring) hipFree(bt->d_ring);
-    if (bt->h_ring) hipHostFree(bt->h_ring);
+    if (bt->arena_tiles) (void)hipFree(bt->arena_tiles);
+    if (bt->d_ptrs) (void)hipFree(bt->d_ptrs);
+    if (bt->d_ring) (void)hipFree(bt->d_ring);
+    if (bt->h_ring) (void)hipHostFree(bt->h_ring);
     for (auto e : bt->half_done)
-        if (e) hipEventDestroy(e);
-    if (bt->t0) hipEventDestroy(bt->t0);
-    if (bt->t1) hipEventDestroy(bt->t1);
-    if (bt->stream) hipStreamDestroy(bt->stream);
+        if (e) (void)hipEventDestroy(e);
+    if (bt->t0) (void)hipEventDestroy(bt->t0);
+    if (bt->t1) (void)hipEventDestroy(bt->t1);
+    if (bt->stream) (void)hipStreamDestroy(bt->stream);
     delete bt;
     return 0;
 }
@@ -320,7 +326,7 @@ int soil_hip_batch_sync(soil_hip_batch* bt) {
     if (rc) return rc;
     if (grown) {
         drop_graph(bt);
-        batch_tiles(bt);
+        return batch_tiles(bt);
     }
     return 0;
 }
@@ -413,8 +419,8 @@ int soil_hip_batch_autotune(soil_hip_batch* bt) {
         }
     }
     for (auto& s : snaps) {
-        hipFree(s.atoms);
-        hipFree(s.pairs);
+        (void)hipFree(s.atoms);
+        (void)hipFree(s.pairs);
     }
     bt->L.maxgrid = best;
     drop_graph(bt);

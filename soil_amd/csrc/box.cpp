@@ -61,25 +61,28 @@ void build_tiles(TileTables& tt, uint32_t b, const Caps& c) {
 
 // Tiles of one space are interleaved across boxes in XCD-friendly order: the
 // tables keep box-major order; kernels stride over them with the grid.
-void upload_tiles(const TileTables& tt, Launch& L, void** arena) {
+int upload_tiles(const TileTables& tt, Launch& L, void** arena) {
     const std::vector<Tile>* lists[11] = {&tt.box,   &tt.local,  &tt.local1,    &tt.atom,       &tt.atom1,   &tt.ghost,
                                           &tt.cell,  &tt.border, &tt.scan_cell, &tt.scan_local, &tt.scan_atom};
     TileSet* sets[11] = {&L.t_box,  &L.t_local,  &L.t_local1,    &L.t_atom,       &L.t_atom1,   &L.t_ghost,
                          &L.t_cell, &L.t_border, &L.t_scan_cell, &L.t_scan_local, &L.t_scan_atom};
     size_t total = 0;
     for (auto* l : lists) total += l->size();
-    if (*arena) hipFree(*arena);
+    if (*arena) (void)hipFree(*arena);
     *arena = nullptr;
-    if (total == 0) return;
-    hipMalloc(arena, total * sizeof(Tile));
+    for (auto* s : sets) *s = TileSet{};
+    if (total == 0) return 0;
+    HIPCHK(hipMalloc(arena, total * sizeof(Tile)));
     Tile* base = (Tile*)*arena;
     size_t off = 0;
     for (int k = 0; k < 11; ++k) {
+        if (!lists[k]->empty())
+            HIPCHK(hipMemcpy(base + off, lists[k]->data(), lists[k]->size() * sizeof(Tile), hipMemcpyHostToDevice));
         sets[k]->d = base + off;
         sets[k]->n = (uint32_t)lists[k]->size();
-        if (!lists[k]->empty()) hipMemcpy(base + off, lists[k]->data(), lists[k]->size() * sizeof(Tile), hipMemcpyHostToDevice);
         off += lists[k]->size();
     }
+    return 0;
 }
 
 }  // namespace soil_hip
@@ -137,7 +140,7 @@ void carve_pairs(Carver& c, BoxDev& d, const Caps& k) {
 }
 
 int alloc_arena(void** arena, size_t bytes) {
-    if (*arena) hipFree(*arena);
+    if (*arena) (void)hipFree(*arena);
     *arena = nullptr;
     hipError_t e = hipMalloc(arena, bytes);
     if (e != hipSuccess) return set_error(SOIL_HIP_EDEVICE, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
@@ -197,7 +200,8 @@ int wait_mirror(soil_hip_box* b) {
 int rebuild_tiles(soil_hip_box* b) {
     TileTables tt;
     build_tiles(tt, 0, b->caps);
-    upload_tiles(tt, b->L, &b->arena_tiles);
+    const int rc = upload_tiles(tt, b->L, &b->arena_tiles);
+    if (rc) return rc;
     b->L.boxes = b->d_ptrs;
     b->L.stream = b->stream;
     b->L.csr_mode = b->h_dev.csr_mode;
@@ -223,8 +227,9 @@ int alloc_atoms(soil_hip_box* b, const Caps& nc, bool keep) {
         const BoxDev& od = b->h_dev;
         const size_t nl = b->nlocal;
         const size_t n = std::min<size_t>((size_t)b->h_st->nlocal + b->h_st->nghost, nc.atoms);
+        hipError_t ce = hipSuccess;  // the first failed copy, reported after them all
         auto cp = [&](void* dst, const void* src, size_t bytes) {
-            if (bytes) hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, b->stream);
+            if (bytes && ce == hipSuccess) ce = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, b->stream);
         };
         cp(nd.x, od.x, n * 8); cp(nd.y, od.y, n * 8); cp(nd.z, od.z, n * 8);
         cp(nd.vx, od.vx, n * 8); cp(nd.vy, od.vy, n * 8); cp(nd.vz, od.vz, n * 8);
@@ -248,12 +253,16 @@ int alloc_atoms(soil_hip_box* b, const Caps& nc, bool keep) {
                                   hipMemcpyHostToDevice, b->stream));
         }
         HIPCHK(hipStreamSynchronize(b->stream));
+        if (ce != hipSuccess) {
+            (void)hipFree(arena);
+            return set_error(SOIL_HIP_EDEVICE, "arena copy: %s", hipGetErrorString(ce));
+        }
     }
     nd.cap_local = nc.local;
     nd.cap_atoms = nc.atoms;
     nd.cap_cells = nc.cells;
     nd.cap_tiles = nc.tiles;
-    if (b->arena_atoms) hipFree(b->arena_atoms);
+    if (b->arena_atoms) (void)hipFree(b->arena_atoms);
     b->arena_atoms = arena;
     b->h_dev = nd;
     b->caps.local = nc.local;
@@ -279,12 +288,12 @@ int alloc_pairs(soil_hip_box* b, uint32_t cap_pairs, bool keep) {
         if (rc) return rc;
         const size_t np = std::min<size_t>(b->h_st->npairs, cap_pairs);
         if (np) {
-            hipMemcpyAsync(nd.indices, b->h_dev.indices, np * 4, hipMemcpyDeviceToDevice, b->stream);
-            hipMemcpyAsync(nd.tsrc, b->h_dev.tsrc, np * 4, hipMemcpyDeviceToDevice, b->stream);
+            HIPCHK(hipMemcpyAsync(nd.indices, b->h_dev.indices, np * 4, hipMemcpyDeviceToDevice, b->stream));
+            HIPCHK(hipMemcpyAsync(nd.tsrc, b->h_dev.tsrc, np * 4, hipMemcpyDeviceToDevice, b->stream));
         }
         HIPCHK(hipStreamSynchronize(b->stream));
     }
-    if (b->arena_pairs) hipFree(b->arena_pairs);
+    if (b->arena_pairs) (void)hipFree(b->arena_pairs);
     b->arena_pairs = arena;
     b->h_dev.indices = nd.indices;
     b->h_dev.ttmp = nd.ttmp;
@@ -343,9 +352,9 @@ void resolve_timing(soil_hip_box* b) {
     for (auto& p : b->pending) {
         if (hipEventQuery(p.e2) == hipSuccess) {
             float ms_all = 0.f, ms_fill = 0.f, ms_count = 0.f;
-            hipEventElapsedTime(&ms_all, p.e0, p.e2);
-            hipEventElapsedTime(&ms_fill, p.e1, p.e2);
-            hipEventElapsedTime(&ms_count, p.e0, p.ec);
+            (void)hipEventElapsedTime(&ms_all, p.e0, p.e2);
+            (void)hipEventElapsedTime(&ms_fill, p.e1, p.e2);
+            (void)hipEventElapsedTime(&ms_count, p.e0, p.ec);
             b->t_builds += 1.0;
             b->t_csr_ms += ms_all;
             b->t_fill_ms += ms_fill;
@@ -369,8 +378,8 @@ hipEvent_t take_event(soil_hip_box* b) {
     }
     // timing only: no system-scope fence at record (it would write back caches
     // in the middle of the step being timed)
-    hipEvent_t e;
-    hipEventCreateWithFlags(&e, hipEventDisableSystemFence);
+    hipEvent_t e = nullptr;
+    if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return nullptr;  // that build goes untimed
     return e;
 }
 
@@ -383,6 +392,11 @@ hipEvent_t take_event(soil_hip_box* b) {
 static void rebuild_launch(soil_hip_box* b, hipEvent_t ev[4]) {
     const bool timed = b->timing && (b->timing_ctr++ % b->timing) == 0;
     for (int k = 0; k < 4; ++k) ev[k] = timed ? take_event(b) : nullptr;
+    if (timed && !(ev[0] && ev[1] && ev[2] && ev[3])) {  // an event could not be made: this build goes untimed
+        for (int k = 0; k < 4; ++k)
+            if (ev[k]) b->ev_pool.push_back(ev[k]);
+        for (int k = 0; k < 4; ++k) ev[k] = nullptr;
+    }
     // borders, ghosts, build (count, scan flags the pair capacity, fill), publish
     phase_rebuild_fused(b->L, ev[0], ev[1], ev[2], ev[3], true);
     b->pub_expect += 1;
@@ -411,8 +425,8 @@ static int rebuild_check(soil_hip_box* b, hipEvent_t ev[4], bool& redone) {
         b->h_st->err &= ~ERR_GHOST_CAPACITY;
         HIPCHK(hipMemcpy(&b->d_st->err, &b->h_st->err, 4, hipMemcpyHostToDevice));
         rc = push_desc(b);
+        if (!rc) rc = rebuild_tiles(b);
         if (rc) return rc;
-        rebuild_tiles(b);
         phase_borders(b->L);
         phase_forward(b->L);
         phase_build_fused(b->L, nullptr, nullptr, nullptr, nullptr);
@@ -428,8 +442,8 @@ static int rebuild_check(soil_hip_box* b, hipEvent_t ev[4], bool& redone) {
         if (rc) return rc;
         b->h_st->err &= ~ERR_PAIR_CAPACITY;
         rc = push_desc(b);
+        if (!rc) rc = rebuild_tiles(b);
         if (rc) return rc;
-        rebuild_tiles(b);
         // rebuilds was counted once by build_prep; the redo re-runs the whole build,
         // so compensate the counter first
         b->h_st->rebuilds -= 1;
@@ -680,25 +694,25 @@ int soil_hip_box_create(soil_hip_ctx* ctx, const soil_hip_box_desc* desc, soil_h
 
 int soil_hip_box_destroy(soil_hip_box* b) {
     if (!b) return 0;
-    if (b->stream) hipStreamSynchronize(b->stream);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
     for (auto& p : b->pending) {
         b->ev_pool.push_back(p.e0);
         b->ev_pool.push_back(p.e1);
         b->ev_pool.push_back(p.e2);
     }
-    for (auto e : b->ev_pool) hipEventDestroy(e);
-    if (b->arena_atoms) hipFree(b->arena_atoms);
-    if (b->arena_pairs) hipFree(b->arena_pairs);
-    if (b->arena_tiles) hipFree(b->arena_tiles);
-    if (b->d_dev) hipFree(b->d_dev);
-    if (b->d_ptrs) hipFree(b->d_ptrs);
-    if (b->d_st) hipFree(b->d_st);
-    if (b->d_view) hipFree(b->d_view);
-    if (b->d_ring) hipFree(b->d_ring);
-    if (b->h_st) hipHostFree(b->h_st);
-    if (b->h_mirror) hipHostFree(b->h_mirror);
-    if (b->h_ring) hipHostFree(b->h_ring);
-    if (b->stream) hipStreamDestroy(b->stream);
+    for (auto e : b->ev_pool) (void)hipEventDestroy(e);
+    if (b->arena_atoms) (void)hipFree(b->arena_atoms);
+    if (b->arena_pairs) (void)hipFree(b->arena_pairs);
+    if (b->arena_tiles) (void)hipFree(b->arena_tiles);
+    if (b->d_dev) (void)hipFree(b->d_dev);
+    if (b->d_ptrs) (void)hipFree(b->d_ptrs);
+    if (b->d_st) (void)hipFree(b->d_st);
+    if (b->d_view) (void)hipFree(b->d_view);
+    if (b->d_ring) (void)hipFree(b->d_ring);
+    if (b->h_st) (void)hipHostFree(b->h_st);
+    if (b->h_mirror) (void)hipHostFree(b->h_mirror);
+    if (b->h_ring) (void)hipHostFree(b->h_ring);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return 0;
 }
@@ -850,8 +864,8 @@ int soil_hip_setup(soil_hip_box* b) {
     b->ring_pos = 0;
     b->L.maxgrid = 4096;
     rc = push_desc(b);
+    if (!rc) rc = rebuild_tiles(b);
     if (rc) return rc;
-    rebuild_tiles(b);
     b->setup_done = true;
     rc = upload_geom_now(b);
     if (rc) return rc;
@@ -1136,8 +1150,8 @@ int soil_hip_bin_neighbor_list(soil_hip_box* b) {
         b->h_st->state = 0;
         HIPCHK(hipMemcpy(b->d_st, b->h_st, sizeof(BoxState), hipMemcpyHostToDevice));
         rc = push_desc(b);
+        if (!rc) rc = rebuild_tiles(b);
         if (rc) return rc;
-        rebuild_tiles(b);
     }
     return check_err_bits(b->h_st->err);
 }
@@ -1488,10 +1502,10 @@ int soil_hip_probe_copy(soil_hip_ctx* ctx, size_t bytes, int reps, double* ms) {
     float t = 0.f;
     HIPCHK(hipEventElapsedTime(&t, e0, e1));
     if (ms) *ms = t / reps;
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
-    hipFree(a);
-    hipFree(b);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
     return 0;
 }
 

@@ -30,7 +30,7 @@ struct Caps {
 };
 
 void build_tiles(TileTables& tt, uint32_t box_index, const Caps& c);
-void upload_tiles(const TileTables& tt, Launch& L, void** arena);
+int upload_tiles(const TileTables& tt, Launch& L, void** arena);
 
 }  // namespace soil_hip
 

@@ -278,10 +278,13 @@ void fail(soil_hip_server* sv, int rc) {
 }
 
 void loop(soil_hip_server* sv) {
-    hipSetDevice(sv->ctx->device);
-    hipEvent_t e0, e1;
-    hipEventCreate(&e0);
-    hipEventCreate(&e1);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipSetDevice(sv->ctx->device) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess) {
+        fail(sv, set_error(SOIL_HIP_EDEVICE, "server: device %d set-up failed", sv->ctx->device));
+        if (e0) (void)hipEventDestroy(e0);
+        return;
+    }
     uint32_t* d_col = nullptr;
     uint32_t* h_col = nullptr;
     size_t col_cap = 0;
@@ -304,11 +307,16 @@ void loop(soil_hip_server* sv) {
             if (rc) { fail(sv, rc); break; }
             const size_t need = 4 * sv->active.size();
             if (need > col_cap) {
-                if (d_col) hipFree(d_col);
-                if (h_col) hipHostFree(h_col);
+                if (d_col) (void)hipFree(d_col);
+                if (h_col) (void)hipHostFree(h_col);
+                d_col = nullptr;
+                h_col = nullptr;
                 col_cap = need * 2;
-                hipMalloc(&d_col, col_cap * 4);
-                hipHostMalloc(&h_col, col_cap * 4, hipHostMallocDefault);
+                if (hipMalloc(&d_col, col_cap * 4) != hipSuccess ||
+                    hipHostMalloc(&h_col, col_cap * 4, hipHostMallocDefault) != hipSuccess) {
+                    fail(sv, set_error(SOIL_HIP_EDEVICE, "server: state read-back buffers (%zu B)", col_cap * 4));
+                    break;
+                }
             }
         }
         // a chunk ends when the first active job reaches its step count
@@ -319,18 +327,25 @@ void loop(soil_hip_server* sv) {
             nparticles += (double)j->n;
         }
         const int n = std::max(1, std::min(sv->chunk, left));
-        hipEventRecord(e0, sv->batch->stream);
+        (void)hipEventRecord(e0, sv->batch->stream);  // timing only
         rc = batch_step(sv->batch, n, sv->use_graph);
         if (rc) { fail(sv, rc); break; }
         phase_collect(sv->batch->L, d_col);
-        hipMemcpyAsync(h_col, d_col, 4 * sv->active.size() * 4, hipMemcpyDeviceToHost, sv->batch->stream);
-        hipEventRecord(e1, sv->batch->stream);
+        if (hipMemcpyAsync(h_col, d_col, 4 * sv->active.size() * 4, hipMemcpyDeviceToHost, sv->batch->stream) !=
+                hipSuccess ||
+            hipEventRecord(e1, sv->batch->stream) != hipSuccess) {
+            fail(sv, set_error(SOIL_HIP_EDEVICE, "server: state read-back after a chunk"));
+            break;
+        }
         // overlaps the chunk: admit queued jobs and set their boxes up
         admit(sv, incoming);
-        hipEventSynchronize(e1);
+        if (hipEventSynchronize(e1) != hipSuccess) {
+            fail(sv, set_error(SOIL_HIP_EDEVICE, "server: chunk did not complete"));
+            break;
+        }
         sv->batch->half_pending[0] = sv->batch->half_pending[1] = false;
         float ms = 0.f;
-        hipEventElapsedTime(&ms, e0, e1);
+        (void)hipEventElapsedTime(&ms, e0, e1);
         sv->busy_s += ms * 1e-3;
         sv->server_steps += (uint64_t)n;
         sv->chunks += 1;
@@ -353,10 +368,10 @@ void loop(soil_hip_server* sv) {
         batch_free(sv->batch, false);
         sv->batch = nullptr;
     }
-    if (d_col) hipFree(d_col);
-    if (h_col) hipHostFree(h_col);
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    if (d_col) (void)hipFree(d_col);
+    if (h_col) (void)hipHostFree(h_col);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
 }
 
 }  // namespace
