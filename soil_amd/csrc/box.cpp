@@ -273,6 +273,9 @@ int alloc_atoms(soil_hip_box* b, const Caps& nc, bool keep) {
 }
 
 int alloc_pairs(soil_hip_box* b, uint32_t cap_pairs, bool keep) {
+    // every growth keeps the row-slot width (the slots are cap_pairs / cap_local)
+    cap_pairs = (uint32_t)std::min<uint64_t>(
+        std::max<uint64_t>(cap_pairs, (uint64_t)b->row_slots * b->caps.local), 0xFFFFFFF0ull);
     Caps nc = b->caps;
     nc.pairs = cap_pairs;
     BoxDev nd = b->h_dev;
@@ -806,7 +809,12 @@ int soil_hip_setup(soil_hip_box* b) {
     b->ring_ok.assign(b->ring_len, 0);
     int rc = alloc_atoms(b, c, false);
     if (rc) return rc;
-    rc = alloc_pairs(b, (uint32_t)std::min<uint64_t>(std::max<uint64_t>(8 * (uint64_t)n, 4096), 0xFFFFFFF0u), false);
+    {
+        const char* m = getenv("SOIL_HIP_ROW_SLOTS");  // bit-neutral tuning knob
+        const int s = m ? atoi(m) : 12;
+        b->row_slots = (uint32_t)(s < 4 ? 4 : (s > 64 ? 64 : s));
+    }
+    rc = alloc_pairs(b, 4096, false);  // raised to row_slots per local
     if (rc) return rc;
     BoxDev& d = b->h_dev;
     // locals (the primary buffers are the ones named without the a- prefix)

@@ -71,6 +71,11 @@ struct soil_hip_box {
     void* arena_atoms = nullptr;
     void* arena_pairs = nullptr;
     void* arena_tiles = nullptr;
+    // entries per local row in the pair arena's row slots: the one-pass CSR
+    // count stores a row's first row_slots pairs there and the fill walks a
+    // longer row again on one lane, so the width sets how rarely a wave waits
+    // on such a lane (1M spheres at phi 0.5: 10^4 rows exceed 8, 254 exceed 12)
+    uint32_t row_slots = 12;
     soil_hip::BoxDev h_dev{};        // host mirror of the device descriptor
     soil_hip::BoxDev* d_dev = nullptr;
     soil_hip::BoxDev** d_ptrs = nullptr;  // one-entry pointer array for single-box launches
