@@ -200,7 +200,7 @@ def main():
                     help="also run the calibration copy kernel over this many MB (traffic counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (each box or job shards "
+        print("bench.py measures one GPU: the library has no multi-GPU path (each box or job shards "
               "to one device); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     parts = set(args.parts.split(","))
