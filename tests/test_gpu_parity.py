@@ -362,3 +362,18 @@ def test_compute_ke_matches_oracle(ctx):
         assert kr > 0 and abs(kg - kr) <= 1e-12 * kr, (kg, kr)
     assert g.compute_ke(np.zeros(n, dtype=bool)) == 0.0
     assert g.compute_ke() == g.compute_ke()  # deterministic
+
+
+@pytest.mark.parametrize("slots", [4, 6])
+def test_narrow_row_slots_take_the_overflow_path(ctx, monkeypatch, slots):
+    # The one-pass CSR count keeps a row's first `slots` entries; a longer row
+    # is walked again by the fill, which also gives its entries transpose
+    # slots. Narrow slots (read at set-up) send most rows of a dense box down
+    # that path; the result must not change.
+    print(f"case: row slots {slots}", flush=True)
+    monkeypatch.setenv("SOIL_HIP_ROW_SLOTS", str(slots))
+    g, r = run_compare(ctx, small(7300 + slots, 4000, 0.4, 0.6, 0.55), 40, 8, k=1000.0)
+    off, _ = r.csr()
+    long_rows = int((np.diff(off.astype(np.int64)) > slots).sum())
+    print(f"rows longer than {slots}: {long_rows}", flush=True)
+    assert long_rows > 20
