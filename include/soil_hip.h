@@ -105,7 +105,11 @@ int soil_hip_upload_column(soil_hip_box* box, int col, const double* data, size_
 
 /* Setup systems: neighbor_setup (neighbor.rs:604-665), comm_setup
  * (comm.rs:253-283) and setup_deform (soil_deform/src/lib.rs:346-420).
- * Sizes device buffers for the uploaded atoms. */
+ * Sizes device buffers for the uploaded atoms. Two environment variables,
+ * read here, tune the CSR build without changing its output:
+ * SOIL_HIP_CSR_MODE (0-3, default 3: one-pass count with wave-staged
+ * candidates) and SOIL_HIP_ROW_SLOTS (4-64, default 12: pair-arena entries
+ * kept per local row by the one-pass count; longer rows are walked again). */
 int soil_hip_setup(soil_hip_box* box);
 
 /* [domain] boundary of one axis (domain.rs:27-34): 0 periodic, 1 fixed,
