@@ -211,6 +211,10 @@ int batch_make(soil_hip_ctx* ctx, soil_hip_box** boxes, int nboxes, soil_hip_bat
                              "box %d has shrink-wrap boundaries, whose bounds need the host every step: step it alone", k);
         for (int j = 0; j < k; ++j)
             if (boxes[j] == boxes[k]) return set_error(SOIL_HIP_EINVAL, "box %d listed twice", k);
+        // one CSR count kernel serves the whole batch, and each box's fill
+        // reads what its own strategy's count leaves behind
+        if (boxes[k]->h_dev.csr_mode != boxes[0]->h_dev.csr_mode)
+            return set_error(SOIL_HIP_EINVAL, "box %d was set up with another SOIL_HIP_CSR_MODE than box 0", k);
     }
     HIPCHK(hipSetDevice(ctx->device));
     soil_hip_batch* bt = new soil_hip_batch();

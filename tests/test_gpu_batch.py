@@ -102,3 +102,22 @@ def test_batch_axis_deform_grows_grid(ctx):
     assert pairs[0][0].grid()[:3].prod() > cells0
     for i, (g, r) in enumerate(pairs):
         assert_same(g, r, what=f"deform box {i}")
+
+
+def test_batch_refuses_mixed_csr_modes(ctx, monkeypatch):
+    # one count kernel serves a batch: boxes set up under different
+    # SOIL_HIP_CSR_MODE values are refused, and each still steps alone
+    print("case: batch of mixed CSR modes", flush=True)
+    boxes = []
+    for k, mode in enumerate(("3", "2")):
+        monkeypatch.setenv("SOIL_HIP_CSR_MODE", mode)
+        d, pos, vel, mass, rad = soil_amd.gen_custom(9100 + k, 600, 0.4, 0.6, 0.3, (1, 1, 1), 0, 0.0, 0.0, 0.1, 0.0)
+        b = soil_amd.Box(ctx, d)
+        b.upload(pos, mass, rad, vel=vel)
+        b.setup()
+        boxes.append(b)
+    with pytest.raises(RuntimeError, match="SOIL_HIP_CSR_MODE"):
+        soil_amd.Batch(ctx, boxes)
+    for b in boxes:
+        b.step(2)
+        b.close()
