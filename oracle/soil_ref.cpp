@@ -34,6 +34,11 @@
 #include <algorithm>
 #include <cfloat>
 
+// memcpy that accepts an empty range (an empty vector's data() may be null)
+static inline void copy_bytes(void* dst, const void* src, size_t bytes) {
+    if (bytes) std::memcpy(dst, src, bytes);
+}
+
 namespace {
 
 enum CommState { FullRebuild = 0, CommunicateOnly = 1 };
@@ -1260,14 +1265,14 @@ int soil_ref_register_column(soil_ref_box* b, int width, unsigned flags) {
 int soil_ref_set_column(soil_ref_box* b, int col, const double* data, size_t n) {
     Column& c = b->cols[col];
     if (n * c.width > c.data.size()) return -1;
-    std::memcpy(c.data.data(), data, n * c.width * sizeof(double));
+    copy_bytes(c.data.data(), data, n * c.width * sizeof(double));
     return 0;
 }
 
 size_t soil_ref_get_column(soil_ref_box* b, int col, double* out, size_t cap) {
     Column& c = b->cols[col];
     size_t n = std::min(cap, c.data.size());
-    if (out) std::memcpy(out, c.data.data(), n * sizeof(double));
+    if (out) copy_bytes(out, c.data.data(), n * sizeof(double));
     return c.data.size();
 }
 
@@ -1421,9 +1426,9 @@ void soil_ref_classify_boundary_atoms(soil_ref_box* b, uint8_t* out) {
 
 size_t soil_ref_csr(soil_ref_box* b, uint32_t* offsets, uint32_t* indices, size_t cap) {
     const Neighbor& nb = b->neighbor;
-    if (offsets) std::memcpy(offsets, nb.neighbor_offsets.data(), nb.neighbor_offsets.size() * 4);
+    if (offsets) copy_bytes(offsets, nb.neighbor_offsets.data(), nb.neighbor_offsets.size() * 4);
     size_t P = nb.neighbor_indices.size();
-    if (indices && cap >= P) std::memcpy(indices, nb.neighbor_indices.data(), P * 4);
+    if (indices && cap >= P) copy_bytes(indices, nb.neighbor_indices.data(), P * 4);
     return P;
 }
 
@@ -1431,10 +1436,10 @@ size_t soil_ref_csr(soil_ref_box* b, uint32_t* offsets, uint32_t* indices, size_
 void soil_ref_bins(soil_ref_box* b, uint32_t* bin_start, uint32_t* sorted_atoms, uint32_t* sorted_idx,
                    uint32_t* atom_cell) {
     const Neighbor& nb = b->neighbor;
-    if (bin_start) std::memcpy(bin_start, nb.bin_start.data(), nb.bin_start.size() * 4);
-    if (sorted_atoms) std::memcpy(sorted_atoms, nb.bin_sorted_atoms.data(), nb.bin_sorted_atoms.size() * 4);
-    if (sorted_idx) std::memcpy(sorted_idx, nb.bin_atom_sorted_idx.data(), nb.bin_atom_sorted_idx.size() * 4);
-    if (atom_cell) std::memcpy(atom_cell, nb.bin_atom_cell.data(), nb.bin_atom_cell.size() * 4);
+    if (bin_start) copy_bytes(bin_start, nb.bin_start.data(), nb.bin_start.size() * 4);
+    if (sorted_atoms) copy_bytes(sorted_atoms, nb.bin_sorted_atoms.data(), nb.bin_sorted_atoms.size() * 4);
+    if (sorted_idx) copy_bytes(sorted_idx, nb.bin_atom_sorted_idx.data(), nb.bin_atom_sorted_idx.size() * 4);
+    if (atom_cell) copy_bytes(atom_cell, nb.bin_atom_cell.data(), nb.bin_atom_cell.size() * 4);
 }
 
 size_t soil_ref_bins_size(soil_ref_box* b) { return b->neighbor.bin_start.size(); }
@@ -1443,7 +1448,7 @@ size_t soil_ref_bins_size(soil_ref_box* b) { return b->neighbor.bin_start.size()
 size_t soil_ref_sort_perm(soil_ref_box* b, uint64_t* out) {
     if (!b->neighbor.last_sort_ran) return 0;
     const auto& p = b->neighbor.last_sort_perm;
-    if (out) std::memcpy(out, p.data(), p.size() * 8);
+    if (out) copy_bytes(out, p.data(), p.size() * 8);
     return p.size();
 }
 
@@ -1452,15 +1457,15 @@ void soil_ref_atoms(soil_ref_box* b, double* pos, double* vel, double* force, do
                     double* inv_mass, uint32_t* tag, int32_t* origin, int32_t* image) {
     const Atom& a = b->atoms;
     size_t n = a.len();
-    if (pos) std::memcpy(pos, a.pos.data(), 3 * n * 8);
-    if (vel) std::memcpy(vel, a.vel.data(), 3 * n * 8);
-    if (force) std::memcpy(force, a.force.data(), 3 * n * 8);
-    if (radius) std::memcpy(radius, a.cutoff_radius.data(), n * 8);
-    if (mass) std::memcpy(mass, a.mass.data(), n * 8);
-    if (inv_mass) std::memcpy(inv_mass, a.inv_mass.data(), n * 8);
-    if (tag) std::memcpy(tag, a.tag.data(), n * 4);
-    if (origin) std::memcpy(origin, a.origin_index.data(), n * 4);
-    if (image) std::memcpy(image, a.image.data(), 3 * n * 4);
+    if (pos) copy_bytes(pos, a.pos.data(), 3 * n * 8);
+    if (vel) copy_bytes(vel, a.vel.data(), 3 * n * 8);
+    if (force) copy_bytes(force, a.force.data(), 3 * n * 8);
+    if (radius) copy_bytes(radius, a.cutoff_radius.data(), n * 8);
+    if (mass) copy_bytes(mass, a.mass.data(), n * 8);
+    if (inv_mass) copy_bytes(inv_mass, a.inv_mass.data(), n * 8);
+    if (tag) copy_bytes(tag, a.tag.data(), n * 4);
+    if (origin) copy_bytes(origin, a.origin_index.data(), n * 4);
+    if (image) copy_bytes(image, a.image.data(), 3 * n * 4);
 }
 
 // swap table: per swap [recv_start, recv_count, nsend, to_proc, from_proc, off xyz, voff xyz]

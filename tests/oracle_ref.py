@@ -7,7 +7,9 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
-LIB_PATH = os.path.join(ORACLE_DIR, "libsoil_ref.so")
+# SOIL_REF_LIB points the tests at another build of the same source (for
+# example the sanitizer build, `make -C oracle asan`); it is used as built
+LIB_PATH = os.environ.get("SOIL_REF_LIB") or os.path.join(ORACLE_DIR, "libsoil_ref.so")
 
 
 class RefDesc(C.Structure):
@@ -38,6 +40,8 @@ P = C.c_void_p
 
 
 def build_oracle():
+    if os.environ.get("SOIL_REF_LIB"):
+        return
     src = os.path.join(ORACLE_DIR, "soil_ref.cpp")
     if not os.path.exists(LIB_PATH) or os.path.getmtime(LIB_PATH) < os.path.getmtime(src):
         subprocess.run(["make", "-C", ORACLE_DIR], check=True, capture_output=True)
