@@ -870,7 +870,12 @@ int soil_hip_setup(soil_hip_box* b) {
     std::memcpy(b->h_st, &st, sizeof(st));
     HIPCHK(hipMemcpy(b->d_st, &st, sizeof(BoxState), hipMemcpyHostToDevice));
     b->ring_pos = 0;
-    b->L.maxgrid = 4096;
+    {
+        // blocks per launch at most (tiles beyond it loop); bit-neutral tuning knob
+        const char* m = getenv("SOIL_HIP_MAX_GRID");
+        const int g = m ? atoi(m) : 4096;
+        b->L.maxgrid = (uint32_t)(g < 256 ? 256 : (g > 65536 ? 65536 : g));
+    }
     rc = push_desc(b);
     if (!rc) rc = rebuild_tiles(b);
     if (rc) return rc;
