@@ -229,6 +229,16 @@ int soil_hip_get_counts(soil_hip_box* box, uint64_t* counts8);
  * on the device, tiles in order on the host), deterministic and within a few
  * ulps of the reference's sequential sum. */
 int soil_hip_compute_ke(soil_hip_box* box, const uint8_t* mask, double* out);
+/* soil_hip_compute_ke without any allocation (no reference counterpart: it
+ * is for callers that cannot allocate, such as code under HIP graph capture
+ * and tight serving loops). Same arguments and the same result, bit for bit;
+ * the mask and the per-tile partial sums go to a fixed workspace of the
+ * library (one per GPU, held until the process ends) instead of the box's
+ * scratch or host vectors. At most SOIL_HIP_NOALLOC_MAX_LOCAL locals
+ * (ECAPACITY above). Calls on one GPU share the workspace: calls that overlap
+ * on the same GPU (from several threads) race. */
+#define SOIL_HIP_NOALLOC_MAX_LOCAL 8388608
+int soil_hip_compute_ke_noalloc(soil_hip_box* box, const uint8_t* mask, double* out);
 /* classify_boundary_atoms (neighbor.rs:1321-1336): mask[i] = 1 when local row
  * i of the current list has a ghost neighbour (index >= nlocal), else 0 — the
  * boundary / interior split that lets interior forces overlap a halo exchange.

@@ -101,6 +101,7 @@ _SIGS = {
     "soil_hip_run_force_hook": ([P], C.c_int),
     "soil_hip_classify_boundary_atoms": ([P, C.POINTER(C.c_uint8), SZ, C.POINTER(SZ)], C.c_int),
     "soil_hip_compute_ke": ([P, C.POINTER(C.c_uint8), C.POINTER(C.c_double)], C.c_int),
+    "soil_hip_compute_ke_noalloc": ([P, C.POINTER(C.c_uint8), C.POINTER(C.c_double)], C.c_int),
     "soil_hip_set_axis_deform": ([P, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double], C.c_int),
     "soil_hip_set_deform_remap": ([P, C.c_int], C.c_int),
     "soil_hip_set_stage_steps": ([P, C.c_uint64], C.c_int),

@@ -663,6 +663,9 @@ int soil_hip_ctx_create(int device, soil_hip_ctx** out) {
     if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return set_error(SOIL_HIP_EDEVICE, "no HIP device");
     if (device < 0 || device >= n) return set_error(SOIL_HIP_EINVAL, "device %d out of range (%d)", device, n);
     HIPCHK(hipSetDevice(device));
+    // the allocation-free entry points' workspace on this device
+    if (scratch_init(device) != 0)
+        return set_error(SOIL_HIP_EDEVICE, "hipGetSymbolAddress of the fixed workspace failed on device %d", device);
     soil_hip_ctx* c = new soil_hip_ctx();
     c->device = device;
     *out = c;
@@ -1295,18 +1298,45 @@ int soil_hip_compute_ke(soil_hip_box* b, const uint8_t* mask, double* out) {
     rc = pull_desc(b);
     if (rc) return rc;
     const size_t nl = b->h_st->nlocal;
-    if (mask && nl) {
-        std::vector<uint32_t> m(nl);
-        for (size_t i = 0; i < nl; ++i) m[i] = mask[i] ? 1u : 0u;
-        HIPCHK(hipMemcpy(b->h_dev.slot, m.data(), nl * 4, hipMemcpyHostToDevice));
-    }
-    phase_ke(b->L, mask != nullptr);
+    // the box's own scratch: the mask bytes in the slot array, the partials in red
+    uint8_t* dmask = mask ? (uint8_t*)b->h_dev.slot : nullptr;
+    if (mask && nl) HIPCHK(hipMemcpy(dmask, mask, nl, hipMemcpyHostToDevice));
+    phase_ke(b->L, dmask, (double*)b->h_dev.red);
     const size_t ntile = (nl + BLK - 1) / BLK;
     std::vector<double> part(ntile);
     HIPCHK(hipStreamSynchronize(b->stream));
     if (ntile) HIPCHK(hipMemcpy(part.data(), b->h_dev.red, ntile * 8, hipMemcpyDeviceToHost));
     double ke = 0.0;
     for (size_t k = 0; k < ntile; ++k) ke += part[k];
+    *out = 0.5 * ke;
+    return 0;
+}
+
+int soil_hip_compute_ke_noalloc(soil_hip_box* b, const uint8_t* mask, double* out) {
+    int rc = check_box(b, true);
+    if (rc) return rc;
+    if (!out) return set_error(SOIL_HIP_EINVAL, "null output");
+    rc = pull_state(b);  // nlocal (a copy into the box's pinned state)
+    if (rc) return rc;
+    const size_t nl = b->h_st->nlocal;
+    // k_ke writes one partial per tile of the box's local capacity (nlocal <= it)
+    if (b->caps.local > NOALLOC_MAX_LOCAL)
+        return set_error(SOIL_HIP_ECAPACITY, "compute_ke_noalloc: %zu locals exceed the fixed workspace of %zu",
+                         (size_t)b->caps.local, (size_t)NOALLOC_MAX_LOCAL);
+    char* ws = (char*)scratch_ptr(b->ctx->device);
+    if (!ws) return set_error(SOIL_HIP_EDEVICE, "compute_ke_noalloc: no workspace on device %d", b->ctx->device);
+    uint8_t* dmask = mask ? (uint8_t*)(ws + NOALLOC_MASK_OFF) : nullptr;
+    double* part = (double*)(ws + NOALLOC_PART_OFF);
+    double* dsum = (double*)(ws + NOALLOC_OUT_OFF);
+    if (mask && nl) HIPCHK(hipMemcpyAsync(dmask, mask, nl, hipMemcpyHostToDevice, b->stream));
+    // every tile below nlocal writes its partial, and the sum writes the
+    // result, so nothing left by an earlier call is read
+    phase_ke(b->L, dmask, part);
+    sum_tiles(part, (uint32_t)((nl + BLK - 1) / BLK), dsum, b->stream);
+    HIPCHK(hipGetLastError());
+    double ke = 0.0;
+    HIPCHK(hipMemcpyAsync(&ke, dsum, sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
     *out = 0.5 * ke;
     return 0;
 }

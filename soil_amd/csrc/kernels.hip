@@ -10,6 +10,9 @@
 // is an ordered wave-ballot compaction.
 
 #include <hip/hip_runtime.h>
+
+#include <mutex>
+
 #include "kernels.hpp"
 
 namespace soil_hip {
@@ -403,10 +406,11 @@ __global__ void __launch_bounds__(BLK) k_classify(KARGS) {
 }
 
 // compute_ke (atom.rs:589-604): each local's m (vx² + vy² + vz²) in the
-// reference's order, summed per 256-atom tile in a fixed tree order; the host
-// adds the tile sums in tile order. `use_mask`: only locals whose slot-scratch
-// word is non-zero (the caller's group mask).
-__global__ void __launch_bounds__(BLK) k_ke(KARGS, int use_mask) {
+// reference's order, summed per 256-atom tile in a fixed tree order into
+// part[tile]; the tile sums are then added in tile order (k_sum_tiles or the
+// host). `mask` (null: every local): one byte per local, non-zero = in the
+// caller's group. Both buffers are scratch the caller provides.
+__global__ void __launch_bounds__(BLK) k_ke(KARGS, const uint8_t* __restrict__ mask, double* __restrict__ part) {
     __shared__ double red[BLK / 64];
     FOR_TILES(t) {
         const Tile tl = tiles[t];
@@ -414,7 +418,7 @@ __global__ void __launch_bounds__(BLK) k_ke(KARGS, int use_mask) {
         if (!live(b)) continue;
         const uint32_t i = tl.start + threadIdx.x;
         double v = 0.0;
-        if (i < b.st->nlocal && (!use_mask || b.slot[i] != 0u)) {
+        if (i < b.st->nlocal && (!mask || mask[i] != 0u)) {
             const double vx = b.vx[i], vy = b.vy[i], vz = b.vz[i];
             v = b.mass[i] * (vx * vx + vy * vy + vz * vz);
         }
@@ -426,10 +430,20 @@ __global__ void __launch_bounds__(BLK) k_ke(KARGS, int use_mask) {
             double s = 0.0;
 #pragma unroll
             for (int w = 0; w < BLK / 64; ++w) s += red[w];
-            b.red[tl.start / BLK] = s;
+            part[tl.start / BLK] = s;
         }
         __syncthreads();
     }
+}
+
+// *out = part[0] + part[1] + ... + part[n-1], added one after another from
+// 0.0 in index order (the same order as a sequential host loop, so the sum is
+// the same bits). One thread.
+__global__ void k_sum_tiles(const double* __restrict__ part, uint32_t n, double* __restrict__ out) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double s = 0.0;
+    for (uint32_t k = 0; k < n; ++k) s += part[k];
+    *out = s;
 }
 
 // counts[0..ncells] = 0 for the step's geometry: the shrink-wrap step stages
@@ -2401,7 +2415,40 @@ void phase_reverse(const Launch& L) {
     }
 }
 void phase_final(const Launch& L) { launch(k_final, L.t_local, L.boxes, L.stream, L.maxgrid); }
-void phase_ke(const Launch& L, bool use_mask) { launch(k_ke, L.t_local, L.boxes, L.stream, L.maxgrid, use_mask ? 1 : 0); }
+void phase_ke(const Launch& L, const uint8_t* mask, double* part) {
+    launch(k_ke, L.t_local, L.boxes, L.stream, L.maxgrid, mask, part);
+}
+void sum_tiles(const double* part, uint32_t n, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_sum_tiles, dim3(1), dim3(64), 0, s, part, n, out);
+}
+
+// ── fixed workspace of the allocation-free entry points ──────────────────
+// One array per GPU, sized at compile time and held until the process ends.
+// Host code never uses the name `scratch` as an address (on the host it is a
+// stand-in symbol, not device memory): scratch_init looks up each device's
+// copy once, with that device current, and the entry points pass that
+// address to the kernels.
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+int scratch_init(int device) {
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) return -1;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return 0;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return -1;
+    g_scratch_ptr[device] = p;
+    return 0;
+}
+
+void* scratch_ptr(int device) {
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch_ptr[device];
+}
 void phase_classify(const Launch& L) { launch(k_classify, L.t_local, L.boxes, L.stream, L.maxgrid); }
 void phase_zero_counts(const Launch& L) { launch(k_zero_counts, L.t_cell, L.boxes, L.stream, L.maxgrid); }
 void phase_sync_view(const Launch& L) { launch(k_sync_view, L.t_box, L.boxes, L.stream, L.maxgrid); }

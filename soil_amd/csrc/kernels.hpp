@@ -47,7 +47,10 @@ void phase_kick(const Launch& L, bool with_final);  // k_initial with the displa
 void phase_decide_fused(const Launch& L);           // k_decide with the sort bookkeeping
 void phase_extent(const Launch& L);                 // locals' extents into BoxState::ext_key
 void phase_sync_view(const Launch& L);              // force hooks' device views
-void phase_ke(const Launch& L, bool use_mask);      // per-tile kinetic-energy partials into BoxDev::red
+// per-tile kinetic-energy partials into part[] (mask: one byte per local, or null)
+void phase_ke(const Launch& L, const uint8_t* mask, double* part);
+// *out = part[0] + ... + part[n-1] in index order, on the device
+void sum_tiles(const double* part, uint32_t n, double* out, hipStream_t s);
 void phase_classify(const Launch& L);               // boundary flags of the local rows into the slot scratch
 void phase_zero_counts(const Launch& L);            // the spatial sort's counts for a regridded step
 void phase_remove_ghosts(const Launch& L);
@@ -81,5 +84,20 @@ void phase_step_end_pending(const Launch& L);
 void phase_final_pending(const Launch& L);
 void phase_collect(const Launch& L, uint32_t* out);
 void copy_probe(const double* src, double* dst, size_t n, hipStream_t s);
+
+// Fixed workspace of the allocation-free entry points: NOALLOC_BYTES of
+// device memory per GPU, from a file-scope __device__ array. Layout: one f64
+// partial per 256-atom tile, the result, then one mask byte per local.
+constexpr size_t NOALLOC_MAX_LOCAL = SOIL_HIP_NOALLOC_MAX_LOCAL;
+constexpr size_t NOALLOC_PART_OFF = 0;
+constexpr size_t NOALLOC_OUT_OFF = NOALLOC_MAX_LOCAL / BLK * sizeof(double);
+constexpr size_t NOALLOC_MASK_OFF = NOALLOC_OUT_OFF + 256;
+constexpr size_t NOALLOC_BYTES = NOALLOC_MASK_OFF + NOALLOC_MAX_LOCAL;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+// Looks up the calling thread's current device's copy of the workspace
+// (hipGetSymbolAddress), once per device; 0 on success.
+int scratch_init(int device);
+// The workspace's device address on `device` (null before scratch_init).
+void* scratch_ptr(int device);
 
 }  // namespace soil_hip
