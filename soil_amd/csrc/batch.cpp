@@ -335,18 +335,60 @@ int soil_hip_batch_sync(soil_hip_batch* bt) {
     return 0;
 }
 
+// Grows each box's cell arrays to the largest grid that its next `nsteps`
+// steps take (the host model advanced on a copy), so that stepping them
+// regrids nothing and keeps every arena where it is. Re-tiles the batch and
+// drops its graph when a box grew.
+static int pregrow_cells(soil_hip_batch* bt, int nsteps) {
+    bool grown = false;
+    for (auto* b : bt->boxes) {
+        HostModel m = b->model;
+        uint64_t maxc = 0;
+        for (int s = 0; s < nsteps; ++s) {
+            if (!m.apply_deform()) break;  // the real step reports the collapse
+            m.sort_prepare();
+            Geom g;
+            if (m.fill(g)) break;
+            maxc = std::max<uint64_t>(maxc, g.ncells);
+        }
+        if (maxc > b->caps.cells) {
+            HIPCHK(hipStreamSynchronize(bt->stream));
+            bt->half_pending[0] = bt->half_pending[1] = false;
+            int rc = ensure_cells(b, maxc);
+            if (rc) return rc;
+            grown = true;
+        }
+    }
+    if (grown) {
+        drop_graph(bt);
+        return batch_tiles(bt);
+    }
+    return 0;
+}
+
 // Bit-neutral launch-shape tuning: the grid-stride cap of every launch. Each
 // candidate runs a few steps on the live batch after a device-side snapshot of
 // every box (columns, lists, state, descriptor) and the host models; the
 // snapshot is restored after each trial, so the batch state is unchanged.
+// The cell arrays are first grown for every grid the trials reach, so no trial
+// reallocates an arena (the snapshot restores arena contents, and its
+// descriptor points into the arenas it was taken from); each trial checks it.
 int soil_hip_batch_autotune(soil_hip_batch* bt) {
     if (!bt) return set_error(SOIL_HIP_EINVAL, "null batch");
     HIPCHK(hipStreamSynchronize(bt->stream));
     bt->half_pending[0] = bt->half_pending[1] = false;
+    const int trial = 4;
+    {
+        const int rc = pregrow_cells(bt, trial + 1);  // a warm-up step and the timed ones
+        if (rc) return rc;
+    }
     struct Snap {
         void* atoms = nullptr;
         void* pairs = nullptr;
         size_t atoms_bytes = 0, pairs_bytes = 0;
+        void* arena_atoms = nullptr;  // the arenas the descriptor points into
+        void* arena_pairs = nullptr;
+        Caps caps;
         BoxDev dev;
         BoxState st;
         HostModel model;
@@ -362,6 +404,9 @@ int soil_hip_batch_autotune(soil_hip_batch* bt) {
         rc = pull_state(b);
         if (rc) return rc;
         s.dev = b->h_dev;
+        s.arena_atoms = b->arena_atoms;
+        s.arena_pairs = b->arena_pairs;
+        s.caps = b->caps;
         s.st = *b->h_st;
         s.model = b->model;
         s.last = b->last_geom;
@@ -383,6 +428,9 @@ int soil_hip_batch_autotune(soil_hip_batch* bt) {
         for (size_t k = 0; k < bt->boxes.size(); ++k) {
             soil_hip_box* b = bt->boxes[k];
             Snap& s = snaps[k];
+            if (b->arena_atoms != s.arena_atoms || b->arena_pairs != s.arena_pairs ||
+                std::memcmp(&b->caps, &s.caps, sizeof(Caps)) != 0)
+                return set_error(SOIL_HIP_ESTATE, "autotune: a trial reallocated box %zu (state not restorable)", k);
             HIPCHK(hipMemcpy(b->arena_atoms, s.atoms, s.atoms_bytes, hipMemcpyDeviceToDevice));
             HIPCHK(hipMemcpy(b->arena_pairs, s.pairs, s.pairs_bytes, hipMemcpyDeviceToDevice));
             b->h_dev = s.dev;
@@ -400,7 +448,6 @@ int soil_hip_batch_autotune(soil_hip_batch* bt) {
     const uint32_t cands[5] = {1024, 2048, 4096, 8192, 16384};
     uint32_t best = bt->L.maxgrid;
     double best_ms = 1e30;
-    const int trial = 4;
     for (uint32_t mg : cands) {
         bt->L.maxgrid = mg;
         drop_graph(bt);

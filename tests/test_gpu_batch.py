@@ -104,6 +104,36 @@ def test_batch_axis_deform_grows_grid(ctx):
         assert_same(g, r, what=f"deform box {i}")
 
 
+def test_batch_autotune_with_a_growing_grid(ctx):
+    # autotune's trial steps would regrow the cell arrays of a box that
+    # expands; it grows them before its snapshot, so every trial steps in the
+    # same arenas and the restored state is exact
+    print("case: autotune with an expanding box", flush=True)
+    gens = ensemble(3, 3100, 3000)
+    pairs = []
+    for k, gen in enumerate(gens):
+        d = gen[0]
+        L = d.hi[0] - d.lo[0]
+        spec = (0, "vel", 0.6 * L / (60 * d.dt)) if k == 0 else None
+
+        def f(b, spec=spec):
+            if spec:
+                b.set_axis_deform(spec[0], spec[1], spec[2])
+                b.set_stage_steps(60)
+        pairs.append(make_pair(ctx, *gen, k=1000.0, pre_setup=f))
+    batch = soil_amd.Batch(ctx, [p[0] for p in pairs])
+    batch.step(12, graph=True)
+    batch.sync()
+    batch.autotune()
+    batch.step(48, graph=True)
+    batch.sync()
+    batch.close()
+    for g, r in pairs:
+        r.step(60)
+    for i, (g, r) in enumerate(pairs):
+        assert_same(g, r, what=f"autotuned deform box {i}")
+
+
 def test_batch_refuses_mixed_csr_modes(ctx, monkeypatch):
     # one count kernel serves a batch: boxes set up under different
     # SOIL_HIP_CSR_MODE values are refused, and each still steps alone
