@@ -19,12 +19,27 @@
 //                                     completeness (no in-cutoff pair missed)
 //   bin_neighbor_list               : brute-force minimum-image pair sets (both
 //                                     inclusions), neighbor.rs:1457-1619 fixtures
-//   borders/forward/reverse comm    : brute-force image enumeration, conservation
-//   sort_atoms_by_bin               : stable-sort invariants vs numpy argsort
+//   borders                         : brute-force image enumeration, swap rules
+//   forward_comm_borders            : every ghost = source + the build's fixed
+//                                     offset, pos and vel, chained swap by swap
+//                                     (comm.rs:400-432; tests/pins.py)
+//   reverse_send_force + columns    : integer-valued #[reverse] ghost values sum
+//                                     exactly onto their chain roots; #[forward]
+//                                     ghost = root value; #[zero] cleared; a
+//                                     tag-valued column follows its atom through
+//                                     every sort (docs/SOIL_ATOMDATA_CONTRACT.md
+//                                     :87-143); force balance is exact
+//   sort_atoms_by_bin               : stable-sort invariants vs numpy argsort; a
+//                                     periodic sort rebuilds exactly when that
+//                                     order is not the identity (neighbor.rs:989-996)
+//   decide_rebuild / every / check  : every=E, check off rebuilds on steps 1,
+//                                     E+2, 2E+3, ... (neighbor.rs:762-784); the
+//                                     threshold fires at ((sf-1) min_r)^2
 //   apply_deform (LE xy)            : closed form soil_deform/src/lib.rs:576-583
-//   step state machine              : parity unpinned across multi-step ordering
-//                                     beyond the invariants above (grass run_if
-//                                     timing is external to the reference tree).
+//   step state machine              : the run_if placement is read from the
+//                                     plugin sources (tests/test_integration_doc.py
+//                                     parses them); the grass scheduler's dispatch
+//                                     timing is external to the reference tree.
 
 #include <cmath>
 #include <cstdint>

@@ -1,0 +1,99 @@
+"""Invariants that the reference's text fixes, checked on any box's outputs.
+
+They pin what bit-for-bit agreement between the device and the oracle
+cannot: a misreading shared by both would pass a device-vs-oracle test, but
+not these. Each states the reference's rule directly:
+
+* forward comm (crates/soil_core/src/comm.rs:400-432, 496-592): on a
+  CommunicateOnly step every ghost is its source atom plus the swap's offset
+  FIXED at the last build, pos and vel, applied swap by swap, so a corner
+  ghost adds onto its already refreshed source ghost;
+* the AtomData column contract (docs/SOIL_ATOMDATA_CONTRACT.md:87-143,
+  soil_derive/src/lib.rs:392-500): #[forward] columns of a ghost are its
+  source's values, #[reverse] ghost values are added onto their owners (a
+  ghost of a ghost first onto its source), #[zero] columns are zero after
+  zero_all_forces, and every column follows its atom through the sorts;
+* the rebuild schedule (crates/soil_core/src/neighbor.rs:762-784, 842-848,
+  989-996): with `every` = E and `check` off, rebuilds land on steps
+  1, E+2, 2E+3, ...; a periodic sort at `sort_every` forces a rebuild
+  exactly when the stable cell order of the locals is not the identity.
+"""
+import numpy as np
+
+
+def ghost_swaps(swaps, nlocal, nghost):
+    """Swap index of every ghost (rows of the swap table: recv_start, recv_count, ...)."""
+    sw = np.full(nghost, -1, dtype=np.int64)
+    for s, row in enumerate(swaps):
+        start, count = int(row[0]), int(row[1])
+        if count:
+            assert nlocal <= start and start + count <= nlocal + nghost
+            sw[start - nlocal:start - nlocal + count] = s
+    assert (sw >= 0).all(), "a ghost belongs to no swap"
+    return sw
+
+
+def chain_depth(origin, nlocal):
+    """Swaps between each ghost and its local root (locals: 0)."""
+    n = len(origin)
+    depth = np.zeros(n, dtype=np.int64)
+    for g in range(nlocal, n):  # a ghost's source precedes it, so its depth is known
+        src = int(origin[g])
+        assert 0 <= src < g
+        depth[g] = depth[src] + 1
+    return depth
+
+
+def chain_roots(origin, nlocal):
+    n = len(origin)
+    root = np.arange(n)
+    for g in range(nlocal, n):
+        root[g] = root[int(origin[g])]
+    return root
+
+
+def check_forward(a, swaps):
+    """Every ghost = source + its swap's fixed offset (pos and vel), exactly.
+    Returns the chain depth of every atom."""
+    nl, ng = a["nlocal"], a["nghost"]
+    sw = ghost_swaps(swaps, nl, ng)
+    off = swaps[:, 5:8]
+    voff = swaps[:, 8:11]
+    src = a["origin"][nl:].astype(np.int64)
+    want_p = a["pos"][src] + off[sw]
+    want_v = a["vel"][src] + voff[sw]
+    bad = np.nonzero((a["pos"][nl:] != want_p).any(axis=1) | (a["vel"][nl:] != want_v).any(axis=1))[0]
+    assert bad.size == 0, f"ghosts {bad[:5] + nl} are not source + fixed offset"
+    # the same, unrolled from the root: root + offsets of the chain, added root first
+    depth = chain_depth(a["origin"], nl)
+    for g in np.nonzero(depth >= 2)[0][:200]:
+        chain = []
+        x = int(g)
+        while x >= nl:
+            chain.append(x)
+            x = int(a["origin"][x])
+        p = a["pos"][x].copy()
+        for y in reversed(chain):
+            p = p + off[sw[y - nl]]
+        assert np.array_equal(p, a["pos"][g]), f"ghost {g}: chained offsets"
+    return depth
+
+
+def rebuild_steps_every(every, nsteps):
+    """Steps (1-based) that rebuild with `every` = E, check off and no other trigger."""
+    return [s for s in range(1, nsteps + 1) if (s - 1) % (every + 1) == 0]
+
+
+def cells(pos, grid):
+    """The cell of each position under a grid record (bin_count, bin_size,
+    bin_origin, ...): floor((p - o) / size) clamped into the grid."""
+    n = grid[0:3].astype(np.int64)
+    inv = 1.0 / grid[3:6]
+    c = np.floor((pos - grid[6:9]) * inv)
+    c = np.clip(c, 0, n - 1).astype(np.int64)
+    return c[:, 0] * n[1] * n[2] + c[:, 1] * n[2] + c[:, 2]
+
+
+def sort_is_identity(pos, grid):
+    cl = cells(pos, grid)
+    return bool(np.array_equal(np.argsort(cl, kind="stable"), np.arange(len(cl))))

@@ -1,0 +1,106 @@
+"""The device against the rules the reference's text fixes (pins.py): forward
+comm, the #[forward] column contract and the rebuild schedule, asserted on
+device runs directly rather than through agreement with the oracle."""
+import numpy as np
+import pytest
+
+import soil_amd
+from pins import check_forward, rebuild_steps_every, sort_is_identity
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return soil_amd.Context(0)
+
+
+def box(ctx, seed, n, phi, vel_sigma, every=0, check=1, sort_every=1000, columns=(), col_data=()):
+    d, pos, vel, mass, rad = soil_amd.gen_custom(seed, n, 0.45, 0.55, phi, (1, 1, 1), 0, 0.0, 0.0, vel_sigma, 0.0)
+    d.every, d.check, d.sort_every = every, check, sort_every
+    b = soil_amd.Box(ctx, d)
+    for w, f in columns:
+        b.register_column(w, f)
+    b.upload(pos, mass, rad, vel=vel)
+    for c, data in enumerate(col_data):
+        b.upload_column(c, data)
+    b.setup()
+    return b, d
+
+
+def test_forward_comm_on_device():
+    ctx = soil_amd.Context(0)
+    print("case: device forward comm pin", flush=True)
+    for tilt in (0.0, 1.7):
+        b, d = box(ctx, 7001, 1200, 0.35, 2.0)
+        if tilt:
+            b.set_tilt(tilt, 0.3)
+        b.step(1)
+        comm_only = 0
+        for s in range(8):
+            before = b.counts()["rebuilds"]
+            b.step(1)
+            if b.counts()["rebuilds"] != before:
+                continue
+            comm_only += 1
+            depth = check_forward(b.atoms(), b.swaps())
+            assert depth.max() == 3
+        assert comm_only >= 3
+        b.close()
+
+
+def test_forward_column_follows_its_atom_on_device(ctx):
+    # a #[forward] column set from the tag at upload stays tied to its atom
+    # through every sort, and every ghost carries its source's value
+    print("case: device forward column pin", flush=True)
+    n = 1500
+    t = np.arange(n, dtype=np.float64)
+    b, _ = box(ctx, 7003, n, 0.4, 0.5, columns=((2, soil_amd._ffi.COL_FORWARD),),
+               col_data=(np.stack([t, 2.0 * t + 0.5], axis=1).ravel(),))
+    for s in range(3):
+        b.step(4)
+        a = b.atoms()
+        assert a["nghost"] > 0
+        c = b.column(0).reshape(-1, 2)
+        tag = a["tag"].astype(np.float64)
+        assert np.array_equal(c[:, 0], tag) and np.array_equal(c[:, 1], 2.0 * tag + 0.5)
+    b.close()
+
+
+@pytest.mark.parametrize("every", [1, 2, 5])
+def test_every_schedule_on_device(ctx, every):
+    print(f"case: device every={every} schedule", flush=True)
+    nsteps = 3 * (every + 1) + 2
+    b, _ = box(ctx, 7010 + every, 1000, 0.3, 3.0, every=every, check=0, sort_every=0)
+    got = []
+    for s in range(1, nsteps + 1):
+        before = b.counts()["rebuilds"]
+        b.step(1)
+        if b.counts()["rebuilds"] != before:
+            got.append(s)
+    assert got == rebuild_steps_every(every, nsteps)
+    b.close()
+
+
+@pytest.mark.parametrize("sigma", [0.0, 0.6])
+def test_periodic_sort_rule_on_device(ctx, sigma):
+    print(f"case: device periodic sort sigma={sigma}", flush=True)
+    S = 5
+    b, d = box(ctx, 7030, 2000, 0.3, sigma, every=0, check=1, sort_every=S)
+    b.step(1)
+    sorts = []
+    for s in range(2, 4 * S + 1):
+        a = b.atoms()
+        nl = a["nlocal"]
+        x = a["pos"][:nl] + a["vel"][:nl] * d.dt
+        before = b.counts()["rebuilds"]
+        b.step(1)
+        rebuilt = b.counts()["rebuilds"] != before
+        if s % S == 0:
+            ident = sort_is_identity(x, b.grid())
+            assert rebuilt == (not ident), (s, ident, rebuilt)
+            sorts.append(ident)
+        else:
+            assert not rebuilt
+    assert all(sorts) if sigma == 0.0 else not all(sorts)
+    b.close()

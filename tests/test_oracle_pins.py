@@ -1,0 +1,199 @@
+"""The oracle against the rules the reference's text fixes (see pins.py):
+forward comm, the AtomData column contract and the rebuild schedule. These
+are the oracle functions that the KATs and brute-force checks leave open."""
+import numpy as np
+import pytest
+
+import soil_amd
+from oracle_ref import RefBox
+from pins import (cells, chain_depth, chain_roots, check_forward, ghost_swaps, rebuild_steps_every,
+                  sort_is_identity)
+
+FWD, REV, ZERO = 1, 2, 4
+
+
+def small_box(seed, n, phi, vel_sigma, shear=0.0, every=0, check=1, sort_every=1000, columns=()):
+    d, pos, vel, mass, rad = soil_amd.gen_custom(seed, n, 0.45, 0.55, phi, (1, 1, 1), 0, 0.0, 0.0, vel_sigma, shear)
+    d.every, d.check, d.sort_every = every, check, sort_every
+    r = RefBox(d)
+    for w, f in columns:
+        r.register_column(w, f)
+    r.add_atoms(pos, mass, rad, vel=vel)
+    return r, d
+
+
+def test_forward_comm_ghosts_are_source_plus_fixed_offset():
+    # a small dense cube: many corner ghosts (three swaps deep); fast atoms
+    # cross faces between builds, where only the build's fixed offset is right
+    # (a sheared box rebuilds every step, so the tilted case is a static tilt
+    # with a boundary velocity: tilted y shifts and velocity offsets)
+    for tilt in (0.0, 1.7):
+        r, d = small_box(7001, 1200, 0.35, 2.0)
+        r.setup()
+        if tilt:
+            r.set_tilt(tilt, 0.3)
+        r.step(1)
+        comm_only = 0
+        crossed = False
+        for s in range(8):
+            before = r.counts()["rebuilds"]
+            r.step(1)
+            if r.counts()["rebuilds"] != before:
+                continue
+            comm_only += 1
+            a = r.atoms()
+            depth = check_forward(a, r.swaps())
+            assert depth.max() == 3
+            nl = a["nlocal"]
+            lo, hi = np.array(d.lo), np.array(d.hi)
+            crossed |= bool(((a["pos"][:nl] < lo) | (a["pos"][:nl] >= hi)).any())
+        assert comm_only >= 3, "too few CommunicateOnly steps to check"
+        assert crossed, "no local crossed a face between builds"
+
+
+def test_forward_comm_rejects_a_planted_offset_mistake():
+    # the check is sensitive: recomputing a ghost from its source's wrapped
+    # (current-image) position instead of the fixed offset breaks it
+    r, d = small_box(7002, 1200, 0.35, 2.0)
+    r.setup()
+    r.step(3)
+    a = r.atoms()
+    check_forward(a, r.swaps())
+    nl = a["nlocal"]
+    L = np.array(d.hi) - np.array(d.lo)
+    bad = dict(a)
+    bad["pos"] = a["pos"].copy()
+    sw = ghost_swaps(r.swaps(), nl, a["nghost"])
+    g = nl + int(np.nonzero(sw == 0)[0][0])  # a ghost of the first swap: shift +Lx
+    bad["pos"][g, 0] -= L[0]
+    with pytest.raises(AssertionError):
+        check_forward(bad, r.swaps())
+
+
+def test_column_contract_forward_reverse_zero():
+    cols = ((2, FWD), (1, REV), (1, ZERO), (1, REV | ZERO))
+    r, d = small_box(7003, 1500, 0.4, 0.5, columns=cols)
+    rng = np.random.default_rng(3)
+    n = 1500
+    # a #[forward] column tied to the atom (tag): it must follow every sort and reach the ghosts
+    tag0 = np.arange(n, dtype=np.float64)
+    r.set_column(0, np.stack([tag0, 2.0 * tag0 + 0.5], axis=1).ravel(), 2)
+    r.set_column(1, np.zeros(n), 1)
+    r.set_column(2, np.zeros(n), 1)
+    r.set_column(3, np.zeros(n), 1)
+    r.setup()
+    r.step(6)
+    a = r.atoms()
+    nl, ng = a["nlocal"], a["nghost"]
+    assert ng > 0 and r.counts()["rebuilds"] >= 1
+    c0 = r.column(0).reshape(-1, 2)
+    t = a["tag"].astype(np.float64)
+    assert np.array_equal(c0[:, 0], t) and np.array_equal(c0[:, 1], 2.0 * t + 0.5), "forward column left its atom"
+
+    # #[forward]: a CommunicateOnly refresh copies each ghost's source value (chained)
+    vals = rng.normal(size=(nl + ng, 2))
+    vals[nl:] = 7.0  # stale ghost values, overwritten by the refresh
+    r.set_column(0, vals.ravel(), 2)
+    r.phase("forward_comm_borders")
+    c0 = r.column(0).reshape(-1, 2)
+    root = chain_roots(a["origin"], nl)
+    assert np.array_equal(c0[nl:], vals[root[nl:]]), "forward column is not the source's value"
+    assert np.array_equal(c0[:nl], vals[:nl])
+
+    # #[reverse]: integer-valued ghost entries (exact in any order) land on
+    # their roots, chain by chain: owner = own + sum over every ghost rooted at it
+    rv = np.zeros(nl + ng)
+    rv[:nl] = rng.integers(0, 1000, nl)
+    rv[nl:] = rng.integers(1, 1 << 20, ng)
+    r.set_column(1, rv, 1)
+    r.set_column(3, rv, 1)
+    r.phase("reverse_send_force")
+    want = rv[:nl].copy()
+    np.add.at(want, root[nl:], rv[nl:])
+    for c in (1, 3):
+        got = r.column(c)
+        assert np.array_equal(got[:nl], want), f"reverse column {c}"
+    # a ghost that is the source of others also received their values first
+    depth = chain_depth(a["origin"], nl)
+    assert depth.max() >= 2
+
+    # #[zero]: zero_all_forces clears the zero columns on every atom and leaves the others
+    zv = rng.normal(size=nl + ng)
+    r.set_column(2, zv, 1)
+    r.set_column(3, zv, 1)
+    keep0, keep1 = r.column(0).copy(), r.column(1).copy()
+    r.phase("zero_all_forces")
+    assert not r.column(2).any() and not r.column(3).any()
+    assert np.array_equal(r.column(0), keep0) and np.array_equal(r.column(1), keep1)
+    assert not r.atoms()["force"].any()
+
+
+@pytest.mark.parametrize("every", [1, 2, 5])
+def test_every_without_check_rebuilds_on_the_schedule(every):
+    # check off: the displacements (fast atoms) must not trigger anything
+    nsteps = 3 * (every + 1) + 2
+    r, _ = small_box(7010 + every, 1000, 0.3, 3.0, every=every, check=0, sort_every=0)
+    r.setup()
+    got = []
+    for s in range(1, nsteps + 1):
+        before = r.counts()["rebuilds"]
+        r.step(1)
+        if r.counts()["rebuilds"] != before:
+            got.append(s)
+    assert got == rebuild_steps_every(every, nsteps)
+
+
+def test_every_with_check_adds_displacement_rebuilds():
+    # check on: the same schedule, plus rebuilds wherever an atom moved past
+    # the threshold ((sf - 1) min_r)^2 since the last build
+    r, _ = small_box(7020, 1000, 0.3, 3.0, every=4, check=1, sort_every=0)
+    r.setup()
+    got = []
+    for s in range(1, 40):
+        before = r.counts()["rebuilds"]
+        r.step(1)
+        if r.counts()["rebuilds"] != before:
+            got.append(s)
+    gaps = np.diff(got)
+    assert got[0] == 1 and gaps.max() <= 5  # never longer than `every` + 1
+    assert gaps.min() < 5                   # displacement rebuilds come earlier
+
+
+@pytest.mark.parametrize("sigma", [0.0, 0.6])
+def test_periodic_sort_forces_a_rebuild_only_for_a_moved_order(sigma):
+    # no forces (no consumer) and few steps: no displacement trigger, so only
+    # the periodic sort can rebuild. Its stable cell order is computed here
+    # from the positions the step's kick-drift will produce (x + v dt: v is
+    # constant without forces).
+    S = 5
+    r, d = small_box(7030, 2000, 0.3, sigma, every=0, check=1, sort_every=S)
+    r.setup()
+    r.step(1)
+    sorts = []
+    for s in range(2, 4 * S + 1):
+        a = r.atoms()
+        nl = a["nlocal"]
+        x = a["pos"][:nl] + a["vel"][:nl] * d.dt
+        before = r.counts()["rebuilds"]
+        r.step(1)
+        rebuilt = r.counts()["rebuilds"] != before
+        if s % S == 0:
+            ident = sort_is_identity(x, r.grid())
+            assert rebuilt == (not ident), (s, ident, rebuilt)
+            sorts.append(ident)
+        else:
+            assert not rebuilt, f"step {s} rebuilt without a trigger"
+    if sigma == 0.0:
+        assert all(sorts)
+    else:
+        assert not all(sorts)
+
+
+def test_cells_match_the_oracle_bins():
+    # the cell formula used above is the oracle's own binning (bins pin it to a stable counting sort)
+    r, _ = small_box(7040, 1500, 0.35, 0.5)
+    r.setup()
+    r.step(2)
+    a = r.atoms()
+    _, _, _, atom_cell = r.bins()
+    assert np.array_equal(cells(a["pos"], r.grid()), atom_cell.astype(np.int64))
