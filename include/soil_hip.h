@@ -103,13 +103,22 @@ int soil_hip_upload(soil_hip_box* box, size_t n, const double* pos, const double
 /* Column data for locals [0, n): n*width f64 */
 int soil_hip_upload_column(soil_hip_box* box, int col, const double* data, size_t n);
 
+/* Device bytes of a box: *estimate = what its set-up allocates (before set-up:
+ * from the uploaded atoms; what the server's HBM admission counts), and
+ * *allocated = what its arenas hold now (0 before set-up; grows with the
+ * ghost and pair capacities). Either pointer may be NULL. No reference
+ * counterpart. */
+int soil_hip_box_device_bytes(soil_hip_box* box, size_t* estimate, size_t* allocated);
+
 /* Setup systems: neighbor_setup (neighbor.rs:604-665), comm_setup
  * (comm.rs:253-283) and setup_deform (soil_deform/src/lib.rs:346-420).
  * Sizes device buffers for the uploaded atoms. Two environment variables,
  * read here, tune the CSR build without changing its output:
- * SOIL_HIP_CSR_MODE (0-3, default 3: one-pass count with wave-staged
- * candidates) and SOIL_HIP_ROW_SLOTS (4-64, default 12: pair-arena entries
- * kept per local row by the one-pass count; longer rows are walked again);
+ * SOIL_HIP_CSR_MODE (0-4, default 4: each wave's rows' candidates flattened
+ * over its 64 lanes, accepted pairs compacted into a dense region per wave;
+ * 3: one thread per row with wave-staged candidates) and SOIL_HIP_ROW_SLOTS
+ * (4-64, default 12: pair-arena entries kept per local row, the size of a
+ * wave's region in mode 4; a wave or row beyond it is walked again);
  * SOIL_HIP_MAX_GRID (256-65536, default 4096) caps the blocks of a box's
  * launches (on 10^6 spheres 4096 is fastest: 1024 costs 14 %, 8192 1 %). */
 int soil_hip_setup(soil_hip_box* box);
@@ -274,9 +283,18 @@ int soil_hip_grid_for(const soil_hip_box_desc* desc, const double* radius, size_
 /* Boxes must be set up and not used individually while in the batch. */
 int soil_hip_batch_create(soil_hip_ctx* ctx, soil_hip_box** boxes, int nboxes, soil_hip_batch** out);
 int soil_hip_batch_destroy(soil_hip_batch* batch);
-/* Autotune launch shapes for this batch (bit-neutral knobs only); may run steps
- * on a scratch copy, never on the batch state. */
+/* Autotunes the batch's bit-neutral knobs: the CSR build's strategy (3: one
+ * thread per row walking its candidates; 4: a wave's 64 lanes sharing its
+ * rows' flattened candidates) and the blocks per launch (1024-16384), by
+ * timing a few steps of each setting on the live batch between snapshots of
+ * every box, which are restored after each trial (the batch state is
+ * unchanged). Bin size is not a knob: the grid fixes the CSR row order. */
 int soil_hip_batch_autotune(soil_hip_batch* batch);
+/* What the last autotune chose and measured: out[0] blocks per launch,
+ * out[1] CSR mode, out[2] ms per step of that setting, out[3] trials, then
+ * per trial (mode, blocks per launch, ms per step). *n receives the length
+ * (out may be NULL to query it). */
+int soil_hip_batch_get_tuning(soil_hip_batch* batch, double* out, size_t cap, size_t* n);
 /* n timesteps of every box; use_graph = 1 replays one captured step graph. */
 int soil_hip_batch_step(soil_hip_batch* batch, int nsteps, int use_graph);
 int soil_hip_batch_sync(soil_hip_batch* batch);
@@ -319,18 +337,26 @@ int soil_hip_server_set_output(soil_hip_server* server, const char* dir, int for
 int soil_hip_server_evict(soil_hip_server* server, int job_id, const char* path, int format);
 /* Waits for every submitted job, then stops the loop. */
 int soil_hip_server_drain(soil_hip_server* server);
-/* out21: jobs_done, jobs_failed, particle_steps, busy_s, wall_s, step_p50_ms, step_p99_ms,
- * completion_p50_s, completion_p99_s, wait_p50_s, wait_p99_s, mean_active_jobs, reconfigs,
- * reconfig_s, budget_gb, chunks, step_mean_ms, completion_max_s, wait_max_s, server_steps,
- * jobs_evicted */
-int soil_hip_server_stats(soil_hip_server* server, double* out21);
+/* out[SOIL_HIP_SERVER_NSTATS]: 0 jobs_done, 1 jobs_failed, 2 particle_steps,
+ * 3 busy_s (device time of the chunks), 4 wall_s, 5-6 step p50/p99 (ms),
+ * 7-8 completion p50/p99 (s), 9-10 queue wait p50/p99 (s), 11 mean_active_jobs,
+ * 12 reconfigs, 13 reconfig_s, 14 budget_gb, 15 chunks, 16 step mean (ms),
+ * 17 completion max (s), 18 wait max (s), 19 server_steps, 20 jobs_evicted,
+ * 21-22 device step p50/p99 (ms), 23 update gap p99 (ms).
+ * Step latency is per job and per step: the wall time between the job's
+ * consecutive state updates (its admission, then each chunk that moved it),
+ * re-batching, admission and read-back gaps included, over the steps that
+ * update brought. The device step is a chunk's GPU time over its steps; the
+ * update gap is the wall time between a job's consecutive updates. */
+#define SOIL_HIP_SERVER_NSTATS 24
+int soil_hip_server_stats(soil_hip_server* server, double* out);
 /* Final local positions/velocities of a finished job submitted with keep_result. */
 int soil_hip_server_result(soil_hip_server* server, int job_id, double* pos, double* vel, size_t cap, size_t* n);
 int soil_hip_server_destroy(soil_hip_server* server);
 /* Benchmark driver: njobs jobs of the served-stream workload (config 5) arrive
  * as a Poisson process of `rate` jobs/s and are submitted by a client thread. */
 int soil_hip_server_bench_stream(soil_hip_ctx* ctx, int njobs, double rate, uint64_t seed, double budget_gb,
-                                 int max_slots, int chunk_steps, int use_graph, double* stats_out21);
+                                 int max_slots, int chunk_steps, int use_graph, double* stats_out);
 
 /* ── restart and dump files (soil_print) ───────────────────────────────── */
 /* RestartData (soil_print/src/lib.rs:317-398): bincode 1 as written by

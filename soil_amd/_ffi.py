@@ -100,6 +100,7 @@ _SIGS = {
     "soil_hip_get_device_view": ([P, C.POINTER(DeviceView)], C.c_int),
     "soil_hip_run_force_hook": ([P], C.c_int),
     "soil_hip_classify_boundary_atoms": ([P, C.POINTER(C.c_uint8), SZ, C.POINTER(SZ)], C.c_int),
+    "soil_hip_box_device_bytes": ([P, SZP, SZP], C.c_int),
     "soil_hip_compute_ke": ([P, C.POINTER(C.c_uint8), C.POINTER(C.c_double)], C.c_int),
     "soil_hip_compute_ke_noalloc": ([P, C.POINTER(C.c_uint8), C.POINTER(C.c_double)], C.c_int),
     "soil_hip_set_axis_deform": ([P, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double], C.c_int),
@@ -143,6 +144,7 @@ _SIGS = {
     "soil_hip_batch_create": ([P, C.POINTER(P), C.c_int, C.POINTER(P)], C.c_int),
     "soil_hip_batch_destroy": ([P], C.c_int),
     "soil_hip_batch_autotune": ([P], C.c_int),
+    "soil_hip_batch_get_tuning": ([P, DP, SZ, SZP], C.c_int),
     "soil_hip_batch_step": ([P, C.c_int, C.c_int], C.c_int),
     "soil_hip_batch_sync": ([P], C.c_int),
     "soil_hip_set_timing": ([P, C.c_int], C.c_int),

@@ -34,35 +34,27 @@ static int batch_tiles(soil_hip_batch* bt) {
     bt->L.boxes = bt->d_ptrs;
     bt->L.stream = bt->stream;
     bt->L.maxgrid = maxgrid;
-    bt->L.csr_mode = bt->boxes.empty() ? 3 : bt->boxes[0]->h_dev.csr_mode;
+    bt->L.csr_mode = bt->boxes.empty() ? 4 : bt->boxes[0]->h_dev.csr_mode;
     return 0;
 }
 
-// One step of every box. Force hooks are called here too, so under capture
-// their launches are recorded into the graph; they read counts and buffers
-// from their device views, refreshed by k_sync_view just before.
-static int launch_step(soil_hip_batch* bt) {
-    const Launch& L = bt->L;
-    phase_initial_decide_pending(L);  // the previous step's final, the step's deform work, the kick-drift
-    bool zero = false;        // forces of boxes without a consumer, #[zero] columns
-    for (auto* b : bt->boxes) {
-        zero |= !b->consumer;
-        for (const auto& c : b->cols) zero |= (c.flags & SOIL_HIP_COL_ZERO) != 0;
-    }
-    if (zero) phase_zero_fused(L);
-    phase_pbc_sort_fused(L);
-    phase_rebuild_fused(L, nullptr, nullptr, nullptr, nullptr, false);
+// The Force phase to the step's end (consumer, hooks, reverse comm, the
+// step's end with its final integration left pending) for `boxes` launched
+// through L. Force hooks are called here too, so under capture their launches
+// are recorded into the graph; they read counts and buffers from their device
+// views, refreshed by k_sync_view just before.
+static int launch_tail(const Launch& L, const std::vector<soil_hip_box*>& boxes) {
     phase_force_fused(L);
     int rc = 0;
     bool hooks = false;
-    for (auto* b : bt->boxes) hooks |= b->hook != nullptr;
+    for (auto* b : boxes) hooks |= b->hook != nullptr;
     if (hooks) {
         phase_sync_view(L);
-        for (auto* b : bt->boxes) {
+        for (auto* b : boxes) {
             if (!b->hook) continue;
             soil_hip_device_view v;
             fill_view(b, v, false, 0);
-            const int r = b->hook(b->hook_user, &v, b->h_dev.view, (void*)bt->stream);
+            const int r = b->hook(b->hook_user, &v, b->h_dev.view, (void*)L.stream);
             if (r && !rc) rc = set_error(SOIL_HIP_EHOOK, "force hook returned %d", r);
         }
     }
@@ -70,6 +62,23 @@ static int launch_step(soil_hip_batch* bt) {
     phase_step_end_pending(L);  // final_integration: the next step's kick-drift or phase_final_pending
     return rc;
 }
+
+// One step of `boxes` launched through L (the batch, or one box of it).
+static int launch_step_on(const Launch& L, const std::vector<soil_hip_box*>& boxes) {
+    phase_initial_decide_pending(L);  // the previous step's final, the step's deform work, the kick-drift
+    bool zero = false;        // forces of boxes without a consumer, #[zero] columns
+    for (auto* b : boxes) {
+        zero |= !b->consumer;
+        for (const auto& c : b->cols) zero |= (c.flags & SOIL_HIP_COL_ZERO) != 0;
+    }
+    if (zero) phase_zero_fused(L);
+    phase_pbc_sort_fused(L);
+    phase_rebuild_fused(L, nullptr, nullptr, nullptr, nullptr, false);
+    return launch_tail(L, boxes);
+}
+
+// One step of every box.
+static int launch_step(soil_hip_batch* bt) { return launch_step_on(bt->L, bt->boxes); }
 
 static void drop_graph(soil_hip_batch* bt) {
     if (bt->graph_exec) (void)hipGraphExecDestroy(bt->graph_exec);
@@ -91,34 +100,162 @@ static int capture_graph(soil_hip_batch* bt) {
     return 0;
 }
 
-// Reads every box's state; grows boxes whose ghosts or pairs approach capacity.
-// Returns >0 when a box was reallocated (tiles and graph must be rebuilt).
-static int batch_check(soil_hip_batch* bt, int* grown) {
-    *grown = 0;
-    for (auto* b : bt->boxes) {
+// Grows box b's ghost capacity (`ghost`) and/or pair capacity (`pairs`, to
+// at least 1.5 need_pairs), keeping its atoms; pushes the descriptor and
+// rebuilds the box's own tile tables. The box's stream must be idle.
+static int grow_box(soil_hip_box* b, bool ghost, bool pairs, uint64_t need_pairs) {
+    if (ghost) {
+        Caps nc = b->caps;
+        const uint64_t ghost_cap = (uint64_t)b->caps.atoms - b->caps.local;
+        nc.atoms = (uint32_t)std::min<uint64_t>((uint64_t)b->caps.local + 2 * ghost_cap + 4096, 0x7FFFFFFFull);
+        nc.tiles = (nc.atoms + 63) / 64;
+        int rc = alloc_atoms(b, nc, true);
+        if (rc) return rc;
+    }
+    if (pairs) {
+        const uint64_t want = std::max<uint64_t>(2ull * b->caps.pairs, need_pairs + need_pairs / 2) + 4096;
+        int rc = alloc_pairs(b, (uint32_t)std::min<uint64_t>(want, 0xFFFFFFF0ull), true);
+        if (rc) return rc;
+    }
+    int rc = push_desc(b);
+    return rc ? rc : rebuild_tiles(b);
+}
+
+// A box frozen in a batch at a step whose borders or build ran out of
+// capacity: grows it and redoes that step's build and tail on the box's own
+// stream, as the plain step does (the phases before the build already ran).
+// The freeze lowered the box's step limit on the device: `limit` (its own
+// limit) is restored first, or every kernel would skip the box. Leaves the
+// step finished (its final integration pending).
+static int redo_failed_step(soil_hip_box* b, uint64_t limit) {
+    for (int attempt = 0; attempt < 6; ++attempt) {
         int rc = pull_state(b);
         if (rc) return rc;
-        const BoxState& st = *b->h_st;
-        if (st.err) return check_err_bits(st.err);
-        const uint64_t ghost_cap = (uint64_t)b->caps.atoms - b->caps.local;
-        if ((uint64_t)st.nghost * 10 > ghost_cap * 8) {
-            Caps nc = b->caps;
-            nc.atoms = (uint32_t)std::min<uint64_t>((uint64_t)b->caps.local + 2 * ghost_cap + 4096, 0x7FFFFFFFull);
-            nc.tiles = (nc.atoms + 63) / 64;
-            rc = alloc_atoms(b, nc, true);
-            if (rc) return rc;
-            *grown = 1;
+        if (b->h_st->step_limit != limit) {
+            b->h_st->step_limit = limit;
+            HIPCHK(hipMemcpy(&b->d_st->step_limit, &b->h_st->step_limit, 8, hipMemcpyHostToDevice));
         }
-        if ((uint64_t)st.npairs * 10 > (uint64_t)b->caps.pairs * 8) {
-            rc = alloc_pairs(b, (uint32_t)std::min<uint64_t>(2ull * b->caps.pairs + 4096, 0xFFFFFFF0ull), true);
+        const uint32_t err = b->h_st->err;
+        if (err & ~(ERR_GHOST_CAPACITY | ERR_PAIR_CAPACITY)) return check_err_bits(err);
+        if (err & ERR_GHOST_CAPACITY) {
+            rc = grow_box(b, true, false, 0);
             if (rc) return rc;
-            *grown = 1;
+            b->h_st->err &= ~ERR_GHOST_CAPACITY;
+            HIPCHK(hipMemcpy(&b->d_st->err, &b->h_st->err, 4, hipMemcpyHostToDevice));
+            phase_borders(b->L);
+            phase_forward(b->L);
+            phase_build_fused(b->L, nullptr, nullptr, nullptr, nullptr);
+            continue;
         }
-        if (*grown) {
-            rc = push_desc(b);
+        if (err & ERR_PAIR_CAPACITY) {
+            rc = grow_box(b, false, true, b->h_st->npairs);
             if (rc) return rc;
+            // the build runs again from its start: it counts the rebuild and
+            // flips the state to CommunicateOnly once more
+            b->h_st->err &= ~ERR_PAIR_CAPACITY;
+            b->h_st->rebuilds -= 1;
+            b->h_st->state = 0;
+            HIPCHK(hipMemcpy(b->d_st, b->h_st, sizeof(BoxState), hipMemcpyHostToDevice));
+            phase_build(b->L, nullptr, nullptr, nullptr, nullptr);
+            continue;
         }
+        rc = launch_tail(b->L, {b});
+        if (rc) return rc;
+        HIPCHK(hipStreamSynchronize(b->stream));
+        return 0;
     }
+    return set_error(SOIL_HIP_ECAPACITY, "capacity growth of a batched box failed");
+}
+
+// Brings the batch's boxes to a consistent point after a run or a call:
+// `col` holds each box's err, nghost, npairs, steps (k_collect; null: read
+// them now). A box that froze on a capacity error is grown, finishes its
+// failed step and is stepped alone (through the batch's geometry ring, whose
+// slots for those steps are still in place) up to the steps the batch ran; a
+// box past 80 % of its ghost or pair capacity is grown before it overflows.
+// `call_ended`: the batch call applied its pending final integrations, so a
+// box stepped here applies its own. Re-tiles the batch and drops its graph
+// when a box grew. Returns 1 when it acted (bt->h_col_now then holds every
+// box's counters afterwards; the stream is idle), 0 when nothing was needed.
+int batch_settle(soil_hip_batch* bt, const uint32_t* col, bool call_ended) {
+    const size_t nb = bt->boxes.size();
+    auto need = [&](size_t k, const uint32_t* c) {
+        const soil_hip_box* b = bt->boxes[k];
+        const uint64_t ghost_cap = (uint64_t)b->caps.atoms - b->caps.local;
+        return c[4 * k] != 0u || (uint64_t)c[4 * k + 1] * 10 > ghost_cap * 8 ||
+               (uint64_t)c[4 * k + 2] * 10 > (uint64_t)b->caps.pairs * 8;
+    };
+    if (col) {
+        bool any = false;
+        for (size_t k = 0; k < nb && !any; ++k) any = need(k, col);
+        if (!any) return 0;
+    }
+    HIPCHK(hipStreamSynchronize(bt->stream));
+    bt->half_pending[0] = bt->half_pending[1] = false;
+    phase_collect(bt->L, bt->d_col);  // the state after the drain
+    HIPCHK(hipMemcpyAsync(bt->h_col_now, bt->d_col, 16 * nb, hipMemcpyDeviceToHost, bt->stream));
+    HIPCHK(hipStreamSynchronize(bt->stream));
+    const uint32_t* c = bt->h_col_now;
+    bool grown = false;
+    for (size_t k = 0; k < nb; ++k) {
+        if (!need(k, c)) continue;
+        soil_hip_box* b = bt->boxes[k];
+        const uint32_t err = c[4 * k];
+        if (err & ~(ERR_GHOST_CAPACITY | ERR_PAIR_CAPACITY)) return check_err_bits(err);
+        grown = true;
+        if (!err) {  // headroom: grow between steps
+            const uint64_t ghost_cap = (uint64_t)b->caps.atoms - b->caps.local;
+            int rc = grow_box(b, (uint64_t)c[4 * k + 1] * 10 > ghost_cap * 8,
+                              (uint64_t)c[4 * k + 2] * 10 > (uint64_t)b->caps.pairs * 8, c[4 * k + 2]);
+            if (rc) return rc;
+            continue;
+        }
+        // frozen at a failed step: finish the step (with its step limit
+        // restored), then step it alone to where the batch is
+        for (int d = 0; d < 3; ++d) b->L.rev_split[d] = 1;  // separate ordered swaps: right for any geometry
+        int rc = redo_failed_step(b, bt->limit[k]);
+        if (!rc) rc = pull_state(b);
+        if (rc) return rc;
+        const uint64_t target =
+            std::min<uint64_t>(bt->limit[k], (uint64_t)((int64_t)bt->steps_done + bt->step_offset[k]));
+        while (b->h_st->steps < target) {
+            const uint64_t before = b->h_st->steps;
+            rc = launch_step_on(b->L, {b});
+            if (!rc) rc = pull_state(b);
+            if (!rc && b->h_st->err) {  // froze again: grow, finish that step
+                rc = redo_failed_step(b, bt->limit[k]);
+                if (!rc) rc = pull_state(b);
+            }
+            if (rc) return rc;
+            if (b->h_st->steps != before + 1)
+                return set_error(SOIL_HIP_ESTATE, "batched box %zu did not advance past step %llu", k,
+                                 (unsigned long long)before);
+        }
+        if (call_ended) phase_final_pending(b->L);
+        HIPCHK(hipStreamSynchronize(b->stream));
+        rc = pull_state(b);
+        if (rc) return rc;
+        if (b->h_st->steps != target)
+            return set_error(SOIL_HIP_ESTATE, "batched box %zu stepped to %llu, not %llu", k,
+                             (unsigned long long)b->h_st->steps, (unsigned long long)target);
+    }
+    if (grown) {
+        drop_graph(bt);
+        int rc = batch_tiles(bt);
+        if (rc) return rc;
+    }
+    phase_collect(bt->L, bt->d_col);
+    HIPCHK(hipMemcpyAsync(bt->h_col_now, bt->d_col, 16 * nb, hipMemcpyDeviceToHost, bt->stream));
+    HIPCHK(hipStreamSynchronize(bt->stream));
+    return grown ? 1 : 0;
+}
+
+// Records each box's err, nghost, npairs and steps of the run just enqueued
+// into the host buffer of half h (read when the host next waits on that half).
+static int record_collect(soil_hip_batch* bt, uint32_t h) {
+    phase_collect(bt->L, bt->d_col + 4 * bt->boxes.size() * (h + 1));
+    HIPCHK(hipMemcpyAsync(bt->h_col[h], bt->d_col + 4 * bt->boxes.size() * (h + 1), 16 * bt->boxes.size(),
+                          hipMemcpyDeviceToHost, bt->stream));
     return 0;
 }
 
@@ -134,6 +271,14 @@ static int batch_run(soil_hip_batch* bt, int nsteps, bool use_graph) {
         if (bt->half_pending[h]) {
             HIPCHK(hipEventSynchronize(bt->half_done[h]));
             bt->half_pending[h] = false;
+            // a box that froze or filled up in that run is handled before its
+            // geometry slots are written again
+            int rc = batch_settle(bt, bt->h_col[h], false);
+            if (rc < 0) return rc;
+            if (use_graph && !bt->graph_exec) {
+                rc = capture_graph(bt);
+                if (rc) return rc;
+            }
         }
         // host geometry for the run: staging rows [slot][box]
         bool regrid = false;
@@ -191,6 +336,8 @@ static int batch_run(soil_hip_batch* bt, int nsteps, bool use_graph) {
             }
         }
         if (done + (int)run >= nsteps) phase_final_pending(bt->L);  // the call's last step
+        int rcc = record_collect(bt, h);
+        if (rcc) return rcc;
         HIPCHK(hipEventRecord(bt->half_done[h], bt->stream));
         bt->half_pending[h] = true;
         bt->steps_done += run;
@@ -248,6 +395,23 @@ int batch_make(soil_hip_ctx* ctx, soil_hip_box** boxes, int nboxes, soil_hip_bat
         b->in_batch = true;
         ptrs[k] = b->d_dev;
     }
+    // per box: the step limit to restore after a freeze, and its step count
+    // relative to the batch's (the step it must reach when the batch is at s)
+    for (uint32_t k = 0; k < nb; ++k) {
+        const int rc = pull_state(boxes[k]);
+        if (rc) {
+            batch_free(bt, false);
+            return rc;
+        }
+        bt->limit.push_back(boxes[k]->h_st->step_limit);
+        bt->step_offset.push_back((int64_t)boxes[k]->h_st->steps);
+    }
+    HIPCHK(hipMalloc(&bt->d_col, 3 * 16 * (size_t)nb));
+    HIPCHK(hipHostMalloc(&bt->h_col[0], 16 * (size_t)nb, hipHostMallocDefault));
+    HIPCHK(hipHostMalloc(&bt->h_col[1], 16 * (size_t)nb, hipHostMallocDefault));
+    HIPCHK(hipHostMalloc(&bt->h_col_now, 16 * (size_t)nb, hipHostMallocDefault));
+    std::memset(bt->h_col[0], 0, 16 * (size_t)nb);
+    std::memset(bt->h_col[1], 0, 16 * (size_t)nb);
     HIPCHK(hipMalloc(&bt->d_ptrs, sizeof(BoxDev*) * nb));
     HIPCHK(hipMemcpy(bt->d_ptrs, ptrs.data(), sizeof(BoxDev*) * nb, hipMemcpyHostToDevice));
     bt->L.maxgrid = 4096;
@@ -284,6 +448,9 @@ int batch_free(soil_hip_batch* bt, bool restore) {
         b->in_batch = false;
     }
     if (bt->arena_tiles) (void)hipFree(bt->arena_tiles);
+    if (bt->d_col) (void)hipFree(bt->d_col);
+    for (auto p : {bt->h_col[0], bt->h_col[1], bt->h_col_now})
+        if (p) (void)hipHostFree(p);
     if (bt->d_ptrs) (void)hipFree(bt->d_ptrs);
     if (bt->d_ring) (void)hipFree(bt->d_ring);
     if (bt->h_ring) (void)hipHostFree(bt->h_ring);
@@ -313,7 +480,12 @@ int soil_hip_batch_create(soil_hip_ctx* ctx, soil_hip_box** boxes, int nboxes, s
     return batch_make(ctx, boxes, nboxes, out);
 }
 
-int soil_hip_batch_destroy(soil_hip_batch* bt) { return batch_free(bt, true); }
+int soil_hip_batch_destroy(soil_hip_batch* bt) {
+    // a box that froze in the last call still finishes its steps before it
+    // is handed back (best effort: the batch goes away either way)
+    if (bt && hipSetDevice(bt->ctx->device) == hipSuccess) (void)batch_settle(bt, nullptr, true);
+    return batch_free(bt, true);
+}
 
 int soil_hip_batch_step(soil_hip_batch* bt, int nsteps, int use_graph) {
     if (!bt || nsteps < 0) return set_error(SOIL_HIP_EINVAL, "bad arguments");
@@ -323,16 +495,9 @@ int soil_hip_batch_step(soil_hip_batch* bt, int nsteps, int use_graph) {
 
 int soil_hip_batch_sync(soil_hip_batch* bt) {
     if (!bt) return set_error(SOIL_HIP_EINVAL, "null batch");
-    HIPCHK(hipStreamSynchronize(bt->stream));
-    bt->half_pending[0] = bt->half_pending[1] = false;
-    int grown = 0;
-    int rc = batch_check(bt, &grown);
-    if (rc) return rc;
-    if (grown) {
-        drop_graph(bt);
-        return batch_tiles(bt);
-    }
-    return 0;
+    HIPCHK(hipSetDevice(bt->ctx->device));
+    const int rc = batch_settle(bt, nullptr, true);
+    return rc < 0 ? rc : 0;
 }
 
 // Grows each box's cell arrays to the largest grid that its next `nsteps`
@@ -445,36 +610,98 @@ int soil_hip_batch_autotune(soil_hip_batch* bt) {
         bt->half_pending[0] = bt->half_pending[1] = false;
         return 0;
     };
-    const uint32_t cands[5] = {1024, 2048, 4096, 8192, 16384};
-    uint32_t best = bt->L.maxgrid;
-    double best_ms = 1e30;
-    for (uint32_t mg : cands) {
+    // the CSR strategy of every box (descriptor field and snapshot alike) and the launch context
+    auto set_mode = [&](int mode) -> int {
+        for (size_t k = 0; k < bt->boxes.size(); ++k) {
+            soil_hip_box* b = bt->boxes[k];
+            b->h_dev.csr_mode = mode;
+            snaps[k].dev.csr_mode = mode;
+            HIPCHK(hipMemcpy(&b->d_dev->csr_mode, &mode, sizeof(int), hipMemcpyHostToDevice));
+            b->L.csr_mode = mode;
+        }
+        bt->L.csr_mode = mode;
+        return 0;
+    };
+    auto trial_ms = [&](uint32_t mg, int mode, double& ms) -> int {
+        int rc = set_mode(mode);
+        if (rc) return rc;
         bt->L.maxgrid = mg;
         drop_graph(bt);
-        int rc = capture_graph(bt);
+        rc = capture_graph(bt);
         if (rc) return rc;
         rc = batch_run(bt, 1, true);  // warm
-        if (rc) return rc;
+        if (rc < 0) return rc;
         HIPCHK(hipStreamSynchronize(bt->stream));
         auto t0 = std::chrono::steady_clock::now();
         rc = batch_run(bt, trial, true);
-        if (rc) return rc;
+        if (rc < 0) return rc;
         HIPCHK(hipStreamSynchronize(bt->stream));
-        const double ms =
-            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-        rc = restore();
-        if (rc) return rc;
+        ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / trial;
+        bt->tune_trials.push_back({(double)mode, (double)mg, ms});
+        return restore();
+    };
+    auto free_snaps = [&]() {
+        for (auto& s : snaps) {
+            (void)hipFree(s.atoms);
+            (void)hipFree(s.pairs);
+        }
+    };
+    bt->tune_trials.clear();
+    // coordinate descent: the CSR strategy at the current launch cap (mode 3:
+    // one thread per row walking its own candidates; mode 4: a wave's 64
+    // lanes sharing its rows' flattened candidates), then the launch cap
+    const int modes[2] = {4, 3};
+    const uint32_t cands[5] = {1024, 2048, 4096, 8192, 16384};
+    const uint32_t mg0 = bt->L.maxgrid;
+    int best_mode = bt->L.csr_mode;
+    uint32_t best = mg0;
+    double best_ms = 1e30;
+    for (int mode : modes) {
+        double ms = 0.0;
+        const int rc = trial_ms(mg0, mode, ms);
+        if (rc) {
+            free_snaps();
+            return rc;
+        }
+        if (ms < best_ms) {
+            best_ms = ms;
+            best_mode = mode;
+        }
+    }
+    for (uint32_t mg : cands) {
+        if (mg == mg0) continue;
+        double ms = 0.0;
+        const int rc = trial_ms(mg, best_mode, ms);
+        if (rc) {
+            free_snaps();
+            return rc;
+        }
         if (ms < best_ms) {
             best_ms = ms;
             best = mg;
         }
     }
-    for (auto& s : snaps) {
-        (void)hipFree(s.atoms);
-        (void)hipFree(s.pairs);
-    }
+    free_snaps();
+    int rc = set_mode(best_mode);
+    if (rc) return rc;
     bt->L.maxgrid = best;
+    bt->tune_ms = best_ms;
     drop_graph(bt);
+    return 0;
+}
+
+int soil_hip_batch_get_tuning(soil_hip_batch* bt, double* out, size_t cap, size_t* n) {
+    if (!bt) return set_error(SOIL_HIP_EINVAL, "null batch");
+    const size_t need = 4 + 3 * bt->tune_trials.size();
+    if (n) *n = need;
+    if (!out) return 0;
+    if (cap < need) return set_error(SOIL_HIP_ECAPACITY, "tuning record needs %zu values", need);
+    out[0] = (double)bt->L.maxgrid;
+    out[1] = (double)bt->L.csr_mode;
+    out[2] = bt->tune_ms;
+    out[3] = (double)bt->tune_trials.size();
+    for (size_t t = 0; t < bt->tune_trials.size(); ++t)
+        for (int q = 0; q < 3; ++q) out[4 + 3 * t + q] = bt->tune_trials[t][q];
     return 0;
 }
 

@@ -127,6 +127,7 @@ void carve_atoms(Carver& c, BoxDev& d, const Caps& k, const std::vector<HostCol>
     c.take(d.offsets, L1);
     c.take(d.tcount, A + 1); c.take(d.tstart, A + 1);
     c.take(d.ctile, (size_t)k.local / BLK + 2);
+    c.take(d.wcount, (size_t)k.local / 64 + 2);
     c.take(d.bcnt, 6 * ((size_t)k.atoms / BORDER_TILE + 2));
     for (size_t i = 0; i < cols.size(); ++i) {
         c.take(d.cols[i].data, A * cols[i].width);
@@ -202,6 +203,10 @@ int rebuild_tiles(soil_hip_box* b) {
     build_tiles(tt, 0, b->caps);
     const int rc = upload_tiles(tt, b->L, &b->arena_tiles);
     if (rc) return rc;
+    b->tiles_bytes = 0;
+    for (auto* l : {&tt.box, &tt.local, &tt.local1, &tt.atom, &tt.atom1, &tt.ghost, &tt.cell, &tt.border,
+                    &tt.scan_cell, &tt.scan_local, &tt.scan_atom})
+        b->tiles_bytes += l->size() * sizeof(Tile);
     b->L.boxes = b->d_ptrs;
     b->L.stream = b->stream;
     b->L.csr_mode = b->h_dev.csr_mode;
@@ -219,6 +224,7 @@ int alloc_atoms(soil_hip_box* b, const Caps& nc, bool keep) {
     if (rc) return rc;
     Carver c{(char*)arena, 0, false};
     carve_atoms(c, nd, nc, b->cols);
+    const size_t atoms_bytes = dry.off + 256;
     if (keep) {
         int r2 = pull_desc(b);
         if (r2) return r2;
@@ -226,7 +232,9 @@ int alloc_atoms(soil_hip_box* b, const Caps& nc, bool keep) {
         if (r2) return r2;
         const BoxDev& od = b->h_dev;
         const size_t nl = b->nlocal;
-        const size_t n = std::min<size_t>((size_t)b->h_st->nlocal + b->h_st->nghost, nc.atoms);
+        // after a ghost overflow nghost may exceed what the old arena holds
+        const size_t n = std::min<size_t>(std::min<size_t>((size_t)b->h_st->nlocal + b->h_st->nghost, nc.atoms),
+                                          b->caps.atoms);
         hipError_t ce = hipSuccess;  // the first failed copy, reported after them all
         auto cp = [&](void* dst, const void* src, size_t bytes) {
             if (bytes && ce == hipSuccess) ce = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, b->stream);
@@ -264,6 +272,7 @@ int alloc_atoms(soil_hip_box* b, const Caps& nc, bool keep) {
     nd.cap_tiles = nc.tiles;
     if (b->arena_atoms) (void)hipFree(b->arena_atoms);
     b->arena_atoms = arena;
+    b->atoms_bytes = atoms_bytes;
     b->h_dev = nd;
     b->caps.local = nc.local;
     b->caps.atoms = nc.atoms;
@@ -272,10 +281,33 @@ int alloc_atoms(soil_hip_box* b, const Caps& nc, bool keep) {
     return 0;
 }
 
+// Entries per local row that the pair arena keeps through every growth
+// (SOIL_HIP_ROW_SLOTS, 4-64, default 12; read at set-up; bit-neutral).
+uint32_t row_slots_setting() {
+    const char* m = getenv("SOIL_HIP_ROW_SLOTS");
+    const int s = m ? atoi(m) : 12;
+    return (uint32_t)(s < 4 ? 4 : (s > 64 ? 64 : s));
+}
+
+// The pair arena's entries for a request of `cap_pairs` on a box of
+// `cap_local` locals: never below row_slots per local (the slots are
+// cap_pairs / cap_local), nor 4096.
+uint32_t pair_capacity(uint64_t cap_pairs, uint32_t row_slots, uint32_t cap_local) {
+    return (uint32_t)std::min<uint64_t>(
+        std::max<uint64_t>(std::max<uint64_t>(cap_pairs, 4096), (uint64_t)row_slots * cap_local), 0xFFFFFFF0ull);
+}
+
+uint32_t initial_pair_capacity(uint32_t row_slots, uint32_t cap_local) { return pair_capacity(4096, row_slots, cap_local); }
+
 int alloc_pairs(soil_hip_box* b, uint32_t cap_pairs, bool keep) {
-    // every growth keeps the row-slot width (the slots are cap_pairs / cap_local)
-    cap_pairs = (uint32_t)std::min<uint64_t>(
-        std::max<uint64_t>(cap_pairs, (uint64_t)b->row_slots * b->caps.local), 0xFFFFFFF0ull);
+    // every growth keeps the row-slot width
+    cap_pairs = pair_capacity(cap_pairs, b->row_slots, b->caps.local);
+    // the host copy of the descriptor must be current: it is pushed back whole
+    // after the growth, and the spatial sort swaps column pointers on the device
+    if (b->arena_pairs) {
+        const int rp = pull_desc(b);
+        if (rp) return rp;
+    }
     Caps nc = b->caps;
     nc.pairs = cap_pairs;
     BoxDev nd = b->h_dev;
@@ -289,7 +321,8 @@ int alloc_pairs(soil_hip_box* b, uint32_t cap_pairs, bool keep) {
     if (keep && b->arena_pairs) {
         rc = pull_state(b);
         if (rc) return rc;
-        const size_t np = std::min<size_t>(b->h_st->npairs, cap_pairs);
+        // after an overflow npairs is what the build needed, beyond the old arena
+        const size_t np = std::min<size_t>(std::min<size_t>(b->h_st->npairs, cap_pairs), b->caps.pairs);
         if (np) {
             HIPCHK(hipMemcpyAsync(nd.indices, b->h_dev.indices, np * 4, hipMemcpyDeviceToDevice, b->stream));
             HIPCHK(hipMemcpyAsync(nd.tsrc, b->h_dev.tsrc, np * 4, hipMemcpyDeviceToDevice, b->stream));
@@ -298,6 +331,7 @@ int alloc_pairs(soil_hip_box* b, uint32_t cap_pairs, bool keep) {
     }
     if (b->arena_pairs) (void)hipFree(b->arena_pairs);
     b->arena_pairs = arena;
+    b->pairs_bytes = dry.off + 256;
     b->h_dev.indices = nd.indices;
     b->h_dev.ttmp = nd.ttmp;
     b->h_dev.tslot = nd.tslot;
@@ -635,13 +669,27 @@ size_t box_bytes_estimate(soil_hip_box* b) {
     c.atoms = (uint32_t)std::min<uint64_t>(b->nlocal + ghost + ghost / 2 + 1024, 0x7FFFFFFFull);
     c.cells = (uint32_t)m.max_cells();
     c.tiles = ceil_div(c.atoms, 64);
-    c.pairs = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(8 * (uint64_t)b->nlocal, 4096), 0xFFFFFFF0u);
+    c.pairs = initial_pair_capacity(row_slots_setting(), c.local);
     BoxDev d{};
     Carver a{nullptr, 0, true};
     carve_atoms(a, d, c, b->cols);
     Carver p{nullptr, 0, true};
     carve_pairs(p, d, c);
-    return a.off + p.off + sizeof(Geom) * b->ring_len + 4096;
+    TileTables tt;
+    build_tiles(tt, 0, c);
+    const std::vector<Tile>* lists[11] = {&tt.box,  &tt.local,  &tt.local1,    &tt.atom,       &tt.atom1,   &tt.ghost,
+                                          &tt.cell, &tt.border, &tt.scan_cell, &tt.scan_local, &tt.scan_atom};
+    size_t tiles = 0;
+    for (auto* l : lists) tiles += l->size();
+    // arenas as alloc_atoms / alloc_pairs size them, the tile tables, the
+    // geometry ring, and the descriptor, state and view records
+    return (a.off + 256) + (p.off + 256) + tiles * sizeof(Tile) + sizeof(Geom) * b->ring_len + 4096;
+}
+
+size_t box_bytes_allocated(soil_hip_box* b) {
+    if (!b->setup_done) return 0;
+    return b->atoms_bytes + b->pairs_bytes + b->tiles_bytes + sizeof(Geom) * b->ring_len + sizeof(BoxDev) +
+           sizeof(BoxState) + sizeof(soil_hip_device_view) + sizeof(BoxDev*);
 }
 }  // namespace soil_hip
 
@@ -781,6 +829,7 @@ int soil_hip_setup(soil_hip_box* b) {
     if (b->setup_done) return set_error(SOIL_HIP_ESTATE, "setup already done");
     HIPCHK(hipSetDevice(b->ctx->device));
     const size_t n = b->nlocal;
+    b->est_bytes = box_bytes_estimate(b);
     b->model.setup(b->hrad.data(), n);
     if (b->model.max_stencil() > SOIL_MAX_STENCIL)
         return set_error(SOIL_HIP_EUNSUPPORTED, "stencil of %zu offsets exceeds %d", b->model.max_stencil(),
@@ -812,11 +861,7 @@ int soil_hip_setup(soil_hip_box* b) {
     b->ring_ok.assign(b->ring_len, 0);
     int rc = alloc_atoms(b, c, false);
     if (rc) return rc;
-    {
-        const char* m = getenv("SOIL_HIP_ROW_SLOTS");  // bit-neutral tuning knob
-        const int s = m ? atoi(m) : 12;
-        b->row_slots = (uint32_t)(s < 4 ? 4 : (s > 64 ? 64 : s));
-    }
+    b->row_slots = row_slots_setting();
     rc = alloc_pairs(b, 4096, false);  // raised to row_slots per local
     if (rc) return rc;
     BoxDev& d = b->h_dev;
@@ -862,8 +907,8 @@ int soil_hip_setup(soil_hip_box* b) {
     d.transpose = (b->consumer || (b->hook && (b->hook_flags & SOIL_HIP_HOOK_TRANSPOSE))) ? 1 : 0;
     {
         const char* m = getenv("SOIL_HIP_CSR_MODE");
-        d.csr_mode = m ? atoi(m) : 3;
-        if (d.csr_mode < 0 || d.csr_mode > 3) d.csr_mode = 3;
+        d.csr_mode = m ? atoi(m) : 4;
+        if (d.csr_mode < 0 || d.csr_mode > 4) d.csr_mode = 4;
     }
     BoxState st{};
     st.nlocal = (uint32_t)n;
@@ -1286,6 +1331,16 @@ int soil_hip_classify_boundary_atoms(soil_hip_box* b, uint8_t* mask, size_t cap,
     std::vector<uint32_t> flags(nl);
     if (nl) HIPCHK(hipMemcpy(flags.data(), b->h_dev.slot, nl * 4, hipMemcpyDeviceToHost));
     for (size_t i = 0; i < nl; ++i) mask[i] = (uint8_t)flags[i];
+    return 0;
+}
+
+int soil_hip_box_device_bytes(soil_hip_box* b, size_t* estimate, size_t* allocated) {
+    if (!b) return set_error(SOIL_HIP_EINVAL, "null box");
+    if (estimate) {
+        if (!b->setup_done && b->nlocal == 0) return set_error(SOIL_HIP_ESTATE, "no atoms uploaded");
+        *estimate = b->setup_done ? b->est_bytes : box_bytes_estimate(b);
+    }
+    if (allocated) *allocated = box_bytes_allocated(b);
     return 0;
 }
 

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <string>
 #include <vector>
 
@@ -39,6 +40,13 @@ struct soil_hip_batch;
 namespace soil_hip {
 int alloc_atoms(soil_hip_box* b, const Caps& nc, bool keep);
 int alloc_pairs(soil_hip_box* b, uint32_t cap_pairs, bool keep);
+uint32_t row_slots_setting();
+uint32_t pair_capacity(uint64_t cap_pairs, uint32_t row_slots, uint32_t cap_local);
+uint32_t initial_pair_capacity(uint32_t row_slots, uint32_t cap_local);
+// Device bytes a box will hold after set-up (its arenas at their initial capacities).
+size_t box_bytes_estimate(soil_hip_box* b);
+// Device bytes soil_hip_setup allocated for the box's arenas (0 before set-up).
+size_t box_bytes_allocated(soil_hip_box* b);
 int push_desc(soil_hip_box* b);
 int pull_desc(soil_hip_box* b);
 int pull_state(soil_hip_box* b);
@@ -51,6 +59,11 @@ int check_err_bits(uint32_t err);
 int batch_make(soil_hip_ctx* ctx, soil_hip_box** boxes, int nboxes, soil_hip_batch** out);
 int batch_free(soil_hip_batch* bt, bool restore);
 int batch_step(soil_hip_batch* bt, int nsteps, bool use_graph);
+// After a batch call or a run: grows boxes near their capacity, and grows,
+// finishes and steps up to the batch every box that froze on a capacity
+// error (col: err, nghost, npairs, steps per box, or null to read them).
+// 1 when it acted (bt->h_col_now then holds the counters), 0 when not, < 0 on error.
+int batch_settle(soil_hip_batch* bt, const uint32_t* col, bool call_ended);
 }  // namespace soil_hip
 
 struct soil_hip_box {
@@ -71,6 +84,8 @@ struct soil_hip_box {
     void* arena_atoms = nullptr;
     void* arena_pairs = nullptr;
     void* arena_tiles = nullptr;
+    size_t atoms_bytes = 0, pairs_bytes = 0, tiles_bytes = 0;  // sizes of the three arenas
+    size_t est_bytes = 0;  // box_bytes_estimate at set-up
     // entries per local row in the pair arena's row slots: the one-pass CSR
     // count stores a row's first row_slots pairs there and the fill walks a
     // longer row again on one lane, so the width sets how rarely a wave waits
@@ -116,6 +131,8 @@ struct soil_hip_batch {
     hipStream_t stream = nullptr;
     soil_hip::BoxDev** d_ptrs = nullptr;
     void* arena_tiles = nullptr;
+    size_t atoms_bytes = 0, pairs_bytes = 0, tiles_bytes = 0;  // sizes of the three arenas
+    size_t est_bytes = 0;  // box_bytes_estimate at set-up
     soil_hip::Launch L;
     soil_hip::Geom* d_ring = nullptr;  // [nbox][K]
     soil_hip::Geom* h_ring = nullptr;  // pinned [2][nbox][K/2] staging halves
@@ -127,6 +144,14 @@ struct soil_hip_batch {
     hipGraphExec_t graph_exec = nullptr;
     std::vector<const soil_hip::Geom*> saved_rings;
     std::vector<uint32_t> saved_ring_len;
+    // per box: err, nghost, npairs, steps after a run (k_collect): [0] now, [1..2] halves
+    uint32_t* d_col = nullptr;
+    uint32_t* h_col[2] = {nullptr, nullptr};
+    uint32_t* h_col_now = nullptr;
+    std::vector<uint64_t> limit;        // each box's step limit (restored after a freeze)
+    std::vector<int64_t> step_offset;   // a box's steps when the batch has run none
+    std::vector<std::array<double, 3>> tune_trials;  // autotune: (csr mode, blocks per launch, ms per step)
+    double tune_ms = 0.0;                             // the chosen setting's ms per step (0: not tuned)
     bool timing = false;
     hipEvent_t t0 = nullptr, t1 = nullptr;
     double t_ms = 0.0;

@@ -926,6 +926,11 @@ __device__ __forceinline__ void scan_view(CBox& b, uint32_t& n, bool& active, GM
         active = st->rebuild_now != 0 && (st->err & ERR_GHOST_CAPACITY) == 0;
         in = b.offsets;
         out = b.offsets;
+    } else if (W == SCAN_WAVES) {
+        n = ((st->nlocal + 63u) >> 6) + 1u;
+        active = st->rebuild_now != 0 && (st->err & ERR_GHOST_CAPACITY) == 0;
+        in = b.wcount;
+        out = b.wcount;
     } else {
         n = st->nlocal + st->nghost + 1;
         active = transpose_active(b);
@@ -993,7 +998,7 @@ __global__ void __launch_bounds__(BLK) k_scan_down(KARGS) {
 #pragma unroll
         for (uint32_t w = 0; w < BLK / 64; ++w) acc += red[w];
         __syncthreads();  // red[] is reused by the next tile
-        if (W == SCAN_CSR && first && threadIdx.x == 0) {
+        if ((W == SCAN_CSR || W == SCAN_WAVES) && first && threadIdx.x == 0) {
             BoxState* st = b.st;
             st->local_cells_valid = 0;  // consumed by this build (neighbor.rs:596)
             st->npairs = acc;
@@ -1923,6 +1928,242 @@ __global__ void __launch_bounds__(BLK) k_csr_count_w(KARGS) {
     }
 }
 
+// ── CSR build, mode 4: flattened per-wave candidates ──────────────────────
+// A wave takes 64 consecutive local rows. A row's candidates are a few
+// contiguous ranges (segments) of the bin-sorted records: the self cell after
+// the row's atom (under newton merged with a first run at offset +1; the whole
+// cell without newton), then each stencil run, clamped to the grid. Laid end
+// to end, row after row and segment after segment, they form one flattened
+// index space per wave, in the order a row's neighbours are appended (self
+// cell, then the stencil cells in order, each in bin order). Lane l tests the
+// flattened candidates l, l+64, ...: every lane stays busy whatever the rows'
+// lengths. A lane finds its segment by one mbcnt over a bitmap of segment
+// starts and reads the segment's row and first record from LDS. Accepted
+// candidates are compacted by ballot into the wave's dense region of the pair
+// arena, which then holds the wave's rows in CSR order; per-row counts are
+// LDS counters, and each row's offset within its wave goes to `offsets`. The
+// offset scan runs over the waves' totals (wcount) and the fill copies each
+// region to its place.
+constexpr uint32_t FLAT_WORDS = 64;  // segment-start bitmap words per wave: 4096 flattened candidates
+constexpr uint32_t FLAT_SEGS = 384;  // non-empty segments per wave
+
+__device__ __forceinline__ uint64_t wave_incl_scan64(uint64_t v, uint32_t lane) {
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint64_t y = __shfl_up(v, o, 64);
+        if (lane >= (uint32_t)o) v += y;
+    }
+    return v;
+}
+
+// Segment k of row i in cell c: k = -1 the self cell, k >= r0 the stencil runs.
+template <bool NEWTON>
+__device__ __forceinline__ void flat_segment(CBox& b, CGeom& g, uint32_t i, uint32_t c, int k, int32_t self_extra,
+                                             uint32_t& m0, uint32_t& m1) {
+    const int32_t ncell = (int32_t)g.ncells;
+    if (k < 0) {
+        if (NEWTON) {
+            int32_t ce = (int32_t)c + self_extra;
+            if (ce > ncell) ce = ncell;
+            m0 = b.sidx[i] + 1u;
+            m1 = b.bin_start[ce];
+        } else {
+            m0 = b.bin_start[c];
+            m1 = b.bin_start[c + 1];
+        }
+        if (m1 < m0) m1 = m0;
+        return;
+    }
+    int32_t c0 = (int32_t)c + g.run_off[k], c1 = c0 + g.run_len[k];
+    if (c0 < 0) c0 = 0;  // only atoms far outside the box reach past the grid
+    if (c1 > ncell) c1 = ncell;
+    if (c0 < c1) {
+        m0 = b.bin_start[c0];
+        m1 = b.bin_start[c1];
+    } else {
+        m0 = m1 = 0u;
+    }
+}
+
+template <bool UNIFORM, bool NEWTON>
+__device__ void csr_wave_flat(CBox& b, CGeom& g, uint32_t nl, uint32_t i0w, uint32_t lane, uint64_t* __restrict__ lbits,
+                              uint64_t* __restrict__ lseg, double4* __restrict__ lrow, uint32_t* __restrict__ lcnt) {
+    const uint32_t i = i0w + lane;
+    const bool act = i < nl;
+    const uint32_t c = act ? b.cell[i] : 0u;
+    const uint32_t w = i0w >> 6;
+    const uint32_t slot = b.cap_pairs / (b.cap_local ? b.cap_local : 1u);
+    const size_t rstart = (size_t)w * 64u * slot;
+    const uint32_t wcap = (uint32_t)min((size_t)64u * slot, (size_t)b.cap_pairs - rstart);
+    GMEM uint32_t* region = b.ttmp + rstart;
+    // the first run merges into the self cell when it starts at the next cell (newton)
+    int r0 = 0;
+    int32_t self_extra = 1;
+    if (NEWTON && g.nruns > 0 && g.run_off[0] == 1) {
+        self_extra = 1 + g.run_len[0];
+        r0 = 1;
+    }
+    const int kfirst = g.has_self ? -1 : r0;
+    // this row's flattened length and non-empty segments
+    uint32_t T = 0, q = 0;
+    if (act) {
+        for (int k = kfirst; k < g.nruns; k = (k < 0 ? r0 : k + 1)) {
+            uint32_t m0, m1;
+            flat_segment<NEWTON>(b, g, i, c, k, self_extra, m0, m1);
+            T += m1 - m0;
+            q += m1 > m0 ? 1u : 0u;
+        }
+    }
+    const uint64_t incl = wave_incl_scan64(((uint64_t)q << 32) | T, lane);
+    const uint64_t tot64 = __shfl(incl, 63, 64);
+    const uint32_t Ttot = (uint32_t)tot64, Qtot = (uint32_t)(tot64 >> 32);
+    uint32_t nout;
+    if (Ttot > FLAT_WORDS * 64u || Qtot > FLAT_SEGS) {
+        // too wide for the wave's tables: each lane walks its own row (count,
+        // then write at its offset within the wave)
+        const uint32_t n = act ? csr_row_g<false, UNIFORM>(b, g, i, nullptr) : 0u;
+        const uint64_t p = wave_incl_scan64(n, lane);
+        const uint32_t loc = (uint32_t)p - n;
+        nout = (uint32_t)__shfl(p, 63, 64);
+        if (act) {
+            b.offsets[i] = loc;
+            if (nout <= wcap && n) csr_row_g<true, UNIFORM>(b, g, i, region + loc);
+        }
+    } else {
+        // segment-start bitmap and segment table
+        const uint32_t nwords = (Ttot + 63u) >> 6;
+        for (uint32_t x = lane; x < nwords; x += 64u) lbits[x] = 0ull;
+        lrow[lane] = act ? ((const double4*)b.xr)[i] : make_double4(0.0, 0.0, 0.0, 0.0);
+        lcnt[lane] = 0u;
+        wave_sync();
+        if (act) {
+            uint32_t F = (uint32_t)incl - T;           // this row's first flattened index
+            uint32_t qk = (uint32_t)(incl >> 32) - q;  // and its first segment
+            uint32_t* bits32 = (uint32_t*)lbits;
+            for (int k = kfirst; k < g.nruns; k = (k < 0 ? r0 : k + 1)) {
+                uint32_t m0, m1;
+                flat_segment<NEWTON>(b, g, i, c, k, self_extra, m0, m1);
+                if (m1 > m0) {
+                    atomicOr(&bits32[F >> 5], 1u << (F & 31u));
+                    lseg[qk++] = ((uint64_t)lane << 32) | (uint32_t)(m0 - F);
+                    F += m1 - m0;
+                }
+            }
+        }
+        wave_sync();
+        const double4* __restrict__ sp4 = (const double4*)b.spos;
+        nout = 0;
+        uint32_t base_rank = 0;
+        for (uint32_t f0 = 0; f0 < Ttot; f0 += 64u) {
+            const uint64_t word = lbits[f0 >> 6];
+            const uint32_t f = f0 + lane;
+            const bool valid = f < Ttot;
+            const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(word >> 32),
+                                                             __builtin_amdgcn_mbcnt_lo((uint32_t)word, 0u));
+            const uint32_t rank = base_rank + below + (uint32_t)((word >> lane) & 1ull) - 1u;
+            base_rank += (uint32_t)__popcll(word);
+            bool acc = false;
+            uint32_t row = 0, j = 0;
+            if (valid) {
+                const uint64_t sv = lseg[rank];
+                row = (uint32_t)(sv >> 32);
+                const uint32_t m = f + (uint32_t)sv;
+                const double4 pr = lrow[row];
+                acc = accept_pair<UNIFORM>(pr.x, pr.y, pr.z, pr.w, sp4[m], g.ucut_sq, g.sf_sq);
+                if (acc) {
+                    j = b.sorted[m];
+                    if (!NEWTON && j == i0w + row) acc = false;  // the full list skips the row atom itself
+                }
+            }
+            const uint64_t am = __ballot(acc);
+            if (acc) {
+                const uint32_t pos = nout + lane_prefix(am);
+                if (pos < wcap) region[pos] = j;
+                atomicAdd(&lcnt[row], 1u);
+            }
+            nout += (uint32_t)__popcll(am);
+        }
+        wave_sync();
+        const uint32_t n = lcnt[lane];
+        const uint64_t p = wave_incl_scan64(n, lane);
+        if (act) b.offsets[i] = (uint32_t)p - n;
+        wave_sync();  // the next tile rewrites the tables
+    }
+    if (lane == 0) b.wcount[w] = nout;
+}
+
+__global__ void __launch_bounds__(BLK) k_csr_count_flat(KARGS) {
+    __shared__ uint64_t lbits[BLK / 64][FLAT_WORDS];
+    __shared__ uint64_t lseg[BLK / 64][FLAT_SEGS];
+    __shared__ double4 lrow[BLK / 64][64];
+    __shared__ uint32_t lcnt[BLK / 64][64];
+    FOR_TILES(t) {
+        const Tile tl = tiles[t];
+        CBox& b = box_at(boxes, tl.box);
+        if (!live(b)) continue;
+        const BoxState* st = b.st;
+        if (!st->rebuild_now || (st->err & ERR_GHOST_CAPACITY)) continue;
+        CGeom& g = geom_of(b);
+        const uint32_t nl = st->nlocal;
+        const uint32_t wid = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+        const uint32_t i0w = tl.start + wid * 64u;
+        // the waves' totals end with a zero entry (its scan is the pair count)
+        if (i0w + lane == nl) b.wcount[(nl + 63u) >> 6] = 0u;
+        if (i0w >= nl) continue;  // wave-uniform; no block barrier in this kernel
+        if (g.newton) {
+            if (g.uniform) csr_wave_flat<true, true>(b, g, nl, i0w, lane, lbits[wid], lseg[wid], lrow[wid], lcnt[wid]);
+            else csr_wave_flat<false, true>(b, g, nl, i0w, lane, lbits[wid], lseg[wid], lrow[wid], lcnt[wid]);
+        } else {
+            if (g.uniform) csr_wave_flat<true, false>(b, g, nl, i0w, lane, lbits[wid], lseg[wid], lrow[wid], lcnt[wid]);
+            else csr_wave_flat<false, false>(b, g, nl, i0w, lane, lbits[wid], lseg[wid], lrow[wid], lcnt[wid]);
+        }
+    }
+}
+
+// Mode 4's fill (after the scan of the waves' totals): each row's offset is
+// its wave's base plus its offset within the wave; each wave's region is
+// copied to its place, one entry per lane, taking the consumer's transpose
+// slots. A wave whose rows overflowed its region is walked again, per lane.
+__global__ void __launch_bounds__(BLK) k_csr_fill_flat(KARGS) {
+    FOR_TILES(t) {
+        const Tile tl = tiles[t];
+        CBox& b = box_at(boxes, tl.box);
+        if (!live(b)) continue;
+        const BoxState* st = b.st;
+        if (!st->rebuild_now || (st->err & (ERR_GHOST_CAPACITY | ERR_PAIR_CAPACITY))) continue;
+        const uint32_t nl = st->nlocal;
+        const uint32_t wid = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+        const uint32_t i0w = tl.start + wid * 64u;
+        const uint32_t i = i0w + lane;
+        if (i == nl) b.offsets[nl] = b.wcount[(nl + 63u) >> 6];
+        if (i0w >= nl) continue;  // wave-uniform
+        const bool act = i < nl;
+        const uint32_t w = i0w >> 6;
+        const uint32_t base = b.wcount[w];
+        const uint32_t tot = b.wcount[w + 1] - base;
+        const uint32_t slot = b.cap_pairs / (b.cap_local ? b.cap_local : 1u);
+        const size_t rstart = (size_t)w * 64u * slot;
+        const uint32_t wcap = (uint32_t)min((size_t)64u * slot, (size_t)b.cap_pairs - rstart);
+        const uint32_t loc = act ? b.offsets[i] : 0u;
+        if (act) b.offsets[i] = base + loc;
+        const bool tr = transpose_active(b);
+        if (tot <= wcap) {
+            const GMEM uint32_t* region = b.ttmp + rstart;
+            for (uint32_t e = lane; e < tot; e += 64u) {
+                const uint32_t j = region[e];
+                b.indices[base + e] = j;
+                if (tr) b.tslot[base + e] = atomicAdd(&b.tcount[j], 1u);  // consumer transpose slot
+            }
+        } else if (act) {
+            CGeom& g = geom_of(b);
+            GMEM uint32_t* out = b.indices + base + loc;
+            const uint32_t n = g.uniform ? csr_row_g<true, true>(b, g, i, out) : csr_row_g<true, false>(b, g, i, out);
+            if (tr)
+                for (uint32_t e = base + loc; e < base + loc + n; ++e) b.tslot[e] = atomicAdd(&b.tcount[b.indices[e]], 1u);
+        }
+    }
+}
+
 // Pass 2 (after the offset scan): move each tile's rows from the temp region
 // to their place (one contiguous copy), or walk overflowed tiles' rows again.
 // Per-row slots (modes 2 and 3): a wave's 64 rows own one contiguous output
@@ -2249,7 +2490,11 @@ __global__ void k_sync_view(KARGS) {
 }
 
 // PEND (batches): the step's final_integration is left to the next step's
-// kick-drift or the end of the call (k_final_pending)
+// kick-drift or the end of the call (k_final_pending). A box that ran out of
+// ghost or pair capacity does not advance: the host grows it and redoes the
+// step's tail. In a batch the box also freezes there (its step limit drops to
+// its step count, so every later launch of the call skips it) until the host,
+// which sees the error between runs, grows it and steps it up to the others.
 template <bool PEND>
 __global__ void k_step_end(KARGS) {
     FOR_TILES(t) {
@@ -2257,7 +2502,10 @@ __global__ void k_step_end(KARGS) {
         CBox& b = box_at(boxes, tiles[t].box);
         if (!live(b)) continue;
         BoxState* st = b.st;
-        if (st->err & (ERR_GHOST_CAPACITY | ERR_PAIR_CAPACITY)) continue;  // the host redoes the step's tail
+        if (st->err & (ERR_GHOST_CAPACITY | ERR_PAIR_CAPACITY)) {
+            if (PEND) st->step_limit = st->steps;
+            continue;
+        }
         if (PEND) st->final_pending = 1u;
         st->steps += 1;
         st->ring_pos = (st->ring_pos + 1u) % b.ring_len;
@@ -2381,12 +2629,20 @@ static void build_rest(const Launch& L, hipEvent_t ev0, hipEvent_t evc, hipEvent
     if (ev0) (void)hipEventRecord(ev0, L.stream);
     if (L.csr_mode == 0) launch(k_csr_count, L.t_local1, L.boxes, L.stream, L.maxgrid);  // tiles + end marker
     else if (L.csr_mode == 3) launch(k_csr_count_w, L.t_local1, L.boxes, L.stream, L.maxgrid);
+    else if (L.csr_mode == 4) launch(k_csr_count_flat, L.t_local1, L.boxes, L.stream, L.maxgrid);
     else launch(k_csr_count_g, L.t_local1, L.boxes, L.stream, L.maxgrid);
     if (evc) (void)hipEventRecord(evc, L.stream);
-    if (publish) run_scan<SCAN_CSR, true>(L, L.t_scan_local);  // publishes the build's counts
-    else run_scan<SCAN_CSR>(L, L.t_scan_local);
+    if (L.csr_mode == 4) {  // the scan of the waves' totals publishes the build's counts
+        if (publish) run_scan<SCAN_WAVES, true>(L, L.t_scan_local);
+        else run_scan<SCAN_WAVES>(L, L.t_scan_local);
+    } else if (publish) {
+        run_scan<SCAN_CSR, true>(L, L.t_scan_local);  // publishes the build's counts
+    } else {
+        run_scan<SCAN_CSR>(L, L.t_scan_local);
+    }
     if (ev1) (void)hipEventRecord(ev1, L.stream);
-    launch(k_csr_fill, L.t_local, L.boxes, L.stream, L.maxgrid);
+    if (L.csr_mode == 4) launch(k_csr_fill_flat, L.t_local1, L.boxes, L.stream, L.maxgrid);
+    else launch(k_csr_fill, L.t_local, L.boxes, L.stream, L.maxgrid);
     if (ev2) (void)hipEventRecord(ev2, L.stream);
     // consumer transpose (tcount was cleared by save_build; modes 1 and 2
     // take the arrival slots in k_csr_fill)

@@ -31,10 +31,6 @@ using namespace soil_hip;
         if (e_ != hipSuccess) return set_error(SOIL_HIP_EDEVICE, "%s: %s", #call, hipGetErrorString(e_)); \
     } while (0)
 
-namespace soil_hip {
-size_t box_bytes_estimate(soil_hip_box* b);
-}
-
 namespace {
 
 using Clock = std::chrono::steady_clock;
@@ -47,6 +43,7 @@ struct Job {
     double k = 1000.0;
     bool keep_result = false;
     double t_arrive = 0, t_admit = 0, t_done = 0;
+    double t_last = 0;  // the job's last state update (admission, then each chunk's completion)
     soil_hip_box* box = nullptr;
     size_t bytes = 0;
     int done_steps = 0;  // steps this job has taken on the device
@@ -101,7 +98,12 @@ struct soil_hip_server {
     // eviction requests for jobs on the device: id -> (restart path, format)
     std::map<int, std::pair<std::string, int>> evict_req;
     // statistics
-    std::vector<double> step_ms;                // one sample per (active job, step)
+    // one sample per (job, step): the wall time between the job's consecutive
+    // state updates (chunk completions, from its admission on) over the steps
+    // each update brought, gaps for re-batching, admission and read-back included
+    std::vector<double> step_ms;
+    std::vector<double> gpu_step_ms;            // the chunk's device time over its steps, per (job, step)
+    std::vector<double> gap_ms;                 // wall time between a job's consecutive updates
     double particle_steps = 0.0;
     double busy_s = 0.0;
     uint64_t reconfigs = 0;
@@ -225,6 +227,7 @@ void admit(soil_hip_server* sv, std::vector<std::unique_ptr<Job>>& incoming) {
             sv->pending.pop_front();
         }
         j->t_admit = sv->now();
+        j->t_last = j->t_admit;
         int rc = soil_hip_setup(j->box);
         if (!rc) {
             const uint64_t limit = (uint64_t)j->steps;  // the box freezes on device after its own steps
@@ -344,6 +347,12 @@ void loop(soil_hip_server* sv) {
             break;
         }
         sv->batch->half_pending[0] = sv->batch->half_pending[1] = false;
+        // a box that ran out of ghost or pair capacity inside the chunk froze
+        // at that step: grow it and step it up to the others (and grow boxes
+        // near their capacity before they overflow)
+        rc = batch_settle(sv->batch, h_col, true);
+        if (rc < 0) { fail(sv, rc); break; }
+        if (rc > 0) std::memcpy(h_col, sv->batch->h_col_now, 16 * sv->active.size());
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, e0, e1);
         sv->busy_s += ms * 1e-3;
@@ -352,10 +361,20 @@ void loop(soil_hip_server* sv) {
         sv->active_sum += (double)sv->active.size();
         sv->particle_steps += nparticles * n;
         const double per = ms / n;
+        const double tnow = sv->now();
         for (size_t k = 0; k < sv->active.size(); ++k) {
             Job* j = sv->active[k].get();
-            j->done_steps = std::min(j->steps, j->done_steps + n);
-            for (int q = 0; q < n; ++q) sv->step_ms.push_back(per);
+            const int took = std::min(j->steps, j->done_steps + n) - j->done_steps;
+            j->done_steps += took;
+            if (took > 0) {
+                const double gap = 1e3 * (tnow - j->t_last);
+                sv->gap_ms.push_back(gap);
+                for (int q = 0; q < took; ++q) {
+                    sv->step_ms.push_back(gap / took);
+                    sv->gpu_step_ms.push_back(per);
+                }
+                j->t_last = tnow;
+            }
             if (h_col[4 * k]) {  // device error bits: the job's results are invalid
                 j->failed = true;
                 check_err_bits(h_col[4 * k]);
@@ -508,9 +527,7 @@ int soil_hip_server_drain(soil_hip_server* sv) {
     return 0;
 }
 
-// out[0..19]: jobs_done, jobs_failed, particle_steps, busy_s, wall_s, step_p50_ms, step_p99_ms,
-// completion_p50_s, completion_p99_s, wait_p50_s, wait_p99_s, mean_active_jobs, reconfigs,
-// reconfig_s, peak_bytes_budget_gb, chunks, step_mean_ms, completion_max_s, wait_max_s, server_steps
+// out[0..23]: as documented with soil_hip_server_stats in soil_hip.h
 int soil_hip_server_stats(soil_hip_server* sv, double* out) {
     if (!sv || !out) return set_error(SOIL_HIP_EINVAL, "null argument");
     std::lock_guard<std::mutex> lk(sv->mu);
@@ -532,7 +549,7 @@ int soil_hip_server_stats(soil_hip_server* sv, double* out) {
         tmax = std::max(tmax, j->t_done);
         tmin = std::min(tmin, j->t_arrive);
     }
-    std::memset(out, 0, 21 * sizeof(double));
+    std::memset(out, 0, SOIL_HIP_SERVER_NSTATS * sizeof(double));
     out[0] = (double)comp.size();
     out[1] = failed;
     out[2] = sv->particle_steps;
@@ -556,6 +573,9 @@ int soil_hip_server_stats(soil_hip_server* sv, double* out) {
     out[18] = wait.empty() ? 0.0 : *std::max_element(wait.begin(), wait.end());
     out[19] = (double)sv->server_steps;
     out[20] = evicted;
+    out[21] = pct(sv->gpu_step_ms, 0.50);
+    out[22] = pct(sv->gpu_step_ms, 0.99);
+    out[23] = pct(sv->gap_ms, 0.99);
     return 0;
 }
 
@@ -598,7 +618,7 @@ int soil_hip_server_destroy(soil_hip_server* sv) {
 // arrival time while the server runs. Inputs are generated before the clock
 // starts. Returns the server statistics (see soil_hip_server_stats).
 int soil_hip_server_bench_stream(soil_hip_ctx* ctx, int njobs, double rate, uint64_t seed, double budget_gb,
-                                 int max_slots, int chunk_steps, int use_graph, double* stats_out) {  // 21 values
+                                 int max_slots, int chunk_steps, int use_graph, double* stats_out) {  // SOIL_HIP_SERVER_NSTATS values
     if (!ctx || njobs < 1 || rate <= 0 || !stats_out) return set_error(SOIL_HIP_EINVAL, "bad arguments");
     std::vector<double> arrival(njobs);
     std::vector<int32_t> steps(njobs);

@@ -16,7 +16,8 @@ from ._ffi import check, lib
 STAT_KEYS = ["jobs_done", "jobs_failed", "particle_steps", "busy_s", "wall_s", "step_p50_ms", "step_p99_ms",
              "completion_p50_s", "completion_p99_s", "wait_p50_s", "wait_p99_s", "mean_active_jobs", "reconfigs",
              "reconfig_s", "budget_gb", "chunks", "step_mean_ms", "completion_max_s", "wait_max_s", "server_steps",
-             "jobs_evicted"]
+             "jobs_evicted", "gpu_step_p50_ms", "gpu_step_p99_ms", "update_gap_p99_ms"]
+NSTATS = len(STAT_KEYS)  # SOIL_HIP_SERVER_NSTATS
 
 
 def _dp(a):
@@ -63,7 +64,7 @@ class Server:
         check(lib().soil_hip_server_drain(self.h))
 
     def stats(self):
-        out = np.zeros(21)
+        out = np.zeros(NSTATS)
         check(lib().soil_hip_server_stats(self.h, _dp(out)))
         return dict(zip(STAT_KEYS, out.tolist()))
 
@@ -87,7 +88,7 @@ class Server:
 
 
 def bench_stream(ctx, njobs, rate, seed=0x50494C, budget_gb=200.0, max_slots=256, chunk_steps=16, graph=True):
-    out = np.zeros(21)
+    out = np.zeros(NSTATS)
     check(lib().soil_hip_server_bench_stream(ctx.h, njobs, rate, seed, budget_gb, max_slots, chunk_steps,
                                              1 if graph else 0, _dp(out)))
     return dict(zip(STAT_KEYS, out.tolist()))

@@ -150,6 +150,13 @@ class Box:
         check(lib().soil_hip_get_device_view(self.h, C.byref(v)))
         return v
 
+    def device_bytes(self):
+        """(estimate, allocated): device bytes set-up allocates (the server's
+        admission figure) and what the arenas hold now."""
+        e, a = C.c_size_t(0), C.c_size_t(0)
+        check(lib().soil_hip_box_device_bytes(self.h, C.byref(e), C.byref(a)))
+        return e.value, a.value
+
     def compute_ke(self, mask=None):
         """atom.rs:589-604: kinetic energy of the locals (mask: bool per local)."""
         out = C.c_double(0.0)
@@ -357,7 +364,19 @@ class Batch:
         check(lib().soil_hip_batch_sync(self.h))
 
     def autotune(self):
+        """Picks the CSR strategy and blocks per launch; returns what it chose and measured."""
         check(lib().soil_hip_batch_autotune(self.h))
+        return self.tuning()
+
+    def tuning(self):
+        n = C.c_size_t(0)
+        check(lib().soil_hip_batch_get_tuning(self.h, None, 0, C.byref(n)))
+        out = np.zeros(n.value)
+        check(lib().soil_hip_batch_get_tuning(self.h, _dp(out), out.size, C.byref(n)))
+        trials = [{"csr_mode": int(out[4 + 3 * t]), "blocks_per_launch": int(out[5 + 3 * t]),
+                   "ms_per_step": float(out[6 + 3 * t])} for t in range(int(out[3]))]
+        return {"blocks_per_launch": int(out[0]), "csr_mode": int(out[1]), "ms_per_step": float(out[2]),
+                "trials": trials}
 
     def set_timing(self, enabled):
         check(lib().soil_hip_batch_set_timing(self.h, 1 if enabled else 0))

@@ -377,3 +377,45 @@ def test_narrow_row_slots_take_the_overflow_path(ctx, monkeypatch, slots):
     long_rows = int((np.diff(off.astype(np.int64)) > slots).sum())
     print(f"rows longer than {slots}: {long_rows}", flush=True)
     assert long_rows > 20
+
+
+@pytest.mark.parametrize("slots", [None, 20])
+def test_device_bytes_estimate_covers_setup(ctx, monkeypatch, slots):
+    # the server admits jobs by the estimate; set-up must not allocate more
+    # (the pair arena keeps SOIL_HIP_ROW_SLOTS entries per local)
+    print(f"case: device bytes estimate slots={slots}", flush=True)
+    if slots:
+        monkeypatch.setenv("SOIL_HIP_ROW_SLOTS", str(slots))
+    for seed, n, phi, shear in ((4601, 3000, 0.3, 0.0), (4602, 20000, 0.55, 0.1), (4603, 500, 0.2, 0.0)):
+        d, pos, vel, mass, rad = soil_amd.gen_custom(seed, n, 0.4, 0.6, phi, (1, 1, 1), 0, 0.0, 0.0, 0.1, shear)
+        b = soil_amd.Box(ctx, d)
+        b.upload(pos, mass, rad, vel=vel)
+        est0, alloc0 = b.device_bytes()
+        assert alloc0 == 0
+        b.setup()
+        est, alloc = b.device_bytes()
+        assert est == est0 and 0 < alloc <= est, (n, est, alloc)
+        pair_bytes = 4 * 4 * (slots or 12) * n  # indices, row slots, transpose slot and source
+        assert est > pair_bytes
+        b.close()
+
+
+def test_pair_growth_after_a_sort_keeps_the_columns(ctx):
+    # an x axis compressed by 85 % over 70 steps: the pair arena (12 entries
+    # per row) overflows at step 67, inside a fused step whose spatial sort
+    # has swapped the column buffers; the growth must keep the swapped
+    # descriptor (the redone build then reads the permuted columns)
+    print("case: pair growth after a sort", flush=True)
+    d, pos, vel, mass, rad = small(5103, 3000, 0.4, 0.6, 0.45)
+    L = d.hi[0] - d.lo[0]
+    g, r = make_pair(ctx, d, pos, vel, mass, rad, k=1000.0,
+                     pre_setup=axis_deform([(0, "vel", -0.85 * L / (70 * d.dt), (0, 0))], stage=70))
+    grew_pairs = False
+    for s in range(1, 71):
+        b0 = g.device_bytes()[1]
+        g.step(1)
+        r.step(1)
+        grew_pairs |= s >= 61 and g.device_bytes()[1] > b0
+        if s >= 60:
+            assert_same(g, r, what=f"compress step {s}")
+    assert grew_pairs

@@ -113,3 +113,10 @@ def test_device_view_mirror_matches_the_header(tmp_path):
     assert out[0] == C.sizeof(_ffi.DeviceView)
     for name, off in zip(fields, out[1:]):
         assert getattr(_ffi.DeviceView, name).offset == off, name
+
+
+def test_server_stat_keys_match_the_header():
+    from soil_amd import serve
+    text = open(os.path.join(ROOT, "include", "soil_hip.h")).read()
+    n = int(re.search(r"#define SOIL_HIP_SERVER_NSTATS (\d+)", text).group(1))
+    assert len(serve.STAT_KEYS) == n == serve.NSTATS
