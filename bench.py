@@ -1,20 +1,26 @@
 """Benchmark: particle-steps/s including rebuilds, batched vs plain, on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--parts plain,batched,served,cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--parts plain,sheared,batched,served,cpu]
 
 Prints ONE JSON line. `value` is the plain single-box path on the 1,000,000
 sphere polydisperse periodic box (phi 0.50, skin fraction 1.1, harmonic
 consumer k=1000, dt 1e-3, trigger-driven rebuilds), inputs resident in HBM
-before the timed region. Extra objects carry the batched ensemble (256 boxes,
-one launch per phase in a captured graph) against the plain path stepping the
-same ensemble one box per launch sequence, the served stream's latencies, the
-CSR-build roofline and the single-core CPU oracle baseline.
+before the timed region. `roofline` is the whole CSR build (count, offset
+scan, fill) on that box, timed with HIP events on its stream; `step_roofline`
+prices every timed step with the algorithmic byte model (soil_amd/roofline.py).
+Extra objects: the 4,000,000-sphere sheared, clustered box (every step
+rebuilds); the 256-box ensemble batched (one launch per phase in a captured
+graph) against the plain path stepping the same boxes (one step per call in
+turn, and n steps per call); the served stream at three arrival rates; and
+the CPU oracle (the restatement of the reference's algorithm) timed on this
+host: one core on the single boxes, K cores on independent boxes.
 """
 import argparse
 import json
 import os
 import platform
 import sys
+import threading
 import time
 
 import numpy as np
@@ -22,10 +28,13 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-HBM_PEAK_GBS = 8000.0
+from soil_amd import roofline as rl  # noqa: E402
+
 # HIP events around every 8th rebuild's CSR build: each event costs a few
 # microseconds of the step, so sampling keeps them out of most timed steps
 TIMING_PERIOD = 8
+LATENCY_BOUND_MS = 5.0   # stated p99 step-latency bound of the served stream
+SERVED_RATES = (6.0, 24.0, 96.0)
 
 
 def log(*a):
@@ -42,59 +51,135 @@ def cpu_model():
     return platform.processor()
 
 
+def host_workers():
+    # worker threads for the K-core CPU lines: the host's cores, at most 16
+    return max(1, min(16, os.cpu_count() or 1))
+
+
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_plain_1M.json")
 
-
 # the count-pass kernel of each CSR strategy (SOIL_HIP_CSR_MODE; 3 is the default)
-COUNT_KERNEL = {0: "k_csr_count", 1: "k_csr_count_g", 2: "k_csr_count_g", 3: "k_csr_count_w"}
+COUNT_KERNEL = {0: "k_csr_count", 1: "k_csr_count_g", 2: "k_csr_count_g", 3: "k_csr_count_w",
+                4: "k_csr_count_flat"}
+FILL_KERNEL = {0: "k_csr_fill", 1: "k_csr_fill", 2: "k_csr_fill", 3: "k_csr_fill", 4: "k_csr_fill_flat"}
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary of this workload (copy-probe calibrated), or None."""
+def csr_mode():
+    return int(os.environ.get("SOIL_HIP_CSR_MODE", "3") or 3)
+
+
+def measured_traffic(kernels):
+    """HBM bytes per launch of `kernels` (summed) from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary of the plain 10^6-sphere box (copy-probe
+    calibrated), or None when it holds none of them."""
     try:
         with open(TRAFFIC_FILE) as fh:
             data = json.load(fh)
     except (OSError, ValueError):
         return None, None
-    for name, v in data.get("kernels", {}).items():
-        if kernel in name:
-            return v["read_bytes"] + v["write_bytes"], os.path.relpath(TRAFFIC_FILE, ROOT)
-    return None, None
+    total, found = 0.0, 0
+    for k in kernels:
+        for name, v in data.get("kernels", {}).items():
+            if k + "(" in name or name.startswith(k) or ("::" + k + "(") in name:
+                total += v["read_bytes"] + v["write_bytes"]
+                found += 1
+                break
+    if found != len(kernels):
+        return None, None
+    return total, os.path.relpath(TRAFFIC_FILE, ROOT)
 
 
-def run_plain(ctx, steps, warmup):
+def box_counts(b):
+    c = b.counts()
+    return c["nlocal"], c["nghost"], c["ncells"], c["npairs"], c["rebuilds"]
+
+
+def run_box(ctx, gen, steps, warmup, period, consumer=True):
+    """Steps one box (plain form); returns its timing and counts."""
     import soil_amd
-    d, pos, vel, mass, rad = soil_amd.gen_box(2)
+    d, pos, vel, mass, rad = gen
     n = len(pos)
     box = soil_amd.Box(ctx, d)
     box.upload(pos, mass, rad, vel=vel)
-    box.set_consumer(1000.0)
+    if consumer:
+        box.set_consumer(1000.0)
     box.setup()
     if warmup:
         box.step(warmup)
     box.sync()
-    c0 = box.counts()
-    box.set_timing(TIMING_PERIOD)
+    r0 = box.counts()["rebuilds"]
+    box.set_timing(period)
     t0 = time.perf_counter()
     box.step(steps)
     box.sync()
-    t1 = time.perf_counter()
+    el = time.perf_counter() - t0
     tm = box.timing()
-    c1 = box.counts()
-    el = t1 - t0
-    res = {
-        "n": n,
-        "seconds": el,
-        "particle_steps_per_s": n * steps / el,
-        "ms_per_step": 1e3 * el / steps,
-        "rebuilds": c1["rebuilds"] - c0["rebuilds"],
-        "nghost": c1["nghost"],
-        "npairs": c1["npairs"],
-        "csr": tm,
-    }
+    N, G, C, P, r1 = box_counts(box)
     box.close()
-    return res, d
+    rebuilds = r1 - r0
+    poly = bool(np.ptp(rad) > 1e-15)
+    le = bool(d.has_shear)
+    sub = rl.run_bytes(N, G, C, P, steps, rebuilds, poly, le, consumer=False)
+    tot = rl.run_bytes(N, G, C, P, steps, rebuilds, poly, le, consumer=consumer)
+    res = {
+        "n": n, "seconds": el, "particle_steps_per_s": n * steps / el, "ms_per_step": 1e3 * el / steps,
+        "rebuilds": rebuilds, "nghost": G, "npairs": P, "ncells": C, "csr": tm,
+        "step_roofline": dict(rl.roofline(tot, el), bytes_per_step=tot / steps,
+                              substrate_frac=rl.roofline(sub, el)["frac"],
+                              model="soil_amd/roofline.py run_bytes: Verlet, trigger, zeroing, comm per step; "
+                                    "pbc, sort, permute, borders, binning, CSR build, save per rebuild; "
+                                    "consumer 84 B per atom and pair"),
+    }
+    return res
+
+
+def csr_roofline(tm):
+    """The whole CSR build (count + offset scan + fill), event-timed, against HBM peak."""
+    if tm["builds"] <= 0 or tm["csr_ms"] <= 0:
+        return None
+    nb = tm["builds"]
+    per = tm["csr_bytes"] / nb
+    build_ms = tm["csr_ms"] / nb
+    count_ms = tm["count_ms"] / nb
+    fill_ms = tm["fill_ms"] / nb
+    mode = csr_mode()
+    traffic, src = measured_traffic([COUNT_KERNEL[mode], FILL_KERNEL[mode]])
+    r = dict(rl.roofline(per, build_ms * 1e-3))
+    r.update({
+        "kernel": f"CSR build: {COUNT_KERNEL[mode]} (candidate tests, row counts and "
+                  f"{'dense wave regions' if mode == 4 else 'row slots'}) + offset scan + "
+                  f"{FILL_KERNEL[mode]} (placement, with the consumer's transpose slots)",
+        "bound": "hbm",
+        "traffic": traffic,
+        "traffic_source": src,
+        "algorithmic_bytes_per_launch": per,
+        "launches_timed": nb,
+        "timing_period": TIMING_PERIOD,
+        "avg_launch_ms": build_ms,
+        "count_ms": count_ms,
+        "scan_ms": build_ms - count_ms - fill_ms,
+        "fill_ms": fill_ms,
+        "count_only_frac": per / (count_ms * 1e-3) / 1e9 / rl.HBM_PEAK_GBS,
+        "csr_mode": mode,
+    })
+    return r
+
+
+def run_sheared(ctx, steps, warmup):
+    import soil_amd
+    log("sheared: 4M box (generating)")
+    gen = soil_amd.gen_box(3)
+    res = run_box(ctx, gen, steps, warmup, 2)
+    out = {
+        "workload": "4,000,000 spheres r~U[0.45,0.55], box 3:1:1, Lees-Edwards xy strain rate 0.1, 64 Gaussian "
+                    "blobs (sigma 4 diameters, half the spheres) over a uniform background, phi 0.40, "
+                    "harmonic consumer; every step rebuilds",
+        "particles": res["n"], "particle_steps_per_s": res["particle_steps_per_s"],
+        "ms_per_step": res["ms_per_step"], "steps": steps, "rebuilds": res["rebuilds"],
+        "nghost": res["nghost"], "npairs": res["npairs"], "step_roofline": res["step_roofline"],
+        "csr_build": csr_roofline(res["csr"]),
+    }
+    return out, gen
 
 
 def ensemble(count=256, config=4):
@@ -114,22 +199,37 @@ def make_boxes(ctx, gens):
     return boxes
 
 
-def run_batched(ctx, steps_batched, steps_plain, warmup):
+def ensemble_bytes(boxes, gens, steps):
+    tot = 0.0
+    for b, g in zip(boxes, gens):
+        N, G, C, P, _ = box_counts(b)
+        tot += rl.run_bytes(N, G, C, P, steps, steps, True, bool(g[0].has_shear), consumer=True)
+    return tot
+
+
+def run_batched(ctx, steps_batched, steps_rr, steps_each, warmup):
     import soil_amd
     gens = ensemble()
     ntot = sum(len(g[1]) for g in gens)
-    # plain: one box per launch sequence, boxes in turn
+    # plain, one step per call: the boxes in turn, each call waiting for its box
     boxes = make_boxes(ctx, gens)
     for b in boxes:
         b.step(1)
         b.sync()
     t0 = time.perf_counter()
-    for _ in range(steps_plain):
+    for _ in range(steps_rr):
         for b in boxes:
             b.step(1)
     for b in boxes:
         b.sync()
-    t_plain = time.perf_counter() - t0
+    t_rr = time.perf_counter() - t0
+    # plain, n steps per call: each box in turn runs its steps in one call
+    t0 = time.perf_counter()
+    for b in boxes:
+        b.step(steps_each)
+    for b in boxes:
+        b.sync()
+    t_each = time.perf_counter() - t0
     for b in boxes:
         b.close()
     del boxes
@@ -138,7 +238,7 @@ def run_batched(ctx, steps_batched, steps_plain, warmup):
     batch = soil_amd.Batch(ctx, boxes)
     batch.step(1, graph=True)
     batch.sync()
-    batch.autotune()
+    tuned = batch.autotune()
     batch.step(max(warmup, 1), graph=True)
     batch.sync()
     t0 = time.perf_counter()
@@ -146,46 +246,133 @@ def run_batched(ctx, steps_batched, steps_plain, warmup):
     batch.sync()
     t_batch = time.perf_counter() - t0
     batch.close()
+    step_b = ensemble_bytes(boxes, gens, steps_batched)
     for b in boxes:
         b.close()
-    plain = ntot * steps_plain / t_plain
+    rr = ntot * steps_rr / t_rr
+    each = ntot * steps_each / t_each
     batched = ntot * steps_batched / t_batch
     return {
         "workload": "256 periodic Lees-Edwards boxes, N log-uniform in [1k,64k], phi U[0.20,0.55], "
                     "r U[0.4,0.6], strain rate U[0,0.2], harmonic consumer, every step rebuilds",
         "particles": ntot,
         "batched_particle_steps_per_s": batched,
-        "plain_particle_steps_per_s": plain,
-        "speedup": batched / plain,
+        "plain_particle_steps_per_s": each,
+        "plain_one_step_per_call_particle_steps_per_s": rr,
+        "speedup": batched / each,
+        "speedup_vs_one_step_per_call": batched / rr,
         "batched_steps": steps_batched,
-        "plain_steps": steps_plain,
+        "plain_steps_per_call": steps_each,
+        "plain_one_step_rounds": steps_rr,
         "batched_ms_per_step": 1e3 * t_batch / steps_batched,
-        "plain_ms_per_step": 1e3 * t_plain / steps_plain,
+        "plain_ms_per_ensemble_step": 1e3 * t_each / steps_each,
+        "step_roofline": dict(rl.roofline(step_b, t_batch), bytes_per_step=step_b / steps_batched),
+        "autotuned": tuned,
+        "note": "plain = each box stepped alone on the single-box path, n steps per call (one host wait per "
+                "step's decision, ~30 launches per step); one_step_per_call adds a state read per call",
     }
 
 
-def run_cpu_baseline(target_s=12.0):
+def run_served(ctx):
+    from soil_amd import serve
+    sweep = []
+    main = None
+    for rate in SERVED_RATES:
+        log(f"served: {rate} jobs/s")
+        st = serve.bench_served(ctx, njobs=120, rate=rate)
+        row = {k: st[k] for k in ("jobs_done", "jobs_failed", "throughput_particle_steps_per_s", "step_p50_ms",
+                                  "step_p99_ms", "gpu_step_p50_ms", "gpu_step_p99_ms", "update_gap_p99_ms",
+                                  "completion_p50_s", "completion_p99_s", "wait_p99_s", "mean_active_jobs",
+                                  "reconfigs", "wall_s")}
+        row["rate_jobs_per_s"] = rate
+        row["p99_within_bound"] = st["step_p99_ms"] < LATENCY_BOUND_MS
+        sweep.append(row)
+        if main is None:
+            main = st
+    main["latency_bound_ms"] = LATENCY_BOUND_MS
+    main["rate_sweep"] = sweep
+    held = [r["rate_jobs_per_s"] for r in sweep if r["p99_within_bound"]]
+    main["bound_holds_up_to_jobs_per_s"] = max(held) if held else None
+    main["step_latency_note"] = ("step latency = per job, the wall time between its consecutive state updates "
+                                 "(admission, then each chunk), re-batching and read-back gaps included, over "
+                                 "the steps each update brought; gpu_step = a chunk's device time over its steps")
+    return main
+
+
+# ── CPU oracle lines (the restatement of the reference's algorithm) ──
+def _ref_box(gen):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import soil_amd
     from oracle_ref import RefBox
-    d, pos, vel, mass, rad = soil_amd.gen_box(2)
-    n = len(pos)
+    d, pos, vel, mass, rad = gen
     r = RefBox(d)
     r.add_atoms(pos, mass, rad, vel=vel)
     r.set_consumer(1000.0)
     r.setup()
+    return r
+
+
+def cpu_one_core(gen, target_s, max_steps):
+    r = _ref_box(gen)
+    n = len(gen[1])
     total, steps = 0.0, 0
-    while total < target_s and steps < 200:
+    while total < target_s and steps < max_steps:
         total += r.time_steps(1)
         steps += 1
-    return {
-        "value": n * steps / total,
-        "unit": "particle-steps/s",
-        "cores": 1,
-        "kind": "port",
-        "sample": f"same 1,000,000-sphere box, first {steps} steps from the initial state "
-                  f"({total:.1f} s, single thread, {cpu_model()})",
-    }
+    return n * steps / total, steps, total
+
+
+def cpu_k_cores(gens, steps, k):
+    """Independent boxes on k threads (each thread steps its share of boxes;
+    the oracle releases the interpreter while it runs)."""
+    boxes = [_ref_box(g) for g in gens]
+    ntot = sum(len(g[1]) for g in gens)
+    shares = [boxes[t::k] for t in range(k)]
+    barrier = threading.Barrier(k + 1)
+
+    def work(share):
+        barrier.wait()
+        for b in share:
+            b.time_steps(steps)
+
+    th = [threading.Thread(target=work, args=(s,)) for s in shares]
+    for t in th:
+        t.start()
+    barrier.wait()
+    t0 = time.perf_counter()
+    for t in th:
+        t.join()
+    el = time.perf_counter() - t0
+    return ntot * steps / el, el
+
+
+def run_cpu(c2_gen, c3_gen):
+    import soil_amd
+    out = {}
+    model = cpu_model()
+    k = host_workers()
+    v, s, t = cpu_one_core(c2_gen, 12.0, 200)
+    main = {"value": v, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": f"same 1,000,000-sphere box, first {s} steps from the initial state ({t:.1f} s, single "
+                      f"thread, {model})"}
+    g1 = soil_amd.gen_box(1)
+    v1, s1, t1 = cpu_one_core(g1, 30.0, 100)
+    out["c1_plain_4k"] = {"value": v1, "cores": 1, "sample": f"4,000 monodisperse spheres, {s1} steps ({t1:.2f} s)"}
+    if c3_gen is not None:
+        v3, s3, t3 = cpu_one_core(c3_gen, 8.0, 20)
+        out["c3_sheared_4M"] = {"value": v3, "cores": 1,
+                                "sample": f"4,000,000-sphere sheared box, first {s3} steps ({t3:.1f} s)"}
+    gens4 = ensemble()
+    v4, t4 = cpu_k_cores(gens4, 4, k)
+    out["c4_ensemble"] = {"value": v4, "cores": k,
+                          "sample": f"the 256-box ensemble, 4 steps per box, boxes dealt over {k} threads "
+                                    f"({t4:.1f} s)"}
+    gens5 = [soil_amd.gen_box(5, j) for j in range(2 * k)]
+    v5, t5 = cpu_k_cores(gens5, 20, k)
+    out["c5_jobs"] = {"value": v5, "cores": k,
+                      "sample": f"{2 * k} served-stream jobs, 20 steps each, over {k} threads ({t5:.1f} s)"}
+    main["other_lines"] = out
+    main["host"] = f"{model}, {os.cpu_count()} cores visible"
+    return main
 
 
 def main():
@@ -193,9 +380,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--parts", default="plain,batched,served,cpu")
+    ap.add_argument("--parts", default="plain,sheared,batched,served,cpu")
+    ap.add_argument("--sheared-steps", type=int, default=20)
     ap.add_argument("--batched-steps", type=int, default=200)
     ap.add_argument("--plain-ensemble-steps", type=int, default=10)
+    ap.add_argument("--plain-steps-per-call", type=int, default=50)
     ap.add_argument("--probe-copy-mb", type=int, default=0,
                     help="also run the calibration copy kernel over this many MB (traffic counters)")
     args = ap.parse_args()
@@ -237,50 +426,34 @@ def main():
                                                                    C.byref(ms)))
         out["probe_copy"] = {"bytes": args.probe_copy_mb * 1000000, "ms": ms.value,
                              "GBps": 2 * args.probe_copy_mb * 1e6 / (ms.value * 1e-3) / 1e9}
+    c2_gen = None
+    if "plain" in parts or "cpu" in parts:
+        c2_gen = soil_amd.gen_box(2)
     if "plain" in parts:
         log("plain: 1M box")
-        res, d = run_plain(ctx, args.steps, args.warmup)
+        res = run_box(ctx, c2_gen, args.steps, args.warmup, TIMING_PERIOD)
         out["value"] = res["particle_steps_per_s"]
         out["ms_per_step"] = res["ms_per_step"]
         out["config"]["rebuilds_in_timed_steps"] = res["rebuilds"]
         out["config"]["nghost"] = res["nghost"]
         out["config"]["npairs"] = res["npairs"]
-        tm = res["csr"]
-        if tm["builds"] > 0 and tm["count_ms"] > 0:
-            per_launch_bytes = tm["csr_bytes"] / tm["builds"]
-            avg_count_ms = tm["count_ms"] / tm["builds"]
-            ach = per_launch_bytes / (avg_count_ms * 1e-3) / 1e9
-            kname = COUNT_KERNEL[int(os.environ.get("SOIL_HIP_CSR_MODE", "3") or 3)]
-            traffic, traffic_src = measured_traffic(kname)
-            out["roofline"] = {
-                "kernel": f"{kname} (CSR count pass: candidate tests, row counts and row entries)",
-                "bound": "hbm",
-                "achieved": ach,
-                "peak": HBM_PEAK_GBS,
-                "unit": "GB/s",
-                "frac": ach / HBM_PEAK_GBS,
-                "traffic": traffic,
-                "traffic_source": traffic_src,
-                "algorithmic_bytes_per_launch": per_launch_bytes,
-                "launches_timed": tm["builds"],
-                "timing_period": TIMING_PERIOD,
-                "avg_launch_ms": avg_count_ms,
-                "build_avg_ms": tm["csr_ms"] / tm["builds"],
-                "build_note": "whole CSR build = count + offset scan + fill; the fill also takes the consumer's "
-                              "transpose slots",
-                "fill_avg_ms": tm["fill_ms"] / tm["builds"],
-            }
+        out["roofline"] = csr_roofline(res["csr"])
+        out["step_roofline"] = res["step_roofline"]
         log(f"plain: {out['value']:.3e} particle-steps/s, {res['ms_per_step']:.3f} ms/step")
+    c3_gen = None
+    if "sheared" in parts:
+        out["sheared"], c3_gen = run_sheared(ctx, args.sheared_steps, 3)
+        log(f"sheared: {out['sheared']['particle_steps_per_s']:.3e} particle-steps/s")
     if "batched" in parts:
         log("batched: 256-box ensemble")
-        out["batched"] = run_batched(ctx, args.batched_steps, args.plain_ensemble_steps, args.warmup)
+        out["batched"] = run_batched(ctx, args.batched_steps, args.plain_ensemble_steps, args.plain_steps_per_call,
+                                     args.warmup)
         log(f"batched: {out['batched']['speedup']:.2f}x over plain")
     if "served" in parts:
-        from soil_amd import serve
-        out["served"] = serve.bench_served(ctx)
+        out["served"] = run_served(ctx)
     if "cpu" in parts:
-        log("cpu baseline")
-        out["cpu_baseline"] = run_cpu_baseline()
+        log("cpu baselines")
+        out["cpu_baseline"] = run_cpu(c2_gen, c3_gen)
     print(json.dumps(out), flush=True)
 
 

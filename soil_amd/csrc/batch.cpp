@@ -34,7 +34,10 @@ static int batch_tiles(soil_hip_batch* bt) {
     bt->L.boxes = bt->d_ptrs;
     bt->L.stream = bt->stream;
     bt->L.maxgrid = maxgrid;
-    bt->L.csr_mode = bt->boxes.empty() ? 4 : bt->boxes[0]->h_dev.csr_mode;
+    bt->L.csr_mode = bt->boxes.empty() ? 3 : bt->boxes[0]->h_dev.csr_mode;
+    bt->L.any_open = false;
+    for (auto* b : bt->boxes)
+        bt->L.any_open |= !(b->model.periodic[0] && b->model.periodic[1] && b->model.periodic[2]);
     return 0;
 }
 

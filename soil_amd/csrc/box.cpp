@@ -210,6 +210,7 @@ int rebuild_tiles(soil_hip_box* b) {
     b->L.boxes = b->d_ptrs;
     b->L.stream = b->stream;
     b->L.csr_mode = b->h_dev.csr_mode;
+    b->L.any_open = !(b->model.periodic[0] && b->model.periodic[1] && b->model.periodic[2]);
     return 0;
 }
 
@@ -349,11 +350,8 @@ int check_box(soil_hip_box* b, bool need_setup) {
 }
 
 int check_err_bits(uint32_t err) {
-    if (err & ERR_ATOM_LEFT_BOX)
-        return set_error(SOIL_HIP_EUNSUPPORTED,
-                         "an atom left the box through a non-periodic face; atom removal (domain.rs:648-661) "
-                         "is not implemented on the device path");
     if (err & ERR_STENCIL) return set_error(SOIL_HIP_EUNSUPPORTED, "stencil exceeds %d offsets", SOIL_MAX_STENCIL);
+    if (err & ERR_REMOVE_PLAN) return set_error(SOIL_HIP_EDEVICE, "atom removal found no kept source for a hole");
     if (err & (ERR_PAIR_CAPACITY | ERR_GHOST_CAPACITY))
         return set_error(SOIL_HIP_ECAPACITY, "device capacity exceeded (err=%u)", err);
     return 0;
@@ -907,8 +905,8 @@ int soil_hip_setup(soil_hip_box* b) {
     d.transpose = (b->consumer || (b->hook && (b->hook_flags & SOIL_HIP_HOOK_TRANSPOSE))) ? 1 : 0;
     {
         const char* m = getenv("SOIL_HIP_CSR_MODE");
-        d.csr_mode = m ? atoi(m) : 4;
-        if (d.csr_mode < 0 || d.csr_mode > 4) d.csr_mode = 4;
+        d.csr_mode = m ? atoi(m) : 3;
+        if (d.csr_mode < 0 || d.csr_mode > 4) d.csr_mode = 3;
     }
     BoxState st{};
     st.nlocal = (uint32_t)n;

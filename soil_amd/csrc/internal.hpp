@@ -28,8 +28,9 @@ namespace soil_hip {
 enum : uint32_t {
     ERR_PAIR_CAPACITY = 1u,
     ERR_GHOST_CAPACITY = 2u,
-    ERR_ATOM_LEFT_BOX = 4u,   // non-periodic removal (not supported on device)
+    ERR_ATOM_LEFT_BOX = 4u,   // (unused: atoms leaving a non-periodic face are removed)
     ERR_STENCIL = 8u,
+    ERR_REMOVE_PLAN = 16u,    // a removal hole found no kept source (an internal inconsistency)
 };
 
 // Host-computed geometry of one box at one step (apply_deform + recompute_bins
@@ -91,6 +92,8 @@ struct BoxState {
     uint64_t step_limit;                        // the box stops (every kernel skips it) at this step count
     uint64_t ext_key[6];                        // locals' min x,y,z then max x,y,z as order-preserving keys
     uint64_t pub_seq;                           // publishes to the host mirror so far
+    uint32_t nremove;                           // locals that left through a non-periodic face this pbc
+    uint32_t removed_total;                     // locals removed so far
 };
 
 // The fields the plain step waits on, published by kernels into pinned host
@@ -172,7 +175,7 @@ struct BoxDev {
     GMEM soil_hip_device_view* view;  // device copy of the hook's view (refreshed by k_sync_view), or null
     GMEM HostMirror* mirror;          // device address of the box's host mirror, or null
     int32_t csr_mode;     // CSR strategy (bit-neutral): 0 LDS tiles, 1 two global passes, 2 one pass + row slots,
-                          // 3 as 2 with wave-staged candidates, 4 (default) flattened per-wave candidates into
+                          // 3 (default) as 2 with wave-staged candidates, 4 flattened per-wave candidates into
                           // dense wave regions
 };
 

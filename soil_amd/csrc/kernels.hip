@@ -557,24 +557,25 @@ __global__ void __launch_bounds__(BLK) k_zero(KARGS) {
 }
 
 // pbc / wrap_periodic / pbc_triclinic: domain.rs:594-729, one local's wrap
-// (FullRebuild only; the caller checks the state and the index)
-__device__ __forceinline__ void pbc_atom(CBox& b, BoxState* st, uint32_t i) {
+// (FullRebuild only; the caller checks the state and the index). Returns true
+// when the atom left through a non-periodic face: pbc removes it
+// (domain.rs:648-656, 700-706), which remove_atoms carries out for all such
+// atoms at once.
+__device__ __forceinline__ bool pbc_atom(CBox& b, uint32_t i) {
     CGeom& g = geom_of(b);
     if (g.triclinic) {
         double l[3];
         x2lamda(g, b.x[i], b.y[i], b.z[i], l[0], l[1], l[2]);
         int delta[3] = {0, 0, 0};
-        bool gone = false;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             if (g.periodic[k]) {
                 if (l[k] < 0.0) { l[k] += 1.0; delta[k] = -1; }
                 else if (l[k] >= 1.0) { l[k] -= 1.0; delta[k] = 1; }
             } else if (l[k] < 0.0 || l[k] >= 1.0) {
-                gone = true;
+                return true;
             }
         }
-        if (gone) { atomicOr(&st->err, ERR_ATOM_LEFT_BOX); return; }
         double x, y, z;
         lamda2x(g, l[0], l[1], l[2], x, y, z);
         b.x[i] = x;
@@ -590,7 +591,7 @@ __device__ __forceinline__ void pbc_atom(CBox& b, BoxState* st, uint32_t i) {
             b.vy[i] -= s * g.bvel[1];
             b.vz[i] -= s * g.bvel[2];
         }
-        return;
+        return false;
     }
     double* p[3] = {b.x, b.y, b.z};
     int32_t* im[3] = {b.imx, b.imy, b.imz};
@@ -604,13 +605,19 @@ __device__ __forceinline__ void pbc_atom(CBox& b, BoxState* st, uint32_t i) {
             if (v < g.lo[k]) { v = v + g.size[k]; p[k][i] = v; im[k][i] -= 1; moved = true; }
             else if (v >= high) { v = v - g.size[k]; p[k][i] = v; im[k][i] += 1; moved = true; }
         } else if (v < g.lo[k] || v >= g.hi[k]) {
-            atomicOr(&st->err, ERR_ATOM_LEFT_BOX);
+            return true;  // the axes after it are not wrapped: the atom goes
         }
         q[k] = v;
     }
     if (moved) put_xr(b, i, q[0], q[1], q[2]);  // xr already holds the unwrapped ones
+    return false;
 }
 
+// pbc with removal. OPEN_ONLY (the fused step): only boxes with a
+// non-periodic axis (fully periodic ones wrap inside the sort's count).
+// A box that can remove writes one flag per local, and 0 at index nlocal,
+// into the slot scratch, and counts its removals.
+template <bool OPEN_ONLY>
 __global__ void __launch_bounds__(BLK) k_pbc(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
@@ -618,9 +625,89 @@ __global__ void __launch_bounds__(BLK) k_pbc(KARGS) {
         if (!live(b)) continue;
         BoxState* st = b.st;
         if (st->state != 0u) continue;
+        const bool open = !geom_of(b).pbc_all;
+        if (OPEN_ONLY && !open) continue;
         const uint32_t i = tl.start + threadIdx.x;
-        if (i >= st->nlocal) continue;
-        pbc_atom(b, st, i);
+        const uint32_t nl = st->nlocal;
+        if (i > nl) continue;
+        const bool rem = i < nl && pbc_atom(b, i);
+        if (!open) continue;
+        b.slot[i] = rem ? 1u : 0u;
+        const uint64_t m = __ballot(rem);
+        if (m && (uint32_t)__lane_id() == (uint32_t)__builtin_ctzll(m)) atomicAdd(&st->nremove, (uint32_t)__popcll(m));
+    }
+}
+
+// Carries out the removals of this pbc (Atom::swap_remove over the locals in
+// descending index order, atom.rs:268-271, 454-458, registry.swap_remove_all
+// atom.rs:291-295 for the AtomData columns) in one parallel pass. With n
+// locals of which m are removed, the k-th removal in that order moves the
+// element then at position n - k into the removed slot, so the hole at a
+// removed index p < n - m (its rank k = removals at or above p) ends up
+// holding what position c = n - k holds when it is taken; when c was itself a
+// removed position, that is what c received: c = n - rank(c), repeated. The
+// chain rises strictly and ends at a kept atom in [n - m, n), so the copy
+// source never lies below n - m and the copies into the holes do not overlap
+// their sources. slot[] holds the exclusive scan of the removal flags
+// (removals below each index), so rank(q) = m - slot[q] and flag(q) =
+// slot[q + 1] - slot[q].
+__device__ __forceinline__ void copy_atom(CBox& b, uint32_t s, uint32_t d) {
+    b.x[d] = b.x[s]; b.y[d] = b.y[s]; b.z[d] = b.z[s];
+    b.vx[d] = b.vx[s]; b.vy[d] = b.vy[s]; b.vz[d] = b.vz[s];
+    b.fx[d] = b.fx[s]; b.fy[d] = b.fy[s]; b.fz[d] = b.fz[s];
+    b.mass[d] = b.mass[s]; b.inv_mass[d] = b.inv_mass[s]; b.rad[d] = b.rad[s];
+    b.tag[d] = b.tag[s]; b.type[d] = b.type[s]; b.origin[d] = b.origin[s];
+    b.imx[d] = b.imx[s]; b.imy[d] = b.imy[s]; b.imz[d] = b.imz[s];
+    const GMEM double4* xs = (const GMEM double4*)b.xr;
+    ((GMEM double4*)b.xr)[d] = xs[s];
+    b.lbx[d] = b.lbx[s]; b.lby[d] = b.lby[s]; b.lbz[d] = b.lbz[s];
+    for (int c = 0; c < b.ncols; ++c) {
+        const int w = b.cols[c].width;
+        GMEM double* col = b.cols[c].data;
+        for (int k = 0; k < w; ++k) col[(size_t)d * w + k] = col[(size_t)s * w + k];
+    }
+}
+
+__global__ void __launch_bounds__(BLK) k_remove_fill(KARGS) {
+    FOR_TILES(t) {
+        const Tile tl = tiles[t];
+        CBox& b = box_at(boxes, tl.box);
+        if (!live(b)) continue;
+        const BoxState* st = b.st;
+        const uint32_t m = st->nremove;
+        if (m == 0u || st->state != 0u) continue;
+        const uint32_t n = st->nlocal, keep = n - m;
+        const uint32_t i = tl.start + threadIdx.x;
+        if (i >= keep || b.slot[i + 1] == b.slot[i]) continue;  // kept in place
+        uint32_t c = n - (m - b.slot[i]);
+        uint32_t hops = 0;  // at most m: the chain rises through removed positions
+        while (b.slot[c + 1] != b.slot[c] && hops < m) {
+            c = n - (m - b.slot[c]);
+            ++hops;
+        }
+        if (c >= n || c < keep || b.slot[c + 1] != b.slot[c]) {  // not a kept tail atom: flag, copy nothing
+            atomicOr(&b.st->err, ERR_REMOVE_PLAN);
+            continue;
+        }
+        copy_atom(b, c, i);
+    }
+}
+
+// nlocal drops by the removals; the state is already FullRebuild (pbc runs
+// only then), so the step rebuilds its ghosts and lists for the new count
+__global__ void k_remove_commit(KARGS) {
+    FOR_TILES(t) {
+        if (threadIdx.x != 0) continue;
+        CBox& b = box_at(boxes, tiles[t].box);
+        if (!live(b)) continue;
+        BoxState* st = b.st;
+        const uint32_t m = st->nremove;
+        if (m == 0u) continue;
+        st->nlocal -= m;
+        st->removed_total += m;
+        st->nremove = 0u;
+        st->last_valid = 0u;
+        st->local_cells_valid = 0u;
     }
 }
 
@@ -702,7 +789,7 @@ __global__ void __launch_bounds__(BLK) k_sort_count(KARGS) {
         if (!live(b)) continue;
         SortView<W> v = sort_view<W>(b);
         const uint32_t i = tl.start + threadIdx.x;
-        if (PBC && b.st->state == 0u && i < b.st->nlocal) pbc_atom(b, b.st, i);
+        if (PBC && b.st->state == 0u && i < b.st->nlocal && geom_of(b).pbc_all) (void)pbc_atom(b, i);
         if (!v.active) continue;
         // Cell sorts: the input is nearly cell-ordered (locals were sorted at the
         // last rebuild, ghosts follow their sources), so lanes form runs of equal
@@ -926,6 +1013,11 @@ __device__ __forceinline__ void scan_view(CBox& b, uint32_t& n, bool& active, GM
         active = st->rebuild_now != 0 && (st->err & ERR_GHOST_CAPACITY) == 0;
         in = b.offsets;
         out = b.offsets;
+    } else if (W == SCAN_REMOVE) {
+        n = st->nlocal + 1;
+        active = st->nremove != 0 && st->state == 0u;
+        in = b.slot;
+        out = b.slot;
     } else if (W == SCAN_WAVES) {
         n = ((st->nlocal + 63u) >> 6) + 1u;
         active = st->rebuild_now != 0 && (st->err & ERR_GHOST_CAPACITY) == 0;
@@ -1985,6 +2077,28 @@ __device__ __forceinline__ void flat_segment(CBox& b, CGeom& g, uint32_t i, uint
     }
 }
 
+// One row walked by its own lane over its segments (the wave's tables cannot
+// hold it): the row's entries in the same order, into out when FILL.
+template <bool FILL, bool UNIFORM, bool NEWTON>
+__device__ uint32_t csr_row_segments(CBox& b, CGeom& g, uint32_t i, uint32_t c, int kfirst, int r0,
+                                     int32_t self_extra, GMEM uint32_t* out) {
+    const double4* __restrict__ sp4 = (const double4*)b.spos;
+    const double4 pi = ((const double4*)b.xr)[i];
+    uint32_t n = 0;
+    for (int k = kfirst; k < g.nruns; k = (k < 0 ? r0 : k + 1)) {
+        uint32_t m0, m1;
+        flat_segment<NEWTON>(b, g, i, c, k, self_extra, m0, m1);
+        for (uint32_t m = m0; m < m1; ++m) {
+            if (!accept_pair<UNIFORM>(pi.x, pi.y, pi.z, pi.w, sp4[m], g.ucut_sq, g.sf_sq)) continue;
+            const uint32_t j = b.sorted[m];
+            if (!NEWTON && j == i) continue;
+            if (FILL) out[n] = j;
+            ++n;
+        }
+    }
+    return n;
+}
+
 template <bool UNIFORM, bool NEWTON>
 __device__ void csr_wave_flat(CBox& b, CGeom& g, uint32_t nl, uint32_t i0w, uint32_t lane, uint64_t* __restrict__ lbits,
                               uint64_t* __restrict__ lseg, double4* __restrict__ lrow, uint32_t* __restrict__ lcnt) {
@@ -2021,13 +2135,15 @@ __device__ void csr_wave_flat(CBox& b, CGeom& g, uint32_t nl, uint32_t i0w, uint
     if (Ttot > FLAT_WORDS * 64u || Qtot > FLAT_SEGS) {
         // too wide for the wave's tables: each lane walks its own row (count,
         // then write at its offset within the wave)
-        const uint32_t n = act ? csr_row_g<false, UNIFORM>(b, g, i, nullptr) : 0u;
+        const uint32_t n =
+            act ? csr_row_segments<false, UNIFORM, NEWTON>(b, g, i, c, kfirst, r0, self_extra, nullptr) : 0u;
         const uint64_t p = wave_incl_scan64(n, lane);
         const uint32_t loc = (uint32_t)p - n;
         nout = (uint32_t)__shfl(p, 63, 64);
         if (act) {
             b.offsets[i] = loc;
-            if (nout <= wcap && n) csr_row_g<true, UNIFORM>(b, g, i, region + loc);
+            if (nout <= wcap && n)
+                csr_row_segments<true, UNIFORM, NEWTON>(b, g, i, c, kfirst, r0, self_extra, region + loc);
         }
     } else {
         // segment-start bitmap and segment table
@@ -2595,7 +2711,15 @@ void phase_remove_ghosts(const Launch& L) { launch(k_remove_ghosts, L.t_box, L.b
 void phase_set_rebuild_now(const Launch& L) { launch(k_set_rebuild_now, L.t_box, L.boxes, L.stream, L.maxgrid); }
 void phase_zero(const Launch& L) { launch(k_zero<false>, L.t_atom, L.boxes, L.stream, L.maxgrid); }
 void phase_zero_fused(const Launch& L) { launch(k_zero<true>, L.t_atom, L.boxes, L.stream, L.maxgrid); }
-void phase_pbc(const Launch& L) { launch(k_pbc, L.t_local, L.boxes, L.stream, L.maxgrid); }
+// pbc with the removals it implies (a box with no non-periodic axis removes none)
+static void run_pbc(const Launch& L, bool open_only) {
+    if (open_only) launch(k_pbc<true>, L.t_local1, L.boxes, L.stream, L.maxgrid);
+    else launch(k_pbc<false>, L.t_local1, L.boxes, L.stream, L.maxgrid);
+    run_scan<SCAN_REMOVE>(L, L.t_scan_local);
+    launch(k_remove_fill, L.t_local, L.boxes, L.stream, L.maxgrid);
+    launch(k_remove_commit, L.t_box, L.boxes, L.stream, L.maxgrid);
+}
+void phase_pbc(const Launch& L) { run_pbc(L, false); }
 void phase_sort(const Launch& L) {
     launch(k_pre_sort, L.t_box, L.boxes, L.stream, L.maxgrid);
     run_sort<SORT_SPATIAL, SCAN_CELLS_SPATIAL>(L, L.t_cell, L.t_local, L.t_scan_cell);
@@ -2605,6 +2729,7 @@ void phase_sort(const Launch& L) {
 }
 // sort + permute + (when rebuilding) the build's bookkeeping in one per-box kernel
 void phase_pbc_sort_fused(const Launch& L) {
+    if (L.any_open) run_pbc(L, true);  // boxes whose pbc may remove atoms
     launch(k_sort_count<SORT_SPATIAL, true>, L.t_local, L.boxes, L.stream, L.maxgrid);  // k_initial zeroed
     run_scan<SCAN_CELLS_SPATIAL>(L, L.t_scan_cell);
     launch(k_sort_scatter<SORT_SPATIAL>, L.t_local, L.boxes, L.stream, L.maxgrid);

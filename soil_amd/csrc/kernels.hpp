@@ -12,7 +12,7 @@ constexpr uint32_t BORDER_PER = 4;                 // 64-atom chunks per wave in
 constexpr uint32_t BORDER_TILE = BLK * BORDER_PER;  // atoms per border-compaction tile
 
 enum { SORT_SPATIAL = 0, SORT_BUILD = 1 };
-enum { SCAN_CELLS_SPATIAL = 0, SCAN_CELLS_BUILD = 1, SCAN_CSR = 2, SCAN_TRANSPOSE = 3, SCAN_WAVES = 4 };
+enum { SCAN_CELLS_SPATIAL = 0, SCAN_CELLS_BUILD = 1, SCAN_CSR = 2, SCAN_TRANSPOSE = 3, SCAN_WAVES = 4, SCAN_REMOVE = 5 };
 
 struct Tile {
     uint32_t box, start;
@@ -32,7 +32,11 @@ struct Launch {
     // reverse comm: 1 = some box may have overlapping lo/hi selections in that
     // dim, so its two swaps need separate ordered launches
     uint8_t rev_split[3] = {1, 1, 1};
-    int csr_mode = 4;
+    int csr_mode = 3;
+    // some box has a non-periodic axis: its pbc may remove atoms, so it runs
+    // as its own phase before the fused sort (fully periodic boxes wrap inside
+    // the sort's count)
+    bool any_open = false;
     TileSet t_box, t_local, t_local1, t_atom, t_atom1, t_ghost, t_cell, t_border;
     TileSet t_scan_cell, t_scan_local, t_scan_atom;
 };

@@ -190,6 +190,7 @@ int retire(soil_hip_server* sv) {
                 j->failed = true;
                 err = rc;
             } else {
+                j->n = (size_t)c[0];  // atoms that left through a non-periodic face are gone
                 j->out_pos.assign(p.begin(), p.begin() + 3 * j->n);
                 j->out_vel.assign(v.begin(), v.begin() + 3 * j->n);
             }

@@ -97,3 +97,31 @@ def cells(pos, grid):
 def sort_is_identity(pos, grid):
     cl = cells(pos, grid)
     return bool(np.array_equal(np.argsort(cl, kind="stable"), np.arange(len(cl))))
+
+
+def removal_order(removed):
+    """Index order left by Atom::swap_remove over the removed locals in
+    descending index order (domain.rs:633-661, atom.rs:454-458), by the
+    closed form the device uses: the hole at a removed index p below n - m
+    (rank k = removals at or above p) takes position c = n - k, followed
+    through c = n - rank(c) while c is itself removed."""
+    removed = np.asarray(removed, dtype=bool)
+    n, m = len(removed), int(removed.sum())
+    below = np.concatenate([[0], np.cumsum(removed)])  # removals below each index
+    order = np.arange(n - m)
+    for p in np.nonzero(removed[: n - m])[0]:
+        c = n - (m - below[p])
+        while removed[c]:
+            c = n - (m - below[c])
+        order[p] = c
+    return order
+
+
+def removal_order_sequential(removed):
+    """The same by the reference's own loop, on an index array."""
+    a = list(range(len(removed)))
+    for i in range(len(removed) - 1, -1, -1):
+        if removed[i]:
+            a[i] = a[-1]
+            a.pop()
+    return np.array(a, dtype=np.int64)

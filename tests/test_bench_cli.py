@@ -16,6 +16,6 @@ def test_more_than_one_gpu_is_refused_with_a_message():
 
 def test_default_parts_cover_the_whole_metric():
     # the metric names plain, batched and the served stream's latencies; the
-    # CPU baseline rides along
+    # sheared 4M box and the CPU baselines ride along
     src = open(os.path.join(ROOT, "bench.py")).read()
-    assert 'ap.add_argument("--parts", default="plain,batched,served,cpu")' in src
+    assert 'ap.add_argument("--parts", default="plain,sheared,batched,served,cpu")' in src

@@ -419,3 +419,26 @@ def test_pair_growth_after_a_sort_keeps_the_columns(ctx):
         if s >= 60:
             assert_same(g, r, what=f"compress step {s}")
     assert grew_pairs
+
+
+@pytest.mark.parametrize("case", ["dense_poly", "lees_edwards", "newton_off", "narrow_regions", "uniform"])
+def test_flat_csr_mode_matches_oracle(ctx, monkeypatch, case):
+    # CSR mode 4 (a wave's candidates flattened over its lanes, dense wave
+    # regions, the scan over wave totals) gives the same lists: dense and
+    # sheared boxes, the full list, regions narrow enough that many waves
+    # overflow into the fill's re-walk, and the uniform-radius test
+    print(f"case: flat CSR mode, {case}", flush=True)
+    monkeypatch.setenv("SOIL_HIP_CSR_MODE", "4")
+    if case == "dense_poly":
+        run_compare(ctx, small(4801, 5000, 0.4, 0.6, 0.55), 30, 10, k=1000.0)
+    elif case == "lees_edwards":
+        run_compare(ctx, small(4802, 4000, 0.4, 0.6, 0.45, (2, 1, 1), 8, 0.5, 3.0, shear=0.15), 20, 10, k=1000.0)
+    elif case == "newton_off":
+        def mod(d):
+            d.newton = 0
+        run_compare(ctx, small(4803, 2000, 0.4, 0.6, 0.3), 20, 10, desc_mod=mod)
+    elif case == "narrow_regions":
+        monkeypatch.setenv("SOIL_HIP_ROW_SLOTS", "4")
+        run_compare(ctx, small(4804, 4000, 0.4, 0.6, 0.55), 30, 10, k=1000.0)
+    else:
+        run_compare(ctx, small(4805, 4000, 0.5, 0.5, 0.4), 30, 10, k=1000.0)

@@ -197,3 +197,50 @@ def test_cells_match_the_oracle_bins():
     a = r.atoms()
     _, _, _, atom_cell = r.bins()
     assert np.array_equal(cells(a["pos"], r.grid()), atom_cell.astype(np.int64))
+
+
+def test_removal_order_closed_form_equals_the_sequential_loop():
+    # the device's one-pass form of the descending swap_remove loop, on random
+    # removal sets of every density (and the empty and full sets)
+    from pins import removal_order, removal_order_sequential
+    rng = np.random.default_rng(11)
+    for n in (1, 2, 5, 40, 300):
+        for frac in (0.0, 0.05, 0.3, 0.7, 1.0):
+            for _ in range(5):
+                rem = rng.random(n) < frac
+                assert np.array_equal(removal_order(rem), removal_order_sequential(rem)), (n, frac)
+
+
+@pytest.mark.parametrize("tilt", [0.0, 1.3])
+def test_oracle_pbc_removes_atoms_outside_fixed_faces(tilt):
+    # atoms placed outside the fixed z faces leave at the first pbc: the
+    # survivors are exactly the inside atoms, in the swap_remove order, with
+    # every column (tag, pos, radius, AtomData) carried along
+    from pins import removal_order
+    d, pos, vel, mass, rad = soil_amd.gen_custom(7050, 1500, 0.45, 0.55, 0.3, (1, 1, 1), 0, 0.0, 0.0, 0.0, 0.0)
+    rng = np.random.default_rng(5)
+    L = d.hi[2] - d.lo[2]
+    out = rng.random(len(pos)) < 0.1
+    pos[out, 2] = np.where(rng.random(out.sum()) < 0.5, d.lo[2] - 0.3, d.hi[2] + 0.2)
+    r = RefBox(d)
+    r.register_column(1, FWD)
+    r.add_atoms(pos, mass, rad, vel=vel)
+    r.set_column(0, np.arange(len(pos), dtype=np.float64) * 3.0, 1)
+    r.set_boundary(2, "f")
+    r.setup()
+    if tilt:
+        r.set_tilt(tilt, 0.0)
+    for ph in ("apply_deform", "initial_integration", "remove_ghost_atoms", "zero_all_forces"):
+        r.phase(ph)
+    a0 = r.atoms()
+    z = a0["pos"][:, 2]
+    gone = (z < d.lo[2]) | (z >= d.hi[2])
+    assert np.array_equal(gone, out)
+    r.phase("pbc")
+    a1 = r.atoms()
+    order = removal_order(gone)
+    assert a1["nlocal"] == len(pos) - out.sum() == len(order)
+    assert np.array_equal(a1["tag"], a0["tag"][order])
+    assert np.array_equal(a1["radius"], a0["radius"][order])
+    assert np.array_equal(r.column(0), a0["tag"][order] * 3.0)
+    assert r.state == 0  # FullRebuild
