@@ -382,6 +382,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--parts", default="plain,sheared,batched,served,cpu")
     ap.add_argument("--sheared-steps", type=int, default=20)
+    ap.add_argument("--csr-steps", type=int, default=200,
+                    help="steps of the plain box without the consumer, timing the CSR build alone")
     ap.add_argument("--batched-steps", type=int, default=200)
     ap.add_argument("--plain-ensemble-steps", type=int, default=10)
     ap.add_argument("--plain-steps-per-call", type=int, default=50)
@@ -439,6 +441,10 @@ def main():
         out["config"]["npairs"] = res["npairs"]
         out["roofline"] = csr_roofline(res["csr"])
         out["step_roofline"] = res["step_roofline"]
+        # the same box without the consumer: the fill is then the CSR's own
+        # placement (no transpose slots), so count + scan + fill is the build alone
+        pure = run_box(ctx, c2_gen, args.csr_steps, args.warmup, 2, consumer=False)
+        out["roofline"]["without_consumer"] = csr_roofline(pure["csr"])
         log(f"plain: {out['value']:.3e} particle-steps/s, {res['ms_per_step']:.3f} ms/step")
     c3_gen = None
     if "sheared" in parts:
