@@ -58,14 +58,22 @@ def host_workers():
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_plain_1M.json")
 
-# the count-pass kernel of each CSR strategy (SOIL_HIP_CSR_MODE; 3 is the default)
+# the count-pass kernel of each CSR strategy (SOIL_HIP_CSR_MODE; 5, one pass, is the default)
 COUNT_KERNEL = {0: "k_csr_count", 1: "k_csr_count_g", 2: "k_csr_count_g", 3: "k_csr_count_w",
-                4: "k_csr_count_flat"}
-FILL_KERNEL = {0: "k_csr_fill", 1: "k_csr_fill", 2: "k_csr_fill", 3: "k_csr_fill", 4: "k_csr_fill_flat"}
+                4: "k_csr_count_flat", 5: "k_csr_tcount"}
+FILL_KERNEL = {0: "k_csr_fill", 1: "k_csr_fill", 2: "k_csr_fill", 3: "k_csr_fill", 4: "k_csr_fill_flat",
+               5: "k_csr_tplace"}
 
 
 def csr_mode():
-    return int(os.environ.get("SOIL_HIP_CSR_MODE", "3") or 3)
+    return int(os.environ.get("SOIL_HIP_CSR_MODE", "5") or 5)
+
+
+def csr_kernel_text(mode):
+    staged = {4: "dense wave regions", 5: "rows kept in LDS, staged densely per tile"}.get(mode, "row slots")
+    scan = {4: "scan of wave totals", 5: "scan of tile totals"}.get(mode, "offset scan")
+    return (f"CSR build: {COUNT_KERNEL[mode]} (candidate tests, row counts and {staged}) + {scan} + "
+            f"{FILL_KERNEL[mode]} (placement, with the consumer's transpose slots)")
 
 
 def measured_traffic(kernels):
@@ -143,12 +151,10 @@ def csr_roofline(tm):
     count_ms = tm["count_ms"] / nb
     fill_ms = tm["fill_ms"] / nb
     mode = csr_mode()
-    traffic, src = measured_traffic([COUNT_KERNEL[mode], FILL_KERNEL[mode]])
+    traffic, src = measured_traffic([k for k in (COUNT_KERNEL[mode], FILL_KERNEL[mode]) if k])
     r = dict(rl.roofline(per, build_ms * 1e-3))
     r.update({
-        "kernel": f"CSR build: {COUNT_KERNEL[mode]} (candidate tests, row counts and "
-                  f"{'dense wave regions' if mode == 4 else 'row slots'}) + offset scan + "
-                  f"{FILL_KERNEL[mode]} (placement, with the consumer's transpose slots)",
+        "kernel": csr_kernel_text(mode),
         "bound": "hbm",
         "traffic": traffic,
         "traffic_source": src,

@@ -114,12 +114,15 @@ int soil_hip_box_device_bytes(soil_hip_box* box, size_t* estimate, size_t* alloc
  * (comm.rs:253-283) and setup_deform (soil_deform/src/lib.rs:346-420).
  * Sizes device buffers for the uploaded atoms. Two environment variables,
  * read here, tune the CSR build without changing its output:
- * SOIL_HIP_CSR_MODE (0-4, default 3: one thread per row with wave-staged
- * candidates; 4: each wave's rows' candidates flattened over its 64 lanes,
- * accepted pairs compacted into a dense region per wave) and
+ * SOIL_HIP_CSR_MODE (0-5, default 5: one thread per row with wave-staged
+ * candidates, each row's first neighbours kept in LDS, each tile's rows
+ * staged densely, a scan over the tiles' totals, then a copy of every tile to
+ * its place; 3: the same count into per-row slots, then a scan over the rows'
+ * counts and a fill; 4: each wave's rows' candidates flattened over its 64
+ * lanes, accepted pairs compacted into a dense region per wave) and
  * SOIL_HIP_ROW_SLOTS (4-64, default 12: pair-arena entries kept per local
- * row, the size of a wave's region in mode 4; a row or wave beyond it is
- * walked again);
+ * row: the slots of modes 2-3, a wave's region in mode 4, a tile's region in
+ * mode 5; a row, wave or tile beyond it is walked again);
  * SOIL_HIP_MAX_GRID (256-65536, default 4096) caps the blocks of a box's
  * launches (on 10^6 spheres 4096 is fastest: 1024 costs 14 %, 8192 1 %). */
 int soil_hip_setup(soil_hip_box* box);

@@ -34,7 +34,7 @@ static int batch_tiles(soil_hip_batch* bt) {
     bt->L.boxes = bt->d_ptrs;
     bt->L.stream = bt->stream;
     bt->L.maxgrid = maxgrid;
-    bt->L.csr_mode = bt->boxes.empty() ? 3 : bt->boxes[0]->h_dev.csr_mode;
+    bt->L.csr_mode = bt->boxes.empty() ? 5 : bt->boxes[0]->h_dev.csr_mode;
     bt->L.any_open = false;
     for (auto* b : bt->boxes)
         bt->L.any_open |= !(b->model.periodic[0] && b->model.periodic[1] && b->model.periodic[2]);
@@ -650,10 +650,12 @@ int soil_hip_batch_autotune(soil_hip_batch* bt) {
         }
     };
     bt->tune_trials.clear();
-    // coordinate descent: the CSR strategy at the current launch cap (mode 3:
-    // one thread per row walking its own candidates; mode 4: a wave's 64
-    // lanes sharing its rows' flattened candidates), then the launch cap
-    const int modes[2] = {4, 3};
+    // coordinate descent: the CSR strategy at the current launch cap (mode 5:
+    // one pass, rows placed at offsets from a look-back; mode 3: one thread
+    // per row into row slots, then an offset scan and a fill; mode 4: a
+    // wave's 64 lanes sharing its rows' flattened candidates), then the
+    // launch cap
+    const int modes[3] = {5, 4, 3};
     const uint32_t cands[5] = {1024, 2048, 4096, 8192, 16384};
     const uint32_t mg0 = bt->L.maxgrid;
     int best_mode = bt->L.csr_mode;

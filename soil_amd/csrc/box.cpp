@@ -905,8 +905,8 @@ int soil_hip_setup(soil_hip_box* b) {
     d.transpose = (b->consumer || (b->hook && (b->hook_flags & SOIL_HIP_HOOK_TRANSPOSE))) ? 1 : 0;
     {
         const char* m = getenv("SOIL_HIP_CSR_MODE");
-        d.csr_mode = m ? atoi(m) : 3;
-        if (d.csr_mode < 0 || d.csr_mode > 4) d.csr_mode = 3;
+        d.csr_mode = m ? atoi(m) : 5;
+        if (d.csr_mode < 0 || d.csr_mode > 5) d.csr_mode = 5;
     }
     BoxState st{};
     st.nlocal = (uint32_t)n;

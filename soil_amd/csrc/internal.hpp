@@ -157,7 +157,8 @@ struct BoxDev {
     GMEM uint32_t* tslot;      // cap_pairs: arrival slot of each CSR entry in its column
     GMEM uint32_t* tsrc;       // cap_pairs: row atom of each incoming pair, grouped by column
     // border compaction scratch: per tile counts of the two swaps
-    GMEM uint32_t* ctile;      // CSR: per local tile, entries staged in the temp region (or overflow mark)
+    GMEM uint32_t* ctile;      // CSR: per local tile, entries staged in the temp region (mode 0: or an
+                               // overflow mark; mode 5: its total, scanned in place into tile bases)
     GMEM uint32_t* wcount;     // CSR (mode 4): entries of each 64-row wave, scanned in place into wave bases
     GMEM uint32_t* bcnt;       // 6 per border tile: selections of the 6 swaps
     ColDev cols[SOIL_MAX_COLS];
@@ -175,8 +176,9 @@ struct BoxDev {
     GMEM soil_hip_device_view* view;  // device copy of the hook's view (refreshed by k_sync_view), or null
     GMEM HostMirror* mirror;          // device address of the box's host mirror, or null
     int32_t csr_mode;     // CSR strategy (bit-neutral): 0 LDS tiles, 1 two global passes, 2 one pass + row slots,
-                          // 3 (default) as 2 with wave-staged candidates, 4 flattened per-wave candidates into
-                          // dense wave regions
+                          // 3 as 2 with wave-staged candidates, 4 flattened per-wave candidates into
+                          // dense wave regions, 5 (default) mode 3's count with rows kept in LDS, each
+                          // tile's rows staged densely, a scan over tile totals, a copy to their place
 };
 
 int set_error(int code, const char* fmt, ...);

@@ -1023,6 +1023,11 @@ __device__ __forceinline__ void scan_view(CBox& b, uint32_t& n, bool& active, GM
         active = st->rebuild_now != 0 && (st->err & ERR_GHOST_CAPACITY) == 0;
         in = b.wcount;
         out = b.wcount;
+    } else if (W == SCAN_TILES) {
+        n = st->nlocal / BLK + 2u;  // the tiles up to the one holding the end marker, and the end entry
+        active = st->rebuild_now != 0 && (st->err & ERR_GHOST_CAPACITY) == 0;
+        in = b.ctile;
+        out = b.ctile;
     } else {
         n = st->nlocal + st->nghost + 1;
         active = transpose_active(b);
@@ -1090,7 +1095,7 @@ __global__ void __launch_bounds__(BLK) k_scan_down(KARGS) {
 #pragma unroll
         for (uint32_t w = 0; w < BLK / 64; ++w) acc += red[w];
         __syncthreads();  // red[] is reused by the next tile
-        if ((W == SCAN_CSR || W == SCAN_WAVES) && first && threadIdx.x == 0) {
+        if ((W == SCAN_CSR || W == SCAN_WAVES || W == SCAN_TILES) && first && threadIdx.x == 0) {
             BoxState* st = b.st;
             st->local_cells_valid = 0;  // consumed by this build (neighbor.rs:596)
             st->npairs = acc;
@@ -1912,13 +1917,22 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <bool UNIFORM, bool NEWTON>
+// LDS (the one-pass build, mode 5): the row's first CSR1_SLOTS accepted
+// candidates are kept as record indices (positions in the bin-sorted order),
+// slot k of lane l at lrow[k * 64 + l]; the count goes on past them. Otherwise
+// the row's first `slot` neighbours go to `out` (global row slots).
+constexpr uint32_t CSR1_SLOTS = 8;
+
+template <bool UNIFORM, bool NEWTON, bool LDS = false>
 __device__ uint32_t csr_row_w(CBox& b, CGeom& g, uint32_t i, bool act, uint32_t c, uint32_t c_first,
                               uint32_t c_last, bool staged, uint32_t lane, double4* __restrict__ lpos,
-                              uint32_t* __restrict__ lj, GMEM uint32_t* out, uint32_t slot) {
+                              uint32_t* __restrict__ lj, GMEM uint32_t* out, uint32_t slot,
+                              uint32_t* __restrict__ lrow = nullptr) {
     const double4* __restrict__ sp4 = (const double4*)b.spos;
     const double4 pi = act ? ((const double4*)b.xr)[i] : make_double4(0.0, 0.0, 0.0, 0.0);
     const int32_t ncell = (int32_t)g.ncells;
+    // the full list skips the row atom itself: its own record
+    const uint32_t self_m = (NEWTON || !act) ? 0xffffffffu : b.sidx[i];
     uint32_t n = 0;
     auto take = [&](uint32_t j) {
         if (NEWTON || j != i) {  // the full list skips the row atom itself
@@ -1926,11 +1940,20 @@ __device__ uint32_t csr_row_w(CBox& b, CGeom& g, uint32_t i, bool act, uint32_t 
             ++n;
         }
     };
+    auto take_m = [&](uint32_t m) {
+        if (NEWTON || m != self_m) {
+            if (n < CSR1_SLOTS) lrow[n * 64u + lane] = m;
+            ++n;
+        }
+    };
     // this lane's range [r0, r1) inside the wave's union [u0, u1) (wave-uniform)
     auto span = [&](uint32_t r0, uint32_t r1, uint32_t u0, uint32_t u1) {
         if (!staged || u1 - u0 > CSRW_MAX_SPAN) {
             for (uint32_t m = r0; m < r1; ++m)
-                if (accept_pair<UNIFORM>(pi.x, pi.y, pi.z, pi.w, sp4[m], g.ucut_sq, g.sf_sq)) take(b.sorted[m]);
+                if (accept_pair<UNIFORM>(pi.x, pi.y, pi.z, pi.w, sp4[m], g.ucut_sq, g.sf_sq)) {
+                    if (LDS) take_m(m);
+                    else take(b.sorted[m]);
+                }
             return;
         }
         for (uint32_t base = u0; base < u1; base += CSRW_CH) {  // empty when u1 <= u0
@@ -1939,14 +1962,17 @@ __device__ uint32_t csr_row_w(CBox& b, CGeom& g, uint32_t i, bool act, uint32_t 
                 const uint32_t m = base + q + lane;
                 if (m < u1) {
                     lpos[q + lane] = sp4[m];
-                    lj[q + lane] = b.sorted[m];
+                    if (!LDS) lj[q + lane] = b.sorted[m];
                 }
             }
             wave_sync();
             const uint32_t lo = r0 > base ? r0 : base;
             const uint32_t hi = r1 < base + CSRW_CH ? r1 : base + CSRW_CH;
             for (uint32_t m = lo; m < hi; ++m)
-                if (accept_pair<UNIFORM>(pi.x, pi.y, pi.z, pi.w, lpos[m - base], g.ucut_sq, g.sf_sq)) take(lj[m - base]);
+                if (accept_pair<UNIFORM>(pi.x, pi.y, pi.z, pi.w, lpos[m - base], g.ucut_sq, g.sf_sq)) {
+                    if (LDS) take_m(m);
+                    else take(lj[m - base]);
+                }
             wave_sync();  // the next chunk overwrites the staging
         }
     };
@@ -2377,6 +2403,192 @@ __global__ void __launch_bounds__(BLK) k_csr_fill(KARGS) {
     }
 }
 
+// ── CSR build, mode 5 (default): tile-staged rows ─────────────────────────
+// Count (k_csr_tcount): a tile's waves count their rows as k_csr_count_w
+// does, keeping each row's first CSR1_SLOTS accepted candidates in LDS (as
+// record indices); the block's exclusive scan gives each row's offset within
+// the tile (offsets[i]) and the tile's total (ctile). The tile's rows then go
+// densely, in CSR order, into the tile's own region of the temp arena: rows
+// that fit their slots are copied one entry per lane, longer rows are walked
+// again by their own lanes (the staging spans only their cells). A tile whose
+// total exceeds its region stages nothing.
+// Scan: an exclusive scan over the tile totals (a few thousand per 10^6 rows)
+// gives each tile's base, and the build's pair count.
+// Place (k_csr_tplace): each tile's region is copied to its base, one entry
+// per thread, taking the consumer's transpose slots; rows' offsets become
+// base + offset in tile. An unstaged tile's rows are walked again here.
+
+// Region of tile `tix` in the temp arena: cap_pairs split evenly over the
+// box's local tiles (at least row-slots entries per row).
+__device__ __forceinline__ uint32_t tile_region(CBox& b) {
+    const uint32_t ntl = b.cap_local / BLK + 1u;
+    return b.cap_pairs / ntl;
+}
+
+template <bool UNIFORM, bool NEWTON>
+__device__ __forceinline__ uint32_t csr1_count(CBox& b, CGeom& g, uint32_t i0w, uint32_t nl, uint32_t lane,
+                                               double4* lpos, uint32_t* lrow) {
+    const uint32_t i = i0w + lane;
+    const bool act = i < nl;
+    const uint32_t c = act ? b.cell[i] : 0u;
+    const uint32_t last = (nl - i0w < 64u ? nl - i0w : 64u) - 1u;
+    const uint32_t c_first = __shfl(c, 0, 64);
+    const uint32_t c_last = __shfl(c, (int)last, 64);
+    const uint32_t c_next = __shfl_down(c, 1, 64);
+    const bool sorted = __ballot(act && lane < last && c > c_next) == 0ull;
+    return csr_row_w<UNIFORM, NEWTON, true>(b, g, i, act, c, c_first, c_last, sorted, lane, lpos, nullptr, nullptr,
+                                            0u, lrow);
+}
+
+// Walks again, by their own lanes, the wave's rows selected by `sel`, each
+// written from out + its offset (loc); the staging covers only their cells.
+template <bool UNIFORM, bool NEWTON>
+__device__ __forceinline__ void csr1_rewalk(CBox& b, CGeom& g, uint32_t i0w, uint32_t nl, uint32_t lane, bool sel,
+                                            GMEM uint32_t* out, uint32_t loc, double4* lpos, uint32_t* lj) {
+    const uint32_t i = i0w + lane;
+    const bool act = i < nl;
+    const uint64_t sm = __ballot(sel);
+    if (sm == 0ull) return;  // wave-uniform
+    const uint32_t last = (nl - i0w < 64u ? nl - i0w : 64u) - 1u;
+    const uint32_t c = act ? b.cell[i] : 0u;
+    const uint32_t c_next = __shfl_down(c, 1, 64);
+    const bool sorted = __ballot(act && lane < last && c > c_next) == 0ull;
+    const uint32_t c_first = __shfl(c, (int)__builtin_ctzll(sm), 64);
+    const uint32_t c_last = __shfl(c, 63 - (int)__builtin_clzll(sm), 64);
+    (void)csr_row_w<UNIFORM, NEWTON>(b, g, i, sel, c, c_first, c_last, sorted, lane, lpos, lj, out + loc,
+                                     sel ? 0xffffffffu : 0u);
+}
+
+template <bool UNIFORM, bool NEWTON>
+__device__ __forceinline__ uint32_t csr1_tile_count(CBox& b, CGeom& g, uint32_t i0, uint32_t nl, uint32_t wid,
+                                                    uint32_t lane, double4* lpos, uint32_t* lrow, uint32_t* ls,
+                                                    uint32_t* lscan) {
+    const uint32_t i0w = i0 + wid * 64u;
+    const uint32_t i = i0w + lane;
+    const bool act = i < nl;
+    uint32_t n = 0;
+    if (i0w < nl) n = csr1_count<UNIFORM, NEWTON>(b, g, i0w, nl, lane, lpos, lrow);  // wave-uniform
+    uint32_t tot;
+    const uint32_t loc = block_excl_scan(n, lscan, tot);
+    if (act) b.offsets[i] = loc;
+    const uint32_t tcap = tile_region(b);
+    if (tot > tcap || i0w >= nl) return tot;  // block-uniform / wave-uniform: nothing staged
+    GMEM uint32_t* region = b.ttmp + (size_t)(i0 / BLK) * tcap;
+    const uint32_t last = (nl - i0w < 64u ? nl - i0w : 64u) - 1u;
+    ls[lane] = loc;
+    wave_sync();
+    const uint32_t o0 = __shfl(loc, 0, 64), o1 = __shfl(loc + n, (int)last, 64);
+    for (uint32_t e = o0 + lane; e < o1; e += 64u) {
+        uint32_t r = 0;  // the last row starting at or before e holds it
+#pragma unroll
+        for (uint32_t step = 32; step > 0; step >>= 1)
+            if (r + step <= last && ls[r + step] <= e) r += step;
+        const uint32_t rs = ls[r];
+        const uint32_t re = r < last ? ls[r + 1] : o1;
+        if (re - rs > CSR1_SLOTS) continue;  // walked again below
+        region[e] = b.sorted[lrow[(e - rs) * 64u + r]];
+    }
+    wave_sync();  // lrow becomes the walk's staging of neighbour indices
+    csr1_rewalk<UNIFORM, NEWTON>(b, g, i0w, nl, lane, act && n > CSR1_SLOTS, region, loc, lpos, lrow);
+    return tot;
+}
+
+__global__ void __launch_bounds__(BLK) k_csr_tcount(KARGS) {
+    __shared__ double4 lpos[BLK / 64][CSRW_CH];
+    __shared__ uint32_t lrow[BLK / 64][CSR1_SLOTS * 64];
+    __shared__ uint32_t lstart[BLK / 64][64];
+    __shared__ uint32_t lscan[BLK / 64];
+    const uint32_t tid = threadIdx.x, wid = tid >> 6, lane = tid & 63u;
+    FOR_TILES(t) {
+        const Tile tl = tiles[t];
+        CBox& b = box_at(boxes, tl.box);
+        if (!live(b)) continue;
+        const BoxState* st = b.st;
+        if (!st->rebuild_now || (st->err & ERR_GHOST_CAPACITY)) continue;
+        const uint32_t nl = st->nlocal;
+        const uint32_t i0 = tl.start;
+        if (i0 > nl) continue;  // block-uniform
+        CGeom& g = geom_of(b);
+        uint32_t tot;
+        if (g.newton) tot = g.uniform ? csr1_tile_count<true, true>(b, g, i0, nl, wid, lane, lpos[wid], lrow[wid], lstart[wid], lscan)
+                                      : csr1_tile_count<false, true>(b, g, i0, nl, wid, lane, lpos[wid], lrow[wid], lstart[wid], lscan);
+        else tot = g.uniform ? csr1_tile_count<true, false>(b, g, i0, nl, wid, lane, lpos[wid], lrow[wid], lstart[wid], lscan)
+                             : csr1_tile_count<false, false>(b, g, i0, nl, wid, lane, lpos[wid], lrow[wid], lstart[wid], lscan);
+        const uint32_t tix = i0 / BLK;
+        if (tid == 0) {
+            b.ctile[tix] = tot;
+            if (nl - i0 < BLK) b.ctile[tix + 1] = 0u;  // the scan's end entry (its result is the pair count)
+        }
+        __syncthreads();  // the next tile rewrites the LDS tables
+    }
+}
+
+template <bool UNIFORM, bool NEWTON>
+__device__ __forceinline__ void csr1_tile_rewalk(CBox& b, CGeom& g, uint32_t i0, uint32_t nl, uint32_t wid,
+                                                 uint32_t lane, uint32_t base, const uint32_t* locs, double4* lpos,
+                                                 uint32_t* lj) {
+    const uint32_t i0w = i0 + wid * 64u;
+    if (i0w >= nl) return;  // wave-uniform
+    const uint32_t i = i0w + lane;
+    csr1_rewalk<UNIFORM, NEWTON>(b, g, i0w, nl, lane, i < nl, b.indices + base, locs[lane], lpos, lj);
+}
+
+__global__ void __launch_bounds__(BLK) k_csr_tplace(KARGS) {
+    __shared__ double4 lpos[BLK / 64][CSRW_CH];
+    __shared__ uint32_t lj[BLK / 64][CSRW_CH];
+    __shared__ uint32_t lloc[BLK / 64][64];
+    const uint32_t tid = threadIdx.x, wid = tid >> 6, lane = tid & 63u;
+    FOR_TILES(t) {
+        const Tile tl = tiles[t];
+        CBox& b = box_at(boxes, tl.box);
+        if (!live(b)) continue;
+        const BoxState* st = b.st;
+        if (!st->rebuild_now || (st->err & (ERR_GHOST_CAPACITY | ERR_PAIR_CAPACITY))) continue;
+        const uint32_t nl = st->nlocal;
+        const uint32_t i0 = tl.start;
+        if (i0 > nl) continue;  // block-uniform
+        const uint32_t tix = i0 / BLK;
+        const uint32_t base = b.ctile[tix];
+        const uint32_t tot = b.ctile[tix + 1] - base;
+        const uint32_t i = i0 + tid;
+        uint32_t loc = 0;
+        if (i < nl) {
+            loc = b.offsets[i];
+            b.offsets[i] = base + loc;
+        } else if (i == nl) {
+            b.offsets[nl] = base + tot;
+        }
+        const bool tr = transpose_active(b);
+        const uint32_t tcap = tile_region(b);
+        GMEM uint32_t* dst = b.indices + base;
+        if (tot <= tcap) {  // block-uniform: the staged region, one entry per thread
+            const GMEM uint32_t* src = b.ttmp + (size_t)tix * tcap;
+            for (uint32_t e = tid; e < tot; e += BLK) {
+                const uint32_t j = src[e];
+                dst[e] = j;
+                if (tr) b.tslot[base + e] = atomicAdd(&b.tcount[j], 1u);  // consumer transpose slot
+            }
+            continue;
+        }
+        // unstaged: every row of the tile walked again, then its transpose slots
+        lloc[wid][lane] = loc;
+        CGeom& g = geom_of(b);
+        if (g.newton) {
+            if (g.uniform) csr1_tile_rewalk<true, true>(b, g, i0, nl, wid, lane, base, lloc[wid], lpos[wid], lj[wid]);
+            else csr1_tile_rewalk<false, true>(b, g, i0, nl, wid, lane, base, lloc[wid], lpos[wid], lj[wid]);
+        } else {
+            if (g.uniform) csr1_tile_rewalk<true, false>(b, g, i0, nl, wid, lane, base, lloc[wid], lpos[wid], lj[wid]);
+            else csr1_tile_rewalk<false, false>(b, g, i0, nl, wid, lane, base, lloc[wid], lpos[wid], lj[wid]);
+        }
+        if (!tr) continue;  // block-uniform
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");  // the rows just written, read back by other threads
+        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        for (uint32_t e = tid; e < tot; e += BLK) b.tslot[base + e] = atomicAdd(&b.tcount[dst[e]], 1u);
+        __syncthreads();  // lloc is rewritten by the next tile
+    }
+}
+
 // ── consumer transpose: incoming pairs of every atom ─────────────────────
 // The harmonic consumer sums exact integers, so the order of a column's
 // entries is free: arrival slots from atomics place them directly.
@@ -2755,18 +2967,24 @@ static void build_rest(const Launch& L, hipEvent_t ev0, hipEvent_t evc, hipEvent
     if (L.csr_mode == 0) launch(k_csr_count, L.t_local1, L.boxes, L.stream, L.maxgrid);  // tiles + end marker
     else if (L.csr_mode == 3) launch(k_csr_count_w, L.t_local1, L.boxes, L.stream, L.maxgrid);
     else if (L.csr_mode == 4) launch(k_csr_count_flat, L.t_local1, L.boxes, L.stream, L.maxgrid);
+    else if (L.csr_mode == 5) launch(k_csr_tcount, L.t_local1, L.boxes, L.stream, L.maxgrid);
     else launch(k_csr_count_g, L.t_local1, L.boxes, L.stream, L.maxgrid);
     if (evc) (void)hipEventRecord(evc, L.stream);
-    if (L.csr_mode == 4) {  // the scan of the waves' totals publishes the build's counts
+    // the scan publishes the build's counts (over the waves' totals in mode 4,
+    // the tiles' totals in mode 5, the rows' counts otherwise)
+    if (L.csr_mode == 4) {
         if (publish) run_scan<SCAN_WAVES, true>(L, L.t_scan_local);
         else run_scan<SCAN_WAVES>(L, L.t_scan_local);
-    } else if (publish) {
-        run_scan<SCAN_CSR, true>(L, L.t_scan_local);  // publishes the build's counts
+    } else if (L.csr_mode == 5) {
+        if (publish) run_scan<SCAN_TILES, true>(L, L.t_scan_local);
+        else run_scan<SCAN_TILES>(L, L.t_scan_local);
     } else {
-        run_scan<SCAN_CSR>(L, L.t_scan_local);
+        if (publish) run_scan<SCAN_CSR, true>(L, L.t_scan_local);
+        else run_scan<SCAN_CSR>(L, L.t_scan_local);
     }
     if (ev1) (void)hipEventRecord(ev1, L.stream);
     if (L.csr_mode == 4) launch(k_csr_fill_flat, L.t_local1, L.boxes, L.stream, L.maxgrid);
+    else if (L.csr_mode == 5) launch(k_csr_tplace, L.t_local1, L.boxes, L.stream, L.maxgrid);
     else launch(k_csr_fill, L.t_local, L.boxes, L.stream, L.maxgrid);
     if (ev2) (void)hipEventRecord(ev2, L.stream);
     // consumer transpose (tcount was cleared by save_build; modes 1 and 2

@@ -12,7 +12,8 @@ constexpr uint32_t BORDER_PER = 4;                 // 64-atom chunks per wave in
 constexpr uint32_t BORDER_TILE = BLK * BORDER_PER;  // atoms per border-compaction tile
 
 enum { SORT_SPATIAL = 0, SORT_BUILD = 1 };
-enum { SCAN_CELLS_SPATIAL = 0, SCAN_CELLS_BUILD = 1, SCAN_CSR = 2, SCAN_TRANSPOSE = 3, SCAN_WAVES = 4, SCAN_REMOVE = 5 };
+enum { SCAN_CELLS_SPATIAL = 0, SCAN_CELLS_BUILD = 1, SCAN_CSR = 2, SCAN_TRANSPOSE = 3, SCAN_WAVES = 4, SCAN_REMOVE = 5,
+       SCAN_TILES = 6 };
 
 struct Tile {
     uint32_t box, start;

@@ -421,14 +421,16 @@ def test_pair_growth_after_a_sort_keeps_the_columns(ctx):
     assert grew_pairs
 
 
+@pytest.mark.parametrize("mode", ["3", "4"])
 @pytest.mark.parametrize("case", ["dense_poly", "lees_edwards", "newton_off", "narrow_regions", "uniform"])
-def test_flat_csr_mode_matches_oracle(ctx, monkeypatch, case):
-    # CSR mode 4 (a wave's candidates flattened over its lanes, dense wave
-    # regions, the scan over wave totals) gives the same lists: dense and
-    # sheared boxes, the full list, regions narrow enough that many waves
-    # overflow into the fill's re-walk, and the uniform-radius test
-    print(f"case: flat CSR mode, {case}", flush=True)
-    monkeypatch.setenv("SOIL_HIP_CSR_MODE", "4")
+def test_other_csr_modes_match_oracle(ctx, monkeypatch, mode, case):
+    # CSR modes 3 (per-row slots, offset scan, fill) and 4 (a wave's
+    # candidates flattened over its lanes, dense wave regions, the scan over
+    # wave totals) give the same lists as the default one-pass mode: dense and
+    # sheared boxes, the full list, slots or regions narrow enough that many
+    # rows overflow into the fill's re-walk, and the uniform-radius test
+    print(f"case: CSR mode {mode}, {case}", flush=True)
+    monkeypatch.setenv("SOIL_HIP_CSR_MODE", mode)
     if case == "dense_poly":
         run_compare(ctx, small(4801, 5000, 0.4, 0.6, 0.55), 30, 10, k=1000.0)
     elif case == "lees_edwards":
@@ -442,3 +444,24 @@ def test_flat_csr_mode_matches_oracle(ctx, monkeypatch, case):
         run_compare(ctx, small(4804, 4000, 0.4, 0.6, 0.55), 30, 10, k=1000.0)
     else:
         run_compare(ctx, small(4805, 4000, 0.5, 0.5, 0.4), 30, 10, k=1000.0)
+
+
+@pytest.mark.parametrize("case", ["clustered_half", "clustered_full", "narrow_tiles", "tiny"])
+def test_tile_staged_csr_long_rows_and_small_boxes(ctx, monkeypatch, case):
+    # the default build (mode 5): blobs whose rows run far past the 8 LDS
+    # slots (walked again by their lanes), under the half and the full list;
+    # tile regions so narrow (4 entries per row) that dense tiles stage
+    # nothing and are walked again at placement; boxes of one partial tile
+    print(f"case: tile-staged CSR, {case}", flush=True)
+    if case == "clustered_half":
+        run_compare(ctx, small(4811, 6000, 0.4, 0.6, 0.4, (2, 1, 1), 6, 0.6, 2.0, shear=0.1), 20, 10, k=1000.0)
+    elif case == "clustered_full":
+        def mod(d):
+            d.newton = 0
+        run_compare(ctx, small(4812, 3000, 0.4, 0.6, 0.35, (1, 1, 1), 4, 0.6, 2.0), 20, 10, desc_mod=mod)
+    elif case == "narrow_tiles":
+        monkeypatch.setenv("SOIL_HIP_ROW_SLOTS", "4")
+        run_compare(ctx, small(4814, 6000, 0.4, 0.6, 0.45, (1, 1, 1), 6, 0.6, 2.0), 20, 10, k=1000.0)
+    else:
+        for n in (1, 63, 64, 200, 256, 257):
+            run_compare(ctx, small(4813 + n, n, 0.4, 0.6, 0.3), 6, 3, k=1000.0)
