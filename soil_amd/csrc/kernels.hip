@@ -1910,6 +1910,7 @@ __global__ void __launch_bounds__(BLK) k_csr_count_g(KARGS) {
 // union is very wide, walks global memory per lane instead.
 constexpr uint32_t CSRW_CH = 128;         // staged records per chunk and wave
 constexpr uint32_t CSRW_MAX_SPAN = 1024;  // wider unions are walked per lane
+static_assert(CSRW_CH % 64u == 0u, "a chunk is staged 64 records (one per lane) at a time");
 
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1917,11 +1918,14 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// LDS (the one-pass build, mode 5): the row's first CSR1_SLOTS accepted
+// LDS (the tile-staged build, mode 5): the row's first CSR1_SLOTS accepted
 // candidates are kept as record indices (positions in the bin-sorted order),
-// slot k of lane l at lrow[k * 64 + l]; the count goes on past them. Otherwise
-// the row's first `slot` neighbours go to `out` (global row slots).
+// slot k of lane l at lrow[k * 64 + l] (lrow holds CSR1_SLOTS + 1 slots, the
+// last one discarding); the count goes on past them. Otherwise the row's
+// first `slot` neighbours go to `out` (global row slots).
 constexpr uint32_t CSR1_SLOTS = 8;
+// the tile-staged count reuses a wave's slots as the neighbour-index staging of a walk
+static_assert((CSR1_SLOTS + 1u) * 64u >= CSRW_CH, "the slots must hold a staged chunk");
 
 template <bool UNIFORM, bool NEWTON, bool LDS = false>
 __device__ uint32_t csr_row_w(CBox& b, CGeom& g, uint32_t i, bool act, uint32_t c, uint32_t c_first,
@@ -1940,20 +1944,22 @@ __device__ uint32_t csr_row_w(CBox& b, CGeom& g, uint32_t i, bool act, uint32_t 
             ++n;
         }
     };
-    auto take_m = [&](uint32_t m) {
-        if (NEWTON || m != self_m) {
-            if (n < CSR1_SLOTS) lrow[n * 64u + lane] = m;
-            ++n;
-        }
+    // branch-free: a rejected candidate (or one past the slots) is written to
+    // the discard slot CSR1_SLOTS
+    auto take_m = [&](bool acc, uint32_t m) {
+        if (!NEWTON) acc = acc && m != self_m;
+        const uint32_t k = acc ? (n < CSR1_SLOTS ? n : CSR1_SLOTS) : CSR1_SLOTS;
+        lrow[k * 64u + lane] = m;
+        n += acc ? 1u : 0u;
     };
     // this lane's range [r0, r1) inside the wave's union [u0, u1) (wave-uniform)
     auto span = [&](uint32_t r0, uint32_t r1, uint32_t u0, uint32_t u1) {
         if (!staged || u1 - u0 > CSRW_MAX_SPAN) {
-            for (uint32_t m = r0; m < r1; ++m)
-                if (accept_pair<UNIFORM>(pi.x, pi.y, pi.z, pi.w, sp4[m], g.ucut_sq, g.sf_sq)) {
-                    if (LDS) take_m(m);
-                    else take(b.sorted[m]);
-                }
+            for (uint32_t m = r0; m < r1; ++m) {
+                const bool acc = accept_pair<UNIFORM>(pi.x, pi.y, pi.z, pi.w, sp4[m], g.ucut_sq, g.sf_sq);
+                if (LDS) take_m(acc, m);
+                else if (acc) take(b.sorted[m]);
+            }
             return;
         }
         for (uint32_t base = u0; base < u1; base += CSRW_CH) {  // empty when u1 <= u0
@@ -1968,11 +1974,21 @@ __device__ uint32_t csr_row_w(CBox& b, CGeom& g, uint32_t i, bool act, uint32_t 
             wave_sync();
             const uint32_t lo = r0 > base ? r0 : base;
             const uint32_t hi = r1 < base + CSRW_CH ? r1 : base + CSRW_CH;
-            for (uint32_t m = lo; m < hi; ++m)
-                if (accept_pair<UNIFORM>(pi.x, pi.y, pi.z, pi.w, lpos[m - base], g.ucut_sq, g.sf_sq)) {
-                    if (LDS) take_m(m);
-                    else take(lj[m - base]);
+            if (LDS) {  // two candidates per pass: both LDS reads in flight before the tests
+                uint32_t m = lo;
+                for (; m + 1u < hi; m += 2u) {
+                    const double4 q0 = lpos[m - base], q1 = lpos[m + 1u - base];
+                    const bool a0 = accept_pair<UNIFORM>(pi.x, pi.y, pi.z, pi.w, q0, g.ucut_sq, g.sf_sq);
+                    const bool a1 = accept_pair<UNIFORM>(pi.x, pi.y, pi.z, pi.w, q1, g.ucut_sq, g.sf_sq);
+                    take_m(a0, m);
+                    take_m(a1, m + 1u);
                 }
+                if (m < hi) take_m(accept_pair<UNIFORM>(pi.x, pi.y, pi.z, pi.w, lpos[m - base], g.ucut_sq, g.sf_sq), m);
+            } else {
+                for (uint32_t m = lo; m < hi; ++m)
+                    if (accept_pair<UNIFORM>(pi.x, pi.y, pi.z, pi.w, lpos[m - base], g.ucut_sq, g.sf_sq))
+                        take(lj[m - base]);
+            }
             wave_sync();  // the next chunk overwrites the staging
         }
     };
@@ -2495,7 +2511,7 @@ __device__ __forceinline__ uint32_t csr1_tile_count(CBox& b, CGeom& g, uint32_t 
 
 __global__ void __launch_bounds__(BLK) k_csr_tcount(KARGS) {
     __shared__ double4 lpos[BLK / 64][CSRW_CH];
-    __shared__ uint32_t lrow[BLK / 64][CSR1_SLOTS * 64];
+    __shared__ uint32_t lrow[BLK / 64][(CSR1_SLOTS + 1) * 64];  // slot CSR1_SLOTS discards
     __shared__ uint32_t lstart[BLK / 64][64];
     __shared__ uint32_t lscan[BLK / 64];
     const uint32_t tid = threadIdx.x, wid = tid >> 6, lane = tid & 63u;
