@@ -2579,7 +2579,23 @@ __global__ void __launch_bounds__(BLK) k_csr_tplace(KARGS) {
         GMEM uint32_t* dst = b.indices + base;
         if (tot <= tcap) {  // block-uniform: the staged region, one entry per thread
             const GMEM uint32_t* src = b.ttmp + (size_t)tix * tcap;
-            for (uint32_t e = tid; e < tot; e += BLK) {
+            uint32_t e = tid;
+            // four entries per thread in flight (a dense tile has many passes)
+            for (; e + 3u * BLK < tot; e += 4u * BLK) {
+                uint32_t j[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) j[u] = src[e + u * BLK];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) dst[e + u * BLK] = j[u];
+                if (tr) {
+                    uint32_t sl[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) sl[u] = atomicAdd(&b.tcount[j[u]], 1u);  // consumer transpose slots
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) b.tslot[base + e + u * BLK] = sl[u];
+                }
+            }
+            for (; e < tot; e += BLK) {
                 const uint32_t j = src[e];
                 dst[e] = j;
                 if (tr) b.tslot[base + e] = atomicAdd(&b.tcount[j], 1u);  // consumer transpose slot
@@ -2600,7 +2616,17 @@ __global__ void __launch_bounds__(BLK) k_csr_tplace(KARGS) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");  // the rows just written, read back by other threads
         __syncthreads();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-        for (uint32_t e = tid; e < tot; e += BLK) b.tslot[base + e] = atomicAdd(&b.tcount[dst[e]], 1u);
+        uint32_t e = tid;
+        for (; e + 3u * BLK < tot; e += 4u * BLK) {
+            uint32_t j[4], sl[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) j[u] = dst[e + u * BLK];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) sl[u] = atomicAdd(&b.tcount[j[u]], 1u);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) b.tslot[base + e + u * BLK] = sl[u];
+        }
+        for (; e < tot; e += BLK) b.tslot[base + e] = atomicAdd(&b.tcount[dst[e]], 1u);
         __syncthreads();  // lloc is rewritten by the next tile
     }
 }
@@ -2638,13 +2664,30 @@ __global__ void __launch_bounds__(BLK) k_tplace(KARGS) {
         uint32_t* ls = lstart[wid];
         ls[lane] = act ? off : o1;
         wave_sync();
-        for (uint32_t e = o0 + lane; e < o1; e += 64u) {
+        auto owner = [&](uint32_t e) {
             uint32_t r = 0;
 #pragma unroll
             for (uint32_t step = 32; step > 0; step >>= 1)
                 if (r + step <= last && ls[r + step] <= e) r += step;
-            b.tsrc[b.tstart[b.indices[e]] + b.tslot[e]] = i0w + r;
+            return r;
+        };
+        // four entries per lane in flight: long rows (clustered boxes) give a
+        // wave many passes, each a chain of dependent loads
+        uint32_t e = o0 + lane;
+        for (; e + 192u < o1; e += 256u) {
+            uint32_t j[4], sl[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                j[u] = b.indices[e + 64u * u];
+                sl[u] = b.tslot[e + 64u * u];
+            }
+            uint32_t ts[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) ts[u] = b.tstart[j[u]];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) b.tsrc[ts[u] + sl[u]] = i0w + owner(e + 64u * u);
         }
+        for (; e < o1; e += 64u) b.tsrc[b.tstart[b.indices[e]] + b.tslot[e]] = i0w + owner(e);
         wave_sync();
     }
 }
@@ -2691,13 +2734,52 @@ __device__ __forceinline__ long long fix_q(double f) {
     return __double2ll_rn(f * kFixScale);
 }
 
+// Adds each lane's (qx, qy, qz) into its row's LDS accumulator. A pass's
+// lanes hold consecutive entries, so each row's lanes are contiguous; when a
+// pass spans few rows (long rows: clustered boxes) the lanes of a row are
+// summed first by a segmented scan and its last lane adds the sum, instead of
+// 64 lanes contending for one accumulator. Integer sums: the same bits either
+// way.
+__device__ __forceinline__ void harmonic_add(unsigned long long (*acc)[3], uint32_t lane, bool valid, uint32_t r,
+                                             long long qx, long long qy, long long qz) {
+    const uint32_t key = valid ? r : 0xffffffffu;
+    const uint32_t kp = __shfl_up(key, 1, 64);
+    const uint64_t heads = __ballot(valid && (lane == 0u || kp != key));
+    if (__popcll(heads) > 8) {  // wave-uniform: many rows, little contention
+        if (valid) {
+            atomicAdd(&acc[r][0], (unsigned long long)qx);
+            atomicAdd(&acc[r][1], (unsigned long long)qy);
+            atomicAdd(&acc[r][2], (unsigned long long)qz);
+        }
+        return;
+    }
+    if (!valid) qx = qy = qz = 0;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t ko = __shfl_up(key, o, 64);
+        const long long x = __shfl_up(qx, o, 64), y = __shfl_up(qy, o, 64), z = __shfl_up(qz, o, 64);
+        if (lane >= (uint32_t)o && ko == key) {
+            qx += x;
+            qy += y;
+            qz += z;
+        }
+    }
+    const uint32_t kn = __shfl_down(key, 1, 64);
+    if (valid && (lane == 63u || kn != key)) {
+        atomicAdd(&acc[r][0], (unsigned long long)qx);
+        atomicAdd(&acc[r][1], (unsigned long long)qy);
+        atomicAdd(&acc[r][2], (unsigned long long)qz);
+    }
+}
+
 // The consumer. Each atom's force is its row's pairs (as the row atom) and its
 // incoming pairs (as the neighbour): a wave's 64 atoms own one contiguous
 // range of row entries and one of incoming entries; each lane evaluates every
 // 64th pair of them (its atom found by a binary search over the wave's
 // starts in LDS) and adds the fixed-point force into the atom's LDS
 // accumulator. The sums are integers, so any evaluation order gives the same
-// bits, and the lanes stay busy whatever the rows' lengths.
+// bits, and the lanes stay busy whatever the rows' lengths. Two passes' loads
+// are issued together (each pass is a chain of dependent loads).
 template <bool FAST>
 __device__ __forceinline__ void harmonic_wave(CBox& b, const BoxState* st, uint32_t a0, uint32_t lane, uint32_t last,
                                               uint32_t* ls, const double4* pw, unsigned long long (*acc)[3]) {
@@ -2717,13 +2799,20 @@ __device__ __forceinline__ void harmonic_wave(CBox& b, const BoxState* st, uint3
         const uint32_t rend = b.offsets[a0 + last + 1u < nl ? a0 + last + 1u : nl];
         ls[lane] = a < nl ? b.offsets[a] : rend;
         wave_sync();
-        for (uint32_t e = ls[0] + lane; e < rend; e += 64u) {
-            const uint32_t r = owner(e);
+        for (uint32_t e0 = ls[0]; e0 < rend; e0 += 128u) {  // wave-uniform
+            const uint32_t e = e0 + lane, e1 = e + 64u;
+            const bool v0 = e < rend, v1 = e1 < rend;
+            double4 q0 = make_double4(0.0, 0.0, 0.0, 0.0), q1 = q0;
+            if (v0) q0 = xr[b.indices[e]];
+            if (v1) q1 = xr[b.indices[e1]];
+            const uint32_t r0 = v0 ? owner(e) : 0u, r1 = v1 ? owner(e1) : 0u;
             double fx, fy, fz;
-            pair_force(pw[r], xr[b.indices[e]], k, fx, fy, fz);
-            atomicAdd(&acc[r][0], (unsigned long long)(-fix_q<FAST>(fx)));
-            atomicAdd(&acc[r][1], (unsigned long long)(-fix_q<FAST>(fy)));
-            atomicAdd(&acc[r][2], (unsigned long long)(-fix_q<FAST>(fz)));
+            pair_force(pw[r0], q0, k, fx, fy, fz);
+            harmonic_add(acc, lane, v0, r0, -fix_q<FAST>(fx), -fix_q<FAST>(fy), -fix_q<FAST>(fz));
+            if (e0 + 64u < rend) {  // wave-uniform
+                pair_force(pw[r1], q1, k, fx, fy, fz);
+                harmonic_add(acc, lane, v1, r1, -fix_q<FAST>(fx), -fix_q<FAST>(fy), -fix_q<FAST>(fz));
+            }
         }
         wave_sync();
     }
@@ -2731,13 +2820,20 @@ __device__ __forceinline__ void harmonic_wave(CBox& b, const BoxState* st, uint3
     const uint32_t tend = b.tstart[a0 + last + 1u];
     ls[lane] = a < nall ? b.tstart[a] : tend;
     wave_sync();
-    for (uint32_t e = ls[0] + lane; e < tend; e += 64u) {
-        const uint32_t r = owner(e);
+    for (uint32_t e0 = ls[0]; e0 < tend; e0 += 128u) {  // wave-uniform
+        const uint32_t e = e0 + lane, e1 = e + 64u;
+        const bool v0 = e < tend, v1 = e1 < tend;
+        double4 q0 = make_double4(0.0, 0.0, 0.0, 0.0), q1 = q0;
+        if (v0) q0 = xr[b.tsrc[e]];
+        if (v1) q1 = xr[b.tsrc[e1]];
+        const uint32_t r0 = v0 ? owner(e) : 0u, r1 = v1 ? owner(e1) : 0u;
         double fx, fy, fz;
-        pair_force(xr[b.tsrc[e]], pw[r], k, fx, fy, fz);
-        atomicAdd(&acc[r][0], (unsigned long long)fix_q<FAST>(fx));
-        atomicAdd(&acc[r][1], (unsigned long long)fix_q<FAST>(fy));
-        atomicAdd(&acc[r][2], (unsigned long long)fix_q<FAST>(fz));
+        pair_force(q0, pw[r0], k, fx, fy, fz);
+        harmonic_add(acc, lane, v0, r0, fix_q<FAST>(fx), fix_q<FAST>(fy), fix_q<FAST>(fz));
+        if (e0 + 64u < tend) {  // wave-uniform
+            pair_force(q1, pw[r1], k, fx, fy, fz);
+            harmonic_add(acc, lane, v1, r1, fix_q<FAST>(fx), fix_q<FAST>(fy), fix_q<FAST>(fz));
+        }
     }
     wave_sync();
 }
