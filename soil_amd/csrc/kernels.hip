@@ -845,11 +845,25 @@ __global__ void __launch_bounds__(BLK) k_sort_rank(KARGS) {
         SortView<W> v = sort_view<W>(b);
         if (!v.active) continue;
         const uint32_t i = tl.start + threadIdx.x;
-        if (i >= v.n) continue;
-        const uint32_t key = v.key[i];
+        const bool act = i < v.n;
+        const uint32_t key = act ? v.key[i] : 0xffffffffu;
+        // the wave's runs of equal keys (lanes), as k_sort_count forms them
+        const uint32_t lane = __lane_id();
+        const uint32_t prev = __shfl_up(key, 1, 64);
+        const uint64_t heads = __ballot(act && (lane == 0u || prev != key));
+        const uint64_t actm = __ballot(act);
+        if (!act) continue;
+        const uint32_t nact = 64u - (uint32_t)__builtin_clzll(actm);
+        const uint64_t upto = lane == 63u ? ~0ull : ((2ull << lane) - 1ull);
+        const uint32_t leader = 63u - (uint32_t)__builtin_clzll((heads & upto) | 1ull);
+        const uint64_t above = heads & ~upto;
+        const uint32_t run_end = above ? (uint32_t)__builtin_ctzll(above) : nact;
         uint32_t s = v.start[key], e = v.start[key + 1];
         uint32_t dst;
-        if (v.fast) {
+        if (!v.fast && e - s == run_end - leader) {
+            // the run is the whole cell: its atoms are consecutive indices, in order
+            dst = s + (lane - leader);
+        } else if (v.fast) {
             const uint32_t ls = b.sstart[key], nloc = b.sstart[key + 1] - ls;
             if (i < b.st->nlocal) {
                 dst = s + (i - ls);
@@ -2753,22 +2767,25 @@ __device__ __forceinline__ void harmonic_add(unsigned long long (*acc)[3], uint3
         }
         return;
     }
-    if (!valid) qx = qy = qz = 0;
+    // the lanes of the same row below each lane, as a mask per shift (one scan
+    // of the keys serves the three components)
+    uint64_t same = 0;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
         const uint32_t ko = __shfl_up(key, o, 64);
-        const long long x = __shfl_up(qx, o, 64), y = __shfl_up(qy, o, 64), z = __shfl_up(qz, o, 64);
-        if (lane >= (uint32_t)o && ko == key) {
-            qx += x;
-            qy += y;
-            qz += z;
-        }
+        if (lane >= (uint32_t)o && ko == key) same |= 1ull << o;
     }
     const uint32_t kn = __shfl_down(key, 1, 64);
-    if (valid && (lane == 63u || kn != key)) {
-        atomicAdd(&acc[r][0], (unsigned long long)qx);
-        atomicAdd(&acc[r][1], (unsigned long long)qy);
-        atomicAdd(&acc[r][2], (unsigned long long)qz);
+    const bool tail = valid && (lane == 63u || kn != key);
+#pragma unroll 1
+    for (int c = 0; c < 3; ++c) {
+        long long q = valid ? (c == 0 ? qx : (c == 1 ? qy : qz)) : 0;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const long long y = __shfl_up(q, o, 64);
+            if (same & (1ull << o)) q += y;
+        }
+        if (tail) atomicAdd(&acc[r][c], (unsigned long long)q);
     }
 }
 
@@ -2778,8 +2795,7 @@ __device__ __forceinline__ void harmonic_add(unsigned long long (*acc)[3], uint3
 // 64th pair of them (its atom found by a binary search over the wave's
 // starts in LDS) and adds the fixed-point force into the atom's LDS
 // accumulator. The sums are integers, so any evaluation order gives the same
-// bits, and the lanes stay busy whatever the rows' lengths. Two passes' loads
-// are issued together (each pass is a chain of dependent loads).
+// bits, and the lanes stay busy whatever the rows' lengths.
 template <bool FAST>
 __device__ __forceinline__ void harmonic_wave(CBox& b, const BoxState* st, uint32_t a0, uint32_t lane, uint32_t last,
                                               uint32_t* ls, const double4* pw, unsigned long long (*acc)[3]) {
@@ -2799,20 +2815,15 @@ __device__ __forceinline__ void harmonic_wave(CBox& b, const BoxState* st, uint3
         const uint32_t rend = b.offsets[a0 + last + 1u < nl ? a0 + last + 1u : nl];
         ls[lane] = a < nl ? b.offsets[a] : rend;
         wave_sync();
-        for (uint32_t e0 = ls[0]; e0 < rend; e0 += 128u) {  // wave-uniform
-            const uint32_t e = e0 + lane, e1 = e + 64u;
-            const bool v0 = e < rend, v1 = e1 < rend;
-            double4 q0 = make_double4(0.0, 0.0, 0.0, 0.0), q1 = q0;
-            if (v0) q0 = xr[b.indices[e]];
-            if (v1) q1 = xr[b.indices[e1]];
-            const uint32_t r0 = v0 ? owner(e) : 0u, r1 = v1 ? owner(e1) : 0u;
+        for (uint32_t e0 = ls[0]; e0 < rend; e0 += 64u) {  // wave-uniform
+            const uint32_t e = e0 + lane;
+            const bool v = e < rend;
+            double4 q = make_double4(0.0, 0.0, 0.0, 0.0);
+            if (v) q = xr[b.indices[e]];
+            const uint32_t r = v ? owner(e) : 0u;
             double fx, fy, fz;
-            pair_force(pw[r0], q0, k, fx, fy, fz);
-            harmonic_add(acc, lane, v0, r0, -fix_q<FAST>(fx), -fix_q<FAST>(fy), -fix_q<FAST>(fz));
-            if (e0 + 64u < rend) {  // wave-uniform
-                pair_force(pw[r1], q1, k, fx, fy, fz);
-                harmonic_add(acc, lane, v1, r1, -fix_q<FAST>(fx), -fix_q<FAST>(fy), -fix_q<FAST>(fz));
-            }
+            pair_force(pw[r], q, k, fx, fy, fz);
+            harmonic_add(acc, lane, v, r, -fix_q<FAST>(fx), -fix_q<FAST>(fy), -fix_q<FAST>(fz));
         }
         wave_sync();
     }
@@ -2820,20 +2831,15 @@ __device__ __forceinline__ void harmonic_wave(CBox& b, const BoxState* st, uint3
     const uint32_t tend = b.tstart[a0 + last + 1u];
     ls[lane] = a < nall ? b.tstart[a] : tend;
     wave_sync();
-    for (uint32_t e0 = ls[0]; e0 < tend; e0 += 128u) {  // wave-uniform
-        const uint32_t e = e0 + lane, e1 = e + 64u;
-        const bool v0 = e < tend, v1 = e1 < tend;
-        double4 q0 = make_double4(0.0, 0.0, 0.0, 0.0), q1 = q0;
-        if (v0) q0 = xr[b.tsrc[e]];
-        if (v1) q1 = xr[b.tsrc[e1]];
-        const uint32_t r0 = v0 ? owner(e) : 0u, r1 = v1 ? owner(e1) : 0u;
+    for (uint32_t e0 = ls[0]; e0 < tend; e0 += 64u) {  // wave-uniform
+        const uint32_t e = e0 + lane;
+        const bool v = e < tend;
+        double4 q = make_double4(0.0, 0.0, 0.0, 0.0);
+        if (v) q = xr[b.tsrc[e]];
+        const uint32_t r = v ? owner(e) : 0u;
         double fx, fy, fz;
-        pair_force(q0, pw[r0], k, fx, fy, fz);
-        harmonic_add(acc, lane, v0, r0, fix_q<FAST>(fx), fix_q<FAST>(fy), fix_q<FAST>(fz));
-        if (e0 + 64u < tend) {  // wave-uniform
-            pair_force(q1, pw[r1], k, fx, fy, fz);
-            harmonic_add(acc, lane, v1, r1, fix_q<FAST>(fx), fix_q<FAST>(fy), fix_q<FAST>(fz));
-        }
+        pair_force(q, pw[r], k, fx, fy, fz);
+        harmonic_add(acc, lane, v, r, fix_q<FAST>(fx), fix_q<FAST>(fy), fix_q<FAST>(fz));
     }
     wave_sync();
 }
