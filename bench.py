@@ -213,7 +213,7 @@ def ensemble_bytes(boxes, gens, steps):
     return tot
 
 
-def run_batched(ctx, steps_batched, steps_rr, steps_each, warmup):
+def run_batched(ctx, steps_batched, steps_rr, steps_each, warmup, graph=True):
     import soil_amd
     gens = ensemble()
     ntot = sum(len(g[1]) for g in gens)
@@ -242,13 +242,13 @@ def run_batched(ctx, steps_batched, steps_rr, steps_each, warmup):
     # batched: one launch per phase for all boxes, graph-captured step
     boxes = make_boxes(ctx, gens)
     batch = soil_amd.Batch(ctx, boxes)
-    batch.step(1, graph=True)
+    batch.step(1, graph=graph)
     batch.sync()
-    tuned = batch.autotune()
-    batch.step(max(warmup, 1), graph=True)
+    tuned = batch.autotune() if graph else None  # the autotuner times captured graphs
+    batch.step(max(warmup, 1), graph=graph)
     batch.sync()
     t0 = time.perf_counter()
-    batch.step(steps_batched, graph=True)
+    batch.step(steps_batched, graph=graph)
     batch.sync()
     t_batch = time.perf_counter() - t0
     batch.close()
@@ -388,10 +388,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--parts", default="plain,sheared,batched,served,cpu")
     ap.add_argument("--sheared-steps", type=int, default=20)
-    ap.add_argument("--csr-steps", type=int, default=200,
+    ap.add_argument("--csr-steps", type=int, default=1000,
                     help="steps of the plain box without the consumer, timing the CSR build alone")
     ap.add_argument("--batched-steps", type=int, default=200)
     ap.add_argument("--plain-ensemble-steps", type=int, default=10)
+    ap.add_argument("--batched-graph", type=int, default=1,
+                    help="0: launch the batch's phases directly, without the captured step graph or the "
+                         "autotuner (for profilers that cannot follow graph launches)")
     ap.add_argument("--plain-steps-per-call", type=int, default=50)
     ap.add_argument("--probe-copy-mb", type=int, default=0,
                     help="also run the calibration copy kernel over this many MB (traffic counters)")
@@ -449,7 +452,7 @@ def main():
         out["step_roofline"] = res["step_roofline"]
         # the same box without the consumer: the fill is then the CSR's own
         # placement (no transpose slots), so count + scan + fill is the build alone
-        pure = run_box(ctx, c2_gen, args.csr_steps, args.warmup, 2, consumer=False)
+        pure = run_box(ctx, c2_gen, args.csr_steps, args.warmup, 1, consumer=False)  # every build timed
         out["roofline"]["without_consumer"] = csr_roofline(pure["csr"])
         log(f"plain: {out['value']:.3e} particle-steps/s, {res['ms_per_step']:.3f} ms/step")
     c3_gen = None
@@ -459,7 +462,7 @@ def main():
     if "batched" in parts:
         log("batched: 256-box ensemble")
         out["batched"] = run_batched(ctx, args.batched_steps, args.plain_ensemble_steps, args.plain_steps_per_call,
-                                     args.warmup)
+                                     args.warmup, graph=bool(args.batched_graph))
         log(f"batched: {out['batched']['speedup']:.2f}x over plain")
     if "served" in parts:
         out["served"] = run_served(ctx)

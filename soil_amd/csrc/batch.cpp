@@ -563,7 +563,6 @@ int soil_hip_batch_autotune(soil_hip_batch* bt) {
         Geom last;
     };
     std::vector<Snap> snaps(bt->boxes.size());
-    // arena sizes: recompute from the carve layout via the allocation sizes
     for (size_t k = 0; k < bt->boxes.size(); ++k) {
         soil_hip_box* b = bt->boxes[k];
         Snap& s = snaps[k];
@@ -578,12 +577,11 @@ int soil_hip_batch_autotune(soil_hip_batch* bt) {
         s.st = *b->h_st;
         s.model = b->model;
         s.last = b->last_geom;
-        size_t sz = 0;
-        hipDeviceptr_t base;
-        if (hipMemGetAddressRange(&base, &sz, (hipDeviceptr_t)b->arena_atoms) != hipSuccess) sz = 0;
-        s.atoms_bytes = sz;
-        if (hipMemGetAddressRange(&base, &sz, (hipDeviceptr_t)b->arena_pairs) != hipSuccess) sz = 0;
-        s.pairs_bytes = sz;
+        // the arenas' own sizes (an address-range query can report a larger
+        // block the runtime suballocated the arena from, which does not
+        // start at the arena)
+        s.atoms_bytes = b->atoms_bytes;
+        s.pairs_bytes = b->pairs_bytes;
         if (!s.atoms_bytes || !s.pairs_bytes) return set_error(SOIL_HIP_EDEVICE, "autotune: arena size unknown");
         HIPCHK(hipMalloc(&s.atoms, s.atoms_bytes));
         HIPCHK(hipMalloc(&s.pairs, s.pairs_bytes));

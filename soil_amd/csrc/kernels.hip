@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <mutex>
+#include <type_traits>
 
 #include "kernels.hpp"
 
@@ -2754,12 +2755,16 @@ __device__ __forceinline__ long long fix_q(double f) {
 // summed first by a segmented scan and its last lane adds the sum, instead of
 // 64 lanes contending for one accumulator. Integer sums: the same bits either
 // way.
+template <bool DENSE>
 __device__ __forceinline__ void harmonic_add(unsigned long long (*acc)[3], uint32_t lane, bool valid, uint32_t r,
                                              long long qx, long long qy, long long qz) {
     const uint32_t key = valid ? r : 0xffffffffu;
-    const uint32_t kp = __shfl_up(key, 1, 64);
-    const uint64_t heads = __ballot(valid && (lane == 0u || kp != key));
-    if (__popcll(heads) > 8) {  // wave-uniform: many rows, little contention
+    bool direct = !DENSE;
+    if (DENSE) {
+        const uint32_t kp = __shfl_up(key, 1, 64);
+        direct = __popcll(__ballot(valid && (lane == 0u || kp != key))) > 8;  // many rows, little contention
+    }
+    if (direct) {  // wave-uniform
         if (valid) {
             atomicAdd(&acc[r][0], (unsigned long long)qx);
             atomicAdd(&acc[r][1], (unsigned long long)qy);
@@ -2815,32 +2820,42 @@ __device__ __forceinline__ void harmonic_wave(CBox& b, const BoxState* st, uint3
         const uint32_t rend = b.offsets[a0 + last + 1u < nl ? a0 + last + 1u : nl];
         ls[lane] = a < nl ? b.offsets[a] : rend;
         wave_sync();
-        for (uint32_t e0 = ls[0]; e0 < rend; e0 += 64u) {  // wave-uniform
-            const uint32_t e = e0 + lane;
-            const bool v = e < rend;
-            double4 q = make_double4(0.0, 0.0, 0.0, 0.0);
-            if (v) q = xr[b.indices[e]];
-            const uint32_t r = v ? owner(e) : 0u;
-            double fx, fy, fz;
-            pair_force(pw[r], q, k, fx, fy, fz);
-            harmonic_add(acc, lane, v, r, -fix_q<FAST>(fx), -fix_q<FAST>(fy), -fix_q<FAST>(fz));
-        }
+        auto rows = [&](auto dense) {
+            for (uint32_t e0 = ls[0]; e0 < rend; e0 += 64u) {  // wave-uniform
+                const uint32_t e = e0 + lane;
+                const bool v = e < rend;
+                double4 q = make_double4(0.0, 0.0, 0.0, 0.0);
+                if (v) q = xr[b.indices[e]];
+                const uint32_t r = v ? owner(e) : 0u;
+                double fx, fy, fz;
+                pair_force(pw[r], q, k, fx, fy, fz);
+                harmonic_add<decltype(dense)::value>(acc, lane, v, r, -fix_q<FAST>(fx), -fix_q<FAST>(fy),
+                                                     -fix_q<FAST>(fz));
+            }
+        };
+        // rows longer than 8 entries on average: passes span few rows
+        if (rend - ls[0] > 8u * (last + 1u)) rows(std::true_type{});
+        else rows(std::false_type{});
         wave_sync();
     }
     // incoming pairs: +q to the neighbour, the row atom evaluated first
     const uint32_t tend = b.tstart[a0 + last + 1u];
     ls[lane] = a < nall ? b.tstart[a] : tend;
     wave_sync();
-    for (uint32_t e0 = ls[0]; e0 < tend; e0 += 64u) {  // wave-uniform
-        const uint32_t e = e0 + lane;
-        const bool v = e < tend;
-        double4 q = make_double4(0.0, 0.0, 0.0, 0.0);
-        if (v) q = xr[b.tsrc[e]];
-        const uint32_t r = v ? owner(e) : 0u;
-        double fx, fy, fz;
-        pair_force(q, pw[r], k, fx, fy, fz);
-        harmonic_add(acc, lane, v, r, fix_q<FAST>(fx), fix_q<FAST>(fy), fix_q<FAST>(fz));
-    }
+    auto incoming = [&](auto dense) {
+        for (uint32_t e0 = ls[0]; e0 < tend; e0 += 64u) {  // wave-uniform
+            const uint32_t e = e0 + lane;
+            const bool v = e < tend;
+            double4 q = make_double4(0.0, 0.0, 0.0, 0.0);
+            if (v) q = xr[b.tsrc[e]];
+            const uint32_t r = v ? owner(e) : 0u;
+            double fx, fy, fz;
+            pair_force(q, pw[r], k, fx, fy, fz);
+            harmonic_add<decltype(dense)::value>(acc, lane, v, r, fix_q<FAST>(fx), fix_q<FAST>(fy), fix_q<FAST>(fz));
+        }
+    };
+    if (tend - ls[0] > 8u * (last + 1u)) incoming(std::true_type{});
+    else incoming(std::false_type{});
     wave_sync();
 }
 
