@@ -1532,7 +1532,7 @@ __global__ void __launch_bounds__(BLK) k_reverse(KARGS, int d, int part) {
         if (!live(b)) continue;
         const BoxState* st = b.st;
         CGeom& g = geom_of(b);
-        // a failed build is redone by the host, and this round with it
+        // a failed build is redone by the host, and this reverse pass with it
         if (!g.newton || (st->err & (ERR_GHOST_CAPACITY | ERR_PAIR_CAPACITY))) continue;
         const bool both = g.disjoint[d] != 0;
         uint32_t k = tl.start + threadIdx.x;
