@@ -389,7 +389,7 @@ def main():
     ap.add_argument("--parts", default="plain,sheared,batched,served,cpu")
     ap.add_argument("--sheared-steps", type=int, default=20)
     ap.add_argument("--csr-steps", type=int, default=1000,
-                    help="steps of the plain box without the consumer, timing the CSR build alone")
+                    help="steps of the plain box without the consumer, timing the CSR build alone (0: skip)")
     ap.add_argument("--batched-steps", type=int, default=200)
     ap.add_argument("--plain-ensemble-steps", type=int, default=10)
     ap.add_argument("--batched-graph", type=int, default=1,
@@ -452,8 +452,9 @@ def main():
         out["step_roofline"] = res["step_roofline"]
         # the same box without the consumer: the fill is then the CSR's own
         # placement (no transpose slots), so count + scan + fill is the build alone
-        pure = run_box(ctx, c2_gen, args.csr_steps, args.warmup, 1, consumer=False)  # every build timed
-        out["roofline"]["without_consumer"] = csr_roofline(pure["csr"])
+        if args.csr_steps > 0:
+            pure = run_box(ctx, c2_gen, args.csr_steps, args.warmup, 1, consumer=False)  # every build timed
+            out["roofline"]["without_consumer"] = csr_roofline(pure["csr"])
         log(f"plain: {out['value']:.3e} particle-steps/s, {res['ms_per_step']:.3f} ms/step")
     c3_gen = None
     if "sheared" in parts:
