@@ -127,6 +127,7 @@ void carve_atoms(Carver& c, BoxDev& d, const Caps& k, const std::vector<HostCol>
     c.take(d.offsets, L1);
     c.take(d.tcount, A + 1); c.take(d.tstart, A + 1);
     c.take(d.ctile, (size_t)k.local / BLK + 2);
+    c.take(d.cgroup, (size_t)k.local / (BLK * CSR1_GROUP) + 2);
     c.take(d.wcount, (size_t)k.local / 64 + 2);
     c.take(d.bcnt, 6 * ((size_t)k.atoms / BORDER_TILE + 2));
     for (size_t i = 0; i < cols.size(); ++i) {

@@ -158,7 +158,8 @@ struct BoxDev {
     GMEM uint32_t* tsrc;       // cap_pairs: row atom of each incoming pair, grouped by column
     // border compaction scratch: per tile counts of the two swaps
     GMEM uint32_t* ctile;      // CSR: per local tile, entries staged in the temp region (mode 0: or an
-                               // overflow mark; mode 5: its total, scanned in place into tile bases)
+                               // overflow mark; mode 5: the tile's total)
+    GMEM uint32_t* cgroup;     // CSR (mode 5): per group of 64 tiles, the sum of their totals
     GMEM uint32_t* wcount;     // CSR (mode 4): entries of each 64-row wave, scanned in place into wave bases
     GMEM uint32_t* bcnt;       // 6 per border tile: selections of the 6 swaps
     ColDev cols[SOIL_MAX_COLS];

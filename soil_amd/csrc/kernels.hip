@@ -132,6 +132,19 @@ __device__ __forceinline__ uint32_t lane_prefix(uint64_t mask) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
+// Block-wide sum of one u64 per thread (BLK threads, wave64), in every thread.
+__device__ uint64_t block_sum64(uint64_t v, uint64_t* lds) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63u) == 0u) lds[threadIdx.x >> 6] = v;
+    __syncthreads();
+    uint64_t tot = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < BLK / 64; ++w) tot += lds[w];
+    __syncthreads();  // lds is reused by the caller's next call
+    return tot;
+}
+
 // Block-wide exclusive scan of one u32 per thread (BLK threads, wave64).
 __device__ uint32_t block_excl_scan(uint32_t v, uint32_t* lds, uint32_t& total) {
     const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
@@ -1038,11 +1051,6 @@ __device__ __forceinline__ void scan_view(CBox& b, uint32_t& n, bool& active, GM
         active = st->rebuild_now != 0 && (st->err & ERR_GHOST_CAPACITY) == 0;
         in = b.wcount;
         out = b.wcount;
-    } else if (W == SCAN_TILES) {
-        n = st->nlocal / BLK + 2u;  // the tiles up to the one holding the end marker, and the end entry
-        active = st->rebuild_now != 0 && (st->err & ERR_GHOST_CAPACITY) == 0;
-        in = b.ctile;
-        out = b.ctile;
     } else {
         n = st->nlocal + st->nghost + 1;
         active = transpose_active(b);
@@ -1110,7 +1118,11 @@ __global__ void __launch_bounds__(BLK) k_scan_down(KARGS) {
 #pragma unroll
         for (uint32_t w = 0; w < BLK / 64; ++w) acc += red[w];
         __syncthreads();  // red[] is reused by the next tile
-        if ((W == SCAN_CSR || W == SCAN_WAVES || W == SCAN_TILES) && first && threadIdx.x == 0) {
+        // the tile-staged CSR build's group sums start from zero (its count adds
+        // into them; this scan runs before it in every build)
+        if (W == SCAN_CELLS_BUILD && first)
+            for (uint32_t q = threadIdx.x; q < b.cap_local / (BLK * CSR1_GROUP) + 2u; q += BLK) b.cgroup[q] = 0u;
+        if ((W == SCAN_CSR || W == SCAN_WAVES) && first && threadIdx.x == 0) {
             BoxState* st = b.st;
             st->local_cells_valid = 0;  // consumed by this build (neighbor.rs:596)
             st->npairs = acc;
@@ -1935,12 +1947,12 @@ __device__ __forceinline__ void wave_sync() {
 
 // LDS (the tile-staged build, mode 5): the row's first CSR1_SLOTS accepted
 // candidates are kept as record indices (positions in the bin-sorted order),
-// slot k of lane l at lrow[k * 64 + l] (lrow holds CSR1_SLOTS + 1 slots, the
-// last one discarding); the count goes on past them. Otherwise the row's
-// first `slot` neighbours go to `out` (global row slots).
-constexpr uint32_t CSR1_SLOTS = 8;
+// slot k of lane l at lrow[k * 64 + l], and one word after the slots that
+// takes the wave's discarded writes; the count goes on past them. Otherwise
+// the row's first `slot` neighbours go to `out` (global row slots).
+constexpr uint32_t CSR1_SLOTS = 10;
 // the tile-staged count reuses a wave's slots as the neighbour-index staging of a walk
-static_assert((CSR1_SLOTS + 1u) * 64u >= CSRW_CH, "the slots must hold a staged chunk");
+static_assert(CSR1_SLOTS * 64u >= CSRW_CH, "the slots must hold a staged chunk");
 
 template <bool UNIFORM, bool NEWTON, bool LDS = false>
 __device__ uint32_t csr_row_w(CBox& b, CGeom& g, uint32_t i, bool act, uint32_t c, uint32_t c_first,
@@ -1959,12 +1971,13 @@ __device__ uint32_t csr_row_w(CBox& b, CGeom& g, uint32_t i, bool act, uint32_t 
             ++n;
         }
     };
-    // branch-free: a rejected candidate (or one past the slots) is written to
-    // the discard slot CSR1_SLOTS
+    // branch-free: every candidate is written to slot n (the discard word
+    // once the slots are full) and kept by counting it; a rejected one is
+    // overwritten by the next candidate
     auto take_m = [&](bool acc, uint32_t m) {
         if (!NEWTON) acc = acc && m != self_m;
-        const uint32_t k = acc ? (n < CSR1_SLOTS ? n : CSR1_SLOTS) : CSR1_SLOTS;
-        lrow[k * 64u + lane] = m;
+        const uint32_t k = n * 64u + lane;
+        lrow[k < CSR1_SLOTS * 64u ? k : CSR1_SLOTS * 64u] = m;
         n += acc ? 1u : 0u;
     };
     // this lane's range [r0, r1) inside the wave's union [u0, u1) (wave-uniform)
@@ -1978,13 +1991,20 @@ __device__ uint32_t csr_row_w(CBox& b, CGeom& g, uint32_t i, bool act, uint32_t 
             return;
         }
         for (uint32_t base = u0; base < u1; base += CSRW_CH) {  // empty when u1 <= u0
+            // all of the chunk's loads in flight at once: lanes past the
+            // union's end load its last record again (staging never read)
+            double4 v[CSRW_CH / 64u];
+            uint32_t jv[CSRW_CH / 64u];
 #pragma unroll
             for (uint32_t q = 0; q < CSRW_CH; q += 64u) {
-                const uint32_t m = base + q + lane;
-                if (m < u1) {
-                    lpos[q + lane] = sp4[m];
-                    if (!LDS) lj[q + lane] = b.sorted[m];
-                }
+                const uint32_t m = base + q + lane < u1 ? base + q + lane : u1 - 1u;
+                v[q / 64u] = sp4[m];
+                if (!LDS) jv[q / 64u] = b.sorted[m];
+            }
+#pragma unroll
+            for (uint32_t q = 0; q < CSRW_CH; q += 64u) {
+                lpos[q + lane] = v[q / 64u];
+                if (!LDS) lj[q + lane] = jv[q / 64u];
             }
             wave_sync();
             const uint32_t lo = r0 > base ? r0 : base;
@@ -2492,8 +2512,7 @@ __device__ __forceinline__ void csr1_rewalk(CBox& b, CGeom& g, uint32_t i0w, uin
 
 template <bool UNIFORM, bool NEWTON>
 __device__ __forceinline__ uint32_t csr1_tile_count(CBox& b, CGeom& g, uint32_t i0, uint32_t nl, uint32_t wid,
-                                                    uint32_t lane, double4* lpos, uint32_t* lrow, uint32_t* ls,
-                                                    uint32_t* lscan) {
+                                                    uint32_t lane, double4* lpos, uint32_t* lrow, uint32_t* lscan) {
     const uint32_t i0w = i0 + wid * 64u;
     const uint32_t i = i0w + lane;
     const bool act = i < nl;
@@ -2505,20 +2524,15 @@ __device__ __forceinline__ uint32_t csr1_tile_count(CBox& b, CGeom& g, uint32_t 
     const uint32_t tcap = tile_region(b);
     if (tot > tcap || i0w >= nl) return tot;  // block-uniform / wave-uniform: nothing staged
     GMEM uint32_t* region = b.ttmp + (size_t)(i0 / BLK) * tcap;
-    const uint32_t last = (nl - i0w < 64u ? nl - i0w : 64u) - 1u;
-    ls[lane] = loc;
-    wave_sync();
-    const uint32_t o0 = __shfl(loc, 0, 64), o1 = __shfl(loc + n, (int)last, 64);
-    for (uint32_t e = o0 + lane; e < o1; e += 64u) {
-        uint32_t r = 0;  // the last row starting at or before e holds it
+    // each lane writes its own row's slots, every neighbour-index load in
+    // flight at once (a lane past its row's count loads record 0's)
+    const uint32_t kn = act && n <= CSR1_SLOTS ? n : 0u;  // longer rows are walked again below
+    uint32_t jv[CSR1_SLOTS];
 #pragma unroll
-        for (uint32_t step = 32; step > 0; step >>= 1)
-            if (r + step <= last && ls[r + step] <= e) r += step;
-        const uint32_t rs = ls[r];
-        const uint32_t re = r < last ? ls[r + 1] : o1;
-        if (re - rs > CSR1_SLOTS) continue;  // walked again below
-        region[e] = b.sorted[lrow[(e - rs) * 64u + r]];
-    }
+    for (uint32_t k = 0; k < CSR1_SLOTS; ++k) jv[k] = b.sorted[k < kn ? lrow[k * 64u + lane] : 0u];
+#pragma unroll
+    for (uint32_t k = 0; k < CSR1_SLOTS; ++k)
+        if (k < kn) region[loc + k] = jv[k];
     wave_sync();  // lrow becomes the walk's staging of neighbour indices
     csr1_rewalk<UNIFORM, NEWTON>(b, g, i0w, nl, lane, act && n > CSR1_SLOTS, region, loc, lpos, lrow);
     return tot;
@@ -2526,8 +2540,7 @@ __device__ __forceinline__ uint32_t csr1_tile_count(CBox& b, CGeom& g, uint32_t 
 
 __global__ void __launch_bounds__(BLK) k_csr_tcount(KARGS) {
     __shared__ double4 lpos[BLK / 64][CSRW_CH];
-    __shared__ uint32_t lrow[BLK / 64][(CSR1_SLOTS + 1) * 64];  // slot CSR1_SLOTS discards
-    __shared__ uint32_t lstart[BLK / 64][64];
+    __shared__ uint32_t lrow[BLK / 64][CSR1_SLOTS * 64 + 1];  // the last word discards
     __shared__ uint32_t lscan[BLK / 64];
     const uint32_t tid = threadIdx.x, wid = tid >> 6, lane = tid & 63u;
     FOR_TILES(t) {
@@ -2541,14 +2554,14 @@ __global__ void __launch_bounds__(BLK) k_csr_tcount(KARGS) {
         if (i0 > nl) continue;  // block-uniform
         CGeom& g = geom_of(b);
         uint32_t tot;
-        if (g.newton) tot = g.uniform ? csr1_tile_count<true, true>(b, g, i0, nl, wid, lane, lpos[wid], lrow[wid], lstart[wid], lscan)
-                                      : csr1_tile_count<false, true>(b, g, i0, nl, wid, lane, lpos[wid], lrow[wid], lstart[wid], lscan);
-        else tot = g.uniform ? csr1_tile_count<true, false>(b, g, i0, nl, wid, lane, lpos[wid], lrow[wid], lstart[wid], lscan)
-                             : csr1_tile_count<false, false>(b, g, i0, nl, wid, lane, lpos[wid], lrow[wid], lstart[wid], lscan);
+        if (g.newton) tot = g.uniform ? csr1_tile_count<true, true>(b, g, i0, nl, wid, lane, lpos[wid], lrow[wid], lscan)
+                                      : csr1_tile_count<false, true>(b, g, i0, nl, wid, lane, lpos[wid], lrow[wid], lscan);
+        else tot = g.uniform ? csr1_tile_count<true, false>(b, g, i0, nl, wid, lane, lpos[wid], lrow[wid], lscan)
+                             : csr1_tile_count<false, false>(b, g, i0, nl, wid, lane, lpos[wid], lrow[wid], lscan);
         const uint32_t tix = i0 / BLK;
         if (tid == 0) {
             b.ctile[tix] = tot;
-            if (nl - i0 < BLK) b.ctile[tix + 1] = 0u;  // the scan's end entry (its result is the pair count)
+            if (tot) atomicAdd(&b.cgroup[tix / CSR1_GROUP], tot);  // zeroed by the build's cell scan
         }
         __syncthreads();  // the next tile rewrites the LDS tables
     }
@@ -2564,23 +2577,57 @@ __device__ __forceinline__ void csr1_tile_rewalk(CBox& b, CGeom& g, uint32_t i0,
     csr1_rewalk<UNIFORM, NEWTON>(b, g, i0w, nl, lane, i < nl, b.indices + base, locs[lane], lpos, lj);
 }
 
+template <bool PUB>
 __global__ void __launch_bounds__(BLK) k_csr_tplace(KARGS) {
     __shared__ double4 lpos[BLK / 64][CSRW_CH];
     __shared__ uint32_t lj[BLK / 64][CSRW_CH];
     __shared__ uint32_t lloc[BLK / 64][64];
+    __shared__ uint64_t lsum[BLK / 64];
     const uint32_t tid = threadIdx.x, wid = tid >> 6, lane = tid & 63u;
     FOR_TILES(t) {
         const Tile tl = tiles[t];
         CBox& b = box_at(boxes, tl.box);
         if (!live(b)) continue;
-        const BoxState* st = b.st;
-        if (!st->rebuild_now || (st->err & (ERR_GHOST_CAPACITY | ERR_PAIR_CAPACITY))) continue;
+        BoxState* st = b.st;
         const uint32_t nl = st->nlocal;
         const uint32_t i0 = tl.start;
-        if (i0 > nl) continue;  // block-uniform
-        const uint32_t tix = i0 / BLK;
-        const uint32_t base = b.ctile[tix];
-        const uint32_t tot = b.ctile[tix + 1] - base;
+        if (!st->rebuild_now || (st->err & (ERR_GHOST_CAPACITY | ERR_PAIR_CAPACITY)) || i0 > nl) {  // block-uniform
+            if (PUB && i0 == 0u && tid == 0u)  // the box's state goes to the host mirror, built or not
+                publish_vals(b, __hip_atomic_load(&st->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), st->npairs);
+            continue;
+        }
+        // the tile's base: the totals of the earlier groups of CSR1_GROUP
+        // tiles, then of the group's earlier tiles; the pair count: every group
+        const uint32_t tix = i0 / BLK, grp = tix / CSR1_GROUP;
+        const uint32_t ngrp = (nl / BLK) / CSR1_GROUP + 1u;  // the groups of tiles 0 .. nl / BLK
+        uint32_t pre = 0, all = 0;
+        for (uint32_t q = tid; q < ngrp; q += BLK) {
+            const uint32_t v = b.cgroup[q];
+            all += v;
+            pre += q < grp ? v : 0u;
+        }
+        if (tid < CSR1_GROUP && grp * CSR1_GROUP + tid < tix) pre += b.ctile[grp * CSR1_GROUP + tid];
+        const uint64_t sums = block_sum64(((uint64_t)all << 32) | pre, lsum);  // neither half exceeds 32 bits
+        const uint32_t base = (uint32_t)sums, npairs = (uint32_t)(sums >> 32);
+        if (i0 == 0u && tid == 0u) {  // the build's bookkeeping, once per box
+            st->local_cells_valid = 0;  // consumed by this build (neighbor.rs:596)
+            st->npairs = npairs;
+            uint32_t err;  // the bits as they stand after this update
+            if (npairs > b.cap_pairs) {
+                err = atomicOr(&st->err, ERR_PAIR_CAPACITY) | ERR_PAIR_CAPACITY;
+            } else {
+                err = __hip_atomic_load(&st->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                CGeom& g = geom_of(b);
+                const uint64_t na = (uint64_t)nl + st->nghost;
+                const uint64_t per = 24u + (g.uniform ? 0u : 8u) + 4u;
+                st->csr_bytes += per * na + 4u * ((uint64_t)g.ncells + 1u) + 4u * ((uint64_t)nl + 1u) +
+                                 4u * (uint64_t)npairs;
+                st->csr_builds += 1;
+            }
+            if (PUB) publish_vals(b, err, npairs);
+        }
+        if (npairs > b.cap_pairs) continue;  // block-uniform: the host grows the pairs and builds again
+        const uint32_t tot = b.ctile[tix];
         const uint32_t i = i0 + tid;
         uint32_t loc = 0;
         if (i < nl) {
@@ -3093,7 +3140,8 @@ void phase_build(const Launch& L, hipEvent_t ev0, hipEvent_t evc, hipEvent_t ev1
     phase_build_fused(L, ev0, evc, ev1, ev2);
 }
 // the build after save_build: bins, CSR count, offsets, fill and transpose;
-// with `publish` the offset scan writes the state into the host mirror
+// with `publish` the offset scan (mode 5: the placement) writes the state
+// into the host mirror
 static void build_rest(const Launch& L, hipEvent_t ev0, hipEvent_t evc, hipEvent_t ev1, hipEvent_t ev2, bool publish) {
     run_sort<SORT_BUILD, SCAN_CELLS_BUILD, false>(L, L.t_cell, L.t_atom, L.t_scan_cell);  // save_build zeroed
     if (ev0) (void)hipEventRecord(ev0, L.stream);
@@ -3104,20 +3152,20 @@ static void build_rest(const Launch& L, hipEvent_t ev0, hipEvent_t evc, hipEvent
     else launch(k_csr_count_g, L.t_local1, L.boxes, L.stream, L.maxgrid);
     if (evc) (void)hipEventRecord(evc, L.stream);
     // the scan publishes the build's counts (over the waves' totals in mode 4,
-    // the tiles' totals in mode 5, the rows' counts otherwise)
+    // the rows' counts in modes 0-3)
     if (L.csr_mode == 4) {
         if (publish) run_scan<SCAN_WAVES, true>(L, L.t_scan_local);
         else run_scan<SCAN_WAVES>(L, L.t_scan_local);
     } else if (L.csr_mode == 5) {
-        if (publish) run_scan<SCAN_TILES, true>(L, L.t_scan_local);
-        else run_scan<SCAN_TILES>(L, L.t_scan_local);
+        // no scan: the placement sums the tiles' totals itself
     } else {
         if (publish) run_scan<SCAN_CSR, true>(L, L.t_scan_local);
         else run_scan<SCAN_CSR>(L, L.t_scan_local);
     }
     if (ev1) (void)hipEventRecord(ev1, L.stream);
     if (L.csr_mode == 4) launch(k_csr_fill_flat, L.t_local1, L.boxes, L.stream, L.maxgrid);
-    else if (L.csr_mode == 5) launch(k_csr_tplace, L.t_local1, L.boxes, L.stream, L.maxgrid);
+    else if (L.csr_mode == 5 && publish) launch(k_csr_tplace<true>, L.t_local1, L.boxes, L.stream, L.maxgrid);
+    else if (L.csr_mode == 5) launch(k_csr_tplace<false>, L.t_local1, L.boxes, L.stream, L.maxgrid);
     else launch(k_csr_fill, L.t_local, L.boxes, L.stream, L.maxgrid);
     if (ev2) (void)hipEventRecord(ev2, L.stream);
     // consumer transpose (tcount was cleared by save_build; modes 1 and 2

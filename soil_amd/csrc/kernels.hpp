@@ -10,10 +10,10 @@ constexpr int SCAN_PER_THREAD = 8;
 constexpr uint32_t SCAN_TILE = BLK * SCAN_PER_THREAD;
 constexpr uint32_t BORDER_PER = 4;                 // 64-atom chunks per wave in a border tile
 constexpr uint32_t BORDER_TILE = BLK * BORDER_PER;  // atoms per border-compaction tile
+constexpr uint32_t CSR1_GROUP = 64;                 // tiles per group sum of the tile-staged CSR build
 
 enum { SORT_SPATIAL = 0, SORT_BUILD = 1 };
-enum { SCAN_CELLS_SPATIAL = 0, SCAN_CELLS_BUILD = 1, SCAN_CSR = 2, SCAN_TRANSPOSE = 3, SCAN_WAVES = 4, SCAN_REMOVE = 5,
-       SCAN_TILES = 6 };
+enum { SCAN_CELLS_SPATIAL = 0, SCAN_CELLS_BUILD = 1, SCAN_CSR = 2, SCAN_TRANSPOSE = 3, SCAN_WAVES = 4, SCAN_REMOVE = 5 };
 
 struct Tile {
     uint32_t box, start;
