@@ -649,26 +649,34 @@ int soil_hip_batch_autotune(soil_hip_batch* bt) {
     };
     bt->tune_trials.clear();
     // coordinate descent: the CSR strategy at the current launch cap (mode 5:
-    // one pass, rows placed at offsets from a look-back; mode 3: one thread
-    // per row into row slots, then an offset scan and a fill; mode 4: a
-    // wave's 64 lanes sharing its rows' flattened candidates), then the
-    // launch cap
+    // rows staged per 256-row tile, then placed at the tiles' bases; mode 3:
+    // one thread per row into row slots, then an offset scan and a fill;
+    // mode 4: a wave's 64 lanes sharing its rows' flattened candidates), then
+    // the launch cap. The strategies are timed in two interleaved rounds and
+    // each keeps its faster round, so that no strategy's figure rests on one
+    // few-step sample taken first or last.
     const int modes[3] = {5, 4, 3};
     const uint32_t cands[5] = {1024, 2048, 4096, 8192, 16384};
     const uint32_t mg0 = bt->L.maxgrid;
     int best_mode = bt->L.csr_mode;
     uint32_t best = mg0;
     double best_ms = 1e30;
-    for (int mode : modes) {
-        double ms = 0.0;
-        const int rc = trial_ms(mg0, mode, ms);
-        if (rc) {
-            free_snaps();
-            return rc;
+    double mode_ms[3] = {1e30, 1e30, 1e30};
+    for (int round = 0; round < 2; ++round) {
+        for (int k = 0; k < 3; ++k) {
+            double ms = 0.0;
+            const int rc = trial_ms(mg0, modes[k], ms);
+            if (rc) {
+                free_snaps();
+                return rc;
+            }
+            mode_ms[k] = std::min(mode_ms[k], ms);
         }
-        if (ms < best_ms) {
-            best_ms = ms;
-            best_mode = mode;
+    }
+    for (int k = 0; k < 3; ++k) {
+        if (mode_ms[k] < best_ms) {
+            best_ms = mode_ms[k];
+            best_mode = modes[k];
         }
     }
     for (uint32_t mg : cands) {

@@ -3035,13 +3035,16 @@ __global__ void k_collect(KARGS, uint32_t* out) {
 
 // Streams n doubles from src to dst: a copy of known size that calibrates the
 // HBM traffic counters (FETCH_SIZE, WRITE_SIZE) against algorithmic bytes.
+// one element per thread over a grid that covers the copy (on MI355X this
+// plain form streams faster than a capped grid-stride loop)
 __global__ void __launch_bounds__(BLK) k_copy_probe(const double* __restrict__ src, double* __restrict__ dst,
                                                     size_t n) {
-    for (size_t i = (size_t)blockIdx.x * BLK + threadIdx.x; i < n; i += (size_t)gridDim.x * BLK) dst[i] = src[i];
+    const size_t i = (size_t)blockIdx.x * BLK + threadIdx.x;
+    if (i < n) dst[i] = src[i];
 }
 
 void copy_probe(const double* src, double* dst, size_t n, hipStream_t s) {
-    hipLaunchKernelGGL(k_copy_probe, dim3(4096), dim3(BLK), 0, s, src, dst, n);
+    hipLaunchKernelGGL(k_copy_probe, dim3((unsigned)((n + BLK - 1) / BLK)), dim3(BLK), 0, s, src, dst, n);
 }
 
 // ── launch helpers ─────────────────────────────────────────────────────────
