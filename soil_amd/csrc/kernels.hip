@@ -1107,6 +1107,12 @@ __global__ void __launch_bounds__(BLK) k_scan_down(KARGS) {
         }
         const uint32_t tix = tl.start / SCAN_TILE;
         const bool first = tix == 0u;
+        // the tile's own entries first: their loads are in flight while the
+        // earlier tiles' totals are summed
+        uint32_t v[SCAN_PER_THREAD];
+        const uint32_t base = tl.start + threadIdx.x * SCAN_PER_THREAD;
+#pragma unroll
+        for (int k = 0; k < SCAN_PER_THREAD; ++k) v[k] = base + k < n ? in[base + k] : 0u;
         const uint32_t lim = first ? (n + SCAN_TILE - 1) / SCAN_TILE : tix;
         uint32_t acc = 0;
         for (uint32_t k = threadIdx.x; k < lim; k += BLK) acc += b.scan_part[k];
@@ -1141,14 +1147,9 @@ __global__ void __launch_bounds__(BLK) k_scan_down(KARGS) {
             if (PUB) publish_vals(b, err, acc);
         }
         const uint32_t prefix = first ? 0u : acc;
-        uint32_t v[SCAN_PER_THREAD];
         uint32_t sum = 0;
-        const uint32_t base = tl.start + threadIdx.x * SCAN_PER_THREAD;
 #pragma unroll
-        for (int k = 0; k < SCAN_PER_THREAD; ++k) {
-            v[k] = base + k < n ? in[base + k] : 0u;
-            sum += v[k];
-        }
+        for (int k = 0; k < SCAN_PER_THREAD; ++k) sum += v[k];
         uint32_t tot;
         uint32_t pre = block_excl_scan(sum, lds, tot) + prefix;
 #pragma unroll
@@ -1168,19 +1169,6 @@ __device__ __forceinline__ double border_coord(CGeom& g, double x, double y, dou
         return d == 0 ? lx : (d == 1 ? ly : lz);
     }
     return d == 0 ? x : (d == 1 ? y : z);
-}
-
-__device__ __forceinline__ void border_preds(CGeom& g, CBox& b, uint32_t i, int d, bool& p0, bool& p1) {
-    double v;
-    if (g.triclinic) {
-        double lx, ly, lz;
-        x2lamda(g, b.x[i], b.y[i], b.z[i], lx, ly, lz);
-        v = d == 0 ? lx : (d == 1 ? ly : lz);
-    } else {
-        v = d == 0 ? b.x[i] : (d == 1 ? b.y[i] : b.z[i]);
-    }
-    p0 = v < g.thr_lo[d];
-    p1 = v >= g.thr_hi[d];
 }
 
 // Border tiles: BORDER_TILE consecutive atoms per block; wave w reads
@@ -1348,6 +1336,25 @@ __global__ void __launch_bounds__(BLK) k_border_write(KARGS, int d) {
         const uint32_t tix = tl.start / BORDER_TILE;
         if (st->err & ERR_GHOST_CAPACITY) continue;
         if (!g.periodic[d] && tix != 0) continue;  // only the totals are needed there (block-uniform)
+        const uint32_t wid = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+        // the tile's coordinates first (lamda needs all three): their loads are
+        // in flight while the dim's counts are summed
+        const bool places = g.periodic[d] && tl.start < scan_end;  // block-uniform
+        double cx[BORDER_PER], cy[BORDER_PER], cz[BORDER_PER];
+#pragma unroll
+        for (int c = 0; c < BORDER_PER; ++c) {
+            const uint32_t i = border_atom(tl.start, wid, c, lane);
+            cx[c] = cy[c] = cz[c] = 0.0;
+            if (places && i < scan_end) {
+                if (g.triclinic) {
+                    cx[c] = b.x[i];
+                    cy[c] = b.y[i];
+                    cz[c] = b.z[i];
+                } else {
+                    cx[c] = d == 0 ? b.x[i] : (d == 1 ? b.y[i] : b.z[i]);
+                }
+            }
+        }
         // totals of the dim and this tile's exclusive prefixes
         const uint32_t nt = (scan_end + BORDER_TILE - 1) / BORDER_TILE;
         uint32_t pb0 = 0, pb1 = 0, tot0 = 0, tot1 = 0;
@@ -1367,7 +1374,6 @@ __global__ void __launch_bounds__(BLK) k_border_write(KARGS, int d) {
             tot0 += __shfl_xor(tot0, o, 64);
             tot1 += __shfl_xor(tot1, o, 64);
         }
-        const uint32_t wid = threadIdx.x >> 6, lane = threadIdx.x & 63u;
         if (lane == 0u) {
             red[0][wid] = pb0;
             red[1][wid] = pb1;
@@ -1419,7 +1425,11 @@ __global__ void __launch_bounds__(BLK) k_border_write(KARGS, int d) {
         for (int c = 0; c < BORDER_PER; ++c) {
             const uint32_t i = border_atom(tl.start, wid, c, lane);
             bool p0 = false, p1 = false;
-            if (i < scan_end) border_preds(g, b, i, d, p0, p1);
+            if (i < scan_end) {
+                const double v = g.triclinic ? border_coord(g, cx[c], cy[c], cz[c], d) : cx[c];
+                p0 = v < g.thr_lo[d];
+                p1 = v >= g.thr_hi[d];
+            }
             m0[c] = __ballot(p0);
             m1[c] = __ballot(p1);
             c0 += (uint32_t)__popcll(m0[c]);
