@@ -1981,14 +1981,16 @@ __device__ uint32_t csr_row_w(CBox& b, CGeom& g, uint32_t i, bool act, uint32_t 
             ++n;
         }
     };
-    // branch-free: every candidate is written to slot n (the discard word
-    // once the slots are full) and kept by counting it; a rejected one is
-    // overwritten by the next candidate
+    // branch-free: every candidate is written to the lane's next slot (the
+    // discard word once the slots are full) and kept by moving the slot
+    // pointer on; a rejected one is overwritten by the next candidate. The
+    // row's count is the pointer's distance at the end.
+    uint32_t* sp = LDS ? lrow + lane : nullptr;
+    uint32_t* const spmax = LDS ? lrow + CSR1_SLOTS * 64u : nullptr;
     auto take_m = [&](bool acc, uint32_t m) {
         if (!NEWTON) acc = acc && m != self_m;
-        const uint32_t k = n * 64u + lane;
-        lrow[k < CSR1_SLOTS * 64u ? k : CSR1_SLOTS * 64u] = m;
-        n += acc ? 1u : 0u;
+        *(sp < spmax ? sp : spmax) = m;
+        sp += acc ? 64u : 0u;
     };
     // this lane's range [r0, r1) inside the wave's union [u0, u1) (wave-uniform)
     auto span = [&](uint32_t r0, uint32_t r1, uint32_t u0, uint32_t u1) {
@@ -2021,14 +2023,19 @@ __device__ uint32_t csr_row_w(CBox& b, CGeom& g, uint32_t i, bool act, uint32_t 
             const uint32_t hi = r1 < base + CSRW_CH ? r1 : base + CSRW_CH;
             if (LDS) {  // two candidates per pass: both LDS reads in flight before the tests
                 uint32_t m = lo;
-                for (; m + 1u < hi; m += 2u) {
-                    const double4 q0 = lpos[m - base], q1 = lpos[m + 1u - base];
+                const double4* lp = lpos + (lo - base);
+                uint32_t m1 = m + 1u;
+                while (m1 < hi) {
+                    const double4 q0 = lp[0], q1 = lp[1];
                     const bool a0 = accept_pair<UNIFORM>(pi.x, pi.y, pi.z, pi.w, q0, g.ucut_sq, g.sf_sq);
                     const bool a1 = accept_pair<UNIFORM>(pi.x, pi.y, pi.z, pi.w, q1, g.ucut_sq, g.sf_sq);
                     take_m(a0, m);
-                    take_m(a1, m + 1u);
+                    take_m(a1, m1);
+                    lp += 2;
+                    m += 2u;
+                    m1 = m + 1u;
                 }
-                if (m < hi) take_m(accept_pair<UNIFORM>(pi.x, pi.y, pi.z, pi.w, lpos[m - base], g.ucut_sq, g.sf_sq), m);
+                if (m < hi) take_m(accept_pair<UNIFORM>(pi.x, pi.y, pi.z, pi.w, lp[0], g.ucut_sq, g.sf_sq), m);
             } else {
                 for (uint32_t m = lo; m < hi; ++m)
                     if (accept_pair<UNIFORM>(pi.x, pi.y, pi.z, pi.w, lpos[m - base], g.ucut_sq, g.sf_sq))
@@ -2068,6 +2075,7 @@ __device__ uint32_t csr_row_w(CBox& b, CGeom& g, uint32_t i, bool act, uint32_t 
         }
         span(rs, re, b.bin_start[uc0], b.bin_start[uc1]);
     }
+    if (LDS) n = (uint32_t)(sp - (lrow + lane)) / 64u;
     return n;
 }
 
