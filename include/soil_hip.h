@@ -112,19 +112,27 @@ int soil_hip_box_device_bytes(soil_hip_box* box, size_t* estimate, size_t* alloc
 
 /* Setup systems: neighbor_setup (neighbor.rs:604-665), comm_setup
  * (comm.rs:253-283) and setup_deform (soil_deform/src/lib.rs:346-420).
- * Sizes device buffers for the uploaded atoms. Two environment variables,
- * read here, tune the CSR build without changing its output:
+ * Sizes device buffers for the uploaded atoms. Environment variables read
+ * here tune the CSR build and the consumer's transpose without changing
+ * their output:
  * SOIL_HIP_CSR_MODE (0-5, default 5: one thread per row with wave-staged
  * candidates, each row's first neighbours kept in LDS, each tile's rows
- * staged densely, a scan over the tiles' totals, then a copy of every tile to
- * its place; 3: the same count into per-row slots, then a scan over the rows'
+ * staged densely, tile bases from per-group sums of the tiles' totals, then
+ * a copy of every tile to its place; 3: the same count into per-row slots, then a scan over the rows'
  * counts and a fill; 4: each wave's rows' candidates flattened over its 64
  * lanes, accepted pairs compacted into a dense region per wave) and
  * SOIL_HIP_ROW_SLOTS (4-64, default 12: pair-arena entries kept per local
  * row: the slots of modes 2-3, a wave's region in mode 4, a tile's region in
  * mode 5; a row, wave or tile beyond it is walked again);
  * SOIL_HIP_MAX_GRID (256-65536, default 4096) caps the blocks of a box's
- * launches (on 10^6 spheres 4096 is fastest: 1024 costs 14 %, 8192 1 %). */
+ * launches (on 10^6 spheres 4096 is fastest: 1024 costs 14 %, 8192 1 %);
+ * SOIL_HIP_TRANSPOSE (atomic, the default: arrival slots from atomics, a
+ * scan and a scatter; pull: with mode 5, each 256-atom tile gathers its
+ * incoming pairs from the rows of its backward stencil without atomics;
+ * auto: pull while the last build had at most 3 pairs per row, atomic above
+ * 4) picks how the incoming-pair transpose is built (the consumer's and a
+ * hook's that asks for it); pull and auto are opt-in until run on the
+ * device. */
 int soil_hip_setup(soil_hip_box* box);
 
 /* [domain] boundary of one axis (domain.rs:27-34): 0 periodic, 1 fixed,

@@ -38,6 +38,7 @@ static int batch_tiles(soil_hip_batch* bt) {
     bt->L.any_open = false;
     for (auto* b : bt->boxes)
         bt->L.any_open |= !(b->model.periodic[0] && b->model.periodic[1] && b->model.periodic[2]);
+    set_transpose_launch(bt->L, bt->boxes.data(), bt->boxes.size());
     return 0;
 }
 
@@ -619,8 +620,10 @@ int soil_hip_batch_autotune(soil_hip_batch* bt) {
             snaps[k].dev.csr_mode = mode;
             HIPCHK(hipMemcpy(&b->d_dev->csr_mode, &mode, sizeof(int), hipMemcpyHostToDevice));
             b->L.csr_mode = mode;
+            set_transpose_launch(b->L, &b, 1);  // the pull transpose follows the default build only
         }
         bt->L.csr_mode = mode;
+        set_transpose_launch(bt->L, bt->boxes.data(), bt->boxes.size());
         return 0;
     };
     auto trial_ms = [&](uint32_t mg, int mode, double& ms) -> int {

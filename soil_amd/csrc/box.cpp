@@ -126,6 +126,7 @@ void carve_atoms(Carver& c, BoxDev& d, const Caps& k, const std::vector<HostCol>
     c.take(d.scell, k.local); c.take(d.sstart, C1); c.take(d.perm, k.local);
     c.take(d.offsets, L1);
     c.take(d.tcount, A + 1); c.take(d.tstart, A + 1);
+    c.take(d.ttile, A / BLK + 2); c.take(d.tgroup, A / (BLK * TP_GROUP) + 2);
     c.take(d.ctile, (size_t)k.local / BLK + 2);
     c.take(d.cgroup, (size_t)k.local / (BLK * CSR1_GROUP) + 2);
     c.take(d.wcount, (size_t)k.local / 64 + 2);
@@ -212,6 +213,41 @@ int rebuild_tiles(soil_hip_box* b) {
     b->L.stream = b->stream;
     b->L.csr_mode = b->h_dev.csr_mode;
     b->L.any_open = !(b->model.periodic[0] && b->model.periodic[1] && b->model.periodic[2]);
+    set_transpose_launch(b->L, &b, 1);
+    return 0;
+}
+
+// Which transpose launches a set of boxes needs (a box without a transpose
+// needs none; the launches skip boxes of the other strategy).
+void set_transpose_launch(Launch& L, soil_hip_box* const* boxes, size_t n) {
+    L.tpull_any = false;
+    L.tatomic_any = false;
+    for (size_t k = 0; k < n; ++k) {
+        const BoxDev& d = boxes[k]->h_dev;
+        if (!d.transpose) continue;
+        if (d.tpull && d.csr_mode == 5) L.tpull_any = true;
+        else L.tatomic_any = true;
+    }
+}
+
+// The auto transpose strategy after a build: the pull walks about four times
+// the CSR entries of each tile's backward rows, which pays while rows are
+// short; dense boxes keep the atomics.
+int tune_transpose(soil_hip_box* b) {
+    if (b->tmode != 2 || !b->tpull_ok || !b->h_dev.transpose || b->h_dev.csr_mode != 5) return 0;
+    const uint64_t np = b->h_st->npairs, nl = b->h_st->nlocal;
+    int32_t want = b->h_dev.tpull;
+    if (np > 4 * nl) want = 0;
+    else if (np <= 3 * nl) want = 1;
+    if (want == b->h_dev.tpull) return 0;
+    b->h_dev.tpull = want;
+    // The box's stream is non-blocking, so a copy on the null stream is not
+    // ordered after it: this build's transpose kernels (and the step's tail)
+    // may still be queued there and must see the strategy they were launched
+    // for. Drain the stream first; a switch is rare.
+    HIPCHK(hipStreamSynchronize(b->stream));
+    HIPCHK(hipMemcpy(&b->d_dev->tpull, &want, sizeof(int32_t), hipMemcpyHostToDevice));
+    set_transpose_launch(b->L, &b, 1);
     return 0;
 }
 
@@ -636,6 +672,8 @@ int step_plain_loop(soil_hip_box* b, int nsteps, bool& pending_final) {
             bool redone = false;
             rc = rebuild_check(b, ev, redone);
             if (rc) return rc;
+            rc = tune_transpose(b);
+            if (rc) return rc;
             if (!early || redone) {
                 rc = tail();
                 if (rc) return rc;
@@ -908,6 +946,25 @@ int soil_hip_setup(soil_hip_box* b) {
         const char* m = getenv("SOIL_HIP_CSR_MODE");
         d.csr_mode = m ? atoi(m) : 5;
         if (d.csr_mode < 0 || d.csr_mode > 5) d.csr_mode = 5;
+    }
+    {
+        // transpose strategy (bit-neutral): atomic (default), pull or auto;
+        // auto starts from the expected pairs per row of a uniform box,
+        // 4 phi sf^3, and follows each build's count (a clustered box switches
+        // after its first). The pull is opt-in until it has run on the device
+        // with its host-side fixes.
+        const char* m = getenv("SOIL_HIP_TRANSPOSE");
+        b->tmode = 0;
+        if (m && (!strcmp(m, "pull") || !strcmp(m, "1"))) b->tmode = 1;
+        if (m && !strcmp(m, "auto")) b->tmode = 2;
+        b->tpull_ok = b->model.max_stencil() + 1 <= (size_t)TP_MAXR;
+        double vol = 1.0, vs = 0.0;
+        for (int k = 0; k < 3; ++k) vol *= b->model.hi[k] - b->model.lo[k];
+        for (size_t i = 0; i < n; ++i) vs += b->hrad[i] * b->hrad[i] * b->hrad[i];
+        const double phi = vol > 0.0 ? 4.18879020478639 * vs / vol : 1.0;
+        const double sf = b->model.skin_fraction;
+        const bool sparse = 4.0 * phi * sf * sf * sf <= 3.0;
+        d.tpull = (b->tpull_ok && (b->tmode == 1 || (b->tmode == 2 && sparse))) ? 1 : 0;
     }
     BoxState st{};
     st.nlocal = (uint32_t)n;

@@ -41,6 +41,8 @@ namespace soil_hip {
 int alloc_atoms(soil_hip_box* b, const Caps& nc, bool keep);
 int alloc_pairs(soil_hip_box* b, uint32_t cap_pairs, bool keep);
 uint32_t row_slots_setting();
+void set_transpose_launch(Launch& L, soil_hip_box* const* boxes, size_t n);
+int tune_transpose(soil_hip_box* b);
 uint32_t pair_capacity(uint64_t cap_pairs, uint32_t row_slots, uint32_t cap_local);
 uint32_t initial_pair_capacity(uint32_t row_slots, uint32_t cap_local);
 // Device bytes a box will hold after set-up (its arenas at their initial capacities).
@@ -91,6 +93,12 @@ struct soil_hip_box {
     // longer row again on one lane, so the width sets how rarely a wave waits
     // on such a lane (1M spheres at phi 0.5: 10^4 rows exceed 8, 254 exceed 12)
     uint32_t row_slots = 12;
+    // consumer transpose strategy (SOIL_HIP_TRANSPOSE: atomic (default), pull,
+    // auto): auto pulls while the last build had at most 3 pairs per local row
+    // and keeps the atomics above 4; tpull_ok: the box's stencil has few
+    // enough runs for the pull's merged ranges
+    int tmode = 0;
+    bool tpull_ok = false;
     soil_hip::BoxDev h_dev{};        // host mirror of the device descriptor
     soil_hip::BoxDev* d_dev = nullptr;
     soil_hip::BoxDev** d_ptrs = nullptr;  // one-entry pointer array for single-box launches

@@ -94,6 +94,9 @@ struct BoxState {
     uint64_t pub_seq;                           // publishes to the host mirror so far
     uint32_t nremove;                           // locals that left through a non-periodic face this pbc
     uint32_t removed_total;                     // locals removed so far
+    uint32_t sstart_build;                      // the build's locals are in the spatial sort's cell order
+                                                // with its per-cell starts (sstart) valid
+    uint32_t tpull_now;                         // this build's transpose strategy (1 pull), latched by its count
 };
 
 // The fields the plain step waits on, published by kernels into pinned host
@@ -156,6 +159,8 @@ struct BoxDev {
     GMEM uint32_t* tstart;     // cap_atoms + 1
     GMEM uint32_t* tslot;      // cap_pairs: arrival slot of each CSR entry in its column
     GMEM uint32_t* tsrc;       // cap_pairs: row atom of each incoming pair, grouped by column
+    GMEM uint32_t* ttile;      // pull transpose: incoming pairs of each 256-atom tile
+    GMEM uint32_t* tgroup;     // pull transpose: per group of 64 atom tiles, the sum of their totals
     // border compaction scratch: per tile counts of the two swaps
     GMEM uint32_t* ctile;      // CSR: per local tile, entries staged in the temp region (mode 0: or an
                                // overflow mark; mode 5: the tile's total)
@@ -174,6 +179,9 @@ struct BoxDev {
     int32_t consumer;
     int32_t fix_fast;     // consumer forces stay below 2^19 in magnitude (fast fixed-point rounding)
     int32_t transpose;    // build the incoming-pair transpose with the CSR (consumer or a hook that asks)
+    int32_t tpull;        // transpose strategy (bit-neutral): 0 (default) arrival slots from atomics in
+                          // the CSR placement, a scan and a scatter; 1 (opt-in, default build only) each
+                          // 256-atom tile pulls its incoming pairs from the rows of its backward stencil
     GMEM soil_hip_device_view* view;  // device copy of the hook's view (refreshed by k_sync_view), or null
     GMEM HostMirror* mirror;          // device address of the box's host mirror, or null
     int32_t csr_mode;     // CSR strategy (bit-neutral): 0 LDS tiles, 1 two global passes, 2 one pass + row slots,

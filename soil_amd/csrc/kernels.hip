@@ -1025,6 +1025,15 @@ __device__ __forceinline__ bool transpose_active(CBox& b) {
     const BoxState* st = b.st;
     return st->rebuild_now != 0 && b.transpose && (st->err & (ERR_PAIR_CAPACITY | ERR_GHOST_CAPACITY)) == 0;
 }
+// the transpose's two strategies: arrival slots from atomics in the CSR fill or
+// placement, a scan and a scatter (k_tplace); or, for the default build of a
+// box with tpull set, the atomic-free pull (k_tpull_count, k_tpull_place). The
+// count of the default build latches the strategy for the rest of its build,
+// so a descriptor written while the build is queued cannot split it.
+__device__ __forceinline__ bool tpull_on(CBox& b) { return b.tpull != 0 && b.csr_mode == 5; }
+__device__ __forceinline__ bool tpull_build(CBox& b) { return b.csr_mode == 5 && b.st->tpull_now != 0u; }
+__device__ __forceinline__ bool tatomic_active(CBox& b) { return transpose_active(b) && !tpull_build(b); }
+__device__ __forceinline__ bool tpull_active(CBox& b) { return transpose_active(b) && tpull_build(b); }
 
 // ── exclusive scans (reduce tiles -> scan tile sums per box -> down-sweep) ──
 template <int W>
@@ -1053,7 +1062,7 @@ __device__ __forceinline__ void scan_view(CBox& b, uint32_t& n, bool& active, GM
         out = b.wcount;
     } else {
         n = st->nlocal + st->nghost + 1;
-        active = transpose_active(b);
+        active = tatomic_active(b);
         in = b.tcount;
         out = b.tstart;
     }
@@ -2580,6 +2589,11 @@ __global__ void __launch_bounds__(BLK) k_csr_tcount(KARGS) {
         if (tid == 0) {
             b.ctile[tix] = tot;
             if (tot) atomicAdd(&b.cgroup[tix / CSR1_GROUP], tot);  // zeroed by the build's cell scan
+            // read by the pull transpose after the placement consumes local_cells_valid
+            if (i0 == 0u) {
+                b.st->sstart_build = st->local_cells_valid;
+                b.st->tpull_now = tpull_on(b) ? 1u : 0u;
+            }
         }
         __syncthreads();  // the next tile rewrites the LDS tables
     }
@@ -2644,6 +2658,8 @@ __global__ void __launch_bounds__(BLK) k_csr_tplace(KARGS) {
             }
             if (PUB) publish_vals(b, err, npairs);
         }
+        if (i0 == 0u && tpull_build(b))  // the pull transpose's group sums start from zero (it runs next)
+            for (uint32_t q = tid; q < b.cap_atoms / (BLK * TP_GROUP) + 2u; q += BLK) b.tgroup[q] = 0u;
         if (npairs > b.cap_pairs) continue;  // block-uniform: the host grows the pairs and builds again
         const uint32_t tot = b.ctile[tix];
         const uint32_t i = i0 + tid;
@@ -2654,7 +2670,7 @@ __global__ void __launch_bounds__(BLK) k_csr_tplace(KARGS) {
         } else if (i == nl) {
             b.offsets[nl] = base + tot;
         }
-        const bool tr = transpose_active(b);
+        const bool tr = tatomic_active(b);
         const uint32_t tcap = tile_region(b);
         GMEM uint32_t* dst = b.indices + base;
         if (tot <= tcap) {  // block-uniform: the staged region, one entry per thread
@@ -2718,7 +2734,7 @@ __global__ void __launch_bounds__(BLK) k_tcount(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
         CBox& b = box_at(boxes, tl.box);
-        if (!live(b) || !transpose_active(b)) continue;
+        if (!live(b) || !tatomic_active(b)) continue;
         const uint32_t i = tl.start + threadIdx.x;
         if (i >= b.st->nlocal) continue;
         for (uint32_t e = b.offsets[i]; e < b.offsets[i + 1]; ++e) b.tslot[e] = atomicAdd(&b.tcount[b.indices[e]], 1u);
@@ -2731,7 +2747,7 @@ __global__ void __launch_bounds__(BLK) k_tplace(KARGS) {
     FOR_TILES(t) {
         const Tile tl = tiles[t];
         CBox& b = box_at(boxes, tl.box);
-        if (!live(b) || !transpose_active(b)) continue;
+        if (!live(b) || !tatomic_active(b)) continue;
         const uint32_t nl = b.st->nlocal;
         const uint32_t wid = threadIdx.x >> 6, lane = threadIdx.x & 63u;
         const uint32_t i0w = tl.start + wid * 64u;
@@ -2769,6 +2785,225 @@ __global__ void __launch_bounds__(BLK) k_tplace(KARGS) {
         }
         for (; e < o1; e += 64u) b.tsrc[b.tstart[b.indices[e]] + b.tslot[e]] = i0w + owner(e);
         wave_sync();
+    }
+}
+
+// ── consumer transpose, pull strategy (sparse boxes, default build) ───────
+// Row i holds target j exactly when j's cell lies in one of i's windows: i's
+// own cell or cells [ci + o, ci + o + len) of a stencil run (o, len). So the
+// rows that can hold a target in cells [cl, ch] lie in cells [cl, ch] and
+// [cl - o - len + 1, ch - o] for every run. The locals are in cell order, so
+// each merged cell range is one range of rows, and walking the merged ranges
+// sees every CSR entry with a target in [cl, ch] exactly once. A tile of 256
+// targets walks its local targets' rows one row per thread and keeps the
+// entries whose target is its own; a ghost target (ghosts are not in cell
+// order) walks the rows of its single cell's ranges, which are disjoint since
+// the runs and the own cell are disjoint sets of offsets. k_tpull_count
+// counts each target's incoming pairs (LDS counters) and the tile's total
+// (and per-group sums); k_tpull_place turns the totals into tile bases,
+// writes tstart and places tsrc in each column in any order, as the consumer
+// allows. No device-scope atomics but one per tile.
+
+// first local row whose cell is >= c (locals are in cell order); the spatial
+// sort's starts unless they are stale (a regrown arena), then a search
+__device__ __forceinline__ uint32_t tp_row_start(CBox& b, uint32_t c, uint32_t nl, bool fast) {
+    if (fast) return b.sstart[c];
+    uint32_t lo = 0, hi = nl;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (b.cell[mid] < c) lo = mid + 1u;
+        else hi = mid;
+    }
+    return lo;
+}
+
+struct TpRanges {
+    int32_t lo[TP_MAXR], hi[TP_MAXR];  // merged cell ranges, ascending
+    uint32_t r0[TP_MAXR];              // first row of each
+    uint32_t pre[TP_MAXR + 1];         // rows before each (pre[n] = all)
+    uint32_t n;
+};
+
+// The merged row ranges of the tile's local targets [a0, a0 + nloc) (every
+// thread of the block takes part; ends with the ranges visible to all).
+__device__ void tp_setup(CBox& b, CGeom& g, uint32_t a0, uint32_t nloc, uint32_t nl, bool fast, TpRanges& R) {
+    const uint32_t tid = threadIdx.x;
+    if (tid == 0) {
+        uint32_t n = 0;
+        if (nloc > 0) {
+            const int32_t nc = (int32_t)g.ncells;
+            const int32_t cl = (int32_t)b.cell[a0], ch = (int32_t)b.cell[a0 + nloc - 1u];
+            auto add = [&](int32_t a, int32_t z) {
+                if (a < 0) a = 0;
+                if (z > nc - 1) z = nc - 1;
+                if (a > z || n == (uint32_t)TP_MAXR) return;  // (set-up keeps pull boxes within TP_MAXR)
+                uint32_t k = n++;  // insertion by start
+                while (k > 0 && R.lo[k - 1] > a) {
+                    R.lo[k] = R.lo[k - 1];
+                    R.hi[k] = R.hi[k - 1];
+                    --k;
+                }
+                R.lo[k] = a;
+                R.hi[k] = z;
+            };
+            add(cl, ch);  // the targets' own cells
+            for (int k = 0; k < g.nruns; ++k) add(cl - g.run_off[k] - g.run_len[k] + 1, ch - g.run_off[k]);
+            uint32_t m = 0;  // merge overlapping and adjacent ranges
+            for (uint32_t k = 0; k < n; ++k) {
+                if (m > 0 && R.lo[k] <= R.hi[m - 1] + 1) {
+                    if (R.hi[k] > R.hi[m - 1]) R.hi[m - 1] = R.hi[k];
+                } else {
+                    R.lo[m] = R.lo[k];
+                    R.hi[m] = R.hi[k];
+                    ++m;
+                }
+            }
+            n = m;
+        }
+        R.n = n;
+    }
+    __syncthreads();
+    const uint32_t n = R.n;
+    if (tid < n) {
+        const uint32_t r0 = tp_row_start(b, (uint32_t)R.lo[tid], nl, fast);
+        R.r0[tid] = r0;
+        R.pre[tid + 1] = tp_row_start(b, (uint32_t)R.hi[tid] + 1u, nl, fast) - r0;  // rows, summed below
+    }
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t s = 0;
+        R.pre[0] = 0;
+        for (uint32_t k = 1; k <= n; ++k) {
+            s += R.pre[k];
+            R.pre[k] = s;
+        }
+    }
+    __syncthreads();
+}
+
+// Calls f(target - a0, row) for every CSR entry of the merged rows whose
+// target is one of the tile's locals [a0, a0 + nloc); one row per thread.
+template <typename F>
+__device__ __forceinline__ void tp_walk_local(CBox& b, const TpRanges& R, uint32_t a0, uint32_t nloc, F&& f) {
+    const uint32_t rows = R.pre[R.n];
+    for (uint32_t x = threadIdx.x; x < rows; x += BLK) {
+        uint32_t k = 0;
+        while (R.pre[k + 1] <= x) ++k;
+        const uint32_t r = R.r0[k] + (x - R.pre[k]);
+        const uint32_t o0 = b.offsets[r], o1 = b.offsets[r + 1];
+        for (uint32_t e = o0; e < o1; e += 4u) {  // up to four entries' loads in flight
+            uint32_t j[4];
+#pragma unroll
+            for (uint32_t u = 0; u < 4u; ++u) j[u] = e + u < o1 ? b.indices[e + u] : 0xffffffffu;
+#pragma unroll
+            for (uint32_t u = 0; u < 4u; ++u)
+                if (j[u] - a0 < nloc) f(j[u] - a0, r);
+        }
+    }
+}
+
+// Calls f(row) for every CSR entry whose target is the ghost a (in cell cg):
+// the rows of its own cell, then of each run's backward cells.
+template <typename F>
+__device__ __forceinline__ void tp_walk_ghost(CBox& b, CGeom& g, uint32_t a, uint32_t nl, bool fast, F&& f) {
+    const int32_t nc = (int32_t)g.ncells;
+    const int32_t cg = (int32_t)b.cell[a];
+    for (int k = -1; k < g.nruns; ++k) {
+        int32_t lo = k < 0 ? cg : cg - g.run_off[k] - g.run_len[k] + 1;
+        int32_t hi = k < 0 ? cg : cg - g.run_off[k];
+        if (lo < 0) lo = 0;
+        if (hi > nc - 1) hi = nc - 1;
+        if (lo > hi) continue;
+        const uint32_t r0 = tp_row_start(b, (uint32_t)lo, nl, fast);
+        const uint32_t r1 = tp_row_start(b, (uint32_t)hi + 1u, nl, fast);
+        for (uint32_t r = r0; r < r1; ++r) {
+            const uint32_t o0 = b.offsets[r], o1 = b.offsets[r + 1];
+            for (uint32_t e = o0; e < o1; ++e)
+                if (b.indices[e] == a) f(r);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(BLK) k_tpull_count(KARGS) {
+    __shared__ TpRanges R;
+    __shared__ uint32_t cnt[BLK];
+    __shared__ uint32_t lsum[BLK / 64];
+    const uint32_t tid = threadIdx.x;
+    FOR_TILES(t) {
+        const Tile tl = tiles[t];
+        CBox& b = box_at(boxes, tl.box);
+        if (!live(b) || !tpull_active(b)) continue;
+        const BoxState* st = b.st;
+        const uint32_t nl = st->nlocal, na = nl + st->nghost, a0 = tl.start;
+        if (a0 >= na) continue;  // block-uniform
+        CGeom& g = geom_of(b);
+        const uint32_t an = na - a0 < BLK ? na - a0 : BLK;
+        const uint32_t nloc = a0 < nl ? (nl - a0 < an ? nl - a0 : an) : 0u;
+        const bool fast = st->sstart_build != 0u;
+        cnt[tid] = 0u;
+        tp_setup(b, g, a0, nloc, nl, fast, R);  // (its barriers also publish cnt's zeros)
+        tp_walk_local(b, R, a0, nloc, [&](uint32_t k, uint32_t) { atomicAdd(&cnt[k], 1u); });
+        const uint32_t a = a0 + tid;
+        if (tid < an && a >= nl) {
+            uint32_t c = 0;
+            tp_walk_ghost(b, g, a, nl, fast, [&](uint32_t) { ++c; });
+            cnt[tid] = c;
+        }
+        __syncthreads();
+        const uint32_t c = tid < an ? cnt[tid] : 0u;
+        if (tid < an) b.tcount[a] = c;
+        uint32_t tot;
+        (void)block_excl_scan(c, lsum, tot);
+        if (tid == 0) {
+            b.ttile[a0 / BLK] = tot;
+            if (tot) atomicAdd(&b.tgroup[(a0 / BLK) / TP_GROUP], tot);  // zeroed by the CSR placement
+        }
+        __syncthreads();  // the next tile rewrites cnt and the ranges
+    }
+}
+
+__global__ void __launch_bounds__(BLK) k_tpull_place(KARGS) {
+    __shared__ TpRanges R;
+    __shared__ uint32_t ts[BLK], cur[BLK];
+    __shared__ uint32_t lsum[BLK / 64];
+    __shared__ uint64_t lsum64[BLK / 64];
+    const uint32_t tid = threadIdx.x;
+    FOR_TILES(t) {
+        const Tile tl = tiles[t];
+        CBox& b = box_at(boxes, tl.box);
+        if (!live(b) || !tpull_active(b)) continue;
+        const BoxState* st = b.st;
+        const uint32_t nl = st->nlocal, na = nl + st->nghost, a0 = tl.start;
+        if (a0 > na) continue;  // block-uniform (the tile holding index na writes tstart[na])
+        // the tile's base: the totals of the earlier groups of tiles, then of
+        // the group's earlier tiles
+        const uint32_t tix = a0 / BLK, grp = tix / TP_GROUP;
+        uint32_t pre = 0;
+        for (uint32_t q = tid; q < grp; q += BLK) pre += b.tgroup[q];
+        if (tid < TP_GROUP && grp * TP_GROUP + tid < tix) pre += b.ttile[grp * TP_GROUP + tid];
+        const uint32_t base = (uint32_t)block_sum64(pre, lsum64);  // below 2^32: the pair count
+        const uint32_t an = na - a0 < BLK ? na - a0 : BLK;
+        const uint32_t c = tid < an ? b.tcount[a0 + tid] : 0u;
+        uint32_t tot;
+        const uint32_t loc = block_excl_scan(c, lsum, tot) + base;
+        if (tid <= an && a0 + tid <= na) b.tstart[a0 + tid] = loc;  // tstart[na] = every incoming pair
+        if (an == 0u) continue;  // block-uniform
+        CGeom& g = geom_of(b);
+        const uint32_t nloc = a0 < nl ? (nl - a0 < an ? nl - a0 : an) : 0u;
+        const bool fast = st->sstart_build != 0u;
+        ts[tid] = loc;
+        cur[tid] = 0u;
+        tp_setup(b, g, a0, nloc, nl, fast, R);
+        tp_walk_local(b, R, a0, nloc, [&](uint32_t k, uint32_t r) {
+            const uint32_t slot = atomicAdd(&cur[k], 1u);
+            b.tsrc[ts[k] + slot] = r;
+        });
+        const uint32_t a = a0 + tid;
+        if (tid < an && a >= nl) {
+            uint32_t k = loc;
+            tp_walk_ghost(b, g, a, nl, fast, [&](uint32_t r) { b.tsrc[k++] = r; });
+        }
+        __syncthreads();  // the next tile rewrites the LDS tables
     }
 }
 
@@ -3191,9 +3426,15 @@ static void build_rest(const Launch& L, hipEvent_t ev0, hipEvent_t evc, hipEvent
     if (ev2) (void)hipEventRecord(ev2, L.stream);
     // consumer transpose (tcount was cleared by save_build; modes 1 and 2
     // take the arrival slots in k_csr_fill)
-    if (L.csr_mode == 0) launch(k_tcount, L.t_local, L.boxes, L.stream, L.maxgrid);
-    run_scan<SCAN_TRANSPOSE>(L, L.t_scan_atom);
-    launch(k_tplace, L.t_local, L.boxes, L.stream, L.maxgrid);
+    if (L.tpull_any) {
+        launch(k_tpull_count, L.t_atom, L.boxes, L.stream, L.maxgrid);
+        launch(k_tpull_place, L.t_atom1, L.boxes, L.stream, L.maxgrid);
+    }
+    if (L.tatomic_any) {
+        if (L.csr_mode == 0) launch(k_tcount, L.t_local, L.boxes, L.stream, L.maxgrid);
+        run_scan<SCAN_TRANSPOSE>(L, L.t_scan_atom);
+        launch(k_tplace, L.t_local, L.boxes, L.stream, L.maxgrid);
+    }
 }
 // the build after phase_pbc_sort_fused (which already did the bookkeeping)
 void phase_build_fused(const Launch& L, hipEvent_t ev0, hipEvent_t evc, hipEvent_t ev1, hipEvent_t ev2) {

@@ -11,6 +11,8 @@ constexpr uint32_t SCAN_TILE = BLK * SCAN_PER_THREAD;
 constexpr uint32_t BORDER_PER = 4;                 // 64-atom chunks per wave in a border tile
 constexpr uint32_t BORDER_TILE = BLK * BORDER_PER;  // atoms per border-compaction tile
 constexpr uint32_t CSR1_GROUP = 64;                 // tiles per group sum of the tile-staged CSR build
+constexpr uint32_t TP_GROUP = 64;                   // atom tiles per group sum of the pull transpose
+constexpr int TP_MAXR = 64;                         // cell ranges a pull-transpose tile can merge
 
 enum { SORT_SPATIAL = 0, SORT_BUILD = 1 };
 enum { SCAN_CELLS_SPATIAL = 0, SCAN_CELLS_BUILD = 1, SCAN_CSR = 2, SCAN_TRANSPOSE = 3, SCAN_WAVES = 4, SCAN_REMOVE = 5 };
@@ -34,6 +36,9 @@ struct Launch {
     // dim, so its two swaps need separate ordered launches
     uint8_t rev_split[3] = {1, 1, 1};
     int csr_mode = 3;
+    // consumer transpose launches: the pull kernels for boxes with tpull set,
+    // the atomic path's scan and scatter for the others (a batch may have both)
+    bool tpull_any = false, tatomic_any = true;
     // some box has a non-periodic axis: its pbc may remove atoms, so it runs
     // as its own phase before the fused sort (fully periodic boxes wrap inside
     // the sort's count)
