@@ -18,7 +18,10 @@
 //   compute_bin_grid(_triclinic)    : closed-form grids, brute-force stencil
 //                                     completeness (no in-cutoff pair missed)
 //   bin_neighbor_list               : brute-force minimum-image pair sets (both
-//                                     inclusions), neighbor.rs:1457-1619 fixtures
+//                                     inclusions), neighbor.rs:1457-1619 fixtures;
+//                                     row order: own cell first, then the stencil
+//                                     in generation (dx, dy, dz loop) order, bin
+//                                     order within a cell (neighbor.rs:1179-1304)
 //   borders                         : brute-force image enumeration, swap rules
 //   forward_comm_borders            : every ghost = source + the build's fixed
 //                                     offset, pos and vel, chained swap by swap

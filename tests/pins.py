@@ -16,7 +16,13 @@ not these. Each states the reference's rule directly:
 * the rebuild schedule (crates/soil_core/src/neighbor.rs:762-784, 842-848,
   989-996): with `every` = E and `check` off, rebuilds land on steps
   1, E+2, 2E+3, ...; a periodic sort at `sort_every` forces a rebuild
-  exactly when the stable cell order of the locals is not the identity.
+  exactly when the stable cell order of the locals is not the identity;
+* the CSR row order (crates/soil_core/src/neighbor.rs:1175-1304, stencil
+  generation :371-389 and :461-492): row i lists its own cell first (the
+  half list only the records after i's own, the full list all but i), then
+  the stencil's cells in generation order, the dx, dy, dz loops nested
+  outermost to innermost, each cell's records in bin order. The pair SET is
+  pinned by brute force elsewhere; this fixes where each pair sits.
 """
 import numpy as np
 
@@ -125,3 +131,57 @@ def removal_order_sequential(removed):
             a[i] = a[-1]
             a.pop()
     return np.array(a, dtype=np.int64)
+
+
+def stencil_digits(offsets, nbin):
+    """(dx, dy, dz) of each cell offset dx*ny*nz + dy*nz + dz. The grid holds
+    the stencil's reach of ghost bins on both sides of at least one bin, so
+    |dz| <= (nz - 1) // 2 and |dy| <= (ny - 1) // 2, and the digits are unique."""
+    ny, nz = int(nbin[1]), int(nbin[2])
+    hz, hy = (nz - 1) // 2, (ny - 1) // 2
+    off = np.asarray(offsets, dtype=np.int64)
+    dz = (off + hz) % nz - hz
+    rest = (off - dz) // nz
+    dy = (rest + hy) % ny - hy
+    dx = (rest - dy) // ny
+    assert np.array_equal(dx * ny * nz + dy * nz + dz, off)
+    return np.stack([dx, dy, dz], axis=1)
+
+
+def check_stencil_order(stencil, nbin, newton):
+    """The stencil a row walks after its own cell: generation order is
+    lexicographic in (dx, dy, dz); the half list keeps the offsets > 0, the
+    full list every offset but 0."""
+    st = np.asarray(stencil, dtype=np.int64)
+    d = stencil_digits(st, nbin)
+    for k in range(1, len(st)):
+        assert tuple(d[k - 1]) < tuple(d[k]), f"stencil entries {k - 1}, {k} out of generation order"
+    assert (st > 0).all() if newton else (st != 0).all()
+
+
+def check_csr_order(off, idx, bins, stencil, nbin, newton):
+    """Every row of the CSR in the reference's loop order (see the module
+    docstring). `stencil`: the offsets the rows walk after their own cell."""
+    check_stencil_order(stencil, nbin, newton)
+    bin_start, sorted_atoms, sorted_idx, atom_cell = (np.asarray(a, dtype=np.int64) for a in bins)
+    seg_of = {int(o): k + 1 for k, o in enumerate(stencil)}
+    off = np.asarray(off, dtype=np.int64)
+    idx = np.asarray(idx, dtype=np.int64)
+    nl = len(off) - 1
+    rows = np.repeat(np.arange(nl), np.diff(off))
+    dcell = atom_cell[idx] - atom_cell[rows]
+    seg = np.array([0 if d == 0 else seg_of.get(int(d), -1) for d in dcell], dtype=np.int64)
+    assert (seg >= 0).all(), "an entry lies in a cell outside the row's stencil"
+    pos = sorted_idx[idx]  # each entry's place in bin order
+    assert np.array_equal(sorted_atoms[pos], idx)
+    own = seg == 0
+    if newton:
+        # the half list's own cell: only the records after the row's own
+        assert (pos[own] > sorted_idx[rows[own]]).all(), "a self-cell entry at or before its row's record"
+    else:
+        assert (idx[own] != rows[own]).all(), "a row lists itself"
+    # within each row the key (segment, place in bin order) strictly increases
+    same_row = rows[1:] == rows[:-1]
+    key_up = (seg[1:] > seg[:-1]) | ((seg[1:] == seg[:-1]) & (pos[1:] > pos[:-1]))
+    bad = np.nonzero(same_row & ~key_up)[0]
+    assert bad.size == 0, f"row {rows[bad[0]] if bad.size else -1}: entries out of the reference's loop order"

@@ -244,3 +244,69 @@ def test_oracle_pbc_removes_atoms_outside_fixed_faces(tilt):
     assert np.array_equal(a1["radius"], a0["radius"][order])
     assert np.array_equal(r.column(0), a0["tag"][order] * 3.0)
     assert r.state == 0  # FullRebuild
+
+
+def csr_case(case):
+    """Oracle boxes for the row-order rule: uniform and polydisperse radii
+    (the two accept paths), the full list, and a tilted grid."""
+    if case == "uniform":
+        g = soil_amd.gen_custom(7060, 1500, 0.5, 0.5, 0.45, (1, 1, 1), 0, 0.0, 0.0, 0.1, 0.0)
+    elif case == "tilted":
+        g = soil_amd.gen_custom(7061, 1500, 0.4, 0.6, 0.4, (2, 1, 1), 3, 0.5, 2.0, 0.1, 0.2)
+    else:
+        g = soil_amd.gen_custom(7062, 1500, 0.4, 0.6, 0.45, (1, 1, 1), 0, 0.0, 0.0, 0.1, 0.0)
+    d, pos, vel, mass, rad = g
+    if case == "full":
+        d.newton = 0
+    r = RefBox(d)
+    r.add_atoms(pos, mass, rad, vel=vel)
+    r.setup()
+    r.step(2)
+    return r, bool(d.newton)
+
+
+def rows_stencil(r, newton):
+    return r.stencil(True) if newton else np.array([o for o in r.stencil(False) if o != 0], dtype=np.int64)
+
+
+@pytest.mark.parametrize("case", ["uniform", "poly", "full", "tilted"])
+def test_csr_rows_follow_the_reference_loop_order(case):
+    from pins import check_csr_order
+    r, newton = csr_case(case)
+    off, idx = r.csr()
+    assert idx.size > 1000
+    check_csr_order(off, idx, r.bins(), rows_stencil(r, newton), r.grid()[0:3], newton)
+
+
+def test_csr_order_rule_catches_planted_mistakes():
+    # the rule is not vacuous: the same rows with two entries of a row
+    # swapped, a stencil walked in the reverse of the generation loops, or the
+    # own cell walked last, must fail
+    from pins import check_csr_order, check_stencil_order
+    r, newton = csr_case("poly")
+    off, idx = r.csr()
+    bins, nbin, st = r.bins(), r.grid()[0:3], rows_stencil(r, newton)
+    long_row = int(np.nonzero(np.diff(off) >= 3)[0][0])
+    bad = idx.copy()
+    a = int(off[long_row])
+    bad[a], bad[a + 1] = bad[a + 1], bad[a]
+    with pytest.raises(AssertionError):
+        check_csr_order(off, bad, bins, st, nbin, newton)
+    # a forward stencil in generation order (reach 2 in z), then reversed
+    gen = [o for o in (dx * 25 + dy * 5 + dz for dx in (-1, 0, 1) for dy in (-1, 0, 1) for dz in (-2, -1, 0, 1, 2))
+           if o > 0]
+    check_stencil_order(gen, (3, 5, 5), True)
+    with pytest.raises(AssertionError):
+        check_stencil_order(gen[::-1], (3, 5, 5), True)
+    with pytest.raises(AssertionError):
+        check_csr_order(off, idx, bins, st[::-1], nbin, newton)
+    # own cell last: move each row's self-cell entries behind its stencil entries
+    _, _, _, ac = (np.asarray(x, dtype=np.int64) for x in bins)
+    moved = idx.copy()
+    for i in range(len(off) - 1):
+        e = idx[off[i]:off[i + 1]].astype(np.int64)
+        own = ac[e] == ac[i]
+        moved[off[i]:off[i + 1]] = np.concatenate([e[~own], e[own]])
+    assert not np.array_equal(moved, idx)
+    with pytest.raises(AssertionError):
+        check_csr_order(off, moved, bins, st, nbin, newton)
