@@ -21,7 +21,10 @@
 //                                     inclusions), neighbor.rs:1457-1619 fixtures;
 //                                     row order: own cell first, then the stencil
 //                                     in generation (dx, dy, dz loop) order, bin
-//                                     order within a cell (neighbor.rs:1179-1304)
+//                                     order within a cell (neighbor.rs:1179-1304);
+//                                     the uniform vs per-pair cutoff form and its
+//                                     rounding, on a pair placed between the two
+//                                     (neighbor.rs:679-688)
 //   borders                         : brute-force image enumeration, swap rules
 //   forward_comm_borders            : every ghost = source + the build's fixed
 //                                     offset, pos and vel, chained swap by swap

@@ -17,6 +17,10 @@ not these. Each states the reference's rule directly:
   989-996): with `every` = E and `check` off, rebuilds land on steps
   1, E+2, 2E+3, ...; a periodic sort at `sort_every` forces a rebuild
   exactly when the stable cell order of the locals is not the identity;
+* the pair cutoff's two forms (crates/soil_core/src/neighbor.rs:679-688,
+  :1175-1304): when the locals' radii span less than 1e-15 the build tests
+  r2 < c*c with c = 2.0 * min_r * sf, otherwise r2 < (ri+rj)*(ri+rj)*(sf*sf);
+  the two round differently, so a pair can sit between them;
 * the CSR row order (crates/soil_core/src/neighbor.rs:1175-1304, stencil
   generation :371-389 and :461-492): row i lists its own cell first (the
   half list only the records after i's own, the full list all but i), then
@@ -185,3 +189,36 @@ def check_csr_order(off, idx, bins, stencil, nbin, newton):
     key_up = (seg[1:] > seg[:-1]) | ((seg[1:] == seg[:-1]) & (pos[1:] > pos[:-1]))
     bad = np.nonzero(same_row & ~key_up)[0]
     assert bad.size == 0, f"row {rows[bad[0]] if bad.size else -1}: entries out of the reference's loop order"
+
+
+# Two atoms 0.96 apart along x (0.99125 - 0.03125, exact), y = z = 10, in a
+# periodic 20-cube, with sf = 1.2 and radius 0.4: r2 = fl(0.96 * 0.96) =
+# 0.9216 equals the uniform form's c*c = 0.9216 (not below it) and lies below
+# the per-pair form's 0.8*0.8*1.44 = 0.9216000000000002.
+CUT_SF = 1.2
+CUT_R = 0.4
+CUT_POS = np.array([[0.03125, 10.0, 10.0], [0.99125, 10.0, 10.0], [10.0, 10.0, 10.0]])
+
+
+def cutoff_sq_rule(r_i, r_j, radii, sf):
+    """The squared cutoff the build applies to the pair (r_i, r_j) of a box
+    whose locals have `radii` (neighbor.rs:679-688 save_build_positions, then
+    the uniform or per-pair test of bin_neighbor_list)."""
+    mn, mx = min(radii), max(radii)
+    if abs(mx - mn) < 1e-15:
+        c = 2.0 * mn * sf
+        return c * c
+    s = r_i + r_j
+    return s * s * (sf * sf)
+
+
+def cutoff_cases():
+    """(third atom's radius, whether the build lists the pair 0-1): the
+    third atom is far from both and only decides which form applies."""
+    x = CUT_POS[1, 0] - CUT_POS[0, 0]
+    r2 = x * x
+    out = []
+    for r3 in (CUT_R, float(np.nextafter(CUT_R, 1.0)), CUT_R + 2e-15, CUT_R + 0.05):
+        t = cutoff_sq_rule(CUT_R, CUT_R, [CUT_R, CUT_R, r3], CUT_SF)
+        out.append((r3, r2 < t))
+    return out

@@ -128,3 +128,26 @@ def test_csr_row_order_on_device(ctx, case):
     assert idx.size > 1000
     check_csr_order(off, idx, b.bins(), b.stencil(), b.grid()[0:3], True)
     b.close()
+
+
+def test_cutoff_forms_on_device(ctx):
+    # the pair between the two cutoff forms' roundings is listed exactly when
+    # the radii span 1e-15 or more, on the device (pins.py lays the case out)
+    from pins import CUT_POS, CUT_R, CUT_SF, cutoff_cases
+    print("case: device cutoff forms", flush=True)
+    for r3, listed in cutoff_cases():
+        d = soil_amd._ffi.BoxDesc()
+        for k in range(3):
+            d.lo[k], d.hi[k], d.periodic[k] = 0.0, 20.0, 1
+        d.newton, d.skin_fraction, d.bin_min_size = 1, CUT_SF, 0.0
+        d.every, d.check, d.sort_every, d.dt = 0, 1, 1000, 1e-3
+        b = soil_amd.Box(ctx, d)
+        b.upload(CUT_POS.copy(), np.ones(3), np.array([CUT_R, CUT_R, r3]), vel=np.zeros((3, 3)))
+        b.setup()
+        b.step(1)
+        off, idx = b.csr()
+        tags = b.atoms()["tag"]
+        pairs = {tuple(sorted((int(tags[i]), int(tags[j]))))
+                 for i in range(len(off) - 1) for j in idx[off[i]:off[i + 1]]}
+        assert pairs == ({(0, 1)} if listed else set()), (r3, pairs)
+        b.close()

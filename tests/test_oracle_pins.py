@@ -310,3 +310,34 @@ def test_csr_order_rule_catches_planted_mistakes():
     assert not np.array_equal(moved, idx)
     with pytest.raises(AssertionError):
         check_csr_order(off, moved, bins, st, nbin, newton)
+
+
+def cutoff_desc():
+    d = soil_amd._ffi.BoxDesc()
+    for k in range(3):
+        d.lo[k], d.hi[k], d.periodic[k] = 0.0, 20.0, 1
+    d.newton, d.skin_fraction, d.bin_min_size = 1, 1.2, 0.0
+    d.every, d.check, d.sort_every, d.dt = 0, 1, 1000, 1e-3
+    return d
+
+
+def test_cutoff_forms_decide_a_pair_between_their_roundings():
+    # one pair sits between the uniform form's c*c and the per-pair form: it
+    # is listed exactly when the locals' radii span 1e-15 or more (a third,
+    # distant atom decides), and the uniform form takes the smallest radius
+    from pins import CUT_POS, CUT_R, CUT_SF, cutoff_cases
+    cases = cutoff_cases()
+    assert [c[1] for c in cases] == [False, False, True, True]  # the rule's verdicts, as laid out in pins.py
+    for r3, listed in cases:
+        d = cutoff_desc()
+        assert d.skin_fraction == CUT_SF
+        r = RefBox(d)
+        r.add_atoms(CUT_POS.copy(), np.ones(3), np.array([CUT_R, CUT_R, r3]), vel=np.zeros((3, 3)))
+        r.setup()
+        r.step(1)
+        off, idx = r.csr()
+        a = r.atoms()
+        tags = a["tag"]
+        pairs = {tuple(sorted((int(tags[i]), int(tags[j]))))
+                 for i in range(len(off) - 1) for j in idx[off[i]:off[i + 1]]}
+        assert pairs == ({(0, 1)} if listed else set()), (r3, pairs)
