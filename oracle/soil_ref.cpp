@@ -24,7 +24,9 @@
 //                                     order within a cell (neighbor.rs:1179-1304);
 //                                     the uniform vs per-pair cutoff form and its
 //                                     rounding, on a pair placed between the two
-//                                     (neighbor.rs:679-688)
+//                                     (neighbor.rs:679-688); the squared distance's
+//                                     fma nesting, on pairs whose listing flips
+//                                     under a plain sum or the reverse nesting
 //   borders                         : brute-force image enumeration, swap rules
 //   forward_comm_borders            : every ghost = source + the build's fixed
 //                                     offset, pos and vel, chained swap by swap

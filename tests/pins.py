@@ -21,6 +21,10 @@ not these. Each states the reference's rule directly:
   :1175-1304): when the locals' radii span less than 1e-15 the build tests
   r2 < c*c with c = 2.0 * min_r * sf, otherwise r2 < (ri+rj)*(ri+rj)*(sf*sf);
   the two round differently, so a pair can sit between them;
+* the squared distance of the build's test (neighbor.rs:1194-1197 and the
+  same lines of the other three loops): dx.mul_add(dx, dy.mul_add(dy, dz*dz)),
+  two fused multiply-adds nested in that order, which rounds differently
+  from a plain sum or the reverse nesting near the cutoff;
 * the CSR row order (crates/soil_core/src/neighbor.rs:1175-1304, stencil
   generation :371-389 and :461-492): row i lists its own cell first (the
   half list only the records after i's own, the full list all but i), then
@@ -222,3 +226,29 @@ def cutoff_cases():
         t = cutoff_sq_rule(CUT_R, CUT_R, [CUT_R, CUT_R, r3], CUT_SF)
         out.append((r3, r2 < t))
     return out
+
+
+def fma_exact(a, b, c):
+    """a*b + c rounded once (Rust's mul_add), by exact rational arithmetic."""
+    from fractions import Fraction
+    return float(Fraction(a) * Fraction(b) + Fraction(c))
+
+
+def r2_reference(dx, dy, dz):
+    return fma_exact(dx, dx, fma_exact(dy, dy, dz * dz))
+
+
+def r2_alternatives(dx, dy, dz):
+    """Plausible misreadings: a plain sum, and the nesting reversed."""
+    return dx * dx + dy * dy + dz * dz, fma_exact(dz, dz, fma_exact(dy, dy, dx * dx))
+
+
+# Two pairs in a periodic 20-cube with radius 0.4 everywhere and sf = 1.2, so
+# the cutoff is c*c = 0.9216: atom 0 at (0.25, 0.25, 0.25), atom 1 at one of
+# these positions (differences exact). The reference's nesting puts the first
+# at 0.9216 (not listed) and the second at 0.9215999999999999 (listed); the
+# plain sum and the reversed nesting give the opposite verdict for each.
+FMA_R, FMA_SF = 0.4, 1.2
+FMA_PI = (0.25, 0.25, 0.25)
+FMA_PJ = [(0.9818395051144911, 0.7291151231438562, 0.6455497914657826),
+          (0.9118833742240997, 0.6027101868239697, 0.8492544726877895)]

@@ -151,3 +151,27 @@ def test_cutoff_forms_on_device(ctx):
                  for i in range(len(off) - 1) for j in idx[off[i]:off[i + 1]]}
         assert pairs == ({(0, 1)} if listed else set()), (r3, pairs)
         b.close()
+
+
+def test_distance_fma_nesting_on_device(ctx):
+    # the device lists the pair the reference's fma nesting puts inside the
+    # cutoff and not the one it puts on it (pins.py lays the pairs out)
+    from pins import FMA_PI, FMA_PJ, FMA_R, FMA_SF, r2_reference
+    print("case: device distance rounding", flush=True)
+    c = 2.0 * FMA_R * FMA_SF
+    for pj in FMA_PJ:
+        listed = r2_reference(*[pj[k] - FMA_PI[k] for k in range(3)]) < c * c
+        d = soil_amd._ffi.BoxDesc()
+        for k in range(3):
+            d.lo[k], d.hi[k], d.periodic[k] = 0.0, 20.0, 1
+        d.newton, d.skin_fraction, d.bin_min_size = 1, FMA_SF, 0.0
+        d.every, d.check, d.sort_every, d.dt = 0, 1, 1000, 1e-3
+        b = soil_amd.Box(ctx, d)
+        b.upload(np.array([FMA_PI, pj, (10.0, 10.0, 10.0)]), np.ones(3), np.full(3, FMA_R), vel=np.zeros((3, 3)))
+        b.setup()
+        b.step(1)
+        off, idx = b.csr()
+        tags = b.atoms()["tag"]
+        pairs = {tuple(sorted((int(tags[i]), int(tags[j])))) for i in range(len(off) - 1) for j in idx[off[i]:off[i + 1]]}
+        assert pairs == ({(0, 1)} if listed else set()), pj
+        b.close()

@@ -341,3 +341,43 @@ def test_cutoff_forms_decide_a_pair_between_their_roundings():
         pairs = {tuple(sorted((int(tags[i]), int(tags[j]))))
                  for i in range(len(off) - 1) for j in idx[off[i]:off[i + 1]]}
         assert pairs == ({(0, 1)} if listed else set()), (r3, pairs)
+
+
+def fma_box_csr(make, pj):
+    """CSR pairs (by tag) of a 20-cube holding the two atoms of the fma pin
+    and a third, distant one; `make(desc, pos, radii)` -> stepped box."""
+    from pins import FMA_PI, FMA_R, FMA_SF
+    d = cutoff_desc()
+    d.skin_fraction = FMA_SF
+    pos = np.array([FMA_PI, pj, (10.0, 10.0, 10.0)])
+    b = make(d, pos, np.full(3, FMA_R))
+    off, idx = b.csr()
+    tags = b.atoms()["tag"]
+    return {tuple(sorted((int(tags[i]), int(tags[j])))) for i in range(len(off) - 1) for j in idx[off[i]:off[i + 1]]}
+
+
+def test_distance_uses_the_reference_fma_nesting():
+    # two pairs whose listing depends on how the squared distance rounds:
+    # the reference's nesting lists the second and not the first, and each
+    # plausible misreading flips one of them
+    from pins import FMA_PI, FMA_PJ, FMA_R, FMA_SF, r2_alternatives, r2_reference
+    c = 2.0 * FMA_R * FMA_SF
+    cut = c * c
+    verdicts = []
+    for pj in FMA_PJ:
+        dxyz = [pj[k] - FMA_PI[k] for k in range(3)]
+        assert all(dxyz[k] + FMA_PI[k] == pj[k] for k in range(3))  # exact differences
+        listed = r2_reference(*dxyz) < cut
+        assert all((alt < cut) != listed for alt in r2_alternatives(*dxyz))
+        verdicts.append(listed)
+    assert verdicts == [False, True]
+
+    def make(d, pos, rad):
+        r = RefBox(d)
+        r.add_atoms(pos, np.ones(3), rad, vel=np.zeros((3, 3)))
+        r.setup()
+        r.step(1)
+        return r
+
+    for pj, listed in zip(FMA_PJ, verdicts):
+        assert fma_box_csr(make, pj) == ({(0, 1)} if listed else set()), pj
