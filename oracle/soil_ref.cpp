@@ -14,7 +14,9 @@
 // Parity pins (where each function is checked; see tests/):
 //   x2lamda/lamda2x, pbc_triclinic  : domain.rs:1016-1058 KATs
 //   initial/final_integration       : soil_verlet/src/lib.rs:221-233,416-428 KATs,
-//                                     closed-form parabola, O(dt^2) ratio
+//                                     closed-form parabola, O(dt^2) ratio; an atom
+//                                     whose first step a fused kick, a fused drift
+//                                     or h = 0.5*dt/m would round differently
 //   compute_bin_grid(_triclinic)    : closed-form grids, brute-force stencil
 //                                     completeness (no in-cutoff pair missed)
 //   bin_neighbor_list               : brute-force minimum-image pair sets (both

@@ -25,6 +25,9 @@ not these. Each states the reference's rule directly:
   same lines of the other three loops): dx.mul_add(dx, dy.mul_add(dy, dz*dz)),
   two fused multiply-adds nested in that order, which rounds differently
   from a plain sum or the reverse nesting near the cutoff;
+* velocity Verlet's kick and drift (crates/soil_verlet/src/lib.rs:143-160,
+  183-192): h = 0.5 * dt * inv_mass left to right, v0 = v + h*f and
+  x = x + v0*dt, no fused multiply-add anywhere;
 * the CSR row order (crates/soil_core/src/neighbor.rs:1175-1304, stencil
   generation :371-389 and :461-492): row i lists its own cell first (the
   half list only the records after i's own, the full list all but i), then
@@ -252,3 +255,32 @@ FMA_R, FMA_SF = 0.4, 1.2
 FMA_PI = (0.25, 0.25, 0.25)
 FMA_PJ = [(0.9818395051144911, 0.7291151231438562, 0.6455497914657826),
           (0.9118833742240997, 0.6027101868239697, 0.8492544726877895)]
+
+
+# One atom whose first step separates the Verlet expression from three
+# misreadings (a fused kick, a fused drift, h from dividing by the mass):
+# each changes at least one of its velocity or position components.
+VERLET_DT = 1e-3
+VERLET_ATOM = dict(
+    m=0.663704541435512,
+    v=(0.13442849658938183, -0.5672019555172545, 1.5736560485162756),
+    f=(278.13949508158737, -403.1574471379613, 317.914802580167),
+    x=(11.923914911422258, 14.98587806422198, 10.067779050664738))
+
+
+def verlet_reference(x, v, f, m, dt):
+    """initial_integration (soil_verlet/src/lib.rs:143-160) for one component."""
+    im = 1.0 / m
+    h = 0.5 * dt * im
+    v0 = v + h * f
+    return v0, x + v0 * dt
+
+
+def verlet_alternatives(x, v, f, m, dt):
+    im = 1.0 / m
+    h = 0.5 * dt * im
+    fused_kick = fma_exact(h, f, v)
+    fused_drift = v + h * f
+    h_div = 0.5 * dt / m
+    v_div = v + h_div * f
+    return [(fused_kick, x + fused_kick * dt), (fused_drift, fma_exact(fused_drift, dt, x)), (v_div, x + v_div * dt)]

@@ -175,3 +175,25 @@ def test_distance_fma_nesting_on_device(ctx):
         pairs = {tuple(sorted((int(tags[i]), int(tags[j])))) for i in range(len(off) - 1) for j in idx[off[i]:off[i + 1]]}
         assert pairs == ({(0, 1)} if listed else set()), pj
         b.close()
+
+
+def test_verlet_rounding_on_device(ctx):
+    # the device's first step of one atom with an uploaded force: the reference
+    # expression's velocity and position bit for bit (pins.py lays it out)
+    from pins import VERLET_ATOM as A, VERLET_DT, verlet_reference
+    print("case: device Verlet rounding", flush=True)
+    ref = [verlet_reference(A["x"][k], A["v"][k], A["f"][k], A["m"], VERLET_DT) for k in range(3)]
+    d = soil_amd._ffi.BoxDesc()
+    for k in range(3):
+        d.lo[k], d.hi[k], d.periodic[k] = 0.0, 20.0, 1
+    d.newton, d.skin_fraction, d.bin_min_size = 1, 1.1, 0.0
+    d.every, d.check, d.sort_every, d.dt = 0, 1, 1000, VERLET_DT
+    b = soil_amd.Box(ctx, d)
+    b.upload(np.array([A["x"]]), np.array([A["m"]]), np.array([0.5]), vel=np.array([A["v"]]),
+             force=np.array([A["f"]]))
+    b.setup()
+    b.step(1)
+    a = b.atoms()
+    assert np.array_equal(a["vel"][0], np.array([r[0] for r in ref]))
+    assert np.array_equal(a["pos"][0], np.array([r[1] for r in ref]))
+    b.close()

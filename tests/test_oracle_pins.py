@@ -381,3 +381,29 @@ def test_distance_uses_the_reference_fma_nesting():
 
     for pj, listed in zip(FMA_PJ, verdicts):
         assert fma_box_csr(make, pj) == ({(0, 1)} if listed else set()), pj
+
+
+def verlet_expected():
+    from pins import VERLET_ATOM as A, VERLET_DT, verlet_alternatives, verlet_reference
+    ref = [verlet_reference(A["x"][k], A["v"][k], A["f"][k], A["m"], VERLET_DT) for k in range(3)]
+    alts = [verlet_alternatives(A["x"][k], A["v"][k], A["f"][k], A["m"], VERLET_DT) for k in range(3)]
+    for a in range(3):  # every misreading changes some component: the case discriminates
+        assert any(alts[k][a] != ref[k] for k in range(3)), a
+    return np.array([r[0] for r in ref]), np.array([r[1] for r in ref])
+
+
+def test_verlet_kick_and_drift_round_as_the_reference():
+    # one atom with an uploaded force: after the first step (whose final
+    # half-kick sees no force) its velocity and position are the reference
+    # expression's, bit for bit
+    from pins import VERLET_ATOM as A, VERLET_DT
+    v1, x1 = verlet_expected()
+    d = cutoff_desc()
+    d.dt = VERLET_DT
+    r = RefBox(d)
+    r.add_atoms(np.array([A["x"]]), np.array([A["m"]]), np.array([0.5]), vel=np.array([A["v"]]),
+                force=np.array([A["f"]]))
+    r.setup()
+    r.step(1)
+    a = r.atoms()
+    assert np.array_equal(a["vel"][0], v1) and np.array_equal(a["pos"][0], x1)
