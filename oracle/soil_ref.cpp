@@ -18,7 +18,9 @@
 //                                     whose first step a fused kick, a fused drift
 //                                     or h = 0.5*dt/m would round differently
 //   compute_bin_grid(_triclinic)    : closed-form grids, brute-force stencil
-//                                     completeness (no in-cutoff pair missed)
+//                                     completeness (no in-cutoff pair missed);
+//                                     cells by the product with 1/bin_size, on
+//                                     atoms at faces where a division differs
 //   bin_neighbor_list               : brute-force minimum-image pair sets (both
 //                                     inclusions), neighbor.rs:1457-1619 fixtures;
 //                                     row order: own cell first, then the stencil

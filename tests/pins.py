@@ -25,6 +25,10 @@ not these. Each states the reference's rule directly:
   same lines of the other three loops): dx.mul_add(dx, dy.mul_add(dy, dz*dz)),
   two fused multiply-adds nested in that order, which rounds differently
   from a plain sum or the reverse nesting near the cutoff;
+* the cell of a position (crates/soil_core/src/neighbor.rs:886-888,
+  915-922): floor((p - origin) * inv) with inv = 1.0 / bin_size computed
+  once, which differs from dividing by bin_size for positions at some cell
+  faces;
 * velocity Verlet's kick and drift (crates/soil_verlet/src/lib.rs:143-160,
   183-192): h = 0.5 * dt * inv_mass left to right, v0 = v + h*f and
   x = x + v0*dt, no fused multiply-add anywhere;
@@ -284,3 +288,19 @@ def verlet_alternatives(x, v, f, m, dt):
     h_div = 0.5 * dt / m
     v_div = v + h_div * f
     return [(fused_kick, x + fused_kick * dt), (fused_drift, fma_exact(fused_drift, dt, x)), (v_div, x + v_div * dt)]
+
+
+# Positions where the reference's product by the inverse bin size and a
+# division by the bin size give different cells, in a periodic 20-cube of
+# radius-0.5 atoms with sf = 1.1 (24 bins of 20/18 per axis from -10/3).
+CELL_POS = np.array([[7.777777777777778, 11.11111111111111, 17.77777777777778],
+                     [2.2222222222222223, 12.222222222222221, 18.88888888888889],
+                     [15.0, 5.0, 9.0]])
+
+
+def cells_by_division(pos, grid):
+    """The misreading: floor((p - o) / size)."""
+    n = grid[0:3].astype(np.int64)
+    c = np.floor((pos - grid[6:9]) / grid[3:6])
+    c = np.clip(c, 0, n - 1).astype(np.int64)
+    return c[:, 0] * n[1] * n[2] + c[:, 1] * n[2] + c[:, 2]

@@ -197,3 +197,24 @@ def test_verlet_rounding_on_device(ctx):
     assert np.array_equal(a["vel"][0], np.array([r[0] for r in ref]))
     assert np.array_equal(a["pos"][0], np.array([r[1] for r in ref]))
     b.close()
+
+
+def test_cell_formula_on_device(ctx):
+    # atoms on cell faces where the product by the inverse bin size and a
+    # division disagree: the device bins them by the product
+    from pins import CELL_POS, cells, cells_by_division
+    print("case: device cell formula", flush=True)
+    d = soil_amd._ffi.BoxDesc()
+    for k in range(3):
+        d.lo[k], d.hi[k], d.periodic[k] = 0.0, 20.0, 1
+    d.newton, d.skin_fraction, d.bin_min_size = 1, 1.1, 0.0
+    d.every, d.check, d.sort_every, d.dt = 0, 1, 1000, 1e-3
+    b = soil_amd.Box(ctx, d)
+    b.upload(CELL_POS.copy(), np.ones(3), np.full(3, 0.5), vel=np.zeros((3, 3)))
+    b.setup()
+    b.step(1)
+    a, g = b.atoms(), b.grid()
+    _, _, _, atom_cell = b.bins()
+    assert (cells_by_division(CELL_POS[:2], g) != cells(CELL_POS[:2], g)).all()
+    assert np.array_equal(atom_cell[:3].astype(np.int64), cells(a["pos"][:3], g))
+    b.close()

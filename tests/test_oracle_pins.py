@@ -407,3 +407,23 @@ def test_verlet_kick_and_drift_round_as_the_reference():
     r.step(1)
     a = r.atoms()
     assert np.array_equal(a["vel"][0], v1) and np.array_equal(a["pos"][0], x1)
+
+
+def test_cell_uses_the_inverse_bin_size():
+    # atoms on cell faces where the product by the inverse and a division
+    # disagree: the oracle bins them by the product (and the case is one
+    # where the division would move them)
+    from pins import CELL_POS, cells_by_division
+    d = cutoff_desc()
+    d.skin_fraction = 1.1
+    r = RefBox(d)
+    r.add_atoms(CELL_POS.copy(), np.ones(3), np.full(3, 0.5), vel=np.zeros((3, 3)))
+    r.setup()
+    r.step(1)
+    a, g = r.atoms(), r.grid()
+    _, _, _, atom_cell = r.bins()
+    pos = a["pos"][:3]
+    assert np.array_equal(pos[np.argsort(a["tag"][:3])], CELL_POS)  # nothing moved
+    want = cells(pos, g)
+    assert (cells_by_division(CELL_POS[:2], g) != cells(CELL_POS[:2], g)).all()
+    assert np.array_equal(atom_cell[:3].astype(np.int64), want)
