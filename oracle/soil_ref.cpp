@@ -46,7 +46,10 @@
 //                                     order is not the identity (neighbor.rs:989-996)
 //   decide_rebuild / every / check  : every=E, check off rebuilds on steps 1,
 //                                     E+2, 2E+3, ... (neighbor.rs:762-784); the
-//                                     threshold fires at ((sf-1) min_r)^2
+//                                     threshold fires at ((sf-1) min_r)^2; the
+//                                     squared displacement fused when the box is
+//                                     fully periodic, a plain sum otherwise, on
+//                                     atoms whose decision the two forms split
 //   apply_deform (LE xy)            : closed form soil_deform/src/lib.rs:576-583
 //   step state machine              : the run_if placement is read from the
 //                                     plugin sources (tests/test_integration_doc.py

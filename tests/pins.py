@@ -25,6 +25,10 @@ not these. Each states the reference's rule directly:
   same lines of the other three loops): dx.mul_add(dx, dy.mul_add(dy, dz*dz)),
   two fused multiply-adds nested in that order, which rounds differently
   from a plain sum or the reverse nesting near the cutoff;
+* the displacement trigger (crates/soil_core/src/neighbor.rs:693-758): the
+  threshold is t*t with t = (sf - 1) * min_r; a fully periodic box compares
+  dx.mul_add(dx, dy.mul_add(dy, dz*dz)), a box with a non-periodic axis the
+  plain dx*dx + dy*dy + dz*dz;
 * the cell of a position (crates/soil_core/src/neighbor.rs:886-888,
   915-922): floor((p - origin) * inv) with inv = 1.0 / bin_size computed
   once, which differs from dividing by bin_size for positions at some cell
@@ -304,3 +308,28 @@ def cells_by_division(pos, grid):
     c = np.floor((pos - grid[6:9]) / grid[3:6])
     c = np.clip(c, 0, n - 1).astype(np.int64)
     return c[:, 0] * n[1] * n[2] + c[:, 1] * n[2] + c[:, 2]
+
+
+# One atom of radius 0.5 (sf = 1.1, dt = 1e-3, no force) per case, moving
+# about the displacement threshold in one step: after the build of step 1,
+# step 2's decision depends on whether the squared displacement is fused
+# (fully periodic boxes) or a plain sum (boxes with a non-periodic axis).
+DISP_SF, DISP_R, DISP_DT = 1.1, 0.5, 1e-3
+DISP_CASES = [
+    ((0.4211645829016162, 0.42858815844774134, 0.4700675058101765),
+     (49.81585995812322, -3.8877342541753896, 1.8071023771698262)),
+    ((0.5597565455328373, 0.4586275074444136, 0.5460616877157702),
+     (-18.44897763477604, -37.71252681317814, -27.15512006228029)),
+]
+
+
+def disp_decisions(x0, v):
+    """(rebuild at step 2 in a fully periodic box, in a box with a fixed axis)."""
+    t = (DISP_SF - 1.0) * DISP_R
+    thr = t * t
+    x1 = [x0[k] + v[k] * DISP_DT for k in range(3)]
+    x2 = [x1[k] + v[k] * DISP_DT for k in range(3)]
+    d = [x2[k] - x1[k] for k in range(3)]
+    fused = fma_exact(d[0], d[0], fma_exact(d[1], d[1], d[2] * d[2]))
+    plain = d[0] * d[0] + d[1] * d[1] + d[2] * d[2]
+    return fused > thr, plain > thr

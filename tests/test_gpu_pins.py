@@ -218,3 +218,29 @@ def test_cell_formula_on_device(ctx):
     assert (cells_by_division(CELL_POS[:2], g) != cells(CELL_POS[:2], g)).all()
     assert np.array_equal(atom_cell[:3].astype(np.int64), cells(a["pos"][:3], g))
     b.close()
+
+
+def test_displacement_trigger_forms_on_device(ctx):
+    # step 2's decision for an atom between the fused and the plain squared
+    # displacement: fused in a fully periodic box, plain with a fixed z
+    from pins import DISP_CASES, DISP_DT, DISP_R, DISP_SF, disp_decisions
+    print("case: device displacement trigger forms", flush=True)
+    for x0, v in DISP_CASES:
+        want = disp_decisions(x0, v)
+        assert want[0] != want[1]
+        for periodic_z, rebuild in zip((True, False), want):
+            d = soil_amd._ffi.BoxDesc()
+            for k in range(3):
+                d.lo[k], d.hi[k], d.periodic[k] = 0.0, 20.0, 1
+            d.newton, d.skin_fraction, d.bin_min_size = 1, DISP_SF, 0.0
+            d.every, d.check, d.sort_every, d.dt = 0, 1, 1000, DISP_DT
+            b = soil_amd.Box(ctx, d)
+            if not periodic_z:
+                b.set_boundary(2, "f")
+            b.upload(np.array([x0]), np.ones(1), np.array([DISP_R]), vel=np.array([v]))
+            b.setup()
+            b.step(1)
+            n1 = b.counts()["rebuilds"]
+            b.step(1)
+            assert b.counts()["rebuilds"] - n1 == (1 if rebuild else 0), (x0, periodic_z)
+            b.close()

@@ -427,3 +427,34 @@ def test_cell_uses_the_inverse_bin_size():
     want = cells(pos, g)
     assert (cells_by_division(CELL_POS[:2], g) != cells(CELL_POS[:2], g)).all()
     assert np.array_equal(atom_cell[:3].astype(np.int64), want)
+
+
+def disp_desc():
+    d = cutoff_desc()
+    d.skin_fraction = 1.1
+    return d
+
+
+def test_displacement_trigger_fuses_only_in_fully_periodic_boxes():
+    # an atom whose step-2 displacement sits between the fused and the plain
+    # squared length: a fully periodic box decides by the fused one, a box
+    # with a fixed z by the plain one (two cases, one each way)
+    from pins import DISP_CASES, DISP_DT, DISP_R, disp_decisions
+    seen = set()
+    for x0, v in DISP_CASES:
+        want = disp_decisions(x0, v)
+        assert want[0] != want[1]  # the case separates the two forms
+        seen.add(want)
+        for periodic_z, rebuild in zip((True, False), want):
+            d = disp_desc()
+            d.dt = DISP_DT
+            r = RefBox(d)
+            if not periodic_z:
+                r.set_boundary(2, "f")
+            r.add_atoms(np.array([x0]), np.ones(1), np.array([DISP_R]), vel=np.array([v]))
+            r.setup()
+            r.step(1)
+            n1 = r.counts()["rebuilds"]
+            r.step(1)
+            assert r.counts()["rebuilds"] - n1 == (1 if rebuild else 0), (x0, periodic_z)
+    assert seen == {(True, False), (False, True)}
