@@ -13,6 +13,9 @@
 //
 // Parity pins (where each function is checked; see tests/):
 //   x2lamda/lamda2x, pbc_triclinic  : domain.rs:1016-1058 KATs
+//   pbc / wrap_periodic             : the bound low + size computed at the wrap,
+//                                     on atoms at hi and at lo + size where the
+//                                     two differ (domain.rs:593-604)
 //   initial/final_integration       : soil_verlet/src/lib.rs:221-233,416-428 KATs,
 //                                     closed-form parabola, O(dt^2) ratio; an atom
 //                                     whose first step a fused kick, a fused drift

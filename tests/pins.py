@@ -29,6 +29,9 @@ not these. Each states the reference's rule directly:
   threshold is t*t with t = (sf - 1) * min_r; a fully periodic box compares
   dx.mul_add(dx, dy.mul_add(dy, dz*dz)), a box with a non-periodic axis the
   plain dx*dx + dy*dy + dz*dz;
+* the periodic wrap (crates/soil_core/src/domain.rs:593-604): the upper
+  bound is low + size computed at the wrap, which need not equal the box's
+  high bound (size = high - low rounds);
 * the cell of a position (crates/soil_core/src/neighbor.rs:886-888,
   915-922): floor((p - origin) * inv) with inv = 1.0 / bin_size computed
   once, which differs from dividing by bin_size for positions at some cell
@@ -333,3 +336,23 @@ def disp_decisions(x0, v):
     fused = fma_exact(d[0], d[0], fma_exact(d[1], d[1], d[2] * d[2]))
     plain = d[0] * d[0] + d[1] * d[1] + d[2] * d[2]
     return fused > thr, plain > thr
+
+
+# A periodic box whose x bounds give lo + (hi - lo) > hi and whose y bounds
+# give lo + (hi - lo) < hi. Atom 0 sits at x = hi (below lo + size: not
+# wrapped), atom 1 at y = lo + size (below hi, but wrapped).
+WRAP_LO = (1.423, 4.391, 0.0)
+WRAP_HI = (18.282, 23.203, 20.0)
+
+
+def wrap_case():
+    """Positions, and the (position, image) the wrap leaves, per atom."""
+    size = [WRAP_HI[k] - WRAP_LO[k] for k in range(3)]
+    top = [WRAP_LO[k] + size[k] for k in range(3)]
+    assert top[0] > WRAP_HI[0] and top[1] < WRAP_HI[1]
+    pos = np.array([[WRAP_HI[0], 10.0, 10.0], [10.0, top[1], 10.0]])
+    want_pos = pos.copy()
+    want_img = np.zeros((2, 3), dtype=np.int32)
+    want_pos[1, 1] = top[1] - size[1]
+    want_img[1, 1] = 1
+    return pos, want_pos, want_img

@@ -244,3 +244,25 @@ def test_displacement_trigger_forms_on_device(ctx):
             b.step(1)
             assert b.counts()["rebuilds"] - n1 == (1 if rebuild else 0), (x0, periodic_z)
             b.close()
+
+
+def test_periodic_wrap_bound_on_device(ctx):
+    # the wrap's upper bound is low + size: an atom at x = hi (below it)
+    # stays, an atom at y = lo + size (below hi) wraps
+    from pins import WRAP_HI, WRAP_LO, wrap_case
+    print("case: device periodic wrap bound", flush=True)
+    pos, want_pos, want_img = wrap_case()
+    d = soil_amd._ffi.BoxDesc()
+    for k in range(3):
+        d.lo[k], d.hi[k], d.periodic[k] = WRAP_LO[k], WRAP_HI[k], 1
+    d.newton, d.skin_fraction, d.bin_min_size = 1, 1.1, 0.0
+    d.every, d.check, d.sort_every, d.dt = 0, 1, 1000, 1e-3
+    b = soil_amd.Box(ctx, d)
+    b.upload(pos, np.ones(2), np.full(2, 0.5), vel=np.zeros((2, 3)))
+    b.setup()
+    b.step(1)
+    a = b.atoms()
+    o = np.argsort(a["tag"][:2])
+    assert np.array_equal(a["pos"][:2][o], want_pos)
+    assert np.array_equal(a["image"][:2][o], want_img)
+    b.close()

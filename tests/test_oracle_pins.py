@@ -458,3 +458,22 @@ def test_displacement_trigger_fuses_only_in_fully_periodic_boxes():
             r.step(1)
             assert r.counts()["rebuilds"] - n1 == (1 if rebuild else 0), (x0, periodic_z)
     assert seen == {(True, False), (False, True)}
+
+
+def test_periodic_wrap_bound_is_low_plus_size():
+    # an atom at x = hi stays (hi < lo + size), an atom at y = lo + size < hi
+    # wraps: the bound is computed as low + size, not read from high
+    from pins import WRAP_HI, WRAP_LO, wrap_case
+    pos, want_pos, want_img = wrap_case()
+    d = cutoff_desc()
+    d.skin_fraction = 1.1
+    for k in range(3):
+        d.lo[k], d.hi[k] = WRAP_LO[k], WRAP_HI[k]
+    r = RefBox(d)
+    r.add_atoms(pos, np.ones(2), np.full(2, 0.5), vel=np.zeros((2, 3)))
+    r.setup()
+    r.step(1)
+    a = r.atoms()
+    o = np.argsort(a["tag"][:2])
+    assert np.array_equal(a["pos"][:2][o], want_pos)
+    assert np.array_equal(a["image"][:2][o], want_img)
