@@ -34,7 +34,10 @@
 //                                     (neighbor.rs:679-688); the squared distance's
 //                                     fma nesting, on pairs whose listing flips
 //                                     under a plain sum or the reverse nesting
-//   borders                         : brute-force image enumeration, swap rules
+//   borders                         : brute-force image enumeration, swap rules;
+//                                     face thresholds lo + gc / hi - gc rounded
+//                                     once, on atoms where a difference form
+//                                     would not send them (comm.rs:336-347)
 //   forward_comm_borders            : every ghost = source + the build's fixed
 //                                     offset, pos and vel, chained swap by swap
 //                                     (comm.rs:400-432; tests/pins.py)

@@ -32,6 +32,10 @@ not these. Each states the reference's rule directly:
 * the periodic wrap (crates/soil_core/src/domain.rs:593-604): the upper
   bound is low + size computed at the wrap, which need not equal the box's
   high bound (size = high - low rounds);
+* the ghost selection faces (crates/soil_core/src/comm.rs:336-347): an atom
+  is sent by the low swap when pos < lo + gc and by the high swap when
+  pos >= hi - gc, the thresholds rounded once, which a form like
+  pos - lo < gc does not reproduce at every position;
 * the cell of a position (crates/soil_core/src/neighbor.rs:886-888,
   915-922): floor((p - origin) * inv) with inv = 1.0 / bin_size computed
   once, which differs from dividing by bin_size for positions at some cell
@@ -356,3 +360,31 @@ def wrap_case():
     want_pos[1, 1] = top[1] - size[1]
     want_img[1, 1] = 1
     return pos, want_pos, want_img
+
+
+# x bounds whose face thresholds round: with radius 0.5 and sf = 1.1 the
+# ghost cutoff is 1.2000000000000002; atom 0 at x just below lo + gc (where
+# x - lo < gc is false) and atom 1 at x = hi - gc (where hi - x <= gc is
+# false) are both sent.
+FACE_LO = (-1.5206, 0.0, 0.0)
+FACE_HI = (15.0177, 20.0, 20.0)
+FACE_X = (-0.32059999999999983, 13.817699999999999)
+
+
+def face_sends(gc):
+    """(sent low, sent high) by the reference's thresholds, and by the
+    difference form, for the two atoms."""
+    lo, hi = FACE_LO[0], FACE_HI[0]
+    ref = (FACE_X[0] < lo + gc, FACE_X[1] >= hi - gc)
+    alt = (FACE_X[0] - lo < gc, hi - FACE_X[1] <= gc)
+    return ref, alt
+
+
+def sent_by_swap(a, swaps):
+    """Tags of the locals each swap's ghosts come from."""
+    nl, ng = a["nlocal"], a["nghost"]
+    sw = ghost_swaps(swaps, nl, ng)
+    out = {}
+    for g in range(nl, nl + ng):
+        out.setdefault(int(sw[g - nl]), set()).add(int(a["tag"][int(a["origin"][g])]))
+    return out

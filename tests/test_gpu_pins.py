@@ -266,3 +266,23 @@ def test_periodic_wrap_bound_on_device(ctx):
     assert np.array_equal(a["pos"][:2][o], want_pos)
     assert np.array_equal(a["image"][:2][o], want_img)
     b.close()
+
+
+def test_ghost_faces_on_device(ctx):
+    # atoms on rounded face thresholds are sent by the x swaps on the device
+    from pins import FACE_HI, FACE_LO, FACE_X, face_sends, sent_by_swap
+    print("case: device ghost faces", flush=True)
+    d = soil_amd._ffi.BoxDesc()
+    for k in range(3):
+        d.lo[k], d.hi[k], d.periodic[k] = FACE_LO[k], FACE_HI[k], 1
+    d.newton, d.skin_fraction, d.bin_min_size = 1, 1.1, 0.0
+    d.every, d.check, d.sort_every, d.dt = 0, 1, 1000, 1e-3
+    b = soil_amd.Box(ctx, d)
+    b.upload(np.array([[FACE_X[0], 10.0, 10.0], [FACE_X[1], 10.0, 10.0]]), np.ones(2), np.full(2, 0.5),
+             vel=np.zeros((2, 3)))
+    b.setup()
+    b.step(1)
+    ref, alt = face_sends(b.grid()[9])
+    assert ref == (True, True) and alt == (False, False)
+    assert sent_by_swap(b.atoms(), b.swaps()) == {0: {0}, 1: {1}}
+    b.close()

@@ -477,3 +477,22 @@ def test_periodic_wrap_bound_is_low_plus_size():
     o = np.argsort(a["tag"][:2])
     assert np.array_equal(a["pos"][:2][o], want_pos)
     assert np.array_equal(a["image"][:2][o], want_img)
+
+
+def test_ghost_faces_use_rounded_thresholds():
+    # two atoms on the face thresholds of a box whose x bounds make them
+    # round: both are sent (low and high swap of x), where the difference
+    # form would send neither
+    from pins import FACE_HI, FACE_LO, FACE_X, face_sends, sent_by_swap
+    d = cutoff_desc()
+    d.skin_fraction = 1.1
+    for k in range(3):
+        d.lo[k], d.hi[k] = FACE_LO[k], FACE_HI[k]
+    r = RefBox(d)
+    pos = np.array([[FACE_X[0], 10.0, 10.0], [FACE_X[1], 10.0, 10.0]])
+    r.add_atoms(pos, np.ones(2), np.full(2, 0.5), vel=np.zeros((2, 3)))
+    r.setup()
+    r.step(1)
+    ref, alt = face_sends(r.grid()[9])
+    assert ref == (True, True) and alt == (False, False)
+    assert sent_by_swap(r.atoms(), r.swaps()) == {0: {0}, 1: {1}}
