@@ -56,7 +56,9 @@
 //                                     squared displacement fused when the box is
 //                                     fully periodic, a plain sum otherwise, on
 //                                     atoms whose decision the two forms split
-//   apply_deform (LE xy)            : closed form soil_deform/src/lib.rs:576-583
+//   apply_deform (LE xy)            : closed form soil_deform/src/lib.rs:576-583;
+//                                     the left-to-right product of :579, on a rate
+//                                     and step every regrouping rounds differently
 //   step state machine              : the run_if placement is read from the
 //                                     plugin sources (tests/test_integration_doc.py
 //                                     parses them); the grass scheduler's dispatch

@@ -36,6 +36,9 @@ not these. Each states the reference's rule directly:
   is sent by the low swap when pos < lo + gc and by the high swap when
   pos >= hi - gc, the thresholds rounded once, which a form like
   pos - lo < gc does not reproduce at every position;
+* the Lees-Edwards tilt (crates/soil_deform/src/lib.rs:579-580):
+  raw = rate * dt * step * ly, multiplied left to right, then
+  xy = raw - lx * round(raw / lx);
 * the cell of a position (crates/soil_core/src/neighbor.rs:886-888,
   915-922): floor((p - origin) * inv) with inv = 1.0 / bin_size computed
   once, which differs from dividing by bin_size for positions at some cell
@@ -388,3 +391,18 @@ def sent_by_swap(a, swaps):
     for g in range(nl, nl + ng):
         out.setdefault(int(sw[g - nl]), set()).add(int(a["tag"][int(a["origin"][g])]))
     return out
+
+
+# A shear rate whose tilt after 9 steps (dt 1e-3, a 20-cube) the reference's
+# left-to-right product rounds differently from every regrouping.
+TILT_RATE, TILT_STEPS = 0.112785, 9
+
+
+def tilt_reference(rate, dt, step, lx, ly):
+    raw = rate * dt * float(step) * ly
+    return raw - lx * round(raw / lx)
+
+
+def tilt_regroupings(rate, dt, step, ly):
+    s = float(step)
+    return [rate * (dt * s) * ly, (rate * ly) * (dt * s), rate * ly * dt * s, (rate * dt) * (s * ly)]

@@ -286,3 +286,21 @@ def test_ghost_faces_on_device(ctx):
     assert ref == (True, True) and alt == (False, False)
     assert sent_by_swap(b.atoms(), b.swaps()) == {0: {0}, 1: {1}}
     b.close()
+
+
+def test_lees_edwards_tilt_on_device(ctx):
+    # the device box's tilt after 9 steps is the reference's left-to-right product
+    from pins import TILT_RATE, TILT_STEPS, tilt_reference
+    print("case: device LE tilt", flush=True)
+    d = soil_amd._ffi.BoxDesc()
+    for k in range(3):
+        d.lo[k], d.hi[k], d.periodic[k] = 0.0, 20.0, 1
+    d.newton, d.skin_fraction, d.bin_min_size = 1, 1.1, 0.0
+    d.every, d.check, d.sort_every, d.dt = 0, 1, 1000, 1e-3
+    d.has_shear, d.shear_rate = 1, TILT_RATE
+    b = soil_amd.Box(ctx, d)
+    b.upload(np.array([[5.0, 5.0, 5.0], [12.0, 14.0, 9.0]]), np.ones(2), np.full(2, 0.5), vel=np.zeros((2, 3)))
+    b.setup()
+    b.step(TILT_STEPS)
+    assert b.grid()[12] == tilt_reference(TILT_RATE, d.dt, TILT_STEPS, 20.0, 20.0)
+    b.close()

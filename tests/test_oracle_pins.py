@@ -496,3 +496,24 @@ def test_ghost_faces_use_rounded_thresholds():
     ref, alt = face_sends(r.grid()[9])
     assert ref == (True, True) and alt == (False, False)
     assert sent_by_swap(r.atoms(), r.swaps()) == {0: {0}, 1: {1}}
+
+
+def tilt_desc():
+    d = cutoff_desc()
+    d.skin_fraction, d.has_shear, d.shear_rate = 1.1, 1, 0.112785
+    return d
+
+
+def test_lees_edwards_tilt_product_order():
+    # the tilt after 9 steps equals the left-to-right product, which every
+    # regrouping of it rounds differently
+    from pins import TILT_RATE, TILT_STEPS, tilt_reference, tilt_regroupings
+    d = tilt_desc()
+    assert d.shear_rate == TILT_RATE
+    want = tilt_reference(TILT_RATE, d.dt, TILT_STEPS, 20.0, 20.0)
+    assert all(alt != want for alt in tilt_regroupings(TILT_RATE, d.dt, TILT_STEPS, 20.0))
+    r = RefBox(d)
+    r.add_atoms(np.array([[5.0, 5.0, 5.0], [12.0, 14.0, 9.0]]), np.ones(2), np.full(2, 0.5), vel=np.zeros((2, 3)))
+    r.setup()
+    r.step(TILT_STEPS)
+    assert r.grid()[12] == want
