@@ -12,7 +12,11 @@
 // calls that stand for Rust's `mul_add`.
 //
 // Parity pins (where each function is checked; see tests/):
-//   x2lamda/lamda2x, pbc_triclinic  : domain.rs:1016-1058 KATs
+//   x2lamda/lamda2x, pbc_triclinic  : domain.rs:1016-1058 KATs; every local
+//                                     round-tripped through lamda, wrapped or not,
+//                                     and x2lamda's left-to-right subtraction, on
+//                                     atoms those misreadings would move
+//                                     differently (domain.rs:211-229,686-712)
 //   pbc / wrap_periodic             : the bound low + size computed at the wrap,
 //                                     on atoms at hi and at lo + size where the
 //                                     two differ (domain.rs:593-604)

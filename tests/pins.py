@@ -39,6 +39,10 @@ not these. Each states the reference's rule directly:
 * the Lees-Edwards tilt (crates/soil_deform/src/lib.rs:579-580):
   raw = rate * dt * step * ly, multiplied left to right, then
   xy = raw - lx * round(raw / lx);
+* the triclinic wrap (crates/soil_core/src/domain.rs:211-229, 686-712):
+  every local goes through x2lamda, the lamda wrap and lamda2x, wrapped or
+  not (the round trip can move it by an ulp), with x2lamda subtracting left
+  to right, x - x0 - xy*ly - xz*lz;
 * the cell of a position (crates/soil_core/src/neighbor.rs:886-888,
   915-922): floor((p - origin) * inv) with inv = 1.0 / bin_size computed
   once, which differs from dividing by bin_size for positions at some cell
@@ -406,3 +410,40 @@ def tilt_reference(rate, dt, step, lx, ly):
 def tilt_regroupings(rate, dt, step, ly):
     s = float(step)
     return [rate * (dt * s) * ly, (rate * ly) * (dt * s), rate * ly * dt * s, (rate * dt) * (s * ly)]
+
+
+# A box tilted by xy = 1.7 (boundary velocity 0.3) with non-zero x and y
+# lower bounds. Atom A is inside and its lamda round trip moves every
+# coordinate by an ulp; atom B sits on the sheared x face where the
+# reference's x2lamda keeps lx < 1 and x - (x0 + xy*ly) would wrap it.
+TRI_LO = (-1.5206, 0.3, 0.0)
+TRI_HI = (15.0177, 20.3, 20.0)
+TRI_XY, TRI_BVEL = 1.7, 0.3
+TRI_POS = np.array([[3.7638479564302947, 13.354988781294496, 12.359699890685233],
+                    [15.986470841224186, 11.69730401440221, 8.081364575891442]])
+
+
+def tri_pbc(r, regrouped=False, keep=False):
+    """(position, image delta) after pbc_triclinic for one atom of the
+    TRI box; `regrouped` and `keep` are the misreadings."""
+    size = [TRI_HI[k] - TRI_LO[k] for k in range(3)]
+    xy, xz, yz = TRI_XY, 0.0, 0.0
+    lz = (r[2] - TRI_LO[2]) / size[2]
+    ly = (r[1] - TRI_LO[1] - yz * lz) / size[1]
+    if regrouped:
+        lx = (r[0] - (TRI_LO[0] + xy * ly + xz * lz)) / size[0]
+    else:
+        lx = (r[0] - TRI_LO[0] - xy * ly - xz * lz) / size[0]
+    lam, delta = [lx, ly, lz], [0, 0, 0]
+    for d in range(3):
+        if lam[d] < 0.0:
+            lam[d] += 1.0
+            delta[d] = -1
+        elif lam[d] >= 1.0:
+            lam[d] -= 1.0
+            delta[d] = 1
+    if keep and delta == [0, 0, 0]:
+        return list(r), delta
+    return [TRI_LO[0] + size[0] * lam[0] + xy * lam[1] + xz * lam[2],
+            TRI_LO[1] + size[1] * lam[1] + yz * lam[2],
+            TRI_LO[2] + size[2] * lam[2]], delta

@@ -304,3 +304,26 @@ def test_lees_edwards_tilt_on_device(ctx):
     b.step(TILT_STEPS)
     assert b.grid()[12] == tilt_reference(TILT_RATE, d.dt, TILT_STEPS, 20.0, 20.0)
     b.close()
+
+
+def test_triclinic_wrap_on_device(ctx):
+    # every local round-trips through lamda at the triclinic wrap, and
+    # x2lamda subtracts left to right (pins.py lays the two atoms out)
+    from pins import TRI_BVEL, TRI_HI, TRI_LO, TRI_POS, TRI_XY, tri_pbc
+    print("case: device triclinic wrap", flush=True)
+    want = [tri_pbc(list(p)) for p in TRI_POS]
+    d = soil_amd._ffi.BoxDesc()
+    for k in range(3):
+        d.lo[k], d.hi[k], d.periodic[k] = TRI_LO[k], TRI_HI[k], 1
+    d.newton, d.skin_fraction, d.bin_min_size = 1, 1.1, 0.0
+    d.every, d.check, d.sort_every, d.dt = 0, 1, 1000, 1e-3
+    b = soil_amd.Box(ctx, d)
+    b.upload(TRI_POS.copy(), np.ones(2), np.full(2, 0.5), vel=np.zeros((2, 3)))
+    b.setup()
+    b.set_tilt(TRI_XY, TRI_BVEL)
+    b.step(1)
+    a = b.atoms()
+    o = np.argsort(a["tag"][:2])
+    assert np.array_equal(a["pos"][:2][o], np.array([w[0] for w in want]))
+    assert np.array_equal(a["image"][:2][o], np.array([w[1] for w in want], dtype=np.int32))
+    b.close()

@@ -517,3 +517,31 @@ def test_lees_edwards_tilt_product_order():
     r.setup()
     r.step(TILT_STEPS)
     assert r.grid()[12] == want
+
+
+def tri_expected():
+    from pins import TRI_POS, tri_pbc
+    want = [tri_pbc(list(p)) for p in TRI_POS]
+    # the case separates the misreadings: A moves only by the round trip, B
+    # wraps only under the regrouped x2lamda
+    assert want[0][0] != list(TRI_POS[0]) and tri_pbc(list(TRI_POS[0]), keep=True)[0] == list(TRI_POS[0])
+    assert want[1][1] == [0, 0, 0] and tri_pbc(list(TRI_POS[1]), regrouped=True)[1] == [1, 0, 0]
+    return np.array([w[0] for w in want]), np.array([w[1] for w in want], dtype=np.int32)
+
+
+def test_triclinic_wrap_round_trips_every_local():
+    from pins import TRI_BVEL, TRI_HI, TRI_LO, TRI_POS, TRI_XY
+    want_pos, want_img = tri_expected()
+    d = cutoff_desc()
+    d.skin_fraction = 1.1
+    for k in range(3):
+        d.lo[k], d.hi[k] = TRI_LO[k], TRI_HI[k]
+    r = RefBox(d)
+    r.add_atoms(TRI_POS.copy(), np.ones(2), np.full(2, 0.5), vel=np.zeros((2, 3)))
+    r.setup()
+    r.set_tilt(TRI_XY, TRI_BVEL)
+    r.step(1)
+    a = r.atoms()
+    o = np.argsort(a["tag"][:2])
+    assert np.array_equal(a["pos"][:2][o], want_pos)
+    assert np.array_equal(a["image"][:2][o], want_img)
