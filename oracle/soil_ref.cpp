@@ -50,7 +50,11 @@
 //                                     ghost = root value; #[zero] cleared; a
 //                                     tag-valued column follows its atom through
 //                                     every sort (docs/SOIL_ATOMDATA_CONTRACT.md
-//                                     :87-143); force balance is exact
+//                                     :87-143); force balance is exact; with
+//                                     inexact values, the sums in the reference's
+//                                     order (rounds last to first, low swap then
+//                                     high, comm.rs:962-1021) on a box whose z
+//                                     selections overlap and whose ghosts chain
 //   sort_atoms_by_bin               : stable-sort invariants vs numpy argsort; a
 //                                     periodic sort rebuilds exactly when that
 //                                     order is not the identity (neighbor.rs:989-996)

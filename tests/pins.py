@@ -43,6 +43,11 @@ not these. Each states the reference's rule directly:
   every local goes through x2lamda, the lamda wrap and lamda2x, wrapped or
   not (the round trip can move it by an ulp), with x2lamda subtracting left
   to right, x - x0 - xy*ly - xz*lz;
+* reverse comm's order (crates/soil_core/src/comm.rs:962-1021): rounds
+  (dims) from the last to the first, in each the low swap and then the high
+  swap adding their ghosts onto their sources in receive order, so a ghost
+  that is the source of later ghosts has their values before its own is
+  added on; with inexact values the order shows;
 * the cell of a position (crates/soil_core/src/neighbor.rs:886-888,
   915-922): floor((p - origin) * inv) with inv = 1.0 / bin_size computed
   once, which differs from dividing by bin_size for positions at some cell
@@ -447,3 +452,28 @@ def tri_pbc(r, regrouped=False, keep=False):
     return [TRI_LO[0] + size[0] * lam[0] + xy * lam[1] + xz * lam[2],
             TRI_LO[1] + size[1] * lam[1] + yz * lam[2],
             TRI_LO[2] + size[2] * lam[2]], delta
+
+
+def reverse_order(nswaps):
+    """Swap order of reverse_send_force: rounds of two from the last, low swap first."""
+    seq, end = [], nswaps
+    while end > 0:
+        start = max(end - 2, 0)
+        seq += list(range(start, end))
+        end = start
+    return seq
+
+
+def reverse_sequential(vals, swaps, origin, order=None):
+    """reverse_send_force's sums as the reference orders them (reverse_order,
+    or `order`), ghost by ghost in receive order, source += ghost. A round
+    packs both swaps' ghost values before adding either; no ghost of a round
+    is a source within it (its candidates precede the round's ghosts), so
+    adding as it goes is the same. `vals`: one row per atom (copied)."""
+    out = np.array(vals, dtype=np.float64, copy=True)
+    seq = reverse_order(len(swaps)) if order is None else order
+    for s in seq:
+        start, count = int(swaps[s][0]), int(swaps[s][1])
+        for g in range(start, start + count):
+            out[int(origin[g])] += out[g]
+    return out

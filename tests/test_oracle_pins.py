@@ -545,3 +545,35 @@ def test_triclinic_wrap_round_trips_every_local():
     o = np.argsort(a["tag"][:2])
     assert np.array_equal(a["pos"][:2][o], want_pos)
     assert np.array_equal(a["image"][:2][o], want_img)
+
+
+def test_reverse_comm_adds_in_the_reference_order():
+    # inexact #[reverse] values on a box thin in z (its two z selections
+    # overlap, so some owners receive from both z swaps) and with chained
+    # corner ghosts: the owners' sums equal the reference's sequence of adds,
+    # which the high-swap-first order and a per-root sum both miss
+    from pins import chain_roots, reverse_order, reverse_sequential
+    d, pos, vel, mass, rad = soil_amd.gen_custom(7070, 700, 0.45, 0.55, 0.4, (1, 1, 0.07), 0, 0.0, 0.0, 0.3, 0.0)
+    Lz = d.hi[2] - d.lo[2]
+    r = RefBox(d)
+    r.register_column(1, REV)
+    r.add_atoms(pos, mass, rad, vel=vel)
+    r.set_column(0, np.zeros(len(pos)), 1)
+    r.setup()
+    r.step(2)
+    a, sw = r.atoms(), r.swaps()
+    nl, ng = a["nlocal"], a["nghost"]
+    gc = r.grid()[9]
+    assert gc < Lz < 2 * gc, (gc, Lz)  # the z selections overlap and need one image each
+    rng = np.random.default_rng(9)
+    vals = rng.normal(size=nl + ng) * 10.0 ** rng.integers(-6, 7, nl + ng)
+    r.set_column(0, vals, 1)
+    r.phase("reverse_send_force")
+    got = r.column(0)
+    want = reverse_sequential(vals, sw, a["origin"])
+    assert np.array_equal(got, want)
+    assert reverse_order(6) == [4, 5, 2, 3, 0, 1]
+    hi_first = reverse_sequential(vals, sw, a["origin"], order=list(range(len(sw) - 1, -1, -1)))
+    per_root = vals[:nl].copy()
+    np.add.at(per_root, chain_roots(a["origin"], nl)[nl:], vals[nl:])
+    assert not np.array_equal(hi_first[:nl], want[:nl]) and not np.array_equal(per_root, want[:nl])
