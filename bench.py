@@ -30,9 +30,11 @@ sys.path.insert(0, ROOT)
 
 from soil_amd import roofline as rl  # noqa: E402
 
-# HIP events around every 8th rebuild's CSR build: each event costs a few
-# microseconds of the step, so sampling keeps them out of most timed steps
-TIMING_PERIOD = 8
+# The plain leg's throughput is timed with no HIP events in its steps (each
+# event costs a few microseconds of the step); the CSR build is timed in a
+# second run of the same box of at least ROOFLINE_STEPS steps with events
+# around every build.
+ROOFLINE_STEPS = 100
 LATENCY_BOUND_MS = 5.0   # stated p99 step-latency bound of the served stream
 SERVED_RATES = (6.0, 24.0, 96.0)
 
@@ -87,14 +89,24 @@ def measured_traffic(kernels):
         return None, None
     total, found = 0.0, 0
     for k in kernels:
-        for name, v in data.get("kernels", {}).items():
-            if k + "(" in name or name.startswith(k) or ("::" + k + "(") in name:
-                total += v["read_bytes"] + v["write_bytes"]
-                found += 1
-                break
+        # a kernel's entries by its unqualified name (template arguments
+        # aside); of several instantiations, the one dispatched most
+        hits = [v for name, v in data.get("kernels", {}).items() if kernel_base_name(name) == k]
+        if hits:
+            v = max(hits, key=lambda h: h.get("dispatches", 0))
+            total += v["read_bytes"] + v["write_bytes"]
+            found += 1
     if found != len(kernels):
         return None, None
     return total, os.path.relpath(TRAFFIC_FILE, ROOT)
+
+
+def kernel_base_name(name):
+    """'void ns::k_foo<true>(args...)' -> 'k_foo'."""
+    n = name.split("(")[0]
+    if n.startswith("void "):
+        n = n[5:]
+    return n.split("<")[0].rsplit("::", 1)[-1]
 
 
 def box_counts(b):
@@ -122,6 +134,7 @@ def run_box(ctx, gen, steps, warmup, period, consumer=True):
     box.sync()
     el = time.perf_counter() - t0
     tm = box.timing()
+    tm["period"] = period
     N, G, C, P, r1 = box_counts(box)
     box.close()
     rebuilds = r1 - r0
@@ -160,7 +173,7 @@ def csr_roofline(tm):
         "traffic_source": src,
         "algorithmic_bytes_per_launch": per,
         "launches_timed": nb,
-        "timing_period": TIMING_PERIOD,
+        "timing_period": tm["period"],
         "avg_launch_ms": build_ms,
         "count_ms": count_ms,
         "scan_ms": build_ms - count_ms - fill_ms,
@@ -442,17 +455,21 @@ def main():
         c2_gen = soil_amd.gen_box(2)
     if "plain" in parts:
         log("plain: 1M box")
-        res = run_box(ctx, c2_gen, args.steps, args.warmup, TIMING_PERIOD)
+        res = run_box(ctx, c2_gen, args.steps, args.warmup, 0)
+        # the build's roofline: the same box from the same start, every build timed
+        rf = run_box(ctx, c2_gen, max(args.steps, ROOFLINE_STEPS), args.warmup, 1)
         out["value"] = res["particle_steps_per_s"]
         out["ms_per_step"] = res["ms_per_step"]
         out["config"]["rebuilds_in_timed_steps"] = res["rebuilds"]
         out["config"]["nghost"] = res["nghost"]
         out["config"]["npairs"] = res["npairs"]
-        out["roofline"] = csr_roofline(res["csr"])
+        out["roofline"] = csr_roofline(rf["csr"])
+        if out["roofline"] is not None:
+            out["roofline"]["timed_run_steps"] = max(args.steps, ROOFLINE_STEPS)
         out["step_roofline"] = res["step_roofline"]
         # the same box without the consumer: the fill is then the CSR's own
         # placement (no transpose slots), so count + scan + fill is the build alone
-        if args.csr_steps > 0:
+        if args.csr_steps > 0 and out["roofline"] is not None:
             pure = run_box(ctx, c2_gen, args.csr_steps, args.warmup, 1, consumer=False)  # every build timed
             out["roofline"]["without_consumer"] = csr_roofline(pure["csr"])
         log(f"plain: {out['value']:.3e} particle-steps/s, {res['ms_per_step']:.3f} ms/step")
