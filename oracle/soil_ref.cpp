@@ -70,7 +70,10 @@
 //   step state machine              : the run_if placement is read from the
 //                                     plugin sources (tests/test_integration_doc.py
 //                                     parses them); the grass scheduler's dispatch
-//                                     timing is external to the reference tree.
+//                                     timing is external to the reference tree. A
+//                                     periodic sort that rebuilds a CommunicateOnly
+//                                     step leaves an atom past a face unwrapped
+//                                     (pbc's guard was already evaluated)
 
 #include <cmath>
 #include <cstdint>

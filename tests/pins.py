@@ -55,6 +55,11 @@ not these. Each states the reference's rule directly:
 * velocity Verlet's kick and drift (crates/soil_verlet/src/lib.rs:143-160,
   183-192): h = 0.5 * dt * inv_mass left to right, v0 = v + h*f and
   x = x + v0*dt, no fused multiply-add anywhere;
+* a periodic sort on a CommunicateOnly step (neighbor.rs:845-848, 989-996;
+  the run_if guards of pbc and remove_ghost_atoms, domain.rs:441-449,
+  atom.rs:613-619): the sort sets FullRebuild after pbc was skipped, so the
+  step rebuilds from unwrapped positions and an atom past a periodic face
+  stays there, its image unchanged, until a later FullRebuild's pbc;
 * the CSR row order (crates/soil_core/src/neighbor.rs:1175-1304, stencil
   generation :371-389 and :461-492): row i lists its own cell first (the
   half list only the records after i's own, the full list all but i), then
@@ -477,3 +482,20 @@ def reverse_sequential(vals, swaps, origin, order=None):
         for g in range(start, start + count):
             out[int(origin[g])] += out[g]
     return out
+
+
+# Two atoms in a periodic 20-cube (radius 0.5, sf 1.1, sort_every 3): A moves
+# +x at 10 from x = 19.975 and crosses the face on step 3 by less than the
+# displacement threshold; B rests in A's cell column at a larger y, so A's
+# move into the next cell reverses their cell order and step 3's periodic
+# sort rebuilds.
+PSORT_EVERY = 3
+PSORT_POS = np.array([[19.975, 1.0, 10.0], [19.5, 15.0, 10.0]])
+PSORT_VEL = np.array([[10.0, 0.0, 0.0], [0.0, 0.0, 0.0]])
+
+
+def psort_x_after(steps, dt=1e-3):
+    x = PSORT_POS[0, 0]
+    for _ in range(steps):
+        x = x + PSORT_VEL[0, 0] * dt
+    return x

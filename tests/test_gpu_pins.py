@@ -327,3 +327,30 @@ def test_triclinic_wrap_on_device(ctx):
     assert np.array_equal(a["pos"][:2][o], np.array([w[0] for w in want]))
     assert np.array_equal(a["image"][:2][o], np.array([w[1] for w in want], dtype=np.int32))
     b.close()
+
+
+def test_periodic_sort_step_on_device(ctx):
+    # a periodic sort that rebuilds a CommunicateOnly step leaves the atom
+    # that crossed a face unwrapped (pbc was skipped), on the device
+    from pins import PSORT_EVERY, PSORT_POS, PSORT_VEL, psort_x_after
+    print("case: device periodic sort without pbc", flush=True)
+    d = soil_amd._ffi.BoxDesc()
+    for k in range(3):
+        d.lo[k], d.hi[k], d.periodic[k] = 0.0, 20.0, 1
+    d.newton, d.skin_fraction, d.bin_min_size = 1, 1.1, 0.0
+    d.every, d.check, d.sort_every, d.dt = 0, 1, PSORT_EVERY, 1e-3
+    b = soil_amd.Box(ctx, d)
+    b.upload(PSORT_POS.copy(), np.ones(2), np.full(2, 0.5), vel=PSORT_VEL.copy())
+    b.setup()
+    b.step(2)
+    assert b.counts()["rebuilds"] == 1
+    for s, rebuilt in ((3, 2), (4, 2)):
+        b.step(1)
+        assert b.counts()["rebuilds"] == rebuilt, s
+        a = b.atoms()
+        k = int(np.nonzero(a["tag"][:2] == 0)[0][0])
+        assert a["pos"][k, 0] == psort_x_after(s) and a["pos"][k, 0] >= 20.0, s
+        assert a["image"][k, 0] == 0
+        if s == 3:
+            assert k == 1
+    b.close()

@@ -577,3 +577,26 @@ def test_reverse_comm_adds_in_the_reference_order():
     per_root = vals[:nl].copy()
     np.add.at(per_root, chain_roots(a["origin"], nl)[nl:], vals[nl:])
     assert not np.array_equal(hi_first[:nl], want[:nl]) and not np.array_equal(per_root, want[:nl])
+
+
+def test_periodic_sort_step_rebuilds_from_unwrapped_positions():
+    # step 3 is CommunicateOnly until its periodic sort finds A's new cell
+    # out of order: it rebuilds, but pbc already skipped, so A stays past the
+    # face with its image unchanged (and on step 4, no rebuild, still there)
+    from pins import PSORT_EVERY, PSORT_POS, PSORT_VEL, psort_x_after
+    d = cutoff_desc()
+    d.skin_fraction, d.sort_every = 1.1, PSORT_EVERY
+    r = RefBox(d)
+    r.add_atoms(PSORT_POS.copy(), np.ones(2), np.full(2, 0.5), vel=PSORT_VEL.copy())
+    r.setup()
+    r.step(2)
+    assert r.counts()["rebuilds"] == 1
+    for s, rebuilt in ((3, 2), (4, 2)):
+        r.step(1)
+        assert r.counts()["rebuilds"] == rebuilt, s
+        a = r.atoms()
+        k = int(np.nonzero(a["tag"][:2] == 0)[0][0])
+        assert a["pos"][k, 0] == psort_x_after(s) and a["pos"][k, 0] >= 20.0, s
+        assert a["image"][k, 0] == 0
+        if s == 3:
+            assert k == 1  # the sort put A after B
