@@ -72,8 +72,12 @@ def csr_mode():
 
 
 def csr_kernel_text(mode):
-    staged = {4: "dense wave regions", 5: "rows kept in LDS, staged densely per tile"}.get(mode, "row slots")
-    scan = {4: "scan of wave totals", 5: "scan of tile totals"}.get(mode, "offset scan")
+    if mode == 5:  # no scan launch: the placement sums the tile totals itself (scan_ms is the events' gap)
+        return ("CSR build: k_csr_tcount (candidate tests, row counts and rows kept in LDS, staged densely per "
+                "tile; tile totals and group sums) + k_csr_tplace (tile bases from the group sums, placement, with "
+                "the consumer's transpose slots)")
+    staged = {4: "dense wave regions"}.get(mode, "row slots")
+    scan = {4: "scan of wave totals"}.get(mode, "offset scan")
     return (f"CSR build: {COUNT_KERNEL[mode]} (candidate tests, row counts and {staged}) + {scan} + "
             f"{FILL_KERNEL[mode]} (placement, with the consumer's transpose slots)")
 
