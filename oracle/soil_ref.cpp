@@ -66,7 +66,8 @@
 //                                     atoms whose decision the two forms split
 //   apply_deform (LE xy)            : closed form soil_deform/src/lib.rs:576-583;
 //                                     the left-to-right product of :579, on a rate
-//                                     and step every regrouping rounds differently
+//                                     and step every regrouping rounds differently;
+//                                     the streaming profile rate * (y - yc) (:568-574)
 //   step state machine              : the run_if placement is read from the
 //                                     plugin sources (tests/test_integration_doc.py
 //                                     parses them); the grass scheduler's dispatch

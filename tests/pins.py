@@ -48,6 +48,8 @@ not these. Each states the reference's rule directly:
   swap adding their ghosts onto their sources in receive order, so a ghost
   that is the source of later ghosts has their values before its own is
   added on; with inexact values the order shows;
+* the one-time streaming profile (crates/soil_deform/src/lib.rs:568-574):
+  v_x += rate * (y - 0.5 * (ylo + yhi)), the difference taken first;
 * the cell of a position (crates/soil_core/src/neighbor.rs:886-888,
   915-922): floor((p - origin) * inv) with inv = 1.0 / bin_size computed
   once, which differs from dividing by bin_size for positions at some cell
@@ -499,3 +501,12 @@ def psort_x_after(steps, dt=1e-3):
     for _ in range(steps):
         x = x + PSORT_VEL[0, 0] * dt
     return x
+
+
+# An atom at rest in a Lees-Edwards 20-cube (rate TILT_RATE) at a height
+# where rate * (y - yc) and rate * y - rate * yc round differently.
+PROFILE_Y = 4.984913705011256
+
+
+def profile_vx(rate, y, ylo=0.0, yhi=20.0):
+    return 0.0 + rate * (y - 0.5 * (ylo + yhi))

@@ -354,3 +354,21 @@ def test_periodic_sort_step_on_device(ctx):
         if s == 3:
             assert k == 1
     b.close()
+
+
+def test_streaming_profile_on_device(ctx):
+    # the device's one-time shear profile takes y - yc first
+    from pins import PROFILE_Y, TILT_RATE, profile_vx
+    print("case: device streaming profile", flush=True)
+    d = soil_amd._ffi.BoxDesc()
+    for k in range(3):
+        d.lo[k], d.hi[k], d.periodic[k] = 0.0, 20.0, 1
+    d.newton, d.skin_fraction, d.bin_min_size = 1, 1.1, 0.0
+    d.every, d.check, d.sort_every, d.dt = 0, 1, 1000, 1e-3
+    d.has_shear, d.shear_rate = 1, TILT_RATE
+    b = soil_amd.Box(ctx, d)
+    b.upload(np.array([[10.0, PROFILE_Y, 10.0]]), np.ones(1), np.full(1, 0.5), vel=np.zeros((1, 3)))
+    b.setup()
+    b.step(1)
+    assert b.atoms()["vel"][0, 0] == profile_vx(TILT_RATE, PROFILE_Y)
+    b.close()

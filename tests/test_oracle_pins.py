@@ -600,3 +600,16 @@ def test_periodic_sort_step_rebuilds_from_unwrapped_positions():
         assert a["image"][k, 0] == 0
         if s == 3:
             assert k == 1  # the sort put A after B
+
+
+def test_streaming_profile_takes_the_difference_first():
+    # the first sheared step gives a resting atom v_x = rate * (y - yc),
+    # which the expanded form rate * y - rate * yc would round differently
+    from pins import PROFILE_Y, TILT_RATE, profile_vx
+    want = profile_vx(TILT_RATE, PROFILE_Y)
+    assert want != TILT_RATE * PROFILE_Y - TILT_RATE * 10.0
+    r = RefBox(tilt_desc())
+    r.add_atoms(np.array([[10.0, PROFILE_Y, 10.0]]), np.ones(1), np.full(1, 0.5), vel=np.zeros((1, 3)))
+    r.setup()
+    r.step(1)
+    assert r.atoms()["vel"][0, 0] == want
