@@ -41,7 +41,9 @@
 //   borders                         : brute-force image enumeration, swap rules;
 //                                     face thresholds lo + gc / hi - gc rounded
 //                                     once, on atoms where a difference form
-//                                     would not send them (comm.rs:336-347)
+//                                     would not send them (comm.rs:336-347); the
+//                                     tilted low-x face (gc / Lx) * inflate, on an
+//                                     atom gc * inflate / Lx would not send
 //   forward_comm_borders            : every ghost = source + the build's fixed
 //                                     offset, pos and vel, chained swap by swap
 //                                     (comm.rs:400-432; tests/pins.py)

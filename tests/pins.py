@@ -50,6 +50,9 @@ not these. Each states the reference's rule directly:
   added on; with inexact values the order shows;
 * the one-time streaming profile (crates/soil_deform/src/lib.rs:568-574):
   v_x += rate * (y - 0.5 * (ylo + yhi)), the difference taken first;
+* the triclinic ghost faces (crates/soil_core/src/comm.rs:305-320): in
+  lamda, the low x face's threshold is lo + (gc / Lx) * (1 + |xy| / Ly),
+  the quotient taken before the inflation;
 * the cell of a position (crates/soil_core/src/neighbor.rs:886-888,
   915-922): floor((p - origin) * inv) with inv = 1.0 / bin_size computed
   once, which differs from dividing by bin_size for positions at some cell
@@ -510,3 +513,21 @@ PROFILE_Y = 4.984913705011256
 
 def profile_vx(rate, y, ylo=0.0, yhi=20.0):
     return 0.0 + rate * (y - 0.5 * (ylo + yhi))
+
+
+# One atom in a periodic 20-cube tilted by xy = 3.86 whose lamda x lies
+# between the reference's low-face threshold (gc / Lx) * inflate and the
+# regrouped gc * inflate / Lx: the reference sends it, the regrouping would not.
+TRIF_XY = 3.86
+TRIF_POS = np.array([[2.640418637, 6.263309, 10.0]])
+
+
+def trif_lx(p, xy=TRIF_XY, L=20.0):
+    lz = (p[2] - 0.0) / L
+    ly = (p[1] - 0.0 - 0.0 * lz) / L
+    return (p[0] - 0.0 - xy * ly - 0.0 * lz) / L
+
+
+def trif_thresholds(gc, xy=TRIF_XY, L=20.0):
+    inflate = 1.0 + abs(xy) / L
+    return 0.0 + (gc / L) * inflate, gc * inflate / L

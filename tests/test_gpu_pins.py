@@ -372,3 +372,26 @@ def test_streaming_profile_on_device(ctx):
     b.step(1)
     assert b.atoms()["vel"][0, 0] == profile_vx(TILT_RATE, PROFILE_Y)
     b.close()
+
+
+def test_triclinic_ghost_face_on_device(ctx):
+    # the device sends the atom between the two forms of the tilted low-x
+    # face threshold (pins.py lays it out)
+    from pins import TRIF_POS, TRIF_XY, sent_by_swap, trif_lx, trif_thresholds
+    print("case: device triclinic ghost face", flush=True)
+    d = soil_amd._ffi.BoxDesc()
+    for k in range(3):
+        d.lo[k], d.hi[k], d.periodic[k] = 0.0, 20.0, 1
+    d.newton, d.skin_fraction, d.bin_min_size = 1, 1.1, 0.0
+    d.every, d.check, d.sort_every, d.dt = 0, 1, 1000, 1e-3
+    b = soil_amd.Box(ctx, d)
+    b.upload(TRIF_POS.copy(), np.ones(1), np.full(1, 0.5), vel=np.zeros((1, 3)))
+    b.setup()
+    b.set_tilt(TRIF_XY, 0.0)
+    b.step(1)
+    a = b.atoms()
+    t_ref, t_alt = trif_thresholds(b.grid()[9])
+    lx = trif_lx(a["pos"][0])
+    assert lx < t_ref and not lx < t_alt
+    assert sent_by_swap(a, b.swaps()).get(0) == {0}
+    b.close()

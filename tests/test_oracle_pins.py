@@ -613,3 +613,19 @@ def test_streaming_profile_takes_the_difference_first():
     r.setup()
     r.step(1)
     assert r.atoms()["vel"][0, 0] == want
+
+
+def test_triclinic_ghost_face_divides_before_inflating():
+    from pins import TRIF_POS, TRIF_XY, sent_by_swap, trif_lx, trif_thresholds
+    d = cutoff_desc()
+    d.skin_fraction = 1.1
+    r = RefBox(d)
+    r.add_atoms(TRIF_POS.copy(), np.ones(1), np.full(1, 0.5), vel=np.zeros((1, 3)))
+    r.setup()
+    r.set_tilt(TRIF_XY, 0.0)
+    r.step(1)
+    a = r.atoms()
+    t_ref, t_alt = trif_thresholds(r.grid()[9])
+    lx = trif_lx(a["pos"][0])
+    assert lx < t_ref and not lx < t_alt  # the case separates the two forms
+    assert sent_by_swap(a, r.swaps()).get(0) == {0}
